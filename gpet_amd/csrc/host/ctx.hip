@@ -1,0 +1,786 @@
+/*
+ * ctx.hip - the C ABI context: device state, run orchestration, outputs.
+ *
+ * Restates the host side of the reference (init/iniPanel/iniwck in
+ * initialize.cu, simulateParticle/sampleParticle in gPET.cu) around the HIP
+ * kernels of transport.hip and digitizer.hip.  Differences by design:
+ *   - status codes instead of exit(); every HIP call is checked;
+ *   - decays are sampled directly (O(decays), not O(atoms): F8/Q17);
+ *   - all events of a run stay in HBM and the digitizer runs once over them,
+ *     so deadtime and coincidences see one global time order instead of
+ *     per-batch orders (gPET.cu:260-428 digitizes each epoch separately);
+ *   - outputs are truncated, not appended (Q21).
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../../include/gpet.h"
+#include "../common/gp_math.h"
+#include "internal.h"
+
+namespace {
+
+constexpr int64_t DEFAULT_BATCH = 1 << 22;
+
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    int64_t cap = 0;
+};
+
+} // namespace
+
+struct gpet_ctx {
+    gpet_options opt;
+    std::string err;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    std::vector<hipEvent_t> kev; /* per-launch timing events */
+
+    /* tables */
+    bool has_tab = false;
+    int nmat = 0;
+    float eminph = 0, emax = 0;
+    float e0 = 0, idl = 0, lam_idl = 0;
+    std::vector<float> h_lamph; /* host copy for majorant construction */
+    DBuf<float4> d_mu4;
+    DBuf<float> d_surf[2];
+    DevTables dtab;
+
+    /* phantom */
+    bool has_ph = false;
+    DBuf<int2> d_vox;
+    DBuf<float> d_wck_ph;
+    std::vector<float> h_wck_ph;
+    DevPhantom dph;
+
+    /* detector */
+    bool has_det = false;
+    DBuf<DevPanel> d_panels;
+    DBuf<float> d_wck_det;
+    std::vector<float> h_wck_det;
+    std::vector<gpet_panel> panels;
+    DevDetector ddet;
+
+    /* digitizer */
+    bool has_dig = false;
+    gpet_digitizer dig;
+
+    /* run buffers */
+    DBuf<float> q_f[6];
+    DBuf<double> q_t;
+    DBuf<int4> q_info;
+    DBuf<int> q_slot;
+    int64_t q_cap = 0;
+    DBuf<unsigned> d_small;      /* [0] queue count, [1] events, [2] hits */
+    DBuf<unsigned long long> d_cnt;
+    DBuf<gpet_event> d_events;
+    DBuf<gpet_trace> d_trace;
+    DBuf<int32_t> d_hit_id;
+    DBuf<float> d_hit_f;
+    DBuf<int64_t> d_first;
+    DBuf<DevSource> d_src;
+    DBuf<DevIsotope> d_iso;
+    DBuf<double> d_psf;
+    DigiWork *dw = nullptr;
+
+    /* last run */
+    gpet_tallies last;
+    DigiOut dout;
+    int64_t n_events = 0;
+};
+
+namespace {
+
+gpet_status fail(gpet_ctx *c, gpet_status s, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return s;
+}
+
+#define HIPOK(c, x)                                                                                    \
+    do {                                                                                               \
+        hipError_t e_ = (x);                                                                           \
+        if (e_ != hipSuccess)                                                                          \
+            return fail((c), GPET_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+template <class T>
+gpet_status dalloc(gpet_ctx *c, DBuf<T> &b, int64_t n)
+{
+    if (n <= b.cap && b.p) return GPET_OK;
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+    int64_t m = std::max<int64_t>(n, 1);
+    hipError_t e = hipMalloc((void **)&b.p, sizeof(T) * (size_t)m);
+    if (e != hipSuccess) return fail(c, GPET_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+    b.cap = m;
+    return GPET_OK;
+}
+
+template <class T>
+void dfree(DBuf<T> &b)
+{
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+/* Grow keeping the first `keep` elements (stream-ordered copy). */
+template <class T>
+gpet_status dgrow(gpet_ctx *c, DBuf<T> &b, int64_t need, int64_t keep)
+{
+    if (need <= b.cap && b.p) return GPET_OK;
+    DBuf<T> nb;
+    gpet_status st = dalloc(c, nb, std::max<int64_t>(need, b.cap + b.cap / 2));
+    if (st) return st;
+    if (b.p && keep > 0) HIPOK(c, hipMemcpyAsync(nb.p, b.p, sizeof(T) * (size_t)keep, hipMemcpyDeviceToDevice, c->stream));
+    HIPOK(c, hipStreamSynchronize(c->stream));
+    dfree(b);
+    b = nb;
+    return GPET_OK;
+}
+
+template <class T>
+gpet_status upload(gpet_ctx *c, DBuf<T> &b, const T *h, int64_t n)
+{
+    gpet_status st = dalloc(c, b, n);
+    if (st) return st;
+    if (n > 0) HIPOK(c, hipMemcpyAsync(b.p, h, sizeof(T) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    HIPOK(c, hipStreamSynchronize(c->stream));
+    return GPET_OK;
+}
+
+/* ---- Woodcock majorant (iniwck, initialize.cu:773-829, 919-966) ---------
+ * Same node grid as the reference; the node value is 1/max of the linearly
+ * interpolated mu*rho_max over both adjacent intervals (exact maximum of a
+ * piecewise-linear function) shrunk by 1e-5, so that lambda*mu <= 1 holds
+ * everywhere for the lerped tables the kernels use (Q18 fixed). */
+float host_lerp(const float *T, int n, float e0, float idl, float E)
+{
+    float x = (E - e0) * idl;
+    x = std::fmin(std::fmax(x, 0.0f), (float)(n - 1));
+    int j = (int)x;
+    if (j > n - 2) j = n - 2;
+    float f = x - (float)j;
+    return T[j] + f * (T[j + 1] - T[j]);
+}
+
+double mu_lin_max(const gpet_ctx *c, const float *rho_max, double E)
+{
+    double best = 0.0;
+    for (int m = 0; m < c->nmat; ++m) {
+        if (rho_max[m] <= 0.0f) continue;
+        double v = (double)host_lerp(c->h_lamph.data() + (size_t)m * GPET_NTAB, GPET_NTAB, c->e0, c->idl, (float)E) *
+                   rho_max[m];
+        if (v > best) best = v;
+    }
+    return best;
+}
+
+void build_majorant(const gpet_ctx *c, const float *rho_max, std::vector<float> &lam, float &we0, float &widl)
+{
+    we0 = c->eminph;
+    float de = (c->emax * (1.0f - 1.0e-10f) - we0) / (float)GPET_NWCK;
+    widl = 1.0f / de;
+    lam.assign(GPET_NWCK, 0.0f);
+    const double d_node = 1.0 / (double)c->idl;
+    std::vector<double> imax(GPET_NWCK + 1, 0.0);
+    for (int i = 0; i <= GPET_NWCK; ++i) {
+        double lo = (double)we0 + (double)de * i;
+        double hi = (double)we0 + (double)de * (i + 1);
+        double m = std::max(mu_lin_max(c, rho_max, lo), mu_lin_max(c, rho_max, hi));
+        double k0 = std::ceil((lo - c->e0) / d_node);
+        double k1 = std::floor((hi - c->e0) / d_node);
+        for (double k = k0; k <= k1; k += 1.0) m = std::max(m, mu_lin_max(c, rho_max, (double)c->e0 + k * d_node));
+        imax[i] = m;
+    }
+    for (int i = 0; i < GPET_NWCK; ++i) {
+        double m = imax[i];
+        if (i > 0) m = std::max(m, imax[i - 1]);
+        lam[i] = (m > 0.0) ? (float)(1.0 / (m * (1.0 + 1.0e-5))) : 1.0e20f;
+    }
+}
+
+/* ---- source plan (restates the sampling rule documented in gpet.h) ----- */
+
+int64_t binomial_draw(double n, double p, uint64_t seed, int src)
+{
+    if (n <= 0.0 || p <= 0.0) return 0;
+    if (p >= 1.0) return (int64_t)std::floor(n);
+    gp_rng r;
+    gp_rng_init(&r, seed, (uint64_t)src, 15u, 0u);
+    double mean = n * p;
+    double u = gp_rng_uniform_d(&r);
+    if (mean < 200.0) {
+        double pk = std::exp(n * std::log1p(-p));
+        double cdf = pk;
+        int64_t k = 0;
+        while (u > cdf && (double)k < n) {
+            pk = pk * (n - (double)k) / (double)(k + 1) * p / (1.0 - p);
+            k++;
+            cdf += pk;
+            if (pk < 1e-300 && (double)k > mean) break;
+        }
+        return k;
+    }
+    double u2 = gp_rng_uniform_d(&r);
+    double z = std::sqrt(-2.0 * std::log(u)) * std::cos(6.283185307179586 * u2);
+    double k = std::floor(mean + std::sqrt(mean * (1.0 - p)) * z + 0.5);
+    if (k < 0.0) k = 0.0;
+    if (k > n) k = std::floor(n);
+    return (int64_t)k;
+}
+
+gpet_status plan_sources(gpet_ctx *c, const gpet_source *src, int ns, const gpet_isotope *iso, int niso, double t0,
+                         double t1, int64_t n_decays, std::vector<int64_t> &first, std::vector<DevSource> &ds)
+{
+    first.assign(ns + 1, 0);
+    ds.assign(ns, DevSource());
+    std::vector<double> expect(ns, 0.0), navail(ns, 0.0);
+    for (int i = 0; i < ns; ++i) {
+        if (src[i].isotope < 0 || src[i].isotope >= niso) return fail(c, GPET_ERR_ARG, "source isotope out of range");
+        double th = iso[src[i].isotope].half_life_s;
+        if (!(th > 0.0)) return fail(c, GPET_ERR_ARG, "half life must be positive");
+        ds[i].tau = th / std::log(2.0);
+        ds[i].pdec = -std::expm1(-(t1 - t0) / ds[i].tau);
+        ds[i].shape = src[i].shape;
+        ds[i].iso = src[i].isotope;
+        for (int k = 0; k < 6; ++k) ds[i].coef[k] = src[i].coef[k];
+        navail[i] = std::floor(src[i].natom * std::exp2(-t0 / th));
+        expect[i] = navail[i] * ds[i].pdec;
+    }
+    std::vector<int64_t> cnt(ns, 0);
+    if (n_decays > 0) {
+        double tot = 0;
+        for (double e : expect) tot += e;
+        if (tot <= 0) return fail(c, GPET_ERR_ARG, "sources have no activity");
+        int64_t assigned = 0;
+        std::vector<std::pair<double, int>> rem;
+        for (int i = 0; i < ns; ++i) {
+            double share = (double)n_decays * expect[i] / tot;
+            cnt[i] = (int64_t)std::floor(share);
+            assigned += cnt[i];
+            rem.push_back({share - (double)cnt[i], i});
+        }
+        std::stable_sort(rem.begin(), rem.end(),
+                         [](const std::pair<double, int> &a, const std::pair<double, int> &b) { return a.first > b.first; });
+        for (int k = 0; assigned < n_decays; ++k, ++assigned) cnt[rem[k % ns].second]++;
+    } else {
+        for (int i = 0; i < ns; ++i) cnt[i] = binomial_draw(navail[i], ds[i].pdec, c->opt.seed, i);
+    }
+    for (int i = 0; i < ns; ++i) first[i + 1] = first[i] + cnt[i];
+    return GPET_OK;
+}
+
+bool ev_less_id(const gpet_event &a, const gpet_event &b)
+{
+    if (a.parn != b.parn) return a.parn < b.parn;
+    return a.siten < b.siten;
+}
+
+gpet_status check_ready(gpet_ctx *c)
+{
+    if (!c) return GPET_ERR_ARG;
+    if (!c->has_tab) return fail(c, GPET_ERR_STATE, "tables not set");
+    if (!c->has_dig) return fail(c, GPET_ERR_STATE, "digitizer not set");
+    return GPET_OK;
+}
+
+gpet_status write_outputs(gpet_ctx *c, const std::string &dir, const std::vector<gpet_event> &adder,
+                          const std::vector<gpet_event> &singles, const std::vector<gpet_event> &coinc,
+                          const std::vector<int32_t> &hid, const std::vector<float> &hf)
+{
+    gpet_status st = gpet_write_events((dir + "/adder.dat").c_str(), adder.data(), (int64_t)adder.size(), 0);
+    if (!st) st = gpet_write_events((dir + "/singles.dat").c_str(), singles.data(), (int64_t)singles.size(), 0);
+    if (!st) st = gpet_write_events((dir + "/coincidences.dat").c_str(), coinc.data(), (int64_t)coinc.size(), 0);
+    if (!st && c->opt.record_hits)
+        st = gpet_write_hits((dir + "/HitsID.dat").c_str(), (dir + "/Hits.dat").c_str(), hid.data(), hf.data(),
+                             (int64_t)hid.size() / 5, 0);
+    if (st) return fail(c, st, "cannot write outputs to " + dir);
+    return GPET_OK;
+}
+
+/* The batch loop shared by source and PSF runs. */
+gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sink)
+{
+    gpet_status st;
+    const int64_t batch = c->opt.batch > 0 ? c->opt.batch : DEFAULT_BATCH;
+    const int64_t qcap = 2 * std::min<int64_t>(batch, std::max<int64_t>(n_hist, 1));
+    if (qcap > c->q_cap) {
+        for (int k = 0; k < 6; ++k)
+            if ((st = dalloc(c, c->q_f[k], qcap))) return st;
+        if ((st = dalloc(c, c->q_t, qcap)) || (st = dalloc(c, c->q_info, qcap))) return st;
+        c->q_cap = qcap;
+    }
+    const bool want_trace = sink && sink->trace && sink->trace_cap > 0;
+    int64_t ntr = want_trace ? std::min<int64_t>(2 * n_hist, sink->trace_cap) : 0;
+    if (want_trace) {
+        if ((st = dalloc(c, c->d_trace, ntr)) || (st = dalloc(c, c->q_slot, qcap))) return st;
+        HIPOK(c, hipMemsetAsync(c->d_trace.p, 0, sizeof(gpet_trace) * (size_t)ntr, c->stream));
+    }
+    const bool want_hits = c->opt.record_hits != 0;
+    int64_t hits_cap = 0;
+    if (want_hits) {
+        hits_cap = (sink && sink->hit_id && sink->hits_cap > 0) ? sink->hits_cap : (int64_t)1 << 24;
+        if ((st = dalloc(c, c->d_hit_id, 5 * hits_cap)) || (st = dalloc(c, c->d_hit_f, 5 * hits_cap))) return st;
+    }
+    if ((st = dalloc(c, c->d_small, 4)) || (st = dalloc(c, c->d_cnt, CNT_NUM))) return st;
+    HIPOK(c, hipMemsetAsync(c->d_small.p, 0, sizeof(unsigned) * 4, c->stream));
+    HIPOK(c, hipMemsetAsync(c->d_cnt.p, 0, sizeof(unsigned long long) * CNT_NUM, c->stream));
+    if ((st = dgrow(c, c->d_events, std::max<int64_t>(1 << 20, n_hist / 4), 0))) return st;
+
+    DevQueue q;
+    q.y = c->q_f[0].p; q.z = c->q_f[1].p; q.vx = c->q_f[2].p; q.vy = c->q_f[3].p; q.vz = c->q_f[4].p; q.E = c->q_f[5].p;
+    q.t = c->q_t.p;
+    q.info = c->q_info.p;
+    q.trace_slot = want_trace ? c->q_slot.p : nullptr;
+    q.count = (int *)c->d_small.p;
+    q.cap = qcap;
+
+    rp.seed = c->opt.seed;
+    rp.eabs = c->opt.eabs;
+    rp.sigma = c->opt.noncol_sigma;
+    rp.prange = c->opt.positron_range;
+    rp.trace = want_trace ? c->d_trace.p : nullptr;
+    rp.trace_cap = ntr;
+    rp.cnt = c->d_cnt.p;
+
+    PanelParams pp;
+    pp.seed = c->opt.seed;
+    pp.eabs = c->opt.eabs;
+    pp.readout_depth = c->dig.readout_depth;
+    pp.readout_policy = c->dig.readout_policy;
+    pp.record_rayleigh = c->dig.record_rayleigh;
+    pp.n_events = c->d_small.p + 1;
+    pp.hit_id = want_hits ? c->d_hit_id.p : nullptr;
+    pp.hit_f = want_hits ? c->d_hit_f.p : nullptr;
+    pp.n_hits = c->d_small.p + 2;
+    pp.hits_cap = hits_cap;
+    pp.trace = rp.trace;
+    pp.cnt = c->d_cnt.p;
+
+    const double *psf_base = rp.psf;
+    const int64_t nbatch = (n_hist + batch - 1) / batch;
+    while ((int64_t)c->kev.size() < 4 * nbatch) {
+        hipEvent_t e;
+        HIPOK(c, hipEventCreate(&e));
+        c->kev.push_back(e);
+    }
+    int64_t ib = 0;
+    HIPOK(c, hipEventRecord(c->ev0, c->stream));
+    unsigned small[4] = {0, 0, 0, 0};
+    for (int64_t b0 = 0; b0 < n_hist; b0 += batch) {
+        rp.h0 = b0;
+        rp.nh = std::min(batch, n_hist - b0);
+        if (psf_base) rp.psf = psf_base + 8 * b0;
+        HIPOK(c, hipMemsetAsync(c->d_small.p, 0, sizeof(unsigned), c->stream));
+        HIPOK(c, hipEventRecord(c->kev[4 * ib], c->stream));
+        int e = gpet_launch_phantom(rp, c->dtab, c->dph, c->ddet, q, c->stream);
+        if (e) return fail(c, GPET_ERR_DEVICE, std::string("phantom kernel: ") + hipGetErrorString((hipError_t)e));
+        HIPOK(c, hipEventRecord(c->kev[4 * ib + 1], c->stream));
+        HIPOK(c, hipMemcpyAsync(small, c->d_small.p, sizeof(small), hipMemcpyDeviceToHost, c->stream));
+        HIPOK(c, hipStreamSynchronize(c->stream));
+        int64_t nq = std::min<int64_t>(small[0], qcap);
+        int64_t ne = small[1];
+        if ((st = dgrow(c, c->d_events, ne + GPET_MAX_PH_EVENTS * nq, ne))) return st;
+        pp.events = c->d_events.p;
+        pp.events_cap = c->d_events.cap;
+        HIPOK(c, hipEventRecord(c->kev[4 * ib + 2], c->stream));
+        e = gpet_launch_panel(pp, c->dtab, c->ddet, q, nq, c->stream);
+        if (e) return fail(c, GPET_ERR_DEVICE, std::string("panel kernel: ") + hipGetErrorString((hipError_t)e));
+        HIPOK(c, hipEventRecord(c->kev[4 * ib + 3], c->stream));
+        ++ib;
+    }
+    HIPOK(c, hipEventRecord(c->ev1, c->stream));
+    HIPOK(c, hipMemcpyAsync(small, c->d_small.p, sizeof(small), hipMemcpyDeviceToHost, c->stream));
+    unsigned long long cnt[CNT_NUM];
+    HIPOK(c, hipMemcpyAsync(cnt, c->d_cnt.p, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+    HIPOK(c, hipStreamSynchronize(c->stream));
+    c->n_events = std::min<int64_t>(small[1], c->d_events.cap);
+
+    DigiParams dp;
+    dp.seed = c->opt.seed;
+    dp.d = c->dig;
+    dp.moduleN = c->ddet.moduleN;
+    dp.crystalN = c->ddet.crystalN;
+    char err[256] = {0};
+    if (gpet_digitize(c->dw, dp, c->d_events.p, c->n_events, &c->dout, c->stream, err, sizeof(err)))
+        return fail(c, GPET_ERR_DEVICE, std::string("digitizer: ") + err);
+    HIPOK(c, hipEventRecord(c->ev2, c->stream));
+    HIPOK(c, hipEventSynchronize(c->ev2));
+    float ms_t = 0, ms_d = 0;
+    HIPOK(c, hipEventElapsedTime(&ms_t, c->ev0, c->ev1));
+    HIPOK(c, hipEventElapsedTime(&ms_d, c->ev1, c->ev2));
+
+    gpet_tallies &T = c->last;
+    memset(&T, 0, sizeof(T));
+    T.n_histories = n_hist;
+    T.n_photons = (int64_t)cnt[CNT_PHOTONS];
+    T.n_panel_entries = (int64_t)cnt[CNT_PANEL_ENTRIES];
+    T.n_hits = (int64_t)cnt[CNT_HITS];
+    T.n_adder = c->n_events;
+    T.n_threshold = c->dout.n_threshold;
+    T.n_deadtime = c->dout.n_deadtime;
+    T.n_singles = c->dout.n_singles;
+    T.n_coinc = c->dout.n_coinc;
+    T.n_coinc_true = c->dout.n_coinc_true;
+    T.dropped_events = (int64_t)cnt[CNT_DROPPED_EVENTS];
+    T.dropped_hits = (int64_t)cnt[CNT_DROPPED_HITS];
+    T.majorant_violations = (int64_t)cnt[CNT_MAJORANT];
+    T.phantom_steps = (int64_t)cnt[CNT_PHANTOM_STEPS];
+    T.range_steps = (int64_t)cnt[CNT_RANGE_STEPS];
+    T.panel_steps = (int64_t)cnt[CNT_PANEL_STEPS];
+    T.seconds_transport = ms_t * 1e-3;
+    T.seconds_digitizer = ms_d * 1e-3;
+    for (int64_t i = 0; i < ib; ++i) {
+        float a = 0, b = 0;
+        HIPOK(c, hipEventElapsedTime(&a, c->kev[4 * i], c->kev[4 * i + 1]));
+        HIPOK(c, hipEventElapsedTime(&b, c->kev[4 * i + 2], c->kev[4 * i + 3]));
+        T.kernel_seconds[0] += a * 1e-3;
+        T.kernel_seconds[1] += b * 1e-3;
+    }
+    T.kernel_launches[0] = ib;
+    T.kernel_launches[1] = ib;
+
+    if (!sink) return GPET_OK;
+    const bool to_files = sink->out_dir && sink->out_dir[0];
+    std::vector<gpet_event> adder, singles, coinc;
+    std::vector<int32_t> hid;
+    std::vector<float> hf;
+    sink->n_adder = c->n_events;
+    sink->n_singles = c->dout.n_singles;
+    sink->n_coinc = c->dout.n_coinc;
+    sink->n_hits = std::min<int64_t>(small[2], hits_cap);
+    if ((sink->adder && sink->adder_cap > 0) || to_files) {
+        adder.resize((size_t)c->n_events);
+        if (c->n_events)
+            HIPOK(c, hipMemcpy(adder.data(), c->d_events.p, sizeof(gpet_event) * adder.size(), hipMemcpyDeviceToHost));
+        std::sort(adder.begin(), adder.end(), ev_less_id);
+        if (sink->adder)
+            memcpy(sink->adder, adder.data(), sizeof(gpet_event) * (size_t)std::min<int64_t>(sink->adder_cap, c->n_events));
+    }
+    if ((sink->singles && sink->singles_cap > 0) || to_files) {
+        int64_t m = to_files ? c->dout.n_singles : std::min<int64_t>(sink->singles_cap, c->dout.n_singles);
+        singles.resize((size_t)m);
+        if (m) HIPOK(c, hipMemcpy(singles.data(), c->dout.singles, sizeof(gpet_event) * (size_t)m, hipMemcpyDeviceToHost));
+        if (sink->singles)
+            memcpy(sink->singles, singles.data(), sizeof(gpet_event) * (size_t)std::min<int64_t>(sink->singles_cap, m));
+    }
+    if ((sink->coinc && sink->coinc_cap > 0) || to_files) {
+        int64_t m = to_files ? c->dout.n_coinc : std::min<int64_t>(sink->coinc_cap, c->dout.n_coinc);
+        coinc.resize((size_t)(2 * m));
+        if (m) HIPOK(c, hipMemcpy(coinc.data(), c->dout.coinc, sizeof(gpet_event) * (size_t)(2 * m), hipMemcpyDeviceToHost));
+        if (sink->coinc)
+            memcpy(sink->coinc, coinc.data(), sizeof(gpet_event) * (size_t)(2 * std::min<int64_t>(sink->coinc_cap, m)));
+    }
+    if (want_hits && sink->n_hits > 0 && ((sink->hit_id && sink->hit_f) || to_files)) {
+        hid.resize((size_t)(5 * sink->n_hits));
+        hf.resize((size_t)(5 * sink->n_hits));
+        HIPOK(c, hipMemcpy(hid.data(), c->d_hit_id.p, sizeof(int32_t) * hid.size(), hipMemcpyDeviceToHost));
+        HIPOK(c, hipMemcpy(hf.data(), c->d_hit_f.p, sizeof(float) * hf.size(), hipMemcpyDeviceToHost));
+        if (sink->hit_id && sink->hit_f) {
+            int64_t m = std::min<int64_t>(sink->hits_cap, sink->n_hits);
+            memcpy(sink->hit_id, hid.data(), sizeof(int32_t) * (size_t)(5 * m));
+            memcpy(sink->hit_f, hf.data(), sizeof(float) * (size_t)(5 * m));
+        }
+    }
+    if (want_trace) {
+        sink->n_trace = ntr;
+        HIPOK(c, hipMemcpy(sink->trace, c->d_trace.p, sizeof(gpet_trace) * (size_t)ntr, hipMemcpyDeviceToHost));
+    }
+    if (to_files) return write_outputs(c, sink->out_dir, adder, singles, coinc, hid, hf);
+    return GPET_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+gpet_status gpet_create(gpet_ctx **out, const gpet_options *opt)
+{
+    if (!out || !opt) return GPET_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return GPET_ERR_DEVICE;
+    if (opt->device < 0 || opt->device >= ndev) return GPET_ERR_ARG;
+    gpet_ctx *c = new gpet_ctx();
+    c->opt = *opt;
+    c->device = opt->device;
+    memset(&c->dig, 0, sizeof(c->dig));
+    memset(&c->last, 0, sizeof(c->last));
+    memset(&c->dtab, 0, sizeof(c->dtab));
+    memset(&c->dph, 0, sizeof(c->dph));
+    memset(&c->ddet, 0, sizeof(c->ddet));
+    if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreate(&c->ev2) != hipSuccess) {
+        delete c;
+        return GPET_ERR_DEVICE;
+    }
+    c->dw = gpet_digi_work_create();
+    *out = c;
+    return GPET_OK;
+}
+
+gpet_status gpet_destroy(gpet_ctx *c)
+{
+    if (!c) return GPET_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    dfree(c->d_mu4); dfree(c->d_surf[0]); dfree(c->d_surf[1]);
+    dfree(c->d_vox); dfree(c->d_wck_ph); dfree(c->d_panels); dfree(c->d_wck_det);
+    for (auto &b : c->q_f) dfree(b);
+    dfree(c->q_t); dfree(c->q_info); dfree(c->q_slot); dfree(c->d_small); dfree(c->d_cnt);
+    dfree(c->d_events); dfree(c->d_trace); dfree(c->d_hit_id); dfree(c->d_hit_f);
+    dfree(c->d_first); dfree(c->d_src); dfree(c->d_iso); dfree(c->d_psf);
+    gpet_digi_work_destroy(c->dw);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->ev2) (void)hipEventDestroy(c->ev2);
+    for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return GPET_OK;
+}
+
+const char *gpet_last_error(gpet_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+gpet_status gpet_set_tables(gpet_ctx *c, const gpet_tables *t)
+{
+    if (!c || !t) return GPET_ERR_ARG;
+    if (t->nmat <= 0 || t->nmat > GPET_MAXMAT || t->n1d != GPET_NTAB || t->ncp != GPET_NCP || t->ne != GPET_NE)
+        return fail(c, GPET_ERR_FORMAT, "table dimensions");
+    for (int k : {GPET_TAB_COMPT, GPET_TAB_RAYLE})
+        if (t->e0[k] != t->e0[GPET_TAB_LAMPH] || t->idl[k] != t->idl[GPET_TAB_LAMPH])
+            return fail(c, GPET_ERR_FORMAT, "lamph/compt/rayle must share one energy grid");
+    HIPOK(c, hipSetDevice(c->device));
+    c->nmat = t->nmat;
+    c->eminph = t->eminph;
+    c->emax = t->emax;
+    c->e0 = t->e0[GPET_TAB_LAMPH];
+    c->idl = t->idl[GPET_TAB_LAMPH];
+    const size_t n = (size_t)t->nmat * t->n1d;
+    c->h_lamph.assign(t->mu[GPET_TAB_LAMPH], t->mu[GPET_TAB_LAMPH] + n);
+    std::vector<float4> mu4(n);
+    for (size_t i = 0; i < n; ++i)
+        mu4[i] = make_float4(t->mu[GPET_TAB_LAMPH][i], t->mu[GPET_TAB_COMPT][i], t->mu[GPET_TAB_RAYLE][i], 0.0f);
+    gpet_status st = upload(c, c->d_mu4, mu4.data(), (int64_t)n);
+    if (st) return st;
+    const int64_t ns = (int64_t)t->nmat * t->ncp * t->ne;
+    for (int k = 0; k < 2; ++k)
+        if ((st = upload(c, c->d_surf[k], t->surf[k], ns))) return st;
+    DevTables &d = c->dtab;
+    d.nmat = t->nmat; d.n1d = t->n1d; d.ncp = t->ncp; d.ne = t->ne;
+    d.e0 = c->e0; d.idl = c->idl;
+    d.mu4 = c->d_mu4.p;
+    for (int k = 0; k < 2; ++k) {
+        d.idcp[k] = t->idcp[k];
+        d.ide[k] = t->ide[k];
+        d.surf[k] = c->d_surf[k].p;
+    }
+    c->has_tab = true;
+    return GPET_OK;
+}
+
+gpet_status gpet_load_tables(gpet_ctx *c, const char *dir)
+{
+    if (!c || !dir) return GPET_ERR_ARG;
+    gpet_tables t;
+    gpet_status st = gpet_tables_load(dir, &t);
+    if (st) return fail(c, st, std::string("cannot load tables from ") + dir);
+    st = gpet_set_tables(c, &t);
+    gpet_tables_free(&t);
+    return st;
+}
+
+gpet_status gpet_set_phantom(gpet_ctx *c, const int32_t *mat, const float *dens, const int32_t dims[3],
+                             const float off[3], const float size[3])
+{
+    if (!c || !mat || !dens || !dims || !off || !size) return GPET_ERR_ARG;
+    if (!c->has_tab) return fail(c, GPET_ERR_STATE, "tables must be set before the phantom");
+    if (dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0) return fail(c, GPET_ERR_ARG, "phantom dims");
+    HIPOK(c, hipSetDevice(c->device));
+    const size_t nv = (size_t)dims[0] * dims[1] * dims[2];
+    std::vector<int2> vox(nv);
+    float rmax[GPET_MAXMAT] = {0};
+    for (size_t i = 0; i < nv; ++i) {
+        int m = mat[i];
+        if (m < 0 || m >= c->nmat) return fail(c, GPET_ERR_ARG, "phantom material out of range");
+        if (dens[i] > rmax[m]) rmax[m] = dens[i];
+        int b;
+        memcpy(&b, &dens[i], 4);
+        vox[i] = make_int2(b, m);
+    }
+    gpet_status st = upload(c, c->d_vox, vox.data(), (int64_t)nv);
+    if (st) return st;
+    DevPhantom &p = c->dph;
+    build_majorant(c, rmax, c->h_wck_ph, p.wck.e0, p.wck.idl);
+    if ((st = upload(c, c->d_wck_ph, c->h_wck_ph.data(), GPET_NWCK))) return st;
+    p.wck.lam = c->d_wck_ph.p;
+    p.nx = dims[0]; p.ny = dims[1]; p.nz = dims[2];
+    p.offx = off[0]; p.offy = off[1]; p.offz = off[2];
+    p.dx = size[0] / (float)p.nx; p.dy = size[1] / (float)p.ny; p.dz = size[2] / (float)p.nz;
+    p.idx = 1.0f / p.dx; p.idy = 1.0f / p.dy; p.idz = 1.0f / p.dz;
+    p.vox = c->d_vox.p;
+    p.present = 1;
+    c->has_ph = true;
+    return GPET_OK;
+}
+
+gpet_status gpet_set_detector(gpet_ctx *c, const gpet_panel *panels, int32_t n, const gpet_panel_material mats[2],
+                              const float *quad, int32_t nq)
+{
+    if (!c || !panels || n <= 0 || !mats) return GPET_ERR_ARG;
+    if (!c->has_tab) return fail(c, GPET_ERR_STATE, "tables must be set before the detector");
+    if (n > GPET_MAXPANELS) return fail(c, GPET_ERR_CAPACITY, "too many panels");
+    if (nq < 0 || nq > GPET_MAXSURFACE) return fail(c, GPET_ERR_CAPACITY, "too many surfaces");
+    HIPOK(c, hipSetDevice(c->device));
+    c->panels.assign(panels, panels + n);
+    std::vector<DevPanel> dp(n);
+    for (int i = 0; i < n; ++i) {
+        const gpet_panel &s = panels[i];
+        DevPanel &d = dp[i];
+        d.panel = s.panel;
+        d.lenx = s.lenx; d.leny = s.leny; d.lenz = s.lenz;
+        d.mody = s.mody; d.modz = s.modz; d.mspy = s.mspy; d.mspz = s.mspz;
+        d.cry = s.cry; d.crz = s.crz; d.cspy = s.cspy; d.cspz = s.cspz;
+        d.ox = s.ox; d.oy = s.oy; d.oz = s.oz; d.dirx = s.dirx;
+        for (int k = 0; k < 3; ++k) { d.ux[k] = s.ux[k]; d.uy[k] = s.uy[k]; d.uz[k] = s.uz[k]; }
+    }
+    gpet_status st = upload(c, c->d_panels, dp.data(), n);
+    if (st) return st;
+    DevDetector &d = c->ddet;
+    d.npanels = n;
+    d.p = c->d_panels.p;
+    float rmax[GPET_MAXMAT] = {0};
+    for (int i = 0; i < 2; ++i) {
+        int m = mats[i].material;
+        if (m < 0 || m >= c->nmat) return fail(c, GPET_ERR_ARG, "panel material out of range");
+        d.mat[i] = m;
+        d.dens[i] = mats[i].density;
+        if (mats[i].density > rmax[m]) rmax[m] = mats[i].density;
+    }
+    d.nquad = (quad && nq > 0) ? nq : 0;
+    memset(d.quad, 0, sizeof(d.quad));
+    if (d.nquad) memcpy(d.quad, quad, sizeof(float) * 10 * nq);
+    gpet_panel_counts(&panels[0], &d.moduleN, &d.crystalN, &d.moduleNy, &d.crystalNy);
+    build_majorant(c, rmax, c->h_wck_det, d.wck.e0, d.wck.idl);
+    if ((st = upload(c, c->d_wck_det, c->h_wck_det.data(), GPET_NWCK))) return st;
+    d.wck.lam = c->d_wck_det.p;
+    d.present = 1;
+    c->has_det = true;
+    return GPET_OK;
+}
+
+gpet_status gpet_set_digitizer(gpet_ctx *c, const gpet_digitizer *d)
+{
+    if (!c || !d) return GPET_ERR_ARG;
+    if (d->readout_depth < 0 || d->readout_depth > 3 || d->dead_level < 0 || d->dead_level > 3)
+        return fail(c, GPET_ERR_ARG, "digitizer depth/level out of range");
+    c->dig = *d;
+    c->has_dig = true;
+    return GPET_OK;
+}
+
+gpet_status gpet_run_source(gpet_ctx *c, const gpet_source *src, int32_t ns, const gpet_isotope *iso, int32_t niso,
+                            double t0, double t1, int64_t n_decays, gpet_sink *sink)
+{
+    gpet_status st = check_ready(c);
+    if (st) return st;
+    if (!src || ns <= 0 || !iso || niso <= 0 || !(t1 >= t0)) return fail(c, GPET_ERR_ARG, "source arguments");
+    HIPOK(c, hipSetDevice(c->device));
+    std::vector<int64_t> first;
+    std::vector<DevSource> ds;
+    if ((st = plan_sources(c, src, ns, iso, niso, t0, t1, n_decays, first, ds))) return st;
+    std::vector<DevIsotope> di(niso);
+    for (int i = 0; i < niso; ++i) {
+        di[i].ratio = iso[i].beta_ratio;
+        for (int k = 0; k < 8; ++k) di[i].coef[k] = iso[i].coef[k];
+    }
+    if ((st = upload(c, c->d_first, first.data(), ns + 1)) || (st = upload(c, c->d_src, ds.data(), ns)) ||
+        (st = upload(c, c->d_iso, di.data(), niso)))
+        return st;
+    RunParams rp;
+    memset(&rp, 0, sizeof(rp));
+    rp.mode = RUN_DECAY;
+    rp.t0 = t0;
+    rp.nsrc = ns;
+    rp.first = c->d_first.p;
+    rp.src = c->d_src.p;
+    rp.iso = c->d_iso.p;
+    rp.psf = nullptr;
+    return run_common(c, rp, first[ns], sink);
+}
+
+gpet_status gpet_run_psf(gpet_ctx *c, const double *rec8, int64_t n, int32_t ptype, gpet_sink *sink)
+{
+    gpet_status st = check_ready(c);
+    if (st) return st;
+    if (!rec8 || n < 0 || (ptype != 0 && ptype != 1)) return fail(c, GPET_ERR_ARG, "psf arguments");
+    HIPOK(c, hipSetDevice(c->device));
+    if ((st = upload(c, c->d_psf, rec8, 8 * n))) return st;
+    RunParams rp;
+    memset(&rp, 0, sizeof(rp));
+    rp.mode = ptype == 0 ? RUN_PSF_POSITRON : RUN_PSF_PHOTON;
+    rp.psf = c->d_psf.p;
+    return run_common(c, rp, n, sink);
+}
+
+gpet_status gpet_get_tallies(gpet_ctx *c, gpet_tallies *t)
+{
+    if (!c || !t) return GPET_ERR_ARG;
+    int64_t *cs = t->crystal_singles, *eh = t->energy_hist, *sg = t->sinogram;
+    int32_t nc = t->n_crystals, nb = t->n_ebins, nsb = t->n_s, nph = t->n_phi;
+    float elo = t->e_lo, ehi = t->e_hi, smax = t->s_max;
+    *t = c->last;
+    t->crystal_singles = cs; t->n_crystals = nc;
+    t->energy_hist = eh; t->n_ebins = nb; t->e_lo = elo; t->e_hi = ehi;
+    t->sinogram = sg; t->n_s = nsb; t->n_phi = nph; t->s_max = smax;
+    if ((cs && nc > 0) || (eh && nb > 0)) {
+        HIPOK(c, hipSetDevice(c->device));
+        DBuf<unsigned long long> dc, dh;
+        gpet_status st;
+        if (cs && nc > 0) {
+            if ((st = dalloc(c, dc, nc))) return st;
+            HIPOK(c, hipMemsetAsync(dc.p, 0, sizeof(unsigned long long) * nc, c->stream));
+        }
+        if (eh && nb > 0) {
+            if ((st = dalloc(c, dh, nb))) return st;
+            HIPOK(c, hipMemsetAsync(dh.p, 0, sizeof(unsigned long long) * nb, c->stream));
+        }
+        int e = gpet_digi_tallies(c->dw, c->dout.singles, c->dout.n_singles, c->ddet.moduleN, c->ddet.crystalN,
+                                  dc.p, (cs && nc > 0) ? nc : 0, dh.p, (eh && nb > 0) ? nb : 0, elo, ehi, c->stream);
+        if (e) return fail(c, GPET_ERR_DEVICE, "tally kernel");
+        if (dc.p) HIPOK(c, hipMemcpyAsync(cs, dc.p, sizeof(int64_t) * nc, hipMemcpyDeviceToHost, c->stream));
+        if (dh.p) HIPOK(c, hipMemcpyAsync(eh, dh.p, sizeof(int64_t) * nb, hipMemcpyDeviceToHost, c->stream));
+        HIPOK(c, hipStreamSynchronize(c->stream));
+        dfree(dc);
+        dfree(dh);
+    }
+    return GPET_OK;
+}
+
+/* Majorant node values of the phantom (which = 0) or detector (1). */
+gpet_status gpet_get_majorant(gpet_ctx *c, int32_t which, float *out, float *e0, float *idl)
+{
+    if (!c || !out) return GPET_ERR_ARG;
+    const std::vector<float> &v = which == 0 ? c->h_wck_ph : c->h_wck_det;
+    if (v.empty()) return GPET_ERR_STATE;
+    memcpy(out, v.data(), sizeof(float) * GPET_NWCK);
+    const DevWck &w = which == 0 ? c->dph.wck : c->ddet.wck;
+    if (e0) *e0 = w.e0;
+    if (idl) *idl = w.idl;
+    return GPET_OK;
+}
+
+} /* extern "C" */

@@ -1,0 +1,154 @@
+/*
+ * internal.h - device data layouts and kernel launchers of the HIP path
+ * (not part of the public C ABI).
+ *
+ * HBM layout (MI355X, 288 GB): everything a run touches stays resident.
+ *   phantom    int2 per voxel {density bits, material}, x fastest: one 8-byte
+ *              gather per Woodcock step instead of two texture fetches
+ *              (initialize.cu:831-884 uses two 3-D textures).
+ *   tables     float4 per (material, energy node) {mu_tot, mu_compt, mu_rayl, 0}
+ *              so one lerp reads two 16-byte nodes; inverse-CDF surfaces
+ *              [mat][icp][ie]; majorants 4096 floats each.
+ *   queue      SoA arrays of photons that entered a panel (regrouped by the
+ *              phantom kernel so the panel kernel runs dense waves).
+ *   events     gpet_event (48 B) appended with one atomic per wave.
+ */
+#ifndef GPET_INTERNAL_H
+#define GPET_INTERNAL_H
+
+#include <stdint.h>
+#include "../../../include/gpet.h"
+
+#define GPET_MAX_PH_EVENTS 4   /* per-photon adder list (reference: events[4]) */
+
+struct DevWck {
+    float e0, idl;
+    const float *lam;          /* [GPET_NWCK] */
+};
+
+struct DevTables {
+    int nmat, n1d, ncp, ne;
+    float e0, idl;             /* common grid of lamph/compt/rayle            */
+    const float4 *mu4;         /* [nmat][n1d] {tot, compt, rayl, 0}           */
+    float idcp[2], ide[2];
+    const float *surf[2];      /* [nmat][ncp][ne]                             */
+};
+
+struct DevPhantom {
+    int nx, ny, nz;
+    float offx, offy, offz, dx, dy, dz, idx, idy, idz;
+    const int2 *vox;           /* {float bits of density, material}           */
+    DevWck wck;
+    int present;
+};
+
+struct DevPanel {
+    int panel;
+    float lenx, leny, lenz, mody, modz, mspy, mspz, cry, crz, cspy, cspz;
+    float ox, oy, oz, dirx;
+    float ux[3], uy[3], uz[3];
+};
+
+struct DevDetector {
+    int npanels;
+    const DevPanel *p;
+    int mat[2];
+    float dens[2];
+    int nquad;
+    float quad[10 * GPET_MAXSURFACE];
+    int moduleN, crystalN, moduleNy, crystalNy;
+    DevWck wck;
+    int present;
+};
+
+struct DevSource {
+    double tau, pdec;          /* T1/2 / ln 2 (s); 1 - 2^(-(t1-t0)/T1/2)      */
+    int shape, iso;
+    float coef[6];
+};
+
+struct DevIsotope {
+    float ratio;
+    float coef[8];
+};
+
+/* What the phantom kernel produces for the panel kernel. */
+struct DevQueue {
+    float *y, *z, *vx, *vy, *vz, *E;
+    double *t;
+    int4 *info;                /* {panel index, parn, eventid, draws}         */
+    int *trace_slot;           /* trace slot or -1                            */
+    int *count;
+    int64_t cap;
+};
+
+/* Device-side counters, 64-bit, accumulated with atomics. */
+enum {
+    CNT_PHOTONS = 0, CNT_PANEL_ENTRIES, CNT_HITS, CNT_DROPPED_EVENTS, CNT_DROPPED_HITS,
+    CNT_MAJORANT, CNT_PHANTOM_STEPS, CNT_RANGE_STEPS, CNT_PANEL_STEPS, CNT_NUM
+};
+
+enum { RUN_DECAY = 0, RUN_PSF_POSITRON = 1, RUN_PSF_PHOTON = 2 };
+
+struct RunParams {
+    int mode;
+    int64_t h0, nh;            /* histories [h0, h0+nh)                        */
+    uint64_t seed;
+    float eabs, sigma;
+    int prange;
+    double t0;
+    int nsrc;
+    const int64_t *first;      /* [nsrc+1] first history per source            */
+    const DevSource *src;
+    const DevIsotope *iso;
+    const double *psf;         /* mode 1/2: records of 8 doubles for [h0,...)  */
+    gpet_trace *trace;         /* device trace buffer or null                  */
+    int64_t trace_cap;         /* slots: 2*(h - 0) < trace_cap are written     */
+    unsigned long long *cnt;   /* [CNT_NUM]                                    */
+};
+
+struct PanelParams {
+    uint64_t seed;
+    float eabs;
+    int readout_depth, readout_policy, record_rayleigh;
+    gpet_event *events;
+    unsigned int *n_events;
+    int64_t events_cap;
+    int32_t *hit_id;
+    float *hit_f;
+    unsigned int *n_hits;
+    int64_t hits_cap;
+    gpet_trace *trace;
+    unsigned long long *cnt;
+};
+
+/* Launchers (transport.hip). */
+int gpet_launch_phantom(const RunParams &rp, const DevTables &tab, const DevPhantom &ph,
+                        const DevDetector &det, const DevQueue &q, hipStream_t s);
+int gpet_launch_panel(const PanelParams &pp, const DevTables &tab, const DevDetector &det,
+                      const DevQueue &q, int64_t max_items, hipStream_t s);
+
+/* Digitizer (digitizer.hip).  Works on the device event list; returns
+ * counts through the pointers. */
+struct DigiParams {
+    uint64_t seed;
+    gpet_digitizer d;
+    int moduleN, crystalN;
+};
+struct DigiOut {
+    gpet_event *singles;       /* device, owned by the caller of digitize     */
+    int64_t n_singles;
+    gpet_event *coinc;         /* device, 2 per pair                          */
+    int64_t n_coinc, n_coinc_true;
+    int64_t n_threshold, n_deadtime;
+};
+struct DigiWork;               /* scratch owned by the context                */
+DigiWork *gpet_digi_work_create();
+void gpet_digi_work_destroy(DigiWork *w);
+int gpet_digitize(DigiWork *w, const DigiParams &p, const gpet_event *events, int64_t n,
+                  DigiOut *out, hipStream_t s, char *err, int errlen);
+int gpet_digi_tallies(DigiWork *w, const gpet_event *singles, int64_t n, int moduleN, int crystalN,
+                      unsigned long long *crystal, int ncrystal, unsigned long long *ehist, int nebins,
+                      float elo, float ehi, hipStream_t s);
+
+#endif

@@ -1,0 +1,582 @@
+/*
+ * digitizer.hip - digitizer chain and coincidence sorter on the GPU.
+ *
+ * The reference runs blur/threshold/deadtime/window as kernels but compacts
+ * and orders events by copying the whole event array to the host and calling
+ * std::sort four times per batch (gPET.cu:380-424, detector.cu:354-385).
+ * Here the chain stays in HBM:
+ *
+ *   blur + thresholder      one lane per event, Philox stream (parn, siten)
+ *   compaction              block scan + scatter (stable)
+ *   time order              LSD radix sort of (sortable t bits, index):
+ *                           8 passes of 8 bits, ranks inside a wavefront
+ *                           from 8 ballots per digit
+ *   deadtime                stable radix sort by site over the time order,
+ *                           then one lane per chain walks it
+ *                           (gPET_kernals.cu:657-698)
+ *   energy window           flags + compaction in time order -> singles
+ *   coincidences            clusters of singles with gaps <= window are
+ *                           independent; one lane per cluster replays the
+ *                           single-window rule, counts, scan, write.
+ */
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstdio>
+#include <vector>
+#include "../common/gp_math.h"
+#include "../host/internal.h"
+
+namespace {
+
+#define HIPCHK(x)                                                                                      \
+    do {                                                                                               \
+        hipError_t e_ = (x);                                                                           \
+        if (e_ != hipSuccess) {                                                                        \
+            snprintf(err, errlen, "%s:%d %s", __FILE__, __LINE__, hipGetErrorString(e_));              \
+            return -1;                                                                                 \
+        }                                                                                              \
+    } while (0)
+
+constexpr int SCAN_BLOCK = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
+constexpr int RS_BLOCK = 256;
+constexpr int RS_ROUNDS = 16;
+constexpr int RS_TILE = RS_BLOCK * RS_ROUNDS;
+
+__device__ __forceinline__ unsigned long long lanemask_lt()
+{
+    unsigned int l = __lane_id();
+    return (l == 0) ? 0ull : ((~0ull) >> (64 - l));
+}
+
+/* ---- exclusive scan of uint32 ------------------------------------------ */
+
+__device__ unsigned block_exclusive_scan(unsigned v, unsigned *sh, unsigned &total)
+{
+    /* wave inclusive scan via shuffles, then across 4 waves */
+    const int lane = __lane_id(), wid = threadIdx.x >> 6;
+    unsigned x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        unsigned y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned acc = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+            unsigned t = sh[w];
+            sh[w] = acc;
+            acc += t;
+        }
+        sh[8] = acc;
+    }
+    __syncthreads();
+    unsigned res = sh[wid] + x - v;
+    total = sh[8];
+    __syncthreads();
+    return res;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_tiles(const unsigned *in, unsigned *out, unsigned *tile_sums, int64_t n)
+{
+    __shared__ unsigned sh[16];
+    int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    unsigned v[SCAN_ITEMS];
+    unsigned s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        v[k] = (base + k < n) ? in[base + k] : 0u;
+        s += v[k];
+    }
+    unsigned tot;
+    unsigned pre = block_exclusive_scan(s, sh, tot);
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        if (base + k < n) out[base + k] = pre;
+        pre += v[k];
+    }
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
+}
+
+/* Single block scans the tile sums in chunks, carrying the running total. */
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_sums(unsigned *sums, int64_t n, unsigned *grand)
+{
+    __shared__ unsigned sh[16];
+    unsigned carry = 0;
+    for (int64_t c = 0; c < n; c += SCAN_BLOCK) {
+        int64_t i = c + threadIdx.x;
+        unsigned v = (i < n) ? sums[i] : 0u;
+        unsigned tot;
+        unsigned pre = block_exclusive_scan(v, sh, tot);
+        if (i < n) sums[i] = pre + carry;
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && grand) *grand = carry;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_add(unsigned *out, const unsigned *tile_sums, int64_t n)
+{
+    int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    unsigned add = tile_sums[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k)
+        if (base + k < n) out[base + k] += add;
+}
+
+/* ---- radix sort of (uint64 key, uint32 value), stable ------------------ */
+
+__global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const unsigned long long *keys, int64_t n, int shift, unsigned *hist,
+                                                      int nblocks)
+{
+    __shared__ unsigned cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        int64_t i = base + (int64_t)r * RS_BLOCK + threadIdx.x;
+        if (i < n) atomicAdd(&cnt[(unsigned)(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
+}
+
+__global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const unsigned long long *kin, const unsigned *vin,
+                                                         unsigned long long *kout, unsigned *vout, int64_t n, int shift,
+                                                         const unsigned *offs, int nblocks)
+{
+    __shared__ unsigned run[256];
+    __shared__ unsigned wcnt[RS_BLOCK / 64][256];
+    const int lane = __lane_id(), wid = threadIdx.x >> 6;
+    run[threadIdx.x] = offs[(int64_t)threadIdx.x * nblocks + blockIdx.x];
+    for (int w = 0; w < RS_BLOCK / 64; ++w) wcnt[w][threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    const unsigned long long lt = lanemask_lt();
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        int64_t i = base + (int64_t)r * RS_BLOCK + threadIdx.x;
+        bool valid = i < n;
+        unsigned long long key = valid ? kin[i] : 0ull;
+        unsigned val = valid ? vin[i] : 0u;
+        unsigned d = (unsigned)(key >> shift) & 255u;
+        unsigned long long match = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            bool bit = (d >> b) & 1u;
+            unsigned long long bb = __ballot(valid && bit);
+            match &= bit ? bb : ~bb;
+        }
+        unsigned rank = (unsigned)__popcll(match & lt);
+        if (valid && rank == 0) wcnt[wid][d] = (unsigned)__popcll(match);
+        __syncthreads();
+        unsigned pos = 0;
+        if (valid) {
+            pos = run[d] + rank;
+            for (int w = 0; w < wid; ++w) pos += wcnt[w][d];
+        }
+        __syncthreads();
+        {
+            unsigned add = 0;
+            for (int w = 0; w < RS_BLOCK / 64; ++w) {
+                add += wcnt[w][threadIdx.x];
+                wcnt[w][threadIdx.x] = 0;
+            }
+            run[threadIdx.x] += add;
+        }
+        if (valid) {
+            kout[pos] = key;
+            vout[pos] = val;
+        }
+        __syncthreads();
+        (void)lane;
+    }
+}
+
+/* ---- digitizer stages --------------------------------------------------- */
+
+__device__ __forceinline__ unsigned long long sortable_t(double t)
+{
+    unsigned long long b = gp_d2u(t);
+    return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+
+/* blur (gPET_kernals.cu:814-837) + time blur + thresholder flags. */
+__global__ __launch_bounds__(256) void k_blur_threshold(const gpet_event *in, gpet_event *out, unsigned *flag, int64_t n,
+                                                        uint64_t seed, gpet_digitizer g)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    gpet_event e = in[i];
+    gp_rng r;
+    gp_rng_init(&r, seed, (uint64_t)(uint32_t)e.parn, (uint32_t)GP_TAG_DIGI | ((uint32_t)e.siten << 4), 0u);
+    float R = 0.0f;
+    if (g.blur_policy == 0) R = sqrtf(g.e_ref / e.E) * g.r_ref;
+    if (g.blur_policy == 1) R = g.r_ref + g.slope * (e.E - g.e_ref) / 1.0e6f;
+    if (R < 0.0f) R = 0.0f;
+    float nr = gp_rng_normal(&r);
+    e.E = e.E + ((nr * R) * e.E) / 2.35482f;
+    if (g.space_blur > 0.0f) {
+        e.x = e.x + g.space_blur * gp_rng_normal(&r);
+        e.y = e.y + g.space_blur * gp_rng_normal(&r);
+        e.z = e.z + g.space_blur * gp_rng_normal(&r);
+    }
+    if (g.time_blur_fwhm_us > 0.0) {
+        float nt = gp_rng_normal(&r);
+        e.t = e.t + (double)nt * (g.time_blur_fwhm_us / 2.35482);
+    }
+    out[i] = e;
+    flag[i] = (e.E < g.e_threshold || e.E > g.e_upholder) ? 0u : 1u;
+}
+
+__global__ __launch_bounds__(256) void k_compact_events(const gpet_event *in, const unsigned *flag, const unsigned *pos,
+                                                        gpet_event *out, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flag[i]) out[pos[i]] = in[i];
+}
+
+__global__ __launch_bounds__(256) void k_time_keys(const gpet_event *ev, unsigned long long *key, unsigned *val, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        key[i] = sortable_t(ev[i].t);
+        val[i] = (unsigned)i;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_site_keys(const gpet_event *ev, const unsigned *perm, unsigned long long *key,
+                                                   unsigned *val, int64_t n, int level, int moduleN, int crystalN)
+{
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    unsigned idx = perm[j];
+    const gpet_event &e = ev[idx];
+    int s;
+    if (level == 0) s = 0;
+    else if (level == 1) s = e.pann;
+    else if (level == 2) s = e.pann * moduleN + e.modn;
+    else if (level == 3) s = e.siten; /* dlevel 3 keeps the readout site (Q7) */
+    else s = e.pann * moduleN * crystalN + e.modn * crystalN + e.cryn;
+    key[j] = (unsigned long long)(unsigned)s;
+    val[j] = idx;
+}
+
+/* One lane per chain: a chain starts at j == 0, at a site change, or where
+ * t_j > t_{j-1} + tau (from the unmodified times). */
+__global__ __launch_bounds__(256) void k_deadtime(const gpet_event *ev, const unsigned *perm, const unsigned long long *site,
+                                                  unsigned *alive, int64_t n, double tau, int type)
+{
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    unsigned long long sj = site[j];
+    double tj = ev[perm[j]].t;
+    bool start = (j == 0) || site[j - 1] != sj || tj > ev[perm[j - 1]].t + tau;
+    if (!start) return;
+    alive[perm[j]] = 1u;
+    double tdead = tj;
+    for (int64_t k = j + 1; k < n && site[k] == sj; ++k) {
+        double tk = ev[perm[k]].t;
+        if (tk < tdead + tau) {
+            alive[perm[k]] = 0u;
+            if (type == 0) tdead = tk;
+        } else {
+            if (tk > ev[perm[k - 1]].t + tau) break; /* next chain's own lane */
+            alive[perm[k]] = 1u;
+            tdead = tk;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_window_flags(const gpet_event *ev, const unsigned *perm, const unsigned *alive,
+                                                      unsigned *flag, int64_t n, float wmin, float wmax)
+{
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    unsigned idx = perm[j];
+    float E = ev[idx].E;
+    flag[j] = (alive ? alive[idx] : 1u) && !(E < wmin || E > wmax);
+}
+
+__global__ __launch_bounds__(256) void k_gather_singles(const gpet_event *ev, const unsigned *perm, const unsigned *flag,
+                                                        const unsigned *pos, gpet_event *out, int64_t n)
+{
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n && flag[j]) out[pos[j]] = ev[perm[j]];
+}
+
+/* Coincidence windows inside one cluster, as the sequential rule: a window
+ * opens at the first single not inside the previous window and holds the
+ * singles with t - t_open <= W; exactly two -> a pair, more -> policy. */
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_coinc(const gpet_event *s, int64_t n, double W, int policy, unsigned *cnt,
+                                               const unsigned *pos, gpet_event *out, unsigned long long *ntrue)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    bool start = (i == 0) || (s[i].t - s[i - 1].t > W);
+    if (!start) {
+        if (!WRITE) cnt[i] = 0;
+        return;
+    }
+    unsigned np = 0, nt = 0;
+    unsigned o = WRITE ? pos[i] : 0u;
+    int64_t a = i;
+    while (a < n) {
+        int64_t b = a + 1;
+        while (b < n && s[b].t - s[a].t <= W) ++b;
+        int64_t m = b - a;
+        if (m == 2 || (m > 2 && policy == GPET_COINC_TAKE_ALL_GOODS)) {
+            for (int64_t p = a; p < b; ++p)
+                for (int64_t q = p + 1; q < b; ++q) {
+                    if (WRITE) {
+                        out[2 * (int64_t)(o + np)] = s[p];
+                        out[2 * (int64_t)(o + np) + 1] = s[q];
+                        if (s[p].eventid == s[q].eventid) nt++;
+                    }
+                    np++;
+                }
+        }
+        a = b;
+        if (a >= n || s[a].t - s[a - 1].t > W) break;
+    }
+    if (!WRITE) cnt[i] = np;
+    else if (nt) atomicAdd(ntrue, (unsigned long long)nt);
+}
+
+__global__ __launch_bounds__(256) void k_tally(const gpet_event *s, int64_t n, int moduleN, int crystalN,
+                                               unsigned long long *crystal, int ncrystal, unsigned long long *ehist,
+                                               int nebins, float elo, float inv)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const gpet_event &e = s[i];
+    if (crystal) {
+        int64_t g = (int64_t)e.pann * moduleN * crystalN + (int64_t)e.modn * crystalN + e.cryn;
+        if (g >= 0 && g < ncrystal) atomicAdd(&crystal[g], 1ull);
+    }
+    if (ehist) {
+        float x = (e.E - elo) * inv;
+        if (x >= 0.0f && x < (float)nebins) atomicAdd(&ehist[(int)x], 1ull);
+    }
+}
+
+inline int grid_for(int64_t n, int block) { return (int)((n + block - 1) / block); }
+
+} // namespace
+
+struct DigiWork {
+    gpet_event *ev_a = nullptr, *ev_b = nullptr;  /* blurred, thresholded */
+    int64_t ev_cap = 0;
+    unsigned *flag = nullptr, *pos = nullptr, *alive = nullptr;
+    unsigned long long *key_a = nullptr, *key_b = nullptr;
+    unsigned *val_a = nullptr, *val_b = nullptr;
+    int64_t cap = 0;
+    unsigned *tile_sums = nullptr, *hist = nullptr, *grand = nullptr;
+    int64_t tile_cap = 0, hist_cap = 0;
+    gpet_event *singles = nullptr, *coinc = nullptr;
+    int64_t singles_cap = 0, coinc_cap = 0;
+    unsigned long long *ntrue = nullptr;
+};
+
+namespace {
+
+template <class T>
+int ensure(T *&p, int64_t &cap, int64_t need, char *err, int errlen)
+{
+    if (need <= cap && p) return 0;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    int64_t c = std::max<int64_t>(need + need / 4, 1024);
+    HIPCHK(hipMalloc((void **)&p, sizeof(T) * (size_t)c));
+    cap = c;
+    return 0;
+}
+
+/* Exclusive scan of n uint32 into out; returns the total in *total (host). */
+int scan(DigiWork *w, const unsigned *in, unsigned *out, int64_t n, unsigned *total, hipStream_t s, char *err, int errlen)
+{
+    int64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (tiles == 0) {
+        *total = 0;
+        return 0;
+    }
+    if (ensure(w->tile_sums, w->tile_cap, tiles, err, errlen)) return -1;
+    if (!w->grand) HIPCHK(hipMalloc((void **)&w->grand, 64));
+    hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0, s, in, out, w->tile_sums, n);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_BLOCK), 0, s, w->tile_sums, tiles, w->grand);
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0, s, out, w->tile_sums, n);
+    HIPCHK(hipGetLastError());
+    if (total) {
+        HIPCHK(hipMemcpyAsync(total, w->grand, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    return 0;
+}
+
+/* Stable LSD sort of (key, val) over the low `bits` bits; result ends in
+ * key_a/val_a. */
+int radix_sort(DigiWork *w, int64_t n, int bits, hipStream_t s, char *err, int errlen)
+{
+    if (n <= 1) return 0;
+    int nblocks = (int)((n + RS_TILE - 1) / RS_TILE);
+    int64_t hn = (int64_t)256 * nblocks;
+    if (ensure(w->hist, w->hist_cap, hn, err, errlen)) return -1;
+    for (int shift = 0; shift < bits; shift += 8) {
+        hipLaunchKernelGGL(k_rs_hist, dim3(nblocks), dim3(RS_BLOCK), 0, s, w->key_a, n, shift, w->hist, nblocks);
+        HIPCHK(hipGetLastError());
+        if (scan(w, w->hist, w->hist, hn, nullptr, s, err, errlen)) return -1;
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nblocks), dim3(RS_BLOCK), 0, s, w->key_a, w->val_a, w->key_b, w->val_b, n,
+                           shift, w->hist, nblocks);
+        HIPCHK(hipGetLastError());
+        std::swap(w->key_a, w->key_b);
+        std::swap(w->val_a, w->val_b);
+    }
+    return 0;
+}
+
+} // namespace
+
+DigiWork *gpet_digi_work_create() { return new DigiWork(); }
+
+void gpet_digi_work_destroy(DigiWork *w)
+{
+    if (!w) return;
+    void *ptrs[] = {w->ev_a, w->ev_b, w->flag, w->pos, w->alive, w->key_a, w->key_b, w->val_a, w->val_b,
+                    w->tile_sums, w->hist, w->grand, w->singles, w->coinc, w->ntrue};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    delete w;
+}
+
+int gpet_digitize(DigiWork *w, const DigiParams &p, const gpet_event *events, int64_t n, DigiOut *out, hipStream_t s,
+                  char *err, int errlen)
+{
+    const gpet_digitizer &g = p.d;
+    *out = DigiOut();
+    int64_t c1 = w->cap, c2 = w->cap, c3 = w->cap, c4 = w->cap, c5 = w->cap, c6 = w->cap, c7 = w->cap, c8 = w->cap;
+    int64_t need = std::max<int64_t>(n, 1);
+    if (need > w->cap) {
+        if (ensure(w->flag, c1, need, err, errlen) || ensure(w->pos, c2, need, err, errlen) ||
+            ensure(w->alive, c3, need, err, errlen) || ensure(w->key_a, c4, need, err, errlen) ||
+            ensure(w->key_b, c5, need, err, errlen) || ensure(w->val_a, c6, need, err, errlen) ||
+            ensure(w->val_b, c7, need, err, errlen))
+            return -1;
+        w->cap = std::min({c1, c2, c3, c4, c5, c6, c7});
+        (void)c8;
+    }
+    int64_t ce = w->ev_cap, ce2 = w->ev_cap;
+    if (ensure(w->ev_a, ce, need, err, errlen) || ensure(w->ev_b, ce2, need, err, errlen)) return -1;
+    w->ev_cap = std::min(ce, ce2);
+    if (!w->ntrue) HIPCHK(hipMalloc((void **)&w->ntrue, sizeof(unsigned long long)));
+    HIPCHK(hipMemsetAsync(w->ntrue, 0, sizeof(unsigned long long), s));
+
+    /* blur + thresholder, compaction (input order kept) */
+    unsigned nT = 0;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_blur_threshold, dim3(grid_for(n, 256)), dim3(256), 0, s, events, w->ev_b, w->flag, n, p.seed, g);
+        HIPCHK(hipGetLastError());
+        if (scan(w, w->flag, w->pos, n, &nT, s, err, errlen)) return -1;
+        hipLaunchKernelGGL(k_compact_events, dim3(grid_for(n, 256)), dim3(256), 0, s, w->ev_b, w->flag, w->pos, w->ev_a, n);
+        HIPCHK(hipGetLastError());
+    }
+    out->n_threshold = nT;
+    const gpet_event *ev = w->ev_a;
+    const int64_t m = nT;
+
+    /* time order: perm_t in val_a */
+    if (m > 0) {
+        hipLaunchKernelGGL(k_time_keys, dim3(grid_for(m, 256)), dim3(256), 0, s, ev, w->key_a, w->val_a, m);
+        HIPCHK(hipGetLastError());
+        if (radix_sort(w, m, 64, s, err, errlen)) return -1;
+    }
+    /* val_a = perm_t.  Keep it in `pos`-independent storage: copy to val_b? we
+     * sort sites on (key_b,val_b) after moving perm_t aside into `alive` ... */
+    unsigned *perm_t = w->val_a;
+    unsigned *alive = nullptr;
+    int64_t n_dead = m;
+    if (g.dead_time_us > 0.0f && m > 0) {
+        /* move perm_t out of the sort buffers */
+        HIPCHK(hipMemcpyAsync(w->pos, perm_t, sizeof(unsigned) * m, hipMemcpyDeviceToDevice, s));
+        perm_t = w->pos;
+        int level = (g.dead_level == 3) ? 3 : g.dead_level;
+        hipLaunchKernelGGL(k_site_keys, dim3(grid_for(m, 256)), dim3(256), 0, s, ev, perm_t, w->key_a, w->val_a, m, level,
+                           p.moduleN, p.crystalN);
+        HIPCHK(hipGetLastError());
+        if (radix_sort(w, m, 32, s, err, errlen)) return -1;
+        HIPCHK(hipMemsetAsync(w->alive, 0, sizeof(unsigned) * m, s));
+        hipLaunchKernelGGL(k_deadtime, dim3(grid_for(m, 256)), dim3(256), 0, s, ev, w->val_a, w->key_a, w->alive, m,
+                           (double)g.dead_time_us, g.dead_type);
+        HIPCHK(hipGetLastError());
+        alive = w->alive;
+        /* perm_t lives in pos; window flags go to flag, scan into key_b (as uint) */
+    }
+    /* window flags in time order, compaction -> singles */
+    unsigned ns = 0;
+    if (m > 0) {
+        if (alive) {
+            unsigned long long *tmp = w->key_b; /* reuse as a uint buffer of >= m entries */
+            unsigned *flag2 = reinterpret_cast<unsigned *>(tmp);
+            unsigned *pos2 = flag2 + m;
+            if (m * 2 > w->cap * 2) return -1;
+            unsigned nalive = 0;
+            if (scan(w, alive, flag2, m, &nalive, s, err, errlen)) return -1;
+            n_dead = nalive;
+            hipLaunchKernelGGL(k_window_flags, dim3(grid_for(m, 256)), dim3(256), 0, s, ev, perm_t, alive, w->flag, m,
+                               g.win_min, g.win_max);
+            HIPCHK(hipGetLastError());
+            if (scan(w, w->flag, pos2, m, &ns, s, err, errlen)) return -1;
+            if (ensure(w->singles, w->singles_cap, std::max<int64_t>(ns, 1), err, errlen)) return -1;
+            hipLaunchKernelGGL(k_gather_singles, dim3(grid_for(m, 256)), dim3(256), 0, s, ev, perm_t, w->flag, pos2,
+                               w->singles, m);
+        } else {
+            hipLaunchKernelGGL(k_window_flags, dim3(grid_for(m, 256)), dim3(256), 0, s, ev, perm_t, (const unsigned *)nullptr,
+                               w->flag, m, g.win_min, g.win_max);
+            HIPCHK(hipGetLastError());
+            unsigned *pos2 = reinterpret_cast<unsigned *>(w->key_b);
+            if (scan(w, w->flag, pos2, m, &ns, s, err, errlen)) return -1;
+            if (ensure(w->singles, w->singles_cap, std::max<int64_t>(ns, 1), err, errlen)) return -1;
+            hipLaunchKernelGGL(k_gather_singles, dim3(grid_for(m, 256)), dim3(256), 0, s, ev, perm_t, w->flag, pos2,
+                               w->singles, m);
+        }
+        HIPCHK(hipGetLastError());
+    }
+    out->n_deadtime = n_dead;
+    out->n_singles = ns;
+    out->singles = w->singles;
+
+    /* coincidences */
+    unsigned npairs = 0;
+    if (g.coinc_window_us > 0.0 && ns > 1) {
+        unsigned *cnt = w->flag;
+        unsigned *cpos = w->pos;
+        hipLaunchKernelGGL(k_coinc<false>, dim3(grid_for(ns, 256)), dim3(256), 0, s, w->singles, (int64_t)ns,
+                           g.coinc_window_us, g.coinc_policy, cnt, (const unsigned *)nullptr, (gpet_event *)nullptr,
+                           (unsigned long long *)nullptr);
+        HIPCHK(hipGetLastError());
+        if (scan(w, cnt, cpos, ns, &npairs, s, err, errlen)) return -1;
+        if (ensure(w->coinc, w->coinc_cap, std::max<int64_t>(2 * (int64_t)npairs, 2), err, errlen)) return -1;
+        hipLaunchKernelGGL(k_coinc<true>, dim3(grid_for(ns, 256)), dim3(256), 0, s, w->singles, (int64_t)ns,
+                           g.coinc_window_us, g.coinc_policy, cnt, cpos, w->coinc, w->ntrue);
+        HIPCHK(hipGetLastError());
+    }
+    unsigned long long nt = 0;
+    HIPCHK(hipMemcpyAsync(&nt, w->ntrue, sizeof(nt), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    out->n_coinc = npairs;
+    out->n_coinc_true = (int64_t)nt;
+    out->coinc = w->coinc;
+    return 0;
+}
+
+int gpet_digi_tallies(DigiWork *w, const gpet_event *singles, int64_t n, int moduleN, int crystalN,
+                      unsigned long long *crystal, int ncrystal, unsigned long long *ehist, int nebins, float elo,
+                      float ehi, hipStream_t s)
+{
+    (void)w;
+    if (n <= 0) return 0;
+    float inv = (nebins > 0 && ehi > elo) ? (float)nebins / (ehi - elo) : 0.0f;
+    hipLaunchKernelGGL(k_tally, dim3(grid_for(n, 256)), dim3(256), 0, s, singles, n, moduleN, crystalN, crystal, ncrystal,
+                       ehist, nebins, elo, inv);
+    return (int)hipGetLastError();
+}

@@ -1,0 +1,638 @@
+/*
+ * transport.hip - photon transport kernels for gfx950 (MI355X).
+ *
+ * Two kernels per batch of histories, regrouped through a queue:
+ *
+ *   k_phantom  one lane per history: decay sampling (setPosition,
+ *              gPET_kernals.cu:483-561) or PSF record (setPositionForPhoton,
+ *              :563-604), positron range (:347-418), the annihilation pair,
+ *              Woodcock tracking in the voxel phantom (photon, :256-345) and
+ *              the panel entry test (photonde, :958-1009).  Only photons that
+ *              enter a panel (about a tenth) are appended to the queue, one
+ *              atomic per wavefront.
+ *   k_panel    one lane per queued photon: Woodcock tracking in the crystal
+ *              panel with analytic Klein-Nishina, Rayleigh and photoelectric
+ *              (:1011-1193), hit records, adder and readout (:737-813), and
+ *              the event append (one atomic per wavefront).
+ *
+ * Splitting at the panel face keeps the panel kernel's waves full: in the
+ * reference one kernel per stage runs every photon slot, and ~90% of the
+ * lanes of photonde idle on photons that miss the detector.
+ *
+ * Random numbers: every photon owns Philox stream (history, 1 + photon), the
+ * decay owns (history, 0); the queue carries the draw counter, so regrouping
+ * does not change any history's numbers.
+ */
+#include <hip/hip_runtime.h>
+#include "dev_physics.h"
+
+namespace {
+
+__device__ __forceinline__ unsigned long long lanemask_lt()
+{
+    unsigned int l = __lane_id();
+    return (l == 0) ? 0ull : ((~0ull) >> (64 - l));
+}
+
+/* Wave-aggregated append: every active lane calls it; lanes with want get
+ * consecutive slots. */
+__device__ __forceinline__ int wave_append(int *counter, bool want)
+{
+    unsigned long long m = __ballot(want);
+    int base = 0;
+    if (m) {
+        int leader = __ffsll((unsigned long long)m) - 1;
+        if ((int)__lane_id() == leader) base = atomicAdd(counter, (int)__popcll(m));
+        base = __shfl(base, leader);
+    }
+    return base + (int)__popcll(m & lanemask_lt());
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ __forceinline__ void flush_counter(unsigned long long *dst, unsigned long long v)
+{
+    unsigned long long s = wave_sum(v);
+    if (__lane_id() == 0 && s) atomicAdd(dst, s);
+}
+
+/* ---- source ------------------------------------------------------------ */
+
+__device__ float d_sample_ek(const DevIsotope &iso, gp_rng &r)
+{
+    float E = 0.511f;
+    for (int it = 0; it < 100000; ++it) {
+        E = gp_rng_uniform(&r) * (iso.coef[0] - 0.511f) + 0.511f;
+        float u = iso.coef[1] * gp_rng_uniform(&r);
+        float p = iso.coef[2];
+        for (int i = 3; i < 8; ++i) p = p * E + iso.coef[i];
+        if (u <= p) break;
+    }
+    return (E - 0.511f) * 1.0e6f;
+}
+
+__device__ void d_sample_shape(const DevSource &s, gp_rng &r, float &x, float &y, float &z)
+{
+    const float *c = s.coef;
+    int shape = (s.shape < 0 || s.shape > 2) ? 0 : s.shape;
+    if (shape == 0) {
+        x = c[0] + (c[3] * (2.0f * gp_rng_uniform(&r) - 1.0f)) * 0.5f;
+        y = c[1] + (c[4] * (2.0f * gp_rng_uniform(&r) - 1.0f)) * 0.5f;
+        z = c[2] + (c[5] * (2.0f * gp_rng_uniform(&r) - 1.0f)) * 0.5f;
+    } else if (shape == 1) {
+        float sp, cp;
+        gp_sincos2pi(gp_rng_uniform(&r), &sp, &cp);
+        float rr = c[3] * sqrtf(gp_rng_uniform(&r));
+        x = c[0] + rr * cp;
+        y = c[1] + rr * sp;
+        z = c[2] + (c[4] * (2.0f * gp_rng_uniform(&r) - 1.0f)) * 0.5f;
+    } else {
+        float sp, cp;
+        gp_sincos2pi(gp_rng_uniform(&r), &sp, &cp);
+        float ct = 2.0f * gp_rng_uniform(&r) - 1.0f;
+        float rr = c[3] * gp_cbrtf(gp_rng_uniform(&r));
+        float st = sqrtf(1.0f - ct * ct);
+        x = c[0] + (rr * st) * cp;
+        y = c[1] + (rr * st) * sp;
+        z = c[2] + rr * ct;
+    }
+}
+
+/* Positron range walk; voxel crossings gather only the density half of the
+ * voxel record. */
+__device__ void d_positron_range(const DevPhantom &p, float &x, float &y, float &z, float ek_ev, int usedir,
+                                 float vx, float vy, float vz, gp_rng &r, unsigned long long &nsteps)
+{
+    float ekin = ek_ev / 1.0e6f;
+    float rex = ((0.1f * 5.44040782f) * ekin * ekin) / (0.369516529f + ekin);
+    float sigma = rex / 2.0f;
+    float dx = sigma * gp_rng_normal(&r);
+    float dy = sigma * gp_rng_normal(&r);
+    float dz = sigma * gp_rng_normal(&r);
+    float rr = sqrtf(dx * dx + dy * dy + dz * dz);
+    if (usedir) {
+        float tmp = sqrtf(vx * vx + vy * vy + vz * vz);
+        dx = rr * vx / tmp;
+        dy = rr * vy / tmp;
+        dz = rr * vz / tmp;
+    }
+    if (!(rr > 0.0f)) return;
+    int ix = 0, iy = 0, iz = 0;
+    bool inside = p.present && d_vox(p, x, y, z, ix, iy, iz);
+    float s = 0.0f;
+    const float ex = p.offx + (float)p.nx * p.dx, ey = p.offy + (float)p.ny * p.dy, ez = p.offz + (float)p.nz * p.dz;
+    for (int it = 0; it < D_STEP_CAP && s < rr; ++it) {
+        float step;
+        if (inside) {
+            step = 1000.0f;
+            int axis = 0;
+            if (dx != 0.0f) {
+                float b = (p.offx + (float)(ix + (dx > 0.0f)) * p.dx - x) / dx;
+                if (step > b) { step = b; axis = 1; }
+            }
+            if (dy != 0.0f) {
+                float b = (p.offy + (float)(iy + (dy > 0.0f)) * p.dy - y) / dy;
+                if (step > b) { step = b; axis = 2; }
+            }
+            if (dz != 0.0f) {
+                float b = (p.offz + (float)(iz + (dz > 0.0f)) * p.dz - z) / dz;
+                if (step > b) { step = b; axis = 3; }
+            }
+            if (axis == 1) ix += (dx > 0.0f) ? 1 : -1;
+            else if (axis == 2) iy += (dy > 0.0f) ? 1 : -1;
+            else iz += (dz > 0.0f) ? 1 : -1;
+            float rho;
+            if (ix >= 0 && ix < p.nx && iy >= 0 && iy < p.ny && iz >= 0 && iz < p.nz) {
+                rho = __int_as_float(p.vox[((size_t)iz * p.ny + iy) * p.nx + ix].x);
+                nsteps++;
+            } else {
+                rho = D_RHO_OUTSIDE;
+            }
+            step = step * rr;
+            s = s + step * rho;
+            if (s > rr) step = step + (rr - s) / rho;
+        } else {
+            step = (rr - s) / D_RHO_OUTSIDE;
+            s = rr + 100.0f;
+        }
+        x = x + (step * dx) / rr;
+        y = y + (step * dy) / rr;
+        z = z + (step * dz) / rr;
+        if (inside) {
+            if (x < p.offx || x > ex || y < p.offy || y > ey || z < p.offz || z > ez) inside = false;
+        }
+    }
+}
+
+__device__ void d_emit_pair(float sigma, gp_rng &r, float v[2][3], float e[2])
+{
+    float ct = 2.0f * gp_rng_uniform(&r) - 1.0f;
+    float sp, cp;
+    gp_sincos2pi(gp_rng_uniform(&r), &sp, &cp);
+    float st = sqrtf(1.0f - ct * ct);
+    v[0][0] = st * cp;
+    v[0][1] = st * sp;
+    v[0][2] = ct;
+    float uphi = gp_rng_uniform(&r);
+    float delta = gp_rng_normal(&r) * sigma;
+    e[0] = D_MC2F + (delta * D_MC2F) / 2.0f;
+    e[1] = D_MC2F - (delta * D_MC2F) / 2.0f;
+    v[1][0] = v[0][0];
+    v[1][1] = v[0][1];
+    v[1][2] = v[0][2];
+    d_rotate(v[1][0], v[1][1], v[1][2], -gp_cosf(delta), uphi);
+}
+
+/* Woodcock tracking in the phantom; returns false when absorbed. */
+__device__ bool d_phantom_walk(const DevTables &tab, const DevPhantom &p, float eabs, float &x, float &y, float &z,
+                               float &vx, float &vy, float &vz, float &E, double &tof, gp_rng &r, int &nsteps,
+                               int &ncomp, int &nrayl, unsigned long long &nmaj)
+{
+    for (int it = 0; it < D_STEP_CAP; ++it) {
+        float lammin = d_lam(p.wck, E);
+        float s = -lammin * gp_logf(gp_rng_uniform(&r));
+        x = x + s * vx;
+        y = y + s * vy;
+        z = z + s * vz;
+        tof = tof + (double)s / D_C_LIGHT;
+        int ix, iy, iz;
+        if (!d_vox(p, x, y, z, ix, iy, iz)) return true;
+        nsteps++;
+        int2 vv = p.vox[((size_t)iz * p.ny + iy) * p.nx + ix];
+        float den = __int_as_float(vv.x);
+        int mat = vv.y;
+        float3 mu = d_mu3(tab, mat, E);
+        float lamden = lammin * den;
+        float prob = 1.0f - lamden * mu.x;
+        if (prob < 0.0f) { prob = 0.0f; nmaj++; }
+        float rn = gp_rng_uniform(&r);
+        if (rn < prob) continue;
+        prob = prob + lamden * mu.y;
+        if (rn < prob) {
+            float cth = d_surf(tab, GPET_SURF_COMPTON, mat, E, gp_rng_uniform(&r));
+            float efrac = 1.0f / (1.0f + (E * D_IMC2F) * (1.0f - cth));
+            float uphi = gp_rng_uniform(&r);
+            ncomp++;
+            E = E * efrac;
+            if (E < eabs) return false;
+            d_rotate(vx, vy, vz, cth, uphi);
+            continue;
+        }
+        prob = prob + lamden * mu.z;
+        if (rn < prob) {
+            float cth = d_surf(tab, GPET_SURF_RAYLEIGH, mat, E, gp_rng_uniform(&r));
+            float uphi = gp_rng_uniform(&r);
+            nrayl++;
+            d_rotate(vx, vy, vz, cth, uphi);
+            continue;
+        }
+        return false;
+    }
+    return false;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_phantom(RunParams rp, DevTables tab, DevPhantom ph, DevDetector det, DevQueue q)
+{
+    unsigned long long c_ph = 0, c_ent = 0, c_maj = 0, c_ps = 0, c_rs = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < rp.nh; i += stride) {
+        const int64_t h = rp.h0 + i;
+        float px = 0.f, py = 0.f, pz = 0.f;
+        float v[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+        float e[2] = {0.f, 0.f};
+        double tstart = 0.0;
+        int nph = 0;
+        if (MODE == RUN_DECAY) {
+            int lo = 0, hi = rp.nsrc; /* largest si with first[si] <= h */
+            while (hi - lo > 1) {
+                int mid = (lo + hi) >> 1;
+                if (rp.first[mid] <= h) lo = mid; else hi = mid;
+            }
+            const DevSource S = rp.src[lo];
+            const DevIsotope I = rp.iso[S.iso];
+            gp_rng r;
+            gp_rng_init(&r, rp.seed, (uint64_t)h, GP_TAG_DECAY, 0u);
+            double u = gp_rng_uniform_d(&r);
+            double ts = rp.t0 + S.tau * (-gp_log1p_d(-u * S.pdec));
+            tstart = ts * 1.0e6;
+            float rb = gp_rng_uniform(&r);
+            if (rb < I.ratio) {
+                d_sample_shape(S, r, px, py, pz);
+                if (rp.prange) {
+                    float ek = d_sample_ek(I, r);
+                    d_positron_range(ph, px, py, pz, ek, 0, 0.f, 0.f, 0.f, r, c_rs);
+                }
+                d_emit_pair(rp.sigma, r, v, e);
+                nph = 2;
+            }
+        } else {
+            const double *rec = rp.psf + 8 * i;
+            px = (float)rec[0];
+            py = (float)rec[1];
+            pz = (float)rec[2];
+            tstart = rec[3];
+            float rvx = (float)rec[4], rvy = (float)rec[5], rvz = (float)rec[6], rE = (float)rec[7];
+            if (MODE == RUN_PSF_PHOTON) {
+                v[0][0] = rvx; v[0][1] = rvy; v[0][2] = rvz; e[0] = rE;
+                nph = 1;
+            } else {
+                gp_rng r;
+                gp_rng_init(&r, rp.seed, (uint64_t)h, GP_TAG_DECAY, 0u);
+                if (rp.prange) d_positron_range(ph, px, py, pz, rE, 1, rvx, rvy, rvz, r, c_rs);
+                d_emit_pair(rp.sigma, r, v, e);
+                nph = 2;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const bool active = k < nph;
+            const int64_t slot = 2 * h + k;
+            gpet_trace *tr = (rp.trace && slot < rp.trace_cap) ? rp.trace + slot : nullptr;
+            float x = px, y = py, z = pz, vx = v[k][0], vy = v[k][1], vz = v[k][2], E = e[k];
+            double t = tstart;
+            gp_rng r;
+            gp_rng_init(&r, rp.seed, (uint64_t)h, (uint32_t)(GP_TAG_PHOTON + k), 0u);
+            int ns = 0, nc = 0, nr = 0;
+            bool alive = active;
+            if (active) {
+                c_ph++;
+                if (ph.present) alive = d_phantom_walk(tab, ph, rp.eabs, x, y, z, vx, vy, vz, E, t, r, ns, nc, nr, c_maj);
+            }
+            c_ps += (unsigned long long)ns;
+            if (tr) {
+                tr->history = h;
+                tr->photon = k;
+                tr->panel = -1;
+                tr->t0 = tstart;
+                if (active) {
+                    tr->status = GPET_TR_ALIVE | (alive ? GPET_TR_ESCAPED : GPET_TR_ABSORBED);
+                    tr->n_steps = ns; tr->n_compton = nc; tr->n_rayleigh = nr;
+                    tr->x0 = px; tr->y0 = py; tr->z0 = pz; tr->E0 = e[k];
+                    tr->x = x; tr->y = y; tr->z = z; tr->vx = vx; tr->vy = vy; tr->vz = vz; tr->E = E;
+                    tr->t = t;
+                    tr->draws = r.n;
+                }
+            }
+            int pa = -1;
+            float y2 = 0.f, z2 = 0.f, vx2 = 0.f, vy2 = 0.f, vz2 = 0.f;
+            if (alive && det.present) {
+                for (int pi = 0; pi < det.npanels; ++pi) {
+                    const DevPanel &P = det.p[pi];
+                    float rx = x - P.ox, ry = y - P.oy, rz = z - P.oz;
+                    float tx = rx * P.ux[0] + ry * P.ux[1] + rz * P.ux[2];
+                    float ty = rx * P.uy[0] + ry * P.uy[1] + rz * P.uy[2];
+                    float tz = rx * P.uz[0] + ry * P.uz[1] + rz * P.uz[2];
+                    float tvx = vx * P.ux[0] + vy * P.ux[1] + vz * P.ux[2];
+                    float tvy = vx * P.uy[0] + vy * P.uy[1] + vz * P.uy[2];
+                    float tvz = vx * P.uz[0] + vy * P.uz[1] + vz * P.uz[2];
+                    if (tvx * P.dirx >= 0.0f) {
+                        float ty2 = ty - (tx / tvx) * tvy;
+                        float tz2 = tz - (tx / tvx) * tvz;
+                        if (fabsf(ty2) < P.leny / 2.0f && fabsf(tz2) < P.lenz / 2.0f) {
+                            y2 = ty2;
+                            z2 = tz2;
+                            t = t + (double)(-tx) / (D_C_LIGHT * (double)tvx);
+                            vx2 = tvx; vy2 = tvy; vz2 = tvz;
+                            pa = pi;
+                            break;
+                        }
+                    }
+                }
+            }
+            if (tr && pa >= 0) {
+                tr->panel = pa;
+                tr->status |= GPET_TR_PANEL;
+            }
+            const bool want = pa >= 0;
+            int idx = wave_append(q.count, want);
+            if (want) {
+                c_ent++;
+                if (idx < q.cap) {
+                    q.y[idx] = y2; q.z[idx] = z2;
+                    q.vx[idx] = vx2; q.vy[idx] = vy2; q.vz[idx] = vz2;
+                    q.E[idx] = E;
+                    q.t[idx] = t;
+                    int parn = (MODE == RUN_PSF_PHOTON) ? (int)h : (int)(2 * h + k);
+                    q.info[idx] = make_int4(pa, parn, (int)h, (int)(r.n | ((unsigned)k << 31)));
+                    if (q.trace_slot) q.trace_slot[idx] = tr ? (int)slot : -1;
+                }
+            }
+        }
+    }
+    flush_counter(rp.cnt + CNT_PHOTONS, c_ph);
+    flush_counter(rp.cnt + CNT_PANEL_ENTRIES, c_ent);
+    flush_counter(rp.cnt + CNT_MAJORANT, c_maj);
+    flush_counter(rp.cnt + CNT_PHANTOM_STEPS, c_ps);
+    flush_counter(rp.cnt + CNT_RANGE_STEPS, c_rs);
+}
+
+/* ---- panel -------------------------------------------------------------- */
+
+struct EvList {
+    int site[GPET_MAX_PH_EVENTS], mod[GPET_MAX_PH_EVENTS], cry[GPET_MAX_PH_EVENTS];
+    double t[GPET_MAX_PH_EVENTS];
+    float E[GPET_MAX_PH_EVENTS], x[GPET_MAX_PH_EVENTS], y[GPET_MAX_PH_EVENTS], z[GPET_MAX_PH_EVENTS];
+};
+
+__device__ __forceinline__ void d_adder(EvList &L, int &n, int site, int mod, int cry, double t, float E, float x,
+                                        float y, float z, unsigned long long &dropped)
+{
+    bool merged = false;
+#pragma unroll
+    for (int i = 0; i < GPET_MAX_PH_EVENTS; ++i) {
+        if (!merged && i < n && L.site[i] == site) {
+            float sE = L.E[i] + E;
+            L.x[i] = (L.x[i] * L.E[i] + x * E) / sE;
+            L.y[i] = (L.y[i] * L.E[i] + y * E) / sE;
+            L.z[i] = (L.z[i] * L.E[i] + z * E) / sE;
+            L.E[i] = sE;
+            merged = true;
+        }
+    }
+    if (merged) return;
+    if (n >= GPET_MAX_PH_EVENTS) {
+        dropped++;
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < GPET_MAX_PH_EVENTS; ++i) {
+        if (i == n) {
+            L.site[i] = site; L.mod[i] = mod; L.cry[i] = cry; L.t[i] = t;
+            L.E[i] = E; L.x[i] = x; L.y[i] = y; L.z[i] = z;
+        }
+    }
+    n++;
+}
+
+/* readout (gPET_kernals.cu:756-813) on the register list; O gets the output. */
+__device__ __forceinline__ int d_readout(const EvList &L, int n, int depth, int policy, int pann, int moduleN, EvList &O)
+{
+    if (depth == 3) {
+        O = L;
+        return n;
+    }
+    if (policy == 1) depth = 2;
+    int s[GPET_MAX_PH_EVENTS];
+    bool dead[GPET_MAX_PH_EVENTS];
+#pragma unroll
+    for (int i = 0; i < GPET_MAX_PH_EVENTS; ++i) {
+        s[i] = (depth == 0) ? 0 : (depth == 1 ? pann : pann * moduleN + L.mod[i]);
+        dead[i] = false;
+    }
+    int out = 0;
+#pragma unroll
+    for (int i = 0; i < GPET_MAX_PH_EVENTS; ++i) {
+        if (i < n && !dead[i]) {
+            int cmod = L.mod[i], ccry = L.cry[i];
+            double ct = L.t[i];
+            float cE = L.E[i], cx = L.x[i], cy = L.y[i], cz = L.z[i];
+#pragma unroll
+            for (int j = i + 1; j < GPET_MAX_PH_EVENTS; ++j) {
+                if (j < n && s[j] == s[i]) {
+                    if (policy == 1) {
+                        float sE = cE + L.E[j];
+                        cx = (cx * cE + L.x[j] * L.E[j]) / sE;
+                        cy = (cy * cE + L.y[j] * L.E[j]) / sE;
+                        cz = (cz * cE + L.z[j] * L.E[j]) / sE;
+                        cE = sE;
+                    } else if (!(cE > L.E[j])) {
+                        cmod = L.mod[j]; ccry = L.cry[j]; ct = L.t[j];
+                        cE = L.E[j]; cx = L.x[j]; cy = L.y[j]; cz = L.z[j];
+                    }
+                    dead[j] = true;
+                }
+            }
+#pragma unroll
+            for (int o = 0; o < GPET_MAX_PH_EVENTS; ++o) {
+                if (o == out) {
+                    O.site[o] = s[i]; O.mod[o] = cmod; O.cry[o] = ccry; O.t[o] = ct;
+                    O.E[o] = cE; O.x[o] = cx; O.y[o] = cy; O.z[o] = cz;
+                }
+            }
+            out++;
+        }
+    }
+    return out;
+}
+
+__global__ __launch_bounds__(256) void k_panel(PanelParams pp, DevTables tab, DevDetector det, DevQueue q, int nq)
+{
+    unsigned long long c_hits = 0, c_drop = 0, c_droph = 0, c_maj = 0, c_steps = 0;
+    const int stride = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += stride) {
+        float x2 = 0.0f, y2 = q.y[i], z2 = q.z[i];
+        float vx2 = q.vx[i], vy2 = q.vy[i], vz2 = q.vz[i];
+        float E = q.E[i];
+        double tof = q.t[i];
+        int4 info = q.info[i];
+        const int pa = info.x, parn = info.y, eid = info.z;
+        const unsigned draws = (unsigned)info.w & 0x7fffffffu;
+        const unsigned k = (unsigned)info.w >> 31;
+        gp_rng r;
+        gp_rng_init(&r, pp.seed, (uint64_t)(int64_t)eid, GP_TAG_PHOTON + k, draws);
+        const DevPanel &P = det.p[pa];
+        const int pann = P.panel;
+        const float hx = P.lenx, hy = P.leny * 0.5f, hz = P.lenz * 0.5f;
+        EvList L;
+        int nev = 0;
+        int nsteps = 0, nint = 0;
+        for (int it = 0; it < D_STEP_CAP; ++it) {
+            float lammin = d_lam(det.wck, E);
+            float s = -lammin * gp_logf(gp_rng_uniform(&r));
+            x2 = x2 + s * vx2;
+            y2 = y2 + s * vy2;
+            z2 = z2 + s * vz2;
+            tof = tof + (double)s / D_C_LIGHT;
+            if (fabsf(y2) > hy || fabsf(z2) > hz || x2 * P.dirx < 0.0f || x2 * P.dirx > hx) break;
+            nsteps++;
+            int M_id = -1, L_id = -1;
+            int m_id = d_crystal(det, P, x2, y2, z2, M_id, L_id);
+            float den = det.dens[m_id];
+            int mat = det.mat[m_id];
+            float3 mu = d_mu3(tab, mat, E);
+            float lamden = lammin * den;
+            float prob = 1.0f - lamden * mu.x;
+            if (prob < 0.0f) { prob = 0.0f; c_maj++; }
+            float rn = gp_rng_uniform(&r);
+            if (rn < prob) continue;
+            nint++;
+            prob = prob + lamden * mu.y;
+            int type = 0;
+            float edep = 0.0f;
+            bool stop = false, rot = false;
+            float cth = 0.0f, uphi = 0.0f;
+            float de2 = 0.0f;
+            bool second = false;
+            if (rn < prob) {
+                float efrac;
+                d_kn(E, r, efrac, cth);
+                float de = E * (1.0f - efrac);
+                uphi = gp_rng_uniform(&r);
+                type = 1;
+                edep = de;
+                E = E - de;
+                if (E < pp.eabs) {
+                    second = true;
+                    de2 = E;
+                    stop = true;
+                } else {
+                    rot = true;
+                }
+            } else {
+                prob = prob + lamden * mu.z;
+                if (rn < prob) {
+                    cth = d_surf(tab, GPET_SURF_RAYLEIGH, mat, E, gp_rng_uniform(&r));
+                    uphi = gp_rng_uniform(&r);
+                    type = pp.record_rayleigh ? 3 : 0;
+                    rot = true;
+                } else {
+                    type = 4;
+                    edep = E;
+                    stop = true;
+                }
+            }
+            if (m_id == 0 && type != 0) {
+                const int site = pann * det.moduleN * det.crystalN + M_id * det.crystalN + L_id;
+                for (int rec = 0; rec < (second ? 2 : 1); ++rec) {
+                    const int ty = rec ? 2 : type;
+                    const float ed = rec ? de2 : (type == 3 ? 0.0f : edep);
+                    c_hits++;
+                    if (pp.hit_id) {
+                        unsigned hidx = atomicAdd(pp.n_hits, 1u);
+                        if ((int64_t)hidx < pp.hits_cap) {
+                            int32_t *hi = pp.hit_id + 5 * (size_t)hidx;
+                            float *hf = pp.hit_f + 5 * (size_t)hidx;
+                            hi[0] = parn; hi[1] = pann; hi[2] = M_id; hi[3] = L_id; hi[4] = ty;
+                            hf[0] = ed; hf[1] = (float)tof; hf[2] = x2; hf[3] = y2; hf[4] = z2;
+                        } else {
+                            c_droph++;
+                        }
+                    }
+                    if (ty != 3) d_adder(L, nev, site, M_id, L_id, tof, ed, x2, y2, z2, c_drop);
+                }
+            }
+            if (stop) break;
+            if (rot) d_rotate(vx2, vy2, vz2, cth, uphi);
+        }
+        EvList O;
+        int nout = 0;
+        if (nev > 0) nout = d_readout(L, nev, pp.readout_depth, pp.readout_policy, pann, det.moduleN, O);
+        /* wave-aggregated append of nout (0..4) events per lane */
+        unsigned long long lt = lanemask_lt();
+        int off = 0, tot = 0;
+#pragma unroll
+        for (int j = 0; j < GPET_MAX_PH_EVENTS; ++j) {
+            unsigned long long b = __ballot(nout > j);
+            off += (int)__popcll(b & lt);
+            tot += (int)__popcll(b);
+        }
+        int base = 0;
+        if (tot) {
+            if (__lane_id() == 0) base = (int)atomicAdd(pp.n_events, (unsigned)tot);
+            base = __shfl(base, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < GPET_MAX_PH_EVENTS; ++j) {
+            if (j < nout) {
+                int64_t dst = (int64_t)base + off + j;
+                if (dst < pp.events_cap) {
+                    gpet_event ev;
+                    ev.parn = parn; ev.pann = pann; ev.modn = O.mod[j]; ev.cryn = O.cry[j];
+                    ev.siten = O.site[j]; ev.eventid = eid; ev.t = O.t[j];
+                    ev.E = O.E[j]; ev.x = O.x[j]; ev.y = O.y[j]; ev.z = O.z[j];
+                    pp.events[dst] = ev;
+                }
+            }
+        }
+        c_steps += (unsigned long long)nsteps;
+        if (q.trace_slot) {
+            int ts = q.trace_slot[i];
+            if (ts >= 0) {
+                gpet_trace *tr = pp.trace + ts;
+                tr->n_steps_panel = nsteps;
+                tr->n_int_panel = nint;
+                tr->n_events = nout;
+                tr->draws = r.n;
+            }
+        }
+    }
+    flush_counter(pp.cnt + CNT_HITS, c_hits);
+    flush_counter(pp.cnt + CNT_DROPPED_EVENTS, c_drop);
+    flush_counter(pp.cnt + CNT_DROPPED_HITS, c_droph);
+    flush_counter(pp.cnt + CNT_MAJORANT, c_maj);
+    flush_counter(pp.cnt + CNT_PANEL_STEPS, c_steps);
+}
+
+} // namespace
+
+int gpet_launch_phantom(const RunParams &rp, const DevTables &tab, const DevPhantom &ph, const DevDetector &det,
+                        const DevQueue &q, hipStream_t s)
+{
+    if (rp.nh <= 0) return 0;
+    const int block = 256;
+    int64_t nb = (rp.nh + block - 1) / block;
+    int grid = (int)(nb < 1048576 ? nb : 1048576);
+    if (rp.mode == RUN_DECAY)
+        hipLaunchKernelGGL(k_phantom<RUN_DECAY>, dim3(grid), dim3(block), 0, s, rp, tab, ph, det, q);
+    else if (rp.mode == RUN_PSF_POSITRON)
+        hipLaunchKernelGGL(k_phantom<RUN_PSF_POSITRON>, dim3(grid), dim3(block), 0, s, rp, tab, ph, det, q);
+    else
+        hipLaunchKernelGGL(k_phantom<RUN_PSF_PHOTON>, dim3(grid), dim3(block), 0, s, rp, tab, ph, det, q);
+    return (int)hipGetLastError();
+}
+
+int gpet_launch_panel(const PanelParams &pp, const DevTables &tab, const DevDetector &det, const DevQueue &q,
+                      int64_t n_items, hipStream_t s)
+{
+    if (n_items <= 0) return 0;
+    const int block = 256;
+    int64_t nb = (n_items + block - 1) / block;
+    int grid = (int)(nb < 1048576 ? nb : 1048576);
+    hipLaunchKernelGGL(k_panel, dim3(grid), dim3(block), 0, s, pp, tab, det, q, (int)n_items);
+    return (int)hipGetLastError();
+}

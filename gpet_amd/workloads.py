@@ -1,0 +1,176 @@
+"""Synthetic workloads: phantoms, scanners, sources and digitizer settings.
+
+The reference ships none of the inputs these need (the cylinder phantom and
+the PSF are missing blobs, and no scanner matches the benchmark geometry),
+so they are generated here from their descriptions, in the reference's file
+formats where one exists (config8.geo text, source.txt rows).
+
+Material indices are rows of data/input4gPET.matter: 0 DryAir, 1 Water,
+2 TissueICRU, 3 CorticalBone, 7 LSO.
+"""
+from __future__ import annotations
+
+import os
+import tempfile
+
+import numpy as np
+
+from . import capi
+from .engine import read_geometry
+
+AIR, WATER, TISSUE, BONE, LSO = 0, 1, 2, 3, 7
+
+
+def water_cube(n: int = 64, voxel: float = 0.05):
+    """Uniform water cube centred on the origin (int32 mat, float32 density)."""
+    mat = np.full((n, n, n), WATER, np.int32)
+    dens = np.ones((n, n, n), np.float32)
+    size = n * voxel
+    return mat, dens, (n, n, n), (-size / 2,) * 3, (size,) * 3
+
+
+def cylinder_phantom(n: int = 200, size: float = 1.0, radius: float = 0.4, rod_radius: float = 0.1,
+                     air_density: float = 0.0012):
+    """Water cylinder with a cortical-bone rod in air; axis along z, arrays
+    indexed [z, y, x] so that x is fastest in memory."""
+    d = size / n
+    c = (np.arange(n) + 0.5) * d - size / 2
+    yy, xx = np.meshgrid(c, c, indexing="ij")
+    r2 = xx * xx + yy * yy
+    sl_mat = np.full((n, n), AIR, np.int32)
+    sl_den = np.full((n, n), air_density, np.float32)
+    sl_mat[r2 <= radius * radius] = WATER
+    sl_den[r2 <= radius * radius] = 1.0
+    sl_mat[r2 <= rod_radius * rod_radius] = BONE
+    sl_den[r2 <= rod_radius * rod_radius] = 1.85
+    mat = np.broadcast_to(sl_mat, (n, n, n)).copy()
+    dens = np.broadcast_to(sl_den, (n, n, n)).copy()
+    return mat, dens, (n, n, n), (-size / 2,) * 3, (size,) * 3
+
+
+def geometry_text(n_panels: int = 8, radius: float = 8.0, ncry: int = 20, pitch: float = 0.15,
+                  gap: float = 0.0005, depth: float = 1.0, crystal_mat: int = LSO, crystal_den: float = 7.4,
+                  rings: int = 1, ring_spacing: float = 0.0) -> str:
+    """A config8.geo-format scanner: n_panels flat panels of ncry x ncry
+    crystals (one module each) on a ring of inner radius `radius`.  The
+    crystal size is pitch - gap with a gap > 0 so the reference's count
+    formula floor(M/(L+s))+1 gives ncry (initialize.cu:1074-1086)."""
+    cry = pitch - gap
+    mod = ncry * pitch - gap
+    ang = 360.0 / n_panels
+    lines = [
+        "panel repetition times", str(n_panels),
+        "panel rotational axis in the global coordinate", "0 0 1",
+        "panel rotational angle (in degree, plus: counterclockwise, minus: clockwise)", "%.10g" % ang,
+        "material density (in the order of crystal material, air)",
+        "%d %.6g" % (crystal_mat, crystal_den), "0 0.001025",
+        "%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%",
+        "starting panel index", "0",
+        "dimension(cm)", "%.6g %.6g %.6g" % (depth, mod, mod),
+        "module size", "%.6g %.6g %.6g" % (depth, mod, mod),
+        "module space", "%.6g %.6g %.6g" % (depth, 0.01, 0.01),
+        "crystal size (cm)", "%.6g %.6g %.6g" % (depth, cry, cry),
+        "crystal space(cm)", "%.6g %.6g %.6g" % (0.0, gap, gap),
+        "panel expansion direction in the local coordinate (x y z)", "-1 1 1",
+        "offset of the panel surface center facing the phantom (cm)", "0 %.6g 0" % (-radius),
+        "unit vector along the local x direction in the global coordinate", "0 1 0",
+        "unit vector along the local y direction in the global coordinate", "1 0 0",
+        "unit vector along the local z direction in the global coordinate", "0 0 -1",
+        "%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%%",
+    ]
+    if rings > 1:
+        lines += ["number of rings and ring spacing along the rotation axis (cm)", "%d %.6g" % (rings, ring_spacing)]
+    return "\n".join(lines) + "\n"
+
+
+def octagon_scanner(**kw):
+    """Panels and materials of the 8-panel LSO octagon, parsed from the
+    config8.geo text by the library's own reader."""
+    txt = geometry_text(**kw)
+    with tempfile.NamedTemporaryFile("w", suffix=".geo", delete=False) as fh:
+        fh.write(txt)
+        path = fh.name
+    try:
+        return read_geometry(path)
+    finally:
+        os.unlink(path)
+
+
+def isotope_f18() -> capi.Isotope:
+    """Row 0 of data/isotopes.txt (F-18)."""
+    iso = capi.Isotope()
+    iso.half_life_s = 6586.26
+    iso.beta_ratio = 0.97
+    for k, v in enumerate([1.144, 0.5209, 0, 0, 10.2088, -30.4551, 28.4376, -7.9828]):
+        iso.coef[k] = v
+    return iso
+
+
+def isotope_pure_pairs() -> capi.Isotope:
+    """A positron emitter that always emits (ratio 1) with a half-life long
+    enough for a flat emission rate; used for 'N back-to-back pairs'."""
+    iso = isotope_f18()
+    iso.half_life_s = 1.0e9
+    iso.beta_ratio = 1.0
+    return iso
+
+
+def source(shape: int, centre=(0.0, 0.0, 0.0), c3=0.0, c4=0.0, c5=0.0, natom=1.0e12, isotope=0) -> capi.Source:
+    s = capi.Source()
+    s.natom = natom
+    s.isotope = isotope
+    s.shape = shape
+    for k, v in enumerate([centre[0], centre[1], centre[2], c3, c4, c5]):
+        s.coef[k] = v
+    return s
+
+
+def digitizer(readout_depth=2, readout_policy=1, e_threshold=50e3, blur_policy=1, e_ref=511e3, r_ref=0.12,
+              slope=0.0, space_blur=0.0, dead_level=3, dead_type=0, dead_time_us=0.0, win_min=350e3,
+              win_max=650e3, time_blur_fwhm_us=300e-6, coinc_window_us=4e-3, coinc_policy=0,
+              record_rayleigh=0) -> capi.Digitizer:
+    """Digitizer of the benchmark scanner: 12% FWHM at 511 keV, 350-650 keV,
+    300 ps time blur, 4 ns coincidence window, no deadtime."""
+    d = capi.Digitizer()
+    d.readout_depth = readout_depth
+    d.readout_policy = readout_policy
+    d.e_threshold = e_threshold
+    d.e_upholder = 2.0e6
+    d.blur_policy = blur_policy
+    d.e_ref = e_ref
+    d.r_ref = r_ref
+    d.slope = slope
+    d.space_blur = space_blur
+    d.dead_level = dead_level
+    d.dead_type = dead_type
+    d.dead_time_us = dead_time_us
+    d.win_min = win_min
+    d.win_max = win_max
+    d.time_blur_fwhm_us = time_blur_fwhm_us
+    d.coinc_window_us = coinc_window_us
+    d.coinc_policy = coinc_policy
+    d.record_rayleigh = record_rayleigh
+    return d
+
+
+def setup_engine(eng, tables, config: int = 2, phantom_n=None, scanner_kw=None, dig=None):
+    """Load tables, phantom, scanner and digitizer of configuration 1 (water
+    cube) or 2 (cylinder with bone rod)."""
+    eng.set_tables(tables)
+    if config == 1:
+        ph = water_cube(phantom_n or 64)
+    else:
+        ph = cylinder_phantom(phantom_n or 200)
+    eng.set_phantom(*ph)
+    panels, mats = octagon_scanner(**(scanner_kw or {}))
+    eng.set_detector(panels, mats, np.array([0, 0, 0, 0, 0, 0, 0, 0, 0, 1], np.float32))
+    eng.set_digitizer(dig or digitizer())
+    return panels, mats
+
+
+def workload_sources(config: int):
+    """(sources, isotopes, t0, t1) of configuration 1 or 2."""
+    if config == 1:
+        return [source(2, natom=1.0e15)], [isotope_pure_pairs()], 0.0, 1.0
+    # F-18 uniform in the r = 0.4 cm water cylinder (full 1 cm height)
+    return [source(1, c3=0.4, c4=1.0, natom=1.0e11)], [isotope_f18()], 0.0, 10.0

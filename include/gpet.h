@@ -1,0 +1,335 @@
+/*
+ * gpet.h - C ABI of the MI355X photon transport and detection path.
+ *
+ * The reference (utaresearch/gPET) is a monolithic executable with no plugin
+ * or FFI API: main.cu:26-229 parses input_PET.in and then calls an internal
+ * host sequence init -> read_file_ro -> iniPanel -> iniwck -> simulateParticle /
+ * sampleParticle (gPET.h:128-166).  This header is that sequence as a C ABI:
+ * plain structs, plain pointers and sizes, status codes instead of exit().
+ *
+ * Conventions (differences from the reference are deliberate):
+ *   - Every function returns gpet_status; 0 is success, negative is an error.
+ *     gpet_last_error() gives the message.  Nothing in the library calls exit().
+ *   - The caller owns every input array; the library copies what it keeps.
+ *   - Units follow the reference: cm, eV, microseconds (time), seconds for the
+ *     decay window t0/t1 (main.cu:138-141).
+ *   - One context per device.  A context is not thread safe; independent
+ *     contexts are.
+ *   - The oracle library exports the same functions with the prefix gpet_cpu_
+ *     (same signatures) so tests can swap implementations.
+ */
+#ifndef GPET_H
+#define GPET_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t gpet_status;
+enum {
+    GPET_OK = 0,
+    GPET_ERR_ARG = -1,         /* bad argument / inconsistent sizes            */
+    GPET_ERR_IO = -2,          /* file missing or unreadable (gPET.h:17)        */
+    GPET_ERR_FORMAT = -3,      /* table/geometry dimension mismatch             */
+    GPET_ERR_DEVICE = -4,      /* HIP runtime error                             */
+    GPET_ERR_CAPACITY = -5,    /* a caller buffer or a library limit exceeded   */
+    GPET_ERR_STATE = -6,       /* call order (e.g. run before tables)           */
+    GPET_ERR_UNSUPPORTED = -7
+};
+
+#define GPET_MAXMAT 10         /* constants.h:9  MAXMAT                         */
+#define GPET_NTAB 4096         /* constants.h:20-25 NLAPH/NCMPT/NPHTE/NRAYL     */
+#define GPET_NWCK 4096         /* constants.h:10 NWCK                           */
+#define GPET_NCP 301           /* constants.h:22,26 NCPCM/NCPRL                 */
+#define GPET_NE 151            /* constants.h:23,27 NECM/NERL                   */
+#define GPET_MAXSURFACE 8      /* constants.h:19 MAXSURFACE (reference: 5)      */
+#define GPET_MAXPANELS 256
+
+/* 48-byte digitizer record; byte-identical to Event, gPET.h:87-92
+ * (t at offset 24, E at offset 32).  siten is the site index at the current
+ * digitizer depth. */
+typedef struct gpet_event {
+    int32_t parn, pann, modn, cryn, siten, eventid;
+    double t;                  /* microseconds                                  */
+    float E, x, y, z;          /* eV; panel-local cm                            */
+} gpet_event;
+
+/* Cross-section tables (data/input4gPET.*).  The four 1-D tables share the
+ * layout of initialize.cu:370-662: nmat blocks of n1d nodes on a linear grid
+ * E_i = e0 + i/idl.  Surfaces follow initialize.cu:483-566,664-748: value
+ * cos(theta) at [mat][icp][ie] with CP = icp/idcp and E = ie/ide. */
+enum { GPET_TAB_LAMPH = 0, GPET_TAB_COMPT = 1, GPET_TAB_PHOTE = 2, GPET_TAB_RAYLE = 3 };
+enum { GPET_SURF_COMPTON = 0, GPET_SURF_RAYLEIGH = 1 };
+typedef struct gpet_tables {
+    int32_t nmat;
+    float eminph, emax;        /* [Emin_ph, Emax] from input4gPET.matter (eV)   */
+    int32_t n1d;               /* nodes per material, 4096                      */
+    float e0[4];               /* first node energy per table                   */
+    float idl[4];              /* inverse node spacing per table               */
+    const float *mu[4];        /* [nmat*n1d], mu/rho in cm^2/g                  */
+    int32_t ncp, ne;           /* 301, 151                                      */
+    float idcp[2], ide[2];     /* 1/dCP, 1/dE per surface                       */
+    const float *surf[2];      /* [nmat*ncp*ne]                                 */
+} gpet_tables;
+
+/* One detector panel: object_t, gPET.h:54-79 (x is the depth axis). */
+typedef struct gpet_panel {
+    int32_t panel;
+    float lenx, leny, lenz;    /* panel dimension                               */
+    float modx, mody, modz;    /* module size                                   */
+    float mspx, mspy, mspz;    /* module spacing                                */
+    float crx, cry, crz;       /* crystal size                                  */
+    float cspx, cspy, cspz;    /* crystal spacing                               */
+    float ox, oy, oz;          /* centre of the face looking at the phantom     */
+    float dirx, diry, dirz;    /* expansion direction in local coordinates      */
+    float ux[3], uy[3], uz[3]; /* local unit vectors in global coordinates      */
+} gpet_panel;
+
+/* object_v, gPET.h:80-85: [0] crystal material, [1] gap material. */
+typedef struct gpet_panel_material {
+    int32_t material;
+    float density;
+} gpet_panel_material;
+
+/* Digitizer chain of gPET.cu:380-424 plus the two stages the reference lacks
+ * (time blur, coincidence sorter).  Field meanings follow input_PET.in:34-45. */
+typedef struct gpet_digitizer {
+    int32_t readout_depth;     /* 0 world, 1 panel, 2 module, 3 crystal         */
+    int32_t readout_policy;    /* 0 winner-takes-all, 1 energy centroid         */
+    float e_threshold;         /* thresholder before deadtime (eV)              */
+    float e_upholder;          /* its upper edge; the reference uses 2 MeV      */
+    int32_t blur_policy;       /* 0: R = Rref*sqrt(Eref/E); 1: R = Rref+slope*dE*/
+    float e_ref, r_ref, slope; /* eV, fraction FWHM, 1/MeV                      */
+    float space_blur;          /* cm, Gaussian sigma on x,y,z                   */
+    int32_t dead_level;        /* 0..3, as setSitenum                           */
+    int32_t dead_type;         /* 0 paralyzable, 1 non-paralyzable              */
+    float dead_time_us;        /* 0 disables                                    */
+    float win_min, win_max;    /* energy window (eV)                            */
+    double time_blur_fwhm_us;  /* Gaussian FWHM on t (0 disables)               */
+    double coinc_window_us;    /* coincidence window (0 disables the sorter)    */
+    int32_t coinc_policy;      /* multiples: 0 kill all, 1 take all pairs       */
+    int32_t record_rayleigh;   /* constants.h:7 RECORDRayleigh                  */
+} gpet_digitizer;
+
+enum {
+    GPET_COINC_KILL_MULTIPLES = 0,
+    GPET_COINC_TAKE_ALL_GOODS = 1
+};
+
+typedef struct gpet_options {
+    int32_t device;
+    uint64_t seed;             /* Philox key; replaces srand(time(NULL))        */
+    float eabs;                /* photon absorption energy (eV), input_PET.in:27*/
+    float noncol_sigma;        /* rad, input_PET.in:3                           */
+    int32_t positron_range;    /* input_PET.in:21                               */
+    int32_t record_hits;       /* constants.h:6 OUTPUTHIT                        */
+    int64_t batch;             /* histories per transport launch; 0 = automatic */
+    int32_t flags;             /* reserved, 0                                   */
+} gpet_options;
+
+/* A source line of source.txt (initialize.cu:116-144).  natom is a double so
+ * that activities beyond uint32 (reference gPET.h:50) are expressible. */
+typedef struct gpet_source {
+    double natom;
+    int32_t isotope;           /* row of the isotope table                      */
+    int32_t shape;             /* 0 box, 1 cylinder (axis z), 2 sphere          */
+    float coef[6];             /* centre xyz + box sizes / cyl r,h / sphere r   */
+} gpet_source;
+
+/* A row of data/isotopes.txt (initialize.cu:10-31). */
+typedef struct gpet_isotope {
+    float half_life_s;
+    float beta_ratio;
+    float coef[8];             /* Emax(total, MeV), envelope max, 6 poly coefs  */
+} gpet_isotope;
+
+/* Per-photon transport record.  Written only when the caller passes a trace
+ * buffer; it exists for per-history parity checks. */
+typedef struct gpet_trace {
+    int64_t history;
+    int32_t photon;            /* 0 or 1 (pairs); 0 for photon-PSF records      */
+    int32_t status;            /* GPET_TR_* bits                                */
+    int32_t n_steps;           /* Woodcock steps in the phantom (real+virtual)  */
+    int32_t n_compton, n_rayleigh;
+    int32_t panel;             /* panel entered, -1 none                        */
+    int32_t n_steps_panel;
+    int32_t n_int_panel;       /* real interactions in the panel                */
+    int32_t n_events;          /* events after readout                          */
+    uint32_t draws;            /* random numbers consumed by this photon        */
+    float x0, y0, z0, E0;      /* start of the photon                           */
+    float x, y, z;             /* phantom exit (or absorption) point            */
+    float vx, vy, vz, E;       /* direction and energy there                    */
+    double t0, t;              /* start time and time there (us)                */
+} gpet_trace;
+
+enum {
+    GPET_TR_ABSORBED = 1,      /* photoelectric or below eabs in the phantom    */
+    GPET_TR_ESCAPED = 2,       /* left the phantom                              */
+    GPET_TR_PANEL = 4,         /* entered a panel                               */
+    GPET_TR_ALIVE = 8          /* photon was emitted (beta+ branch taken)       */
+};
+
+/* Output of one run.  Host buffers may be NULL; counts are always filled and
+ * may exceed the capacities (then only the first cap records are copied). */
+typedef struct gpet_sink {
+    gpet_event *singles; int64_t singles_cap; int64_t n_singles;
+    gpet_event *coinc;   int64_t coinc_cap;   int64_t n_coinc;  /* pairs: 2 events each */
+    gpet_event *adder;   int64_t adder_cap;   int64_t n_adder;  /* after adder+readout  */
+    int32_t *hit_id;     float *hit_f;        int64_t hits_cap; int64_t n_hits; /* 5+5 per hit */
+    gpet_trace *trace;   int64_t trace_cap;   int64_t n_trace;
+    const char *out_dir; /* non-NULL: write singles.dat, adder.dat, coincidences.dat, Hits*.dat */
+} gpet_sink;
+
+/* Run statistics and tallies.  Arrays are caller-provided (may be NULL). */
+typedef struct gpet_tallies {
+    int64_t n_histories;       /* decays (source) or PSF records                */
+    int64_t n_photons;         /* photons started                               */
+    int64_t n_panel_entries;
+    int64_t n_hits;
+    int64_t n_adder;           /* events after adder + readout                  */
+    int64_t n_threshold;       /* after thresholder                             */
+    int64_t n_deadtime;        /* after deadtime                                */
+    int64_t n_singles;         /* after energy window                           */
+    int64_t n_coinc;
+    int64_t n_coinc_true;      /* both singles from the same decay              */
+    int64_t dropped_events;    /* per-photon event list overflow (ref. Q5)      */
+    int64_t dropped_hits;
+    int64_t majorant_violations;
+    int64_t phantom_steps;     /* Woodcock steps landing in the phantom         */
+    int64_t range_steps;       /* voxel crossings of the positron range walk    */
+    int64_t panel_steps;
+    int64_t *crystal_singles;  int32_t n_crystals;       /* per global crystal */
+    int64_t *energy_hist;      int32_t n_ebins; float e_lo, e_hi;
+    int64_t *sinogram;         int32_t n_s, n_phi; float s_max;
+    double seconds_transport, seconds_digitizer;
+    double kernel_seconds[2];  /* summed launch durations: 0 phantom, 1 panel   */
+    int64_t kernel_launches[2];
+} gpet_tallies;
+
+typedef struct gpet_ctx gpet_ctx;
+
+/* ---- host-only helpers: parsers and writers (no device needed) ---------- */
+
+/* Parse data/input4gPET.{matter,lamph,compt,phote,rayle,cmpsf,rayff} from a
+ * directory (initialize.cu:279-748).  The tables own library memory; release
+ * with gpet_tables_free. */
+gpet_status gpet_tables_load(const char *data_dir, gpet_tables *out);
+void gpet_tables_free(gpet_tables *t);
+
+/* config8.geo (detector.cu:64-285): panel 0 from the file, replicated by
+ * rotation.  Optional multi-ring extension lines are described in the
+ * implementation. */
+gpet_status gpet_read_geometry(const char *path, gpet_panel *panels, int32_t cap,
+                               int32_t *n_panels, gpet_panel_material mats[2]);
+
+/* source.txt (initialize.cu:116-144) and isotopes.txt (initialize.cu:10-31). */
+gpet_status gpet_read_sources(const char *path, gpet_source *out, int32_t cap, int32_t *n);
+gpet_status gpet_read_isotopes(const char *path, gpet_isotope *out, int32_t cap, int32_t *n);
+
+/* psf.dat: 64-byte records of 8 doubles x y z t vx vy vz E
+ * (initialize.cu:76-115).  Returns the number of records in *n (clamped to
+ * cap when out is non-NULL). */
+gpet_status gpet_read_psf(const char *path, double *out, int64_t cap, int64_t *n);
+
+/* Number of modules / crystals per panel and along y (initialize.cu:1074-1086). */
+gpet_status gpet_panel_counts(const gpet_panel *p0, int32_t *module_n, int32_t *crystal_n,
+                              int32_t *module_ny, int32_t *crystal_ny);
+
+/* input_PET.in: 20 label/value fields in fixed order (main.cu:50-184).
+ * Optional trailing label/value pairs extend it (all absent = reference
+ * behaviour): time blur FWHM in us; coincidence window in us and multiples
+ * policy; Philox seed. */
+#define GPET_PATH_MAX 256
+typedef struct gpet_input {
+    int32_t device;
+    float noncol_sigma;
+    int32_t dims[3];
+    float offset[3], size[3];
+    char mat_file[GPET_PATH_MAX], den_file[GPET_PATH_MAX];
+    int64_t n_histories;
+    int32_t use_psf;
+    char source_file[GPET_PATH_MAX];
+    int32_t ptype, prange;
+    float t0, t1;
+    float record_sphere[4];
+    float eabs;
+    char geo_file[GPET_PATH_MAX];
+    int32_t n_surfaces;
+    float surfaces[10 * GPET_MAXSURFACE];
+    int32_t readout_depth, readout_policy;
+    float e_threshold;
+    int32_t blur_policy;
+    float e_ref, r_ref, slope, space_blur;
+    int32_t dead_level, dead_type;
+    float dead_time;
+    float win_min, win_max;
+    double time_blur_fwhm_us, coinc_window_us;
+    int32_t coinc_policy;
+    uint64_t seed;
+} gpet_input;
+
+gpet_status gpet_read_input(const char *path, gpet_input *out);
+
+/* Digitizer settings implied by an input file (upholder 2 MeV, gPET.cu:393). */
+void gpet_input_digitizer(const gpet_input *in, gpet_digitizer *out);
+
+/* Binary writers (detector.cu:287-307, gPET.cu:129-141).  append = 0
+ * truncates (the reference always appends, Q21). */
+gpet_status gpet_write_events(const char *path, const gpet_event *ev, int64_t n, int32_t append);
+gpet_status gpet_write_hits(const char *id_path, const char *f_path, const int32_t *hit_id,
+                            const float *hit_f, int64_t n, int32_t append);
+
+const char *gpet_status_string(gpet_status s);
+
+/* ---- context ------------------------------------------------------------ */
+
+gpet_status gpet_create(gpet_ctx **ctx, const gpet_options *opt);
+gpet_status gpet_destroy(gpet_ctx *ctx);
+const char *gpet_last_error(gpet_ctx *ctx);
+
+/* init(): tables (initialize.cu:886-917).  gpet_set_tables copies. */
+gpet_status gpet_set_tables(gpet_ctx *ctx, const gpet_tables *t);
+gpet_status gpet_load_tables(gpet_ctx *ctx, const char *data_dir);
+
+/* loadPhantom + initPhantom + iniwck(phantom) (initialize.cu:32-74,773-884).
+ * mat/dens: nx*ny*nz, x fastest. */
+gpet_status gpet_set_phantom(gpet_ctx *ctx, const int32_t *mat, const float *dens,
+                             const int32_t dims[3], const float offset[3], const float size[3]);
+
+/* iniPanel + iniwck(panel materials) (initialize.cu:919-1224) and the
+ * quadric exclusion surfaces (main.cu:149-161; 10 coefficients each). */
+gpet_status gpet_set_detector(gpet_ctx *ctx, const gpet_panel *panels, int32_t n_panels,
+                              const gpet_panel_material mats[2],
+                              const float *quadrics, int32_t n_quadrics);
+
+gpet_status gpet_set_digitizer(gpet_ctx *ctx, const gpet_digitizer *d);
+
+/* sampleParticle (gPET.cu:201-437).  Decays are sampled directly: per source
+ * a Binomial(natom*2^(-t0/T), 1-2^(-(t1-t0)/T)) count, each decay time a
+ * truncated exponential in [t0,t1).  n_decays > 0 instead fixes the total
+ * number of decays, split over the sources in proportion to their expected
+ * counts.  History h is decay h; its photons are 2h and 2h+1. */
+gpet_status gpet_run_source(gpet_ctx *ctx, const gpet_source *src, int32_t n_src,
+                            const gpet_isotope *iso, int32_t n_iso,
+                            double t0, double t1, int64_t n_decays, gpet_sink *sink);
+
+/* simulateParticle (gPET.cu:13-199): rec8 holds n records of 8 doubles
+ * (x y z t vx vy vz E).  ptype 0: positron records, each becomes a pair;
+ * ptype 1: photon records. */
+gpet_status gpet_run_psf(gpet_ctx *ctx, const double *rec8, int64_t n, int32_t ptype,
+                         gpet_sink *sink);
+
+/* Tallies of the last run. */
+gpet_status gpet_get_tallies(gpet_ctx *ctx, gpet_tallies *t);
+
+/* Woodcock majorant nodes (GPET_NWCK floats) built by gpet_set_phantom
+ * (which = 0) or gpet_set_detector (which = 1); iniwck, initialize.cu:773-829. */
+gpet_status gpet_get_majorant(gpet_ctx *ctx, int32_t which, float *out, float *e0, float *idl);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPET_H */

@@ -1,0 +1,1354 @@
+/*
+ * gpet_oracle.cpp - CPU oracle for the photon transport and detection path.
+ *
+ * TEST INFRASTRUCTURE.  This is a scalar, host-compiled restatement of the
+ * reference's algorithm (utaresearch/gPET) used only to check the HIP path:
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ * The product library never links or calls it.
+ *
+ * It exports the C ABI of include/gpet.h with the prefix gpet_cpu_ so tests
+ * can run the same calls against either implementation.  Random numbers and
+ * elementary functions come from the shared headers gp_philox.h / gp_math.h
+ * (built with -ffp-contract=off), so per-history results are comparable bit
+ * for bit with the GPU path.
+ *
+ * Every physics function cites the reference function it restates.  Where
+ * the restatement departs from the reference on purpose (a quirk fixed) the
+ * comment says so; the quirk list is Appendix B of the survey (Q1..Q22).
+ *
+ * Parity pins (what checks this file besides the GPU path):
+ *   philox / math     - KATs and error bounds (tests/test_rng_math.py)
+ *   table lerp        - node values of input4gPET.* (tests/test_tables.py)
+ *   majorant          - invariant mu*rho_max*lambda <= 1 on a dense grid
+ *   samplers          - chi2/KS against closed forms (tests/test_oracle_physics.py)
+ *   geometry          - brute-force crystal index, closed-form panel entry
+ *   transport         - narrow-beam exp(-mu L) in water and LSO
+ *   digitizer         - brute-force O(n^2) restatements on small lists
+ */
+#include "../include/gpet.h"
+#include "../gpet_amd/csrc/common/gp_math.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr double C_LIGHT = 29979.2458;   /* cm/us, gPET_kernals.cu:264        */
+constexpr float MC2F = 510999.1f;        /* constants.h:34                    */
+constexpr float IMC2F = 1.95695061e-06f; /* constants.h:35                    */
+constexpr float SZERO = 1.0e-4f;         /* constants.h:37                    */
+constexpr float ZEROF = 1.0e-20f;        /* constants.h:36                    */
+constexpr double MAXT = 1e20;            /* constants.h:18                    */
+constexpr float RHO_OUTSIDE = 0.0012905f;/* gPET_kernals.cu:403               */
+constexpr int MAX_PH_EVENTS = 4;         /* reference events[4]; bounded, drops counted (Q5) */
+constexpr int STEP_CAP = 1000000;
+
+struct Wck {
+    float e0 = 0, idl = 0;
+    std::vector<float> lam;
+};
+
+struct Tables {
+    int nmat = 0;
+    float eminph = 0, emax = 0;
+    int n1d = 0;
+    float e0[4] = {0}, idl[4] = {0};
+    std::vector<float> mu[4];
+    int ncp = 0, ne = 0;
+    float idcp[2] = {0}, ide[2] = {0};
+    std::vector<float> surf[2];
+};
+
+struct Phantom {
+    int nx = 0, ny = 0, nz = 0;
+    float offx = 0, offy = 0, offz = 0;
+    float dx = 0, dy = 0, dz = 0, idx = 0, idy = 0, idz = 0;
+    std::vector<int32_t> mat;
+    std::vector<float> dens;
+    Wck wck;
+};
+
+struct Detector {
+    std::vector<gpet_panel> p;
+    gpet_panel_material mats[2];
+    std::vector<float> quad;
+    int nquad = 0;
+    int moduleN = 0, crystalN = 0, moduleNy = 0, crystalNy = 0;
+    Wck wck;
+};
+
+struct Counters {
+    int64_t photons = 0, panel_entries = 0, hits = 0, dropped_events = 0, majorant = 0;
+    int64_t phantom_steps = 0, range_steps = 0, panel_steps = 0;
+    void add(const Counters &o)
+    {
+        photons += o.photons; panel_entries += o.panel_entries; hits += o.hits;
+        dropped_events += o.dropped_events; majorant += o.majorant;
+        phantom_steps += o.phantom_steps; range_steps += o.range_steps; panel_steps += o.panel_steps;
+    }
+};
+
+struct HitRec {
+    int32_t id[5];
+    float f[5];
+};
+
+struct Result {
+    std::vector<gpet_event> adder, singles, coinc;
+    std::vector<HitRec> hits;
+    std::vector<gpet_trace> trace;
+    int64_t n_histories = 0, n_threshold = 0, n_deadtime = 0, n_coinc_true = 0;
+    Counters c;
+    double sec_transport = 0, sec_digi = 0;
+};
+
+} // namespace
+
+struct gpet_ctx {
+    gpet_options opt;
+    Tables tab;
+    bool has_tab = false;
+    Phantom ph;
+    bool has_ph = false;
+    Detector det;
+    bool has_det = false;
+    gpet_digitizer dig;
+    bool has_dig = false;
+    Result last;
+    std::string err;
+    int threads = 0;
+};
+
+namespace {
+
+gpet_status fail(gpet_ctx *c, gpet_status s, const char *msg)
+{
+    if (c) c->err = msg;
+    return s;
+}
+
+/* ---- table interpolation ------------------------------------------------
+ * The reference binds the tables to linear-filtered textures and reads them at
+ * i = idl*(E-e0)+0.5 (gPET_kernals.cu:31-64); here the same lerp is explicit
+ * in FP32 (texture filtering's 9-bit weights are not reproduced). */
+inline float lerp1(const float *T, int n, float e0, float idl, float E)
+{
+    float x = (E - e0) * idl;
+    x = std::fmin(std::fmax(x, 0.0f), (float)(n - 1));
+    int j = (int)x;
+    if (j > n - 2) j = n - 2;
+    float f = x - (float)j;
+    return T[j] + f * (T[j + 1] - T[j]);
+}
+
+inline float mu_tab(const Tables &t, int k, int mat, float E)
+{
+    return lerp1(t.mu[k].data() + (size_t)mat * t.n1d, t.n1d, t.e0[k], t.idl[k], E);
+}
+
+inline float lam_wck(const Wck &w, float E)
+{
+    return lerp1(w.lam.data(), GPET_NWCK, w.e0, w.idl, E);
+}
+
+/* Bilinear read of an inverse-CDF surface: tex3D(s_tex, inde+0.5, indcp+0.5,
+ * mat+0.5) of gPET_kernals.cu:66-88,128-146 with clamp addressing. */
+inline float surf_read(const Tables &t, int k, int mat, float E, float u)
+{
+    const int ne = t.ne, ncp = t.ncp;
+    float x = E * t.ide[k];
+    x = std::fmin(std::fmax(x, 0.0f), (float)(ne - 1));
+    float y = u * t.idcp[k];
+    y = std::fmin(std::fmax(y, 0.0f), (float)(ncp - 1));
+    int ix = (int)x;
+    if (ix > ne - 2) ix = ne - 2;
+    int iy = (int)y;
+    if (iy > ncp - 2) iy = ncp - 2;
+    float fx = x - (float)ix, fy = y - (float)iy;
+    const float *S = t.surf[k].data() + (size_t)mat * ncp * ne;
+    float v00 = S[iy * ne + ix], v01 = S[iy * ne + ix + 1];
+    float v10 = S[(iy + 1) * ne + ix], v11 = S[(iy + 1) * ne + ix + 1];
+    float a = v00 + fx * (v01 - v00);
+    float b = v10 + fx * (v11 - v10);
+    float v = a + fy * (b - a);
+    return std::fmin(std::fmax(v, -1.0f), 1.0f);
+}
+
+/* ---- Woodcock majorant: iniwck (initialize.cu:773-829, 919-966) ---------
+ * Same node grid as the reference (NWCK nodes from eminph with spacing
+ * (emax-eminph)/NWCK).  Q18 fix: the reference fits a cubic spline and reads
+ * the majorant with linear filtering, so lambda*mu can exceed 1.  Here the
+ * node value is 1/max of the linearly interpolated mu*rho_max over the two
+ * adjacent intervals (exact maximum of a piecewise-linear function: interval
+ * ends and the data nodes inside), shrunk by 1e-5 against rounding. */
+double mu_lin_max(const Tables &t, const float *rho_max, double E)
+{
+    double best = 0.0;
+    for (int m = 0; m < t.nmat; ++m) {
+        if (rho_max[m] <= 0.0f) continue;
+        double v = (double)mu_tab(t, GPET_TAB_LAMPH, m, (float)E) * rho_max[m];
+        if (v > best) best = v;
+    }
+    return best;
+}
+
+void build_wck(const Tables &t, const float *rho_max, Wck &w)
+{
+    w.e0 = t.eminph;
+    float de = (t.emax * (1.0f - 1.0e-10f) - w.e0) / (float)GPET_NWCK;
+    w.idl = 1.0f / de;
+    w.lam.assign(GPET_NWCK, 0.0f);
+    const double d_node = 1.0 / (double)t.idl[GPET_TAB_LAMPH];
+    std::vector<double> imax(GPET_NWCK + 1, 0.0); /* max over [E_i, E_i+1] */
+    for (int i = 0; i <= GPET_NWCK; ++i) {
+        double lo = (double)w.e0 + (double)de * i;
+        double hi = (double)w.e0 + (double)de * (i + 1);
+        double m = std::max(mu_lin_max(t, rho_max, lo), mu_lin_max(t, rho_max, hi));
+        double k0 = std::ceil((lo - t.e0[GPET_TAB_LAMPH]) / d_node);
+        double k1 = std::floor((hi - t.e0[GPET_TAB_LAMPH]) / d_node);
+        for (double k = k0; k <= k1; k += 1.0)
+            m = std::max(m, mu_lin_max(t, rho_max, (double)t.e0[GPET_TAB_LAMPH] + k * d_node));
+        imax[i] = m;
+    }
+    for (int i = 0; i < GPET_NWCK; ++i) {
+        double m = imax[i];
+        if (i > 0) m = std::max(m, imax[i - 1]);
+        w.lam[i] = (m > 0.0) ? (float)(1.0 / (m * (1.0 + 1.0e-5))) : 1.0e20f;
+    }
+}
+
+/* ---- geometry helpers -------------------------------------------------- */
+
+/* getAbsVox (gPET_kernals.cu:19-29), Q1 fixed: voxel 0 is inside and
+ * truncation toward zero no longer maps (-1,0) into the grid. */
+inline bool vox_index(const Phantom &p, float x, float y, float z, int &ix, int &iy, int &iz)
+{
+    float fx = (x - p.offx) * p.idx;
+    float fy = (y - p.offy) * p.idy;
+    float fz = (z - p.offz) * p.idz;
+    if (!(fx >= 0.0f && fx < (float)p.nx && fy >= 0.0f && fy < (float)p.ny && fz >= 0.0f &&
+          fz < (float)p.nz))
+        return false;
+    ix = (int)fx; iy = (int)fy; iz = (int)fz;
+    return true;
+}
+
+/* rotate (gPET_kernals.cu:172-254).  The azimuth arrives as a uniform u with
+ * phi = 2 pi u.  rho2 is recomputed after a renormalisation (the reference
+ * keeps the stale value). */
+inline void rotate_dir(float &u, float &v, float &w, float costh, float uphi)
+{
+    float rho2 = u * u + v * v;
+    float norm = rho2 + w * w;
+    if (std::fabs(norm - 1.0f) > SZERO) {
+        norm = 1.0f / std::sqrt(norm);
+        u = u * norm; v = v * norm; w = w * norm;
+        rho2 = u * u + v * v;
+    }
+    float sinphi, cosphi;
+    gp_sincos2pi(uphi, &sinphi, &cosphi);
+    float temp = costh * costh;
+    if (rho2 > ZEROF) {
+        float sthrho = (temp < 1.0f) ? std::sqrt((1.0f - temp) / rho2) : 0.0f;
+        float urho = u * sthrho;
+        float vrho = v * sthrho;
+        float nu = u * costh - vrho * sinphi + (w * urho) * cosphi;
+        float nv = v * costh + urho * sinphi + (w * vrho) * cosphi;
+        float nw = w * costh - (rho2 * sthrho) * cosphi;
+        u = nu; v = nv; w = nw;
+    } else {
+        float sinth = (temp < 1.0f) ? std::sqrt(1.0f - temp) : 0.0f;
+        v = sinth * sinphi;
+        if (w > 0.0f) { u = sinth * cosphi; w = costh; }
+        else { u = -sinth * cosphi; w = -costh; }
+    }
+}
+
+/* Analytic Klein-Nishina sampling (gPET_kernals.cu:90-126), FP32. */
+inline void kn_sample(float E, gp_rng &r, float &efrac, float &costhe)
+{
+    float e0 = E * IMC2F;
+    float twoe = 2.0f * e0;
+    float kmin2 = 1.0f / ((1.0f + twoe) * (1.0f + twoe));
+    float loge = gp_logf(1.0f + twoe);
+    float a1 = loge + (twoe * (1.0f + e0)) * kmin2;
+    for (int it = 0; it < STEP_CAP; ++it) {
+        float u1 = gp_rng_uniform(&r);
+        float u2 = gp_rng_uniform(&r);
+        if (u1 * a1 < loge) efrac = gp_expf(-u2 * loge);
+        else efrac = std::sqrt(kmin2 + u2 * (1.0f - kmin2));
+        float mess = ((e0 * e0) * efrac) * (1.0f + efrac * efrac);
+        float u3 = gp_rng_uniform(&r);
+        if (u3 * mess <= mess - (1.0f - efrac) * ((1.0f + twoe) * efrac - 1.0f)) break;
+    }
+    costhe = 1.0f - (1.0f - efrac) / (efrac * e0);
+}
+
+/* crystalSearch (gPET_kernals.cu:1236-1279): m_id 0 crystal, 1 gap/excluded. */
+inline int crystal_search(const Detector &d, const gpet_panel &P, float x, float y, float z,
+                          int &M_id, int &L_id)
+{
+    for (int q = 0; q < d.nquad; ++q) {
+        const float *s = &d.quad[q * 10];
+        float v = s[0] * x * x + s[1] * y * y + s[2] * z * z + s[3] * x * y + s[4] * x * z +
+                  s[5] * y * z + s[6] * x + s[7] * y + s[8] * z + s[9];
+        if (v < 0.0f) return 1;
+    }
+    float yy = P.leny / 2.0f + y;
+    float zz = P.lenz / 2.0f + z;
+    float mpy = P.mody + P.mspy, mpz = P.modz + P.mspz;
+    int my = std::floor(yy / mpy) > 0.0f ? (int)(yy / mpy) : 0;
+    int mz = std::floor(zz / mpz) > 0.0f ? (int)(zz / mpz) : 0;
+    M_id = mz * d.moduleNy + my;
+    yy = yy - (float)my * mpy;
+    zz = zz - (float)mz * mpz;
+    if (yy > P.mody || zz > P.modz) return 1;
+    float cpy = P.cry + P.cspy, cpz = P.crz + P.cspz;
+    int ly = std::floor(yy / cpy) > 0.0f ? (int)(yy / cpy) : 0;
+    int lz = std::floor(zz / cpz) > 0.0f ? (int)(zz / cpz) : 0;
+    L_id = lz * d.crystalNy + ly;
+    yy = yy - (float)ly * cpy;
+    zz = zz - (float)lz * cpz;
+    if (yy > P.cry || zz > P.crz) return 1;
+    return 0;
+}
+
+/* adder (gPET_kernals.cu:737-755); capacity-checked (Q5). */
+inline void adder(gpet_event *ev, int &n, const gpet_event &e, Counters &c)
+{
+    for (int i = 0; i < n; ++i) {
+        if (ev[i].siten == e.siten) {
+            float sE = ev[i].E + e.E;
+            ev[i].x = (ev[i].x * ev[i].E + e.x * e.E) / sE;
+            ev[i].y = (ev[i].y * ev[i].E + e.y * e.E) / sE;
+            ev[i].z = (ev[i].z * ev[i].E + e.z * e.E) / sE;
+            ev[i].E = sE;
+            return;
+        }
+    }
+    if (n < MAX_PH_EVENTS) ev[n++] = e;
+    else c.dropped_events++;
+}
+
+/* readout (gPET_kernals.cu:756-813).  depth 3 keeps the crystal-level list
+ * (the reference returns without setting the output count). */
+inline int readout(gpet_event *ev, int n, int depth, int policy, int moduleN)
+{
+    if (depth == 3) return n;
+    if (policy == 1) depth = 2;
+    for (int i = 0; i < n; ++i) {
+        if (depth == 0) ev[i].siten = 0;
+        else if (depth == 1) ev[i].siten = ev[i].pann;
+        else ev[i].siten = ev[i].pann * moduleN + ev[i].modn;
+    }
+    int out = 0;
+    for (int i = 0; i < n; ++i) {
+        gpet_event e0 = ev[i];
+        if (e0.t > MAXT * 0.1) continue;
+        for (int j = i + 1; j < n; ++j) {
+            gpet_event e = ev[j];
+            if (e.parn == e0.parn && e.siten == e0.siten) {
+                if (policy == 1) {
+                    float sE = e0.E + e.E;
+                    e0.x = (e0.x * e0.E + e.x * e.E) / sE;
+                    e0.y = (e0.y * e0.E + e.y * e.E) / sE;
+                    e0.z = (e0.z * e0.E + e.z * e.E) / sE;
+                    e0.E = sE;
+                    ev[j].t = MAXT;
+                    continue;
+                }
+                e0 = (e0.E > e.E) ? e0 : e;
+                ev[j].t = MAXT;
+            }
+        }
+        ev[out++] = e0;
+    }
+    return out;
+}
+
+/* ---- per-photon transport ---------------------------------------------- */
+
+struct PhotonOut {
+    gpet_event ev[MAX_PH_EVENTS];
+    int nev = 0;
+    std::vector<HitRec> hits;
+};
+
+/* photon (gPET_kernals.cu:256-345): Woodcock tracking in the phantom.
+ * Returns false if the photon is absorbed.  Q10 fixed: a Compton event that
+ * leaves less than eabs absorbs the photon. */
+bool phantom_transport(const gpet_ctx &C, float &x, float &y, float &z, float &vx, float &vy,
+                       float &vz, float &E, double &tof, gp_rng &r, gpet_trace *tr, Counters &cnt)
+{
+    const Phantom &p = C.ph;
+    const Tables &t = C.tab;
+    for (int it = 0; it < STEP_CAP; ++it) {
+        float lammin = lam_wck(p.wck, E);
+        float s = -lammin * gp_logf(gp_rng_uniform(&r));
+        x = x + s * vx;
+        y = y + s * vy;
+        z = z + s * vz;
+        tof = tof + (double)s / C_LIGHT;
+        int ix, iy, iz;
+        if (!vox_index(p, x, y, z, ix, iy, iz)) return true; /* escaped (Q9: landing point) */
+        if (tr) tr->n_steps++;
+        cnt.phantom_steps++;
+        size_t vid = ((size_t)iz * p.ny + iy) * p.nx + ix;
+        float den = p.dens[vid];
+        int mat = p.mat[vid];
+        float lamden = lammin * den;
+        float prob = 1.0f - lamden * mu_tab(t, GPET_TAB_LAMPH, mat, E);
+        if (prob < 0.0f) { prob = 0.0f; cnt.majorant++; }
+        float rn = gp_rng_uniform(&r);
+        if (rn < prob) continue;
+        prob = prob + lamden * mu_tab(t, GPET_TAB_COMPT, mat, E);
+        if (rn < prob) {
+            float cth = surf_read(t, GPET_SURF_COMPTON, mat, E, gp_rng_uniform(&r));
+            float efrac = 1.0f / (1.0f + (E * IMC2F) * (1.0f - cth));
+            float uphi = gp_rng_uniform(&r);
+            if (tr) tr->n_compton++;
+            E = E * efrac;
+            if (E < C.opt.eabs) return false;
+            rotate_dir(vx, vy, vz, cth, uphi);
+            continue;
+        }
+        prob = prob + lamden * mu_tab(t, GPET_TAB_RAYLE, mat, E);
+        if (rn < prob) {
+            float cth = surf_read(t, GPET_SURF_RAYLEIGH, mat, E, gp_rng_uniform(&r));
+            float uphi = gp_rng_uniform(&r);
+            if (tr) tr->n_rayleigh++;
+            rotate_dir(vx, vy, vz, cth, uphi);
+            continue;
+        }
+        return false; /* photoelectric */
+    }
+    return false;
+}
+
+/* photonde (gPET_kernals.cu:839-1233) for one photon: panel entry, Woodcock
+ * in the panel, hit records, adder, readout. */
+void panel_transport(const gpet_ctx &C, float x, float y, float z, float vx, float vy, float vz,
+                     float E, double tof, int32_t parn, int32_t eid, gp_rng &r, gpet_trace *tr,
+                     PhotonOut &out, Counters &cnt, bool want_hits)
+{
+    const Detector &d = C.det;
+    const Tables &t = C.tab;
+    int pa = -1;
+    float x2 = 0, y2 = 0, z2 = 0, vx2 = 0, vy2 = 0, vz2 = 0;
+    const int np = (int)d.p.size();
+    for (int i = 0; i < np; ++i) {
+        const gpet_panel &P = d.p[i];
+        float rx = x - P.ox, ry = y - P.oy, rz = z - P.oz;
+        float tx = rx * P.ux[0] + ry * P.ux[1] + rz * P.ux[2];
+        float ty = rx * P.uy[0] + ry * P.uy[1] + rz * P.uy[2];
+        float tz = rx * P.uz[0] + ry * P.uz[1] + rz * P.uz[2];
+        float tvx = vx * P.ux[0] + vy * P.ux[1] + vz * P.ux[2];
+        float tvy = vx * P.uy[0] + vy * P.uy[1] + vz * P.uy[2];
+        float tvz = vx * P.uz[0] + vy * P.uz[1] + vz * P.uz[2];
+        if (tvx * P.dirx >= 0.0f) {
+            float ty2 = ty - (tx / tvx) * tvy;
+            float tz2 = tz - (tx / tvx) * tvz;
+            if (std::fabs(ty2) < P.leny / 2.0f && std::fabs(tz2) < P.lenz / 2.0f) {
+                x2 = 0.0f; y2 = ty2; z2 = tz2;
+                tof = tof + (double)(-tx) / (C_LIGHT * (double)tvx);
+                vx2 = tvx; vy2 = tvy; vz2 = tvz;
+                pa = i;
+                break;
+            }
+        }
+    }
+    if (tr) tr->panel = pa;
+    if (pa < 0) return;
+    cnt.panel_entries++;
+    const gpet_panel &P = d.p[pa];
+    const int pann = P.panel;
+    const float hx = P.lenx, hy = P.leny * 0.5f, hz = P.lenz * 0.5f;
+    int nev = 0;
+    gpet_event *ev = out.ev;
+    auto record = [&](int type, float edep, int M_id, int L_id, bool to_adder) {
+        cnt.hits++;
+        if (want_hits) {
+            HitRec h;
+            h.id[0] = parn; h.id[1] = pann; h.id[2] = M_id; h.id[3] = L_id; h.id[4] = type;
+            h.f[0] = edep; h.f[1] = (float)tof; h.f[2] = x2; h.f[3] = y2; h.f[4] = z2;
+            out.hits.push_back(h);
+        }
+        if (!to_adder) return;
+        gpet_event e;
+        e.parn = parn; e.pann = pann; e.modn = M_id; e.cryn = L_id;
+        e.siten = pann * d.moduleN * d.crystalN + M_id * d.crystalN + L_id;
+        e.eventid = eid; e.t = tof; e.E = edep; e.x = x2; e.y = y2; e.z = z2;
+        adder(ev, nev, e, cnt);
+    };
+    for (int it = 0; it < STEP_CAP; ++it) {
+        float lammin = lam_wck(d.wck, E);
+        float s = -lammin * gp_logf(gp_rng_uniform(&r));
+        x2 = x2 + s * vx2;
+        y2 = y2 + s * vy2;
+        z2 = z2 + s * vz2;
+        tof = tof + (double)s / C_LIGHT;
+        if (std::fabs(y2) > hy || std::fabs(z2) > hz || x2 * P.dirx < 0.0f || x2 * P.dirx > hx) break;
+        if (tr) tr->n_steps_panel++;
+        cnt.panel_steps++;
+        int M_id = -1, L_id = -1;
+        int m_id = crystal_search(d, P, x2, y2, z2, M_id, L_id);
+        float den = d.mats[m_id].density;
+        int mat = d.mats[m_id].material;
+        float lamden = lammin * den;
+        float prob = 1.0f - lamden * mu_tab(t, GPET_TAB_LAMPH, mat, E);
+        if (prob < 0.0f) { prob = 0.0f; cnt.majorant++; }
+        float rn = gp_rng_uniform(&r);
+        if (rn < prob) continue;
+        if (tr) tr->n_int_panel++;
+        prob = prob + lamden * mu_tab(t, GPET_TAB_COMPT, mat, E);
+        if (rn < prob) {
+            float efrac, cth;
+            kn_sample(E, r, efrac, cth);
+            float de = E * (1.0f - efrac);
+            float uphi = gp_rng_uniform(&r);
+            if (m_id == 0) record(1, de, M_id, L_id, true);
+            E = E - de;
+            if (E < C.opt.eabs) {
+                if (m_id == 0) record(2, E, M_id, L_id, true);
+                break;
+            }
+            rotate_dir(vx2, vy2, vz2, cth, uphi);
+            continue;
+        }
+        prob = prob + lamden * mu_tab(t, GPET_TAB_RAYLE, mat, E);
+        if (rn < prob) {
+            float cth = surf_read(t, GPET_SURF_RAYLEIGH, mat, E, gp_rng_uniform(&r));
+            float uphi = gp_rng_uniform(&r);
+            if (C.dig.record_rayleigh && m_id == 0) record(3, 0.0f, M_id, L_id, false);
+            rotate_dir(vx2, vy2, vz2, cth, uphi);
+            continue;
+        }
+        if (m_id == 0) record(4, E, M_id, L_id, true);
+        break;
+    }
+    if (nev > 0) nev = readout(ev, nev, C.dig.readout_depth, C.dig.readout_policy, d.moduleN);
+    out.nev = nev;
+    if (tr) tr->n_events = nev;
+}
+
+/* Phantom + panel for one photon, filling the trace record. */
+void photon_full(const gpet_ctx &C, int64_t hist, int k, float x, float y, float z, float vx,
+                 float vy, float vz, float E, double t, int32_t parn, gpet_trace *tr,
+                 PhotonOut &out, Counters &cnt, bool want_hits)
+{
+    gp_rng r;
+    gp_rng_init(&r, C.opt.seed, (uint64_t)hist, (uint32_t)(GP_TAG_PHOTON + k), 0u);
+    cnt.photons++;
+    if (tr) {
+        tr->status |= GPET_TR_ALIVE;
+        tr->x0 = x; tr->y0 = y; tr->z0 = z; tr->E0 = E; tr->t0 = t;
+        tr->panel = -1;
+    }
+    bool alive = true;
+    if (C.has_ph) alive = phantom_transport(C, x, y, z, vx, vy, vz, E, t, r, tr, cnt);
+    if (tr) {
+        tr->status |= alive ? GPET_TR_ESCAPED : GPET_TR_ABSORBED;
+        tr->x = x; tr->y = y; tr->z = z; tr->vx = vx; tr->vy = vy; tr->vz = vz; tr->E = E; tr->t = t;
+    }
+    if (alive && C.has_det) {
+        panel_transport(C, x, y, z, vx, vy, vz, E, t, parn, (int32_t)hist, r, tr, out, cnt, want_hits);
+        if (tr && tr->panel >= 0) tr->status |= GPET_TR_PANEL;
+    }
+    if (tr) tr->draws = r.n;
+}
+
+/* ---- source ------------------------------------------------------------ */
+
+/* sampleEkPositron (gPET_kernals.cu:420-443): rejection from the fitted
+ * spectrum; Horner form of the degree-5 polynomial; returns kinetic eV. */
+float sample_ek(const gpet_isotope &iso, gp_rng &r)
+{
+    float E = 0.511f;
+    for (int it = 0; it < 100000; ++it) {
+        E = gp_rng_uniform(&r) * (iso.coef[0] - 0.511f) + 0.511f;
+        float u = iso.coef[1] * gp_rng_uniform(&r);
+        float p = iso.coef[2];
+        for (int i = 3; i < 8; ++i) p = p * E + iso.coef[i];
+        if (u <= p) break;
+    }
+    return (E - 0.511f) * 1.0e6f;
+}
+
+/* getPositionFromShape (gPET_kernals.cu:445-482). */
+void sample_shape(const gpet_source &s, gp_rng &r, float &x, float &y, float &z)
+{
+    const float *c = s.coef;
+    int shape = (s.shape < 0 || s.shape > 2) ? 0 : s.shape;
+    if (shape == 0) {
+        x = c[0] + (c[3] * (2.0f * gp_rng_uniform(&r) - 1.0f)) * 0.5f;
+        y = c[1] + (c[4] * (2.0f * gp_rng_uniform(&r) - 1.0f)) * 0.5f;
+        z = c[2] + (c[5] * (2.0f * gp_rng_uniform(&r) - 1.0f)) * 0.5f;
+    } else if (shape == 1) {
+        float sp, cp;
+        gp_sincos2pi(gp_rng_uniform(&r), &sp, &cp);
+        float rr = c[3] * std::sqrt(gp_rng_uniform(&r));
+        x = c[0] + rr * cp;
+        y = c[1] + rr * sp;
+        z = c[2] + (c[4] * (2.0f * gp_rng_uniform(&r) - 1.0f)) * 0.5f;
+    } else {
+        float sp, cp;
+        gp_sincos2pi(gp_rng_uniform(&r), &sp, &cp);
+        float ct = 2.0f * gp_rng_uniform(&r) - 1.0f;
+        float rr = c[3] * gp_cbrtf(gp_rng_uniform(&r));
+        float st = std::sqrt(1.0f - ct * ct);
+        x = c[0] + (rr * st) * cp;
+        y = c[1] + (rr * st) * sp;
+        z = c[2] + rr * ct;
+    }
+}
+
+/* setPositronRange (gPET_kernals.cu:347-418).  Fixes: outside the phantom the
+ * remaining range is covered at RHO_OUTSIDE (the reference adds it to a
+ * leftover 1000-unit step); a crossing into a voxel outside the grid uses
+ * RHO_OUTSIDE (the reference reads the clamped border voxel).  Kept (Q8): the
+ * segment just crossed is charged at the density of the voxel entered. */
+void positron_range(const gpet_ctx &C, float &x, float &y, float &z, float ek_ev, int usedir,
+                    float vx, float vy, float vz, gp_rng &r, Counters &cnt)
+{
+    const Phantom &p = C.ph;
+    float ekin = ek_ev / 1.0e6f;
+    float rex = ((0.1f * 5.44040782f) * ekin * ekin) / (0.369516529f + ekin);
+    float sigma = rex / 2.0f;
+    float dx = sigma * gp_rng_normal(&r);
+    float dy = sigma * gp_rng_normal(&r);
+    float dz = sigma * gp_rng_normal(&r);
+    float rr = std::sqrt(dx * dx + dy * dy + dz * dz);
+    if (usedir) {
+        float tmp = std::sqrt(vx * vx + vy * vy + vz * vz);
+        dx = rr * vx / tmp;
+        dy = rr * vy / tmp;
+        dz = rr * vz / tmp;
+    }
+    if (!(rr > 0.0f)) return;
+    int ix = 0, iy = 0, iz = 0;
+    bool inside = C.has_ph && vox_index(p, x, y, z, ix, iy, iz);
+    float s = 0.0f;
+    for (int it = 0; it < STEP_CAP && s < rr; ++it) {
+        float step;
+        if (inside) {
+            step = 1000.0f;
+            int axis = 0;
+            if (dx != 0.0f) {
+                float b = (p.offx + (float)(ix + (dx > 0.0f)) * p.dx - x) / dx;
+                if (step > b) { step = b; axis = 1; }
+            }
+            if (dy != 0.0f) {
+                float b = (p.offy + (float)(iy + (dy > 0.0f)) * p.dy - y) / dy;
+                if (step > b) { step = b; axis = 2; }
+            }
+            if (dz != 0.0f) {
+                float b = (p.offz + (float)(iz + (dz > 0.0f)) * p.dz - z) / dz;
+                if (step > b) { step = b; axis = 3; }
+            }
+            if (axis == 1) ix += (dx > 0.0f) ? 1 : -1;
+            else if (axis == 2) iy += (dy > 0.0f) ? 1 : -1;
+            else iz += (dz > 0.0f) ? 1 : -1;
+            float rho;
+            if (ix >= 0 && ix < p.nx && iy >= 0 && iy < p.ny && iz >= 0 && iz < p.nz) {
+                rho = p.dens[((size_t)iz * p.ny + iy) * p.nx + ix];
+                cnt.range_steps++;
+            } else {
+                rho = RHO_OUTSIDE;
+            }
+            step = step * rr;
+            s = s + step * rho;
+            if (s > rr) step = step + (rr - s) / rho;
+        } else {
+            step = (rr - s) / RHO_OUTSIDE;
+            s = rr + 100.0f;
+        }
+        x = x + (step * dx) / rr;
+        y = y + (step * dy) / rr;
+        z = z + (step * dz) / rr;
+        if (inside) {
+            float ex = p.offx + (float)p.nx * p.dx, ey = p.offy + (float)p.ny * p.dy,
+                  ez = p.offz + (float)p.nz * p.dz;
+            if (x < p.offx || x > ex || y < p.offy || y > ey || z < p.offz || z > ez) inside = false;
+        }
+    }
+}
+
+/* Pair emission (gPET_kernals.cu:536-555): isotropic direction, Gaussian
+ * noncollinearity delta; photon 1 along d with E = mc2 + delta mc2/2,
+ * photon 2 rotated by (cos = -cos delta, phi) with E = mc2 - delta mc2/2. */
+void emit_pair(float sigma, gp_rng &r, float v[2][3], float e[2])
+{
+    float ct = 2.0f * gp_rng_uniform(&r) - 1.0f;
+    float sp, cp;
+    gp_sincos2pi(gp_rng_uniform(&r), &sp, &cp);
+    float st = std::sqrt(1.0f - ct * ct);
+    v[0][0] = st * cp; v[0][1] = st * sp; v[0][2] = ct;
+    float uphi = gp_rng_uniform(&r);
+    float delta = gp_rng_normal(&r) * sigma;
+    e[0] = MC2F + (delta * MC2F) / 2.0f;
+    e[1] = MC2F - (delta * MC2F) / 2.0f;
+    v[1][0] = v[0][0]; v[1][1] = v[0][1]; v[1][2] = v[0][2];
+    rotate_dir(v[1][0], v[1][1], v[1][2], -gp_cosf(delta), uphi);
+}
+
+/* Per-source decay plan shared by all histories of a run. */
+struct SourcePlan {
+    std::vector<int64_t> first;   /* first history of each source (prefix) */
+    std::vector<double> tau, pdec; /* T1/2/ln2 (s) and 1-2^(-(t1-t0)/T1/2) */
+    int64_t total = 0;
+};
+
+/* Binomial(n, p) from one uniform pair: inversion for small means, a
+ * rounded normal approximation otherwise.  Host-only, FP64 libm. */
+int64_t binomial_draw(double n, double p, uint64_t seed, int src)
+{
+    if (n <= 0.0 || p <= 0.0) return 0;
+    if (p >= 1.0) return (int64_t)std::floor(n);
+    gp_rng r;
+    gp_rng_init(&r, seed, (uint64_t)src, 15u, 0u);
+    double mean = n * p;
+    double u = gp_rng_uniform_d(&r);
+    if (mean < 200.0) {
+        double pk = std::exp(n * std::log1p(-p));
+        double cdf = pk;
+        int64_t k = 0;
+        while (u > cdf && (double)k < n) {
+            pk = pk * (n - (double)k) / (double)(k + 1) * p / (1.0 - p);
+            k++;
+            cdf += pk;
+            if (pk < 1e-300 && (double)k > mean) break;
+        }
+        return k;
+    }
+    double u2 = gp_rng_uniform_d(&r);
+    double z = std::sqrt(-2.0 * std::log(u)) * std::cos(6.283185307179586 * u2);
+    double k = std::floor(mean + std::sqrt(mean * (1.0 - p)) * z + 0.5);
+    if (k < 0.0) k = 0.0;
+    if (k > n) k = std::floor(n);
+    return (int64_t)k;
+}
+
+gpet_status plan_sources(const gpet_ctx &C, const gpet_source *src, int ns, const gpet_isotope *iso,
+                         int niso, double t0, double t1, int64_t n_decays, SourcePlan &pl)
+{
+    pl.first.assign(ns + 1, 0);
+    pl.tau.assign(ns, 0.0);
+    pl.pdec.assign(ns, 0.0);
+    std::vector<double> expect(ns, 0.0), navail(ns, 0.0);
+    for (int i = 0; i < ns; ++i) {
+        if (src[i].isotope < 0 || src[i].isotope >= niso) return GPET_ERR_ARG;
+        double th = iso[src[i].isotope].half_life_s;
+        pl.tau[i] = th / std::log(2.0);
+        pl.pdec[i] = -std::expm1(-(t1 - t0) / pl.tau[i]);
+        navail[i] = std::floor(src[i].natom * std::exp2(-t0 / th));
+        expect[i] = navail[i] * pl.pdec[i];
+    }
+    std::vector<int64_t> cnt(ns, 0);
+    if (n_decays > 0) {
+        double tot = 0;
+        for (double e : expect) tot += e;
+        if (tot <= 0) return GPET_ERR_ARG;
+        int64_t assigned = 0;
+        std::vector<std::pair<double, int>> rem;
+        for (int i = 0; i < ns; ++i) {
+            double share = (double)n_decays * expect[i] / tot;
+            cnt[i] = (int64_t)std::floor(share);
+            assigned += cnt[i];
+            rem.push_back({share - (double)cnt[i], i});
+        }
+        std::stable_sort(rem.begin(), rem.end(), [](const std::pair<double, int> &a, const std::pair<double, int> &b) {
+            return a.first > b.first;
+        });
+        for (int k = 0; assigned < n_decays; ++k, ++assigned) cnt[rem[k % ns].second]++;
+    } else {
+        for (int i = 0; i < ns; ++i) cnt[i] = binomial_draw(navail[i], pl.pdec[i], C.opt.seed, i);
+    }
+    for (int i = 0; i < ns; ++i) pl.first[i + 1] = pl.first[i] + cnt[i];
+    pl.total = pl.first[ns];
+    return GPET_OK;
+}
+
+/* One decay history (setPosition, gPET_kernals.cu:483-561, sampled per
+ * decay instead of per atom, F8/Q17). */
+void decay_history(const gpet_ctx &C, const SourcePlan &pl, const gpet_source *src,
+                   const gpet_isotope *iso, double t0, int64_t h, gpet_trace *tr2,
+                   PhotonOut out[2], Counters &cnt, bool want_hits)
+{
+    int si = (int)(std::upper_bound(pl.first.begin(), pl.first.end(), h) - pl.first.begin()) - 1;
+    const gpet_source &S = src[si];
+    const gpet_isotope &I = iso[S.isotope];
+    gp_rng r;
+    gp_rng_init(&r, C.opt.seed, (uint64_t)h, GP_TAG_DECAY, 0u);
+    double u = gp_rng_uniform_d(&r);
+    double ts = t0 + pl.tau[si] * (-gp_log1p_d(-u * pl.pdec[si]));
+    double tus = ts * 1.0e6;
+    float rb = gp_rng_uniform(&r);
+    if (tr2) {
+        for (int k = 0; k < 2; ++k) {
+            tr2[k].history = h; tr2[k].photon = k; tr2[k].panel = -1; tr2[k].t0 = tus;
+        }
+    }
+    if (!(rb < I.beta_ratio)) return;
+    float x, y, z;
+    sample_shape(S, r, x, y, z);
+    if (C.opt.positron_range) {
+        float ek = sample_ek(I, r);
+        positron_range(C, x, y, z, ek, 0, 0.0f, 0.0f, 0.0f, r, cnt);
+    }
+    float v[2][3], e[2];
+    emit_pair(C.opt.noncol_sigma, r, v, e);
+    for (int k = 0; k < 2; ++k)
+        photon_full(C, h, k, x, y, z, v[k][0], v[k][1], v[k][2], e[k], tus, (int32_t)(2 * h + k),
+                    tr2 ? &tr2[k] : nullptr, out[k], cnt, want_hits);
+}
+
+/* One PSF history (simulateParticle / setPositionForPhoton,
+ * gPET_kernals.cu:563-604).  Q3/Q11: the pair's photons both carry the
+ * record time; records with t <= 0 are still transported. */
+void psf_history(const gpet_ctx &C, const double *rec, int ptype, int64_t h, gpet_trace *tr2,
+                 PhotonOut out[2], Counters &cnt, bool want_hits)
+{
+    float x = (float)rec[0], y = (float)rec[1], z = (float)rec[2];
+    double t = rec[3];
+    float vx = (float)rec[4], vy = (float)rec[5], vz = (float)rec[6], E = (float)rec[7];
+    if (tr2) {
+        for (int k = 0; k < 2; ++k) {
+            tr2[k].history = h; tr2[k].photon = k; tr2[k].panel = -1; tr2[k].t0 = t;
+        }
+    }
+    if (ptype == 1) {
+        photon_full(C, h, 0, x, y, z, vx, vy, vz, E, t, (int32_t)h, tr2 ? &tr2[0] : nullptr, out[0],
+                    cnt, want_hits);
+        return;
+    }
+    gp_rng r;
+    gp_rng_init(&r, C.opt.seed, (uint64_t)h, GP_TAG_DECAY, 0u);
+    if (C.opt.positron_range) positron_range(C, x, y, z, E, 1, vx, vy, vz, r, cnt);
+    float v[2][3], e[2];
+    emit_pair(C.opt.noncol_sigma, r, v, e);
+    for (int k = 0; k < 2; ++k)
+        photon_full(C, h, k, x, y, z, v[k][0], v[k][1], v[k][2], e[k], t, (int32_t)(2 * h + k),
+                    tr2 ? &tr2[k] : nullptr, out[k], cnt, want_hits);
+}
+
+/* ---- digitizer (gPET.cu:380-424, gPET_kernals.cu:607-698,814-837) ------ */
+
+inline bool ev_less_t(const gpet_event &a, const gpet_event &b)
+{
+    if (a.t != b.t) return a.t < b.t;
+    if (a.parn != b.parn) return a.parn < b.parn;
+    return a.siten < b.siten;
+}
+
+inline int site_at(const gpet_event &e, int level, const Detector &d)
+{
+    if (level == 0) return 0;
+    if (level == 1) return e.pann;
+    if (level == 2) return e.pann * d.moduleN + e.modn;
+    return e.pann * d.moduleN * d.crystalN + e.modn * d.crystalN + e.cryn;
+}
+
+/* blur (gPET_kernals.cu:814-837) plus the time blur the reference lacks.
+ * Stream: id = parn, aux = GP_TAG_DIGI | siten << 4 (unique per event). */
+void blur_event(const gpet_ctx &C, gpet_event &e)
+{
+    const gpet_digitizer &g = C.dig;
+    gp_rng r;
+    gp_rng_init(&r, C.opt.seed, (uint64_t)(uint32_t)e.parn, (uint32_t)GP_TAG_DIGI | ((uint32_t)e.siten << 4), 0u);
+    float R = 0.0f;
+    if (g.blur_policy == 0) R = std::sqrt(g.e_ref / e.E) * g.r_ref;
+    if (g.blur_policy == 1) R = g.r_ref + g.slope * (e.E - g.e_ref) / 1.0e6f;
+    if (R < 0.0f) R = 0.0f;
+    float n = gp_rng_normal(&r);
+    e.E = e.E + ((n * R) * e.E) / 2.35482f;
+    if (g.space_blur > 0.0f) {
+        e.x = e.x + g.space_blur * gp_rng_normal(&r);
+        e.y = e.y + g.space_blur * gp_rng_normal(&r);
+        e.z = e.z + g.space_blur * gp_rng_normal(&r);
+    }
+    if (g.time_blur_fwhm_us > 0.0) {
+        float nt = gp_rng_normal(&r);
+        e.t = e.t + (double)nt * (g.time_blur_fwhm_us / 2.35482);
+    }
+}
+
+/* deadtime (gPET_kernals.cu:657-698) on (site,t)-sorted events.  Chain
+ * starts are taken from the unmodified times (the reference reads times
+ * other threads may already have overwritten).  Returns alive flags. */
+void deadtime(std::vector<gpet_event> &ev, const std::vector<int> &site, double tau, int type,
+              std::vector<char> &alive)
+{
+    size_t n = ev.size();
+    alive.assign(n, 1);
+    size_t i = 0;
+    while (i < n) {
+        size_t j = i + 1;
+        double tdead = ev[i].t;
+        while (j < n && site[j] == site[i]) {
+            if (ev[j].t < tdead + tau) {
+                alive[j] = 0;
+                if (type == 0) tdead = ev[j].t;
+            } else {
+                tdead = ev[j].t;
+            }
+            ++j;
+        }
+        i = j;
+    }
+}
+
+void run_digitizer(gpet_ctx &C, Result &R)
+{
+    const gpet_digitizer &g = C.dig;
+    const Detector &d = C.det;
+    std::vector<gpet_event> ev;
+    ev.reserve(R.adder.size());
+    for (gpet_event e : R.adder) {
+        blur_event(C, e);
+        if (e.E < g.e_threshold || e.E > g.e_upholder) continue; /* energywindow */
+        ev.push_back(e);
+    }
+    R.n_threshold = (int64_t)ev.size();
+    if (g.dead_time_us > 0.0f && !ev.empty()) {
+        if (g.dead_level != 3)
+            for (auto &e : ev) e.siten = site_at(e, g.dead_level, d); /* setSitenum */
+        std::vector<size_t> idx(ev.size());
+        for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+        std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+            if (ev[a].siten != ev[b].siten) return ev[a].siten < ev[b].siten;
+            return ev_less_t(ev[a], ev[b]);
+        });
+        std::vector<gpet_event> s(ev.size());
+        std::vector<int> site(ev.size());
+        for (size_t i = 0; i < idx.size(); ++i) { s[i] = ev[idx[i]]; site[i] = s[i].siten; }
+        std::vector<char> alive;
+        deadtime(s, site, (double)g.dead_time_us, g.dead_type, alive);
+        ev.clear();
+        for (size_t i = 0; i < s.size(); ++i)
+            if (alive[i]) ev.push_back(s[i]);
+    }
+    R.n_deadtime = (int64_t)ev.size();
+    std::vector<gpet_event> sg;
+    sg.reserve(ev.size());
+    for (auto &e : ev)
+        if (!(e.E < g.win_min || e.E > g.win_max)) sg.push_back(e);
+    std::sort(sg.begin(), sg.end(), ev_less_t);
+    R.singles = sg;
+    R.coinc.clear();
+    R.n_coinc_true = 0;
+    if (g.coinc_window_us > 0.0) {
+        const double W = g.coinc_window_us;
+        size_t n = sg.size(), i = 0;
+        while (i < n) {
+            size_t j = i + 1;
+            while (j < n && sg[j].t - sg[i].t <= W) ++j;
+            size_t m = j - i;
+            if (m == 2 || (m > 2 && g.coinc_policy == GPET_COINC_TAKE_ALL_GOODS)) {
+                for (size_t a = i; a < j; ++a)
+                    for (size_t b = a + 1; b < j; ++b) {
+                        R.coinc.push_back(sg[a]);
+                        R.coinc.push_back(sg[b]);
+                        if (sg[a].eventid == sg[b].eventid) R.n_coinc_true++;
+                    }
+            }
+            i = j;
+        }
+    }
+}
+
+bool ev_less_id(const gpet_event &a, const gpet_event &b)
+{
+    if (a.parn != b.parn) return a.parn < b.parn;
+    return a.siten < b.siten;
+}
+
+/* Run the transport over n histories with OpenMP, collect, digitize. */
+template <class F>
+void run_histories(gpet_ctx &C, int64_t n, F hist_fn, gpet_sink *sink, Result &R)
+{
+    R = Result();
+    R.n_histories = n;
+    const bool want_trace = sink && sink->trace && sink->trace_cap > 0;
+    const bool want_hits = C.opt.record_hits != 0;
+    auto t_begin = std::chrono::steady_clock::now();
+    int nthreads = 1;
+#ifdef _OPENMP
+    nthreads = C.threads > 0 ? C.threads : omp_get_max_threads();
+#endif
+    std::vector<std::vector<gpet_event>> tev(nthreads);
+    std::vector<std::vector<HitRec>> thit(nthreads);
+    std::vector<Counters> tcnt(nthreads);
+    std::vector<gpet_trace> trace;
+    if (want_trace) trace.assign((size_t)std::min<int64_t>(2 * n, sink->trace_cap), gpet_trace());
+    const int64_t ntr = (int64_t)trace.size();
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads)
+#endif
+    for (int64_t h = 0; h < n; ++h) {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        PhotonOut out[2];
+        gpet_trace tr2[2];
+        memset(tr2, 0, sizeof(tr2));
+        bool tracing = want_trace && 2 * h < ntr;
+        hist_fn(h, tracing ? tr2 : nullptr, out, tcnt[tid], want_hits);
+        for (int k = 0; k < 2; ++k) {
+            for (int i = 0; i < out[k].nev; ++i) tev[tid].push_back(out[k].ev[i]);
+            for (auto &hr : out[k].hits) thit[tid].push_back(hr);
+        }
+        if (tracing) {
+            trace[2 * h] = tr2[0];
+            if (2 * h + 1 < ntr) trace[2 * h + 1] = tr2[1];
+        }
+    }
+    for (int t = 0; t < nthreads; ++t) {
+        R.adder.insert(R.adder.end(), tev[t].begin(), tev[t].end());
+        R.hits.insert(R.hits.end(), thit[t].begin(), thit[t].end());
+        R.c.add(tcnt[t]);
+    }
+    std::sort(R.adder.begin(), R.adder.end(), ev_less_id);
+    /* per photon, hits stay in the order they happened */
+    std::stable_sort(R.hits.begin(), R.hits.end(), [](const HitRec &a, const HitRec &b) { return a.id[0] < b.id[0]; });
+    R.trace.swap(trace);
+    auto t_mid = std::chrono::steady_clock::now();
+    run_digitizer(C, R);
+    auto t_end = std::chrono::steady_clock::now();
+    R.sec_transport = std::chrono::duration<double>(t_mid - t_begin).count();
+    R.sec_digi = std::chrono::duration<double>(t_end - t_mid).count();
+}
+
+template <class T>
+void copy_out(const std::vector<T> &v, T *dst, int64_t cap, int64_t &n, int64_t per = 1)
+{
+    n = (int64_t)v.size() / per;
+    if (dst && cap > 0) {
+        int64_t m = std::min<int64_t>(n, cap);
+        memcpy(dst, v.data(), (size_t)(m * per) * sizeof(T));
+    }
+}
+
+void fill_sink(const Result &R, gpet_sink *s)
+{
+    if (!s) return;
+    copy_out(R.singles, s->singles, s->singles_cap, s->n_singles);
+    copy_out(R.coinc, s->coinc, s->coinc_cap, s->n_coinc, 2);
+    copy_out(R.adder, s->adder, s->adder_cap, s->n_adder);
+    s->n_hits = (int64_t)R.hits.size();
+    if (s->hit_id && s->hit_f && s->hits_cap > 0) {
+        int64_t m = std::min<int64_t>(s->n_hits, s->hits_cap);
+        for (int64_t i = 0; i < m; ++i) {
+            memcpy(s->hit_id + 5 * i, R.hits[i].id, 20);
+            memcpy(s->hit_f + 5 * i, R.hits[i].f, 20);
+        }
+    }
+    copy_out(R.trace, s->trace, s->trace_cap, s->n_trace);
+}
+
+gpet_status check_ready(gpet_ctx *c)
+{
+    if (!c) return GPET_ERR_ARG;
+    if (!c->has_tab) return fail(c, GPET_ERR_STATE, "tables not set");
+    if (!c->has_dig) return fail(c, GPET_ERR_STATE, "digitizer not set");
+    return GPET_OK;
+}
+
+} // namespace
+
+/* ======================= C ABI (gpet_cpu_*) ============================== */
+extern "C" {
+
+gpet_status gpet_cpu_create(gpet_ctx **out, const gpet_options *opt)
+{
+    if (!out || !opt) return GPET_ERR_ARG;
+    gpet_ctx *c = new gpet_ctx();
+    c->opt = *opt;
+    memset(&c->dig, 0, sizeof(c->dig));
+    *out = c;
+    return GPET_OK;
+}
+
+gpet_status gpet_cpu_destroy(gpet_ctx *c)
+{
+    delete c;
+    return GPET_OK;
+}
+
+const char *gpet_cpu_last_error(gpet_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+gpet_status gpet_cpu_set_threads(gpet_ctx *c, int32_t n)
+{
+    if (!c) return GPET_ERR_ARG;
+    c->threads = n;
+    return GPET_OK;
+}
+
+gpet_status gpet_cpu_set_tables(gpet_ctx *c, const gpet_tables *t)
+{
+    if (!c || !t) return GPET_ERR_ARG;
+    if (t->nmat <= 0 || t->nmat > GPET_MAXMAT || t->n1d < 2 || t->ncp < 2 || t->ne < 2)
+        return fail(c, GPET_ERR_FORMAT, "table dimensions");
+    Tables &T = c->tab;
+    T.nmat = t->nmat; T.eminph = t->eminph; T.emax = t->emax; T.n1d = t->n1d;
+    for (int k = 0; k < 4; ++k) {
+        T.e0[k] = t->e0[k]; T.idl[k] = t->idl[k];
+        T.mu[k].assign(t->mu[k], t->mu[k] + (size_t)t->nmat * t->n1d);
+    }
+    T.ncp = t->ncp; T.ne = t->ne;
+    for (int k = 0; k < 2; ++k) {
+        T.idcp[k] = t->idcp[k]; T.ide[k] = t->ide[k];
+        T.surf[k].assign(t->surf[k], t->surf[k] + (size_t)t->nmat * t->ncp * t->ne);
+    }
+    c->has_tab = true;
+    return GPET_OK;
+}
+
+gpet_status gpet_cpu_set_phantom(gpet_ctx *c, const int32_t *mat, const float *dens,
+                                 const int32_t dims[3], const float off[3], const float size[3])
+{
+    if (!c || !mat || !dens || !dims || !off || !size) return GPET_ERR_ARG;
+    if (!c->has_tab) return fail(c, GPET_ERR_STATE, "tables must be set before the phantom");
+    Phantom &p = c->ph;
+    p.nx = dims[0]; p.ny = dims[1]; p.nz = dims[2];
+    if (p.nx <= 0 || p.ny <= 0 || p.nz <= 0) return fail(c, GPET_ERR_ARG, "phantom dims");
+    p.offx = off[0]; p.offy = off[1]; p.offz = off[2];
+    p.dx = size[0] / (float)p.nx; p.dy = size[1] / (float)p.ny; p.dz = size[2] / (float)p.nz;
+    p.idx = 1.0f / p.dx; p.idy = 1.0f / p.dy; p.idz = 1.0f / p.dz;
+    size_t nv = (size_t)p.nx * p.ny * p.nz;
+    p.mat.assign(mat, mat + nv);
+    p.dens.assign(dens, dens + nv);
+    float rmax[GPET_MAXMAT] = {0};
+    for (size_t i = 0; i < nv; ++i) {
+        int m = p.mat[i];
+        if (m < 0 || m >= c->tab.nmat) return fail(c, GPET_ERR_ARG, "phantom material out of range");
+        if (p.dens[i] > rmax[m]) rmax[m] = p.dens[i];
+    }
+    build_wck(c->tab, rmax, p.wck);
+    c->has_ph = true;
+    return GPET_OK;
+}
+
+gpet_status gpet_cpu_set_detector(gpet_ctx *c, const gpet_panel *panels, int32_t n,
+                                  const gpet_panel_material mats[2], const float *quad, int32_t nq)
+{
+    if (!c || !panels || n <= 0 || !mats) return GPET_ERR_ARG;
+    if (!c->has_tab) return fail(c, GPET_ERR_STATE, "tables must be set before the detector");
+    if (nq < 0 || nq > GPET_MAXSURFACE) return fail(c, GPET_ERR_CAPACITY, "too many surfaces");
+    Detector &d = c->det;
+    d.p.assign(panels, panels + n);
+    d.mats[0] = mats[0]; d.mats[1] = mats[1];
+    if (quad && nq > 0) d.quad.assign(quad, quad + 10 * nq);
+    else d.quad.clear();
+    d.nquad = (int)d.quad.size() / 10;
+    const gpet_panel &p0 = panels[0];
+    int mny = (int)std::floor(p0.leny / (p0.mody + p0.mspy)) + 1;
+    int cny = (int)std::floor(p0.mody / (p0.cry + p0.cspy)) + 1;
+    d.moduleNy = mny; d.crystalNy = cny;
+    d.moduleN = mny * ((int)std::floor(p0.lenz / (p0.modz + p0.mspz)) + 1);
+    d.crystalN = cny * ((int)std::floor(p0.modz / (p0.crz + p0.cspz)) + 1);
+    float rmax[GPET_MAXMAT] = {0};
+    for (int i = 0; i < 2; ++i) {
+        int m = mats[i].material;
+        if (m < 0 || m >= c->tab.nmat) return fail(c, GPET_ERR_ARG, "panel material out of range");
+        if (mats[i].density > rmax[m]) rmax[m] = mats[i].density;
+    }
+    build_wck(c->tab, rmax, d.wck);
+    c->has_det = true;
+    return GPET_OK;
+}
+
+gpet_status gpet_cpu_set_digitizer(gpet_ctx *c, const gpet_digitizer *d)
+{
+    if (!c || !d) return GPET_ERR_ARG;
+    c->dig = *d;
+    c->has_dig = true;
+    return GPET_OK;
+}
+
+gpet_status gpet_cpu_run_source(gpet_ctx *c, const gpet_source *src, int32_t ns, const gpet_isotope *iso,
+                                int32_t niso, double t0, double t1, int64_t n_decays, gpet_sink *sink)
+{
+    gpet_status st = check_ready(c);
+    if (st) return st;
+    if (!src || ns <= 0 || !iso || niso <= 0 || !(t1 >= t0)) return fail(c, GPET_ERR_ARG, "source arguments");
+    SourcePlan pl;
+    st = plan_sources(*c, src, ns, iso, niso, t0, t1, n_decays, pl);
+    if (st) return fail(c, st, "bad source/isotope");
+    const gpet_ctx &C = *c;
+    run_histories(*c, pl.total,
+                  [&](int64_t h, gpet_trace *tr, PhotonOut *out, Counters &cnt, bool wh) {
+                      decay_history(C, pl, src, iso, t0, h, tr, out, cnt, wh);
+                  },
+                  sink, c->last);
+    fill_sink(c->last, sink);
+    return GPET_OK;
+}
+
+gpet_status gpet_cpu_run_psf(gpet_ctx *c, const double *rec8, int64_t n, int32_t ptype, gpet_sink *sink)
+{
+    gpet_status st = check_ready(c);
+    if (st) return st;
+    if (!rec8 || n < 0 || (ptype != 0 && ptype != 1)) return fail(c, GPET_ERR_ARG, "psf arguments");
+    const gpet_ctx &C = *c;
+    run_histories(*c, n,
+                  [&](int64_t h, gpet_trace *tr, PhotonOut *out, Counters &cnt, bool wh) {
+                      psf_history(C, rec8 + 8 * h, ptype, h, tr, out, cnt, wh);
+                  },
+                  sink, c->last);
+    fill_sink(c->last, sink);
+    return GPET_OK;
+}
+
+gpet_status gpet_cpu_get_tallies(gpet_ctx *c, gpet_tallies *t)
+{
+    if (!c || !t) return GPET_ERR_ARG;
+    const Result &R = c->last;
+    t->n_histories = R.n_histories;
+    t->n_photons = R.c.photons;
+    t->n_panel_entries = R.c.panel_entries;
+    t->n_hits = R.c.hits;
+    t->n_adder = (int64_t)R.adder.size();
+    t->n_threshold = R.n_threshold;
+    t->n_deadtime = R.n_deadtime;
+    t->n_singles = (int64_t)R.singles.size();
+    t->n_coinc = (int64_t)R.coinc.size() / 2;
+    t->n_coinc_true = R.n_coinc_true;
+    t->dropped_events = R.c.dropped_events;
+    t->dropped_hits = 0;
+    t->majorant_violations = R.c.majorant;
+    t->phantom_steps = R.c.phantom_steps;
+    t->range_steps = R.c.range_steps;
+    t->panel_steps = R.c.panel_steps;
+    t->seconds_transport = R.sec_transport;
+    t->seconds_digitizer = R.sec_digi;
+    const Detector &d = c->det;
+    if (t->crystal_singles && t->n_crystals > 0) {
+        memset(t->crystal_singles, 0, sizeof(int64_t) * t->n_crystals);
+        for (auto &e : R.singles) {
+            int64_t g = (int64_t)e.pann * d.moduleN * d.crystalN + (int64_t)e.modn * d.crystalN + e.cryn;
+            if (g >= 0 && g < t->n_crystals) t->crystal_singles[g]++;
+        }
+    }
+    if (t->energy_hist && t->n_ebins > 0 && t->e_hi > t->e_lo) {
+        memset(t->energy_hist, 0, sizeof(int64_t) * t->n_ebins);
+        float inv = (float)t->n_ebins / (t->e_hi - t->e_lo);
+        for (auto &e : R.singles) {
+            float x = (e.E - t->e_lo) * inv;
+            if (x >= 0.0f && x < (float)t->n_ebins) t->energy_hist[(int)x]++;
+        }
+    }
+    return GPET_OK;
+}
+
+/* Majorant table of the last set_phantom (which=0) or set_detector (1),
+ * for the invariant test. */
+gpet_status gpet_cpu_get_majorant(gpet_ctx *c, int32_t which, float *out, float *e0, float *idl)
+{
+    if (!c || !out) return GPET_ERR_ARG;
+    const Wck &w = which == 0 ? c->ph.wck : c->det.wck;
+    if (w.lam.empty()) return GPET_ERR_STATE;
+    memcpy(out, w.lam.data(), sizeof(float) * GPET_NWCK);
+    if (e0) *e0 = w.e0;
+    if (idl) *idl = w.idl;
+    return GPET_OK;
+}
+
+/* Single-function probes used by the CPU physics tests. */
+float gpet_cpu_lerp_mu(gpet_ctx *c, int32_t table, int32_t mat, float E)
+{
+    return mu_tab(c->tab, table, mat, E);
+}
+
+float gpet_cpu_surface(gpet_ctx *c, int32_t which, int32_t mat, float E, float u)
+{
+    return surf_read(c->tab, which, mat, E, u);
+}
+
+void gpet_cpu_sample_kn(float E, uint64_t seed, int64_t n, float *efrac, float *costh)
+{
+    gp_rng r;
+    gp_rng_init(&r, seed, 0, 9u, 0u);
+    for (int64_t i = 0; i < n; ++i) kn_sample(E, r, efrac[i], costh[i]);
+}
+
+void gpet_cpu_rotate(float *uvw, float costh, float uphi)
+{
+    rotate_dir(uvw[0], uvw[1], uvw[2], costh, uphi);
+}
+
+void gpet_cpu_sample_ek(const gpet_isotope *iso, uint64_t seed, int64_t n, float *out)
+{
+    gp_rng r;
+    gp_rng_init(&r, seed, 0, 9u, 0u);
+    for (int64_t i = 0; i < n; ++i) out[i] = sample_ek(*iso, r);
+}
+
+void gpet_cpu_sample_shape(const gpet_source *s, uint64_t seed, int64_t n, float *xyz)
+{
+    gp_rng r;
+    gp_rng_init(&r, seed, 0, 9u, 0u);
+    for (int64_t i = 0; i < n; ++i) sample_shape(*s, r, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+}
+
+void gpet_cpu_emit_pairs(float sigma, uint64_t seed, int64_t n, float *v6, float *e2)
+{
+    gp_rng r;
+    gp_rng_init(&r, seed, 0, 9u, 0u);
+    for (int64_t i = 0; i < n; ++i) {
+        float v[2][3], e[2];
+        emit_pair(sigma, r, v, e);
+        for (int k = 0; k < 2; ++k) {
+            v6[6 * i + 3 * k] = v[k][0]; v6[6 * i + 3 * k + 1] = v[k][1]; v6[6 * i + 3 * k + 2] = v[k][2];
+            e2[2 * i + k] = e[k];
+        }
+    }
+}
+
+/* Positron range displacement for n positrons of kinetic energy ek starting
+ * at xyz0 (needs a phantom). */
+gpet_status gpet_cpu_positron_range(gpet_ctx *c, const float *xyz0, float ek, uint64_t seed, int64_t n, float *xyz)
+{
+    if (!c || !c->has_ph) return GPET_ERR_STATE;
+    gp_rng r;
+    gp_rng_init(&r, seed, 0, 9u, 0u);
+    Counters cnt;
+    for (int64_t i = 0; i < n; ++i) {
+        float x = xyz0[0], y = xyz0[1], z = xyz0[2];
+        positron_range(*c, x, y, z, ek, 0, 0, 0, 0, r, cnt);
+        xyz[3 * i] = x; xyz[3 * i + 1] = y; xyz[3 * i + 2] = z;
+    }
+    return GPET_OK;
+}
+
+/* Crystal index probe: returns m_id and writes module/crystal ids. */
+int32_t gpet_cpu_crystal_search(gpet_ctx *c, int32_t panel, float x, float y, float z, int32_t *M_id, int32_t *L_id)
+{
+    int M = -1, L = -1;
+    int m = crystal_search(c->det, c->det.p[panel], x, y, z, M, L);
+    *M_id = M; *L_id = L;
+    return m;
+}
+
+/* Digitizer stages on a caller list (already adder/readout output). */
+gpet_status gpet_cpu_digitize(gpet_ctx *c, const gpet_event *ev, int64_t n, gpet_sink *sink)
+{
+    gpet_status st = check_ready(c);
+    if (st) return st;
+    c->last = Result();
+    c->last.adder.assign(ev, ev + n);
+    std::sort(c->last.adder.begin(), c->last.adder.end(), ev_less_id);
+    run_digitizer(*c, c->last);
+    fill_sink(c->last, sink);
+    return GPET_OK;
+}
+
+} /* extern "C" */
