@@ -130,7 +130,7 @@ def main():
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         probe = 200000
         rate, _, dt = cpu_baseline(probe, threads)
-        n_sample = int(min(max(rate * args.cpu_seconds, probe), 5e7))
+        n_sample = int(min(max(rate * args.cpu_seconds, probe), 4e8))
         rate, crate, dt = cpu_baseline(n_sample, threads)
         out["cpu_baseline"] = {"value": rate, "unit": "histories/s", "cores": threads, "kind": "port",
                                "sample": "%d decays of the same workload (%.1f s), oracle build -O3 OpenMP" % (n_sample, dt),

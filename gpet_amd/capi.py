@@ -152,6 +152,7 @@ HOST_FUNCS = {
     "gpet_input_digitizer": (None, [P(Input), P(Digitizer)]),
     "gpet_write_events": (C.c_int32, [C.c_char_p, _VP, C.c_int64, C.c_int32]),
     "gpet_write_hits": (C.c_int32, [C.c_char_p, C.c_char_p, _VP, _VP, C.c_int64, C.c_int32]),
+    "gpet_build_majorant": (C.c_int32, [P(Tables), _VP, _VP, P(C.c_float), P(C.c_float)]),
 }
 
 

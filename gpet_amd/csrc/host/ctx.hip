@@ -770,6 +770,27 @@ gpet_status gpet_get_tallies(gpet_ctx *c, gpet_tallies *t)
     return GPET_OK;
 }
 
+/* Host-only: the majorant gpet_set_phantom/gpet_set_detector would build
+ * for these tables and per-material maximum densities. */
+gpet_status gpet_build_majorant(const gpet_tables *t, const float *rho_max, float *out, float *e0, float *idl)
+{
+    if (!t || !rho_max || !out || t->nmat <= 0 || t->nmat > GPET_MAXMAT) return GPET_ERR_ARG;
+    gpet_ctx tmp;
+    tmp.nmat = t->nmat;
+    tmp.eminph = t->eminph;
+    tmp.emax = t->emax;
+    tmp.e0 = t->e0[GPET_TAB_LAMPH];
+    tmp.idl = t->idl[GPET_TAB_LAMPH];
+    tmp.h_lamph.assign(t->mu[GPET_TAB_LAMPH], t->mu[GPET_TAB_LAMPH] + (size_t)t->nmat * t->n1d);
+    std::vector<float> lam;
+    float a = 0, b = 0;
+    build_majorant(&tmp, rho_max, lam, a, b);
+    memcpy(out, lam.data(), sizeof(float) * GPET_NWCK);
+    if (e0) *e0 = a;
+    if (idl) *idl = b;
+    return GPET_OK;
+}
+
 /* Majorant node values of the phantom (which = 0) or detector (1). */
 gpet_status gpet_get_majorant(gpet_ctx *c, int32_t which, float *out, float *e0, float *idl)
 {

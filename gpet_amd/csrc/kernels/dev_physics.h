@@ -127,6 +127,43 @@ __device__ __forceinline__ void d_rotate(float &u, float &v, float &w, float cos
     }
 }
 
+/* rotate with sin(theta) given (pair noncollinearity: avoids the FP32
+ * collapse of sqrt(1 - cos^2) for small angles). */
+__device__ __forceinline__ void d_rotate_cs(float &u, float &v, float &w, float costh, float sinth, float uphi)
+{
+    float rho2 = u * u + v * v;
+    float norm = rho2 + w * w;
+    if (fabsf(norm - 1.0f) > D_SZERO) {
+        norm = 1.0f / sqrtf(norm);
+        u = u * norm;
+        v = v * norm;
+        w = w * norm;
+        rho2 = u * u + v * v;
+    }
+    float sinphi, cosphi;
+    gp_sincos2pi(uphi, &sinphi, &cosphi);
+    if (rho2 > D_ZEROF) {
+        float sthrho = sinth / sqrtf(rho2);
+        float urho = u * sthrho;
+        float vrho = v * sthrho;
+        float nu = u * costh - vrho * sinphi + (w * urho) * cosphi;
+        float nv = v * costh + urho * sinphi + (w * vrho) * cosphi;
+        float nw = w * costh - (rho2 * sthrho) * cosphi;
+        u = nu;
+        v = nv;
+        w = nw;
+    } else {
+        v = sinth * sinphi;
+        if (w > 0.0f) {
+            u = sinth * cosphi;
+            w = costh;
+        } else {
+            u = -sinth * cosphi;
+            w = -costh;
+        }
+    }
+}
+
 /* Klein-Nishina rejection sampling (gPET_kernals.cu:90-126). */
 __device__ __forceinline__ void d_kn(float E, gp_rng &r, float &efrac, float &costhe)
 {

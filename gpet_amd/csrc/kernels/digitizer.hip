@@ -245,20 +245,28 @@ __global__ __launch_bounds__(256) void k_time_keys(const gpet_event *ev, unsigne
     }
 }
 
-__global__ __launch_bounds__(256) void k_site_keys(const gpet_event *ev, const unsigned *perm, unsigned long long *key,
-                                                   unsigned *val, int64_t n, int level, int moduleN, int crystalN)
+/* setSitenum (gPET_kernals.cu:607-640): rewrite siten at the deadtime
+ * level; run whenever dlevel != 3 (dlevel 3 keeps the readout site, Q7). */
+__global__ __launch_bounds__(256) void k_set_site(gpet_event *ev, int64_t n, int level, int moduleN, int crystalN)
 {
-    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    unsigned idx = perm[j];
-    const gpet_event &e = ev[idx];
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    gpet_event &e = ev[i];
     int s;
     if (level == 0) s = 0;
     else if (level == 1) s = e.pann;
     else if (level == 2) s = e.pann * moduleN + e.modn;
-    else if (level == 3) s = e.siten; /* dlevel 3 keeps the readout site (Q7) */
     else s = e.pann * moduleN * crystalN + e.modn * crystalN + e.cryn;
-    key[j] = (unsigned long long)(unsigned)s;
+    e.siten = s;
+}
+
+__global__ __launch_bounds__(256) void k_site_keys(const gpet_event *ev, const unsigned *perm, unsigned long long *key,
+                                                   unsigned *val, int64_t n)
+{
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    unsigned idx = perm[j];
+    key[j] = (unsigned long long)(unsigned)ev[idx].siten;
     val[j] = idx;
 }
 
@@ -483,6 +491,11 @@ int gpet_digitize(DigiWork *w, const DigiParams &p, const gpet_event *events, in
     out->n_threshold = nT;
     const gpet_event *ev = w->ev_a;
     const int64_t m = nT;
+    if (g.dead_level != 3 && m > 0) {
+        hipLaunchKernelGGL(k_set_site, dim3(grid_for(m, 256)), dim3(256), 0, s, w->ev_a, m, g.dead_level, p.moduleN,
+                           p.crystalN);
+        HIPCHK(hipGetLastError());
+    }
 
     /* time order: perm_t in val_a */
     if (m > 0) {
@@ -499,9 +512,7 @@ int gpet_digitize(DigiWork *w, const DigiParams &p, const gpet_event *events, in
         /* move perm_t out of the sort buffers */
         HIPCHK(hipMemcpyAsync(w->pos, perm_t, sizeof(unsigned) * m, hipMemcpyDeviceToDevice, s));
         perm_t = w->pos;
-        int level = (g.dead_level == 3) ? 3 : g.dead_level;
-        hipLaunchKernelGGL(k_site_keys, dim3(grid_for(m, 256)), dim3(256), 0, s, ev, perm_t, w->key_a, w->val_a, m, level,
-                           p.moduleN, p.crystalN);
+        hipLaunchKernelGGL(k_site_keys, dim3(grid_for(m, 256)), dim3(256), 0, s, ev, perm_t, w->key_a, w->val_a, m);
         HIPCHK(hipGetLastError());
         if (radix_sort(w, m, 32, s, err, errlen)) return -1;
         HIPCHK(hipMemsetAsync(w->alive, 0, sizeof(unsigned) * m, s));

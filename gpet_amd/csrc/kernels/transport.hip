@@ -184,7 +184,9 @@ __device__ void d_emit_pair(float sigma, gp_rng &r, float v[2][3], float e[2])
     v[1][0] = v[0][0];
     v[1][1] = v[0][1];
     v[1][2] = v[0][2];
-    d_rotate(v[1][0], v[1][1], v[1][2], -gp_cosf(delta), uphi);
+    float sd, cd;
+    gp_sincosf(delta, &sd, &cd);
+    d_rotate_cs(v[1][0], v[1][1], v[1][2], -cd, fabsf(sd), uphi);
 }
 
 /* Woodcock tracking in the phantom; returns false when absorbed. */

@@ -328,6 +328,10 @@ gpet_status gpet_get_tallies(gpet_ctx *ctx, gpet_tallies *t);
  * (which = 0) or gpet_set_detector (which = 1); iniwck, initialize.cu:773-829. */
 gpet_status gpet_get_majorant(gpet_ctx *ctx, int32_t which, float *out, float *e0, float *idl);
 
+/* Host-only: the majorant the two calls above would build for tables t and
+ * per-material maximum densities rho_max[nmat]. */
+gpet_status gpet_build_majorant(const gpet_tables *t, const float *rho_max, float *out, float *e0, float *idl);
+
 #ifdef __cplusplus
 }
 #endif

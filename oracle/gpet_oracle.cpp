@@ -272,6 +272,35 @@ inline void rotate_dir(float &u, float &v, float &w, float costh, float uphi)
     }
 }
 
+/* rotate with sin(theta) given: used for the pair's noncollinearity, where
+ * sqrt(1 - cos^2) in FP32 would round every |delta| < 3.5e-4 rad to a
+ * collinear pair (the reference computes it that way). */
+inline void rotate_dir_cs(float &u, float &v, float &w, float costh, float sinth, float uphi)
+{
+    float rho2 = u * u + v * v;
+    float norm = rho2 + w * w;
+    if (std::fabs(norm - 1.0f) > SZERO) {
+        norm = 1.0f / std::sqrt(norm);
+        u = u * norm; v = v * norm; w = w * norm;
+        rho2 = u * u + v * v;
+    }
+    float sinphi, cosphi;
+    gp_sincos2pi(uphi, &sinphi, &cosphi);
+    if (rho2 > ZEROF) {
+        float sthrho = sinth / std::sqrt(rho2);
+        float urho = u * sthrho;
+        float vrho = v * sthrho;
+        float nu = u * costh - vrho * sinphi + (w * urho) * cosphi;
+        float nv = v * costh + urho * sinphi + (w * vrho) * cosphi;
+        float nw = w * costh - (rho2 * sthrho) * cosphi;
+        u = nu; v = nv; w = nw;
+    } else {
+        v = sinth * sinphi;
+        if (w > 0.0f) { u = sinth * cosphi; w = costh; }
+        else { u = -sinth * cosphi; w = -costh; }
+    }
+}
+
 /* Analytic Klein-Nishina sampling (gPET_kernals.cu:90-126), FP32. */
 inline void kn_sample(float E, gp_rng &r, float &efrac, float &costhe)
 {
@@ -696,7 +725,9 @@ void emit_pair(float sigma, gp_rng &r, float v[2][3], float e[2])
     e[0] = MC2F + (delta * MC2F) / 2.0f;
     e[1] = MC2F - (delta * MC2F) / 2.0f;
     v[1][0] = v[0][0]; v[1][1] = v[0][1]; v[1][2] = v[0][2];
-    rotate_dir(v[1][0], v[1][1], v[1][2], -gp_cosf(delta), uphi);
+    float sd, cd;
+    gp_sincosf(delta, &sd, &cd);
+    rotate_dir_cs(v[1][0], v[1][1], v[1][2], -cd, std::fabs(sd), uphi);
 }
 
 /* Per-source decay plan shared by all histories of a run. */
@@ -917,9 +948,11 @@ void run_digitizer(gpet_ctx &C, Result &R)
         ev.push_back(e);
     }
     R.n_threshold = (int64_t)ev.size();
+    /* setSitenum runs whenever dlevel != 3, deadtime or not (gPET.cu:164-169);
+     * the rewritten site index is what singles.dat carries */
+    if (g.dead_level != 3)
+        for (auto &e : ev) e.siten = site_at(e, g.dead_level, d);
     if (g.dead_time_us > 0.0f && !ev.empty()) {
-        if (g.dead_level != 3)
-            for (auto &e : ev) e.siten = site_at(e, g.dead_level, d); /* setSitenum */
         std::vector<size_t> idx(ev.size());
         for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
         std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
@@ -1349,6 +1382,55 @@ gpet_status gpet_cpu_digitize(gpet_ctx *c, const gpet_event *ev, int64_t n, gpet
     run_digitizer(*c, c->last);
     fill_sink(c->last, sink);
     return GPET_OK;
+}
+
+} /* extern "C" */
+
+/* ---- probes of the shared RNG / math headers (tests only) -------------- */
+extern "C" {
+
+void gpet_cpu_philox(const uint32_t *ctr4, const uint32_t *key2, uint32_t *out4)
+{
+    gp_u4 c = {ctr4[0], ctr4[1], ctr4[2], ctr4[3]};
+    gp_u4 r = gp_philox4x32_10(c, key2[0], key2[1]);
+    out4[0] = r.x; out4[1] = r.y; out4[2] = r.z; out4[3] = r.w;
+}
+
+/* kind: 0 logf, 1 expf, 2 sin2pi, 3 cos2pi, 4 sinf, 5 cosf, 6 cbrtf */
+void gpet_cpu_mathf(int32_t kind, const float *in, float *out, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        float x = in[i], s, c;
+        switch (kind) {
+        case 0: out[i] = gp_logf(x); break;
+        case 1: out[i] = gp_expf(x); break;
+        case 2: gp_sincos2pi(x, &s, &c); out[i] = s; break;
+        case 3: gp_sincos2pi(x, &s, &c); out[i] = c; break;
+        case 4: gp_sincosf(x, &s, &c); out[i] = s; break;
+        case 5: gp_sincosf(x, &s, &c); out[i] = c; break;
+        case 6: out[i] = gp_cbrtf(x); break;
+        default: out[i] = 0.0f;
+        }
+    }
+}
+
+/* kind: 0 log, 1 log1p */
+void gpet_cpu_mathd(int32_t kind, const double *in, double *out, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i) out[i] = kind == 0 ? gp_log_d(in[i]) : gp_log1p_d(in[i]);
+}
+
+/* Draws of one stream: uniform floats, normals or 53-bit doubles. */
+void gpet_cpu_stream(uint64_t seed, uint64_t id, uint32_t aux, int32_t kind, int64_t n, void *out)
+{
+    gp_rng r;
+    gp_rng_init(&r, seed, id, aux, 0u);
+    for (int64_t i = 0; i < n; ++i) {
+        if (kind == 0) ((float *)out)[i] = gp_rng_uniform(&r);
+        else if (kind == 1) ((float *)out)[i] = gp_rng_normal(&r);
+        else if (kind == 2) ((double *)out)[i] = gp_rng_uniform_d(&r);
+        else ((uint32_t *)out)[i] = gp_rng_u32(&r);
+    }
 }
 
 } /* extern "C" */

@@ -57,22 +57,34 @@ def test_source_parity(tables, config, n, prange):
     assert_parity(a, b, hits=True)
 
 
-def test_deadtime_and_policies(tables):
-    """Paralyzable deadtime per crystal, winner-takes-all readout at crystal
-    depth, take-all-goods multiples and spatial blur, at a rate high enough
-    for deadtime and multiples to matter."""
+DIGI_CASES = {
+    # paralyzable deadtime per crystal, winner-takes-all at crystal depth,
+    # take-all-goods multiples, spatial blur
+    "crystal_paralyzable": dict(readout_depth=3, readout_policy=0, dead_level=3, dead_type=0, dead_time_us=2.0,
+                                space_blur=0.05, coinc_policy=1, coinc_window_us=0.05, win_min=100e3, win_max=700e3),
+    # non-paralyzable per panel (site rewritten by setSitenum), blur policy 0
+    "panel_nonparalyzable": dict(dead_level=1, dead_type=1, dead_time_us=1.0, blur_policy=0, r_ref=0.15),
+    # site rewrite to module level without deadtime, panel-level readout
+    "module_no_deadtime": dict(readout_depth=1, readout_policy=0, dead_level=2, dead_time_us=0.0),
+}
+
+
+@pytest.mark.parametrize("case", sorted(DIGI_CASES))
+def test_deadtime_and_policies(tables, case):
+    """Digitizer variants at a rate high enough for deadtime and multiples to
+    matter."""
     from gpet_amd import engine
     from gpet_amd import workloads as W
 
-    dig = W.digitizer(readout_depth=3, readout_policy=0, dead_level=3, dead_type=0, dead_time_us=2.0,
-                      space_blur=0.05, coinc_policy=1, coinc_window_us=0.05, win_min=100e3, win_max=700e3)
+    dig = W.digitizer(**DIGI_CASES[case])
     opt = engine.make_options(seed=5, positron_range=0)
     src, iso, _, _ = W.workload_sources(1)
     n = 40000
     a, b = run_both(tables, opt, lambda e: W.setup_engine(e, tables, config=1, dig=dig),
                     lambda e: e.run_source(src, iso, 0.0, 1e-3, n_decays=n, singles_cap=n, coinc_cap=4 * n,
                                            adder_cap=4 * n, trace_cap=2 * n))
-    assert a.tallies.n_deadtime < a.tallies.n_threshold
+    if DIGI_CASES[case].get("dead_time_us", 0) > 0:
+        assert a.tallies.n_deadtime < a.tallies.n_threshold
     assert_parity(a, b)
 
 
