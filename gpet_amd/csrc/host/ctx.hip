@@ -656,6 +656,8 @@ gpet_status gpet_set_detector(gpet_ctx *c, const gpet_panel *panels, int32_t n, 
         d.cry = s.cry; d.crz = s.crz; d.cspy = s.cspy; d.cspz = s.cspz;
         d.ox = s.ox; d.oy = s.oy; d.oz = s.oz; d.dirx = s.dirx;
         for (int k = 0; k < 3; ++k) { d.ux[k] = s.ux[k]; d.uy[k] = s.uy[k]; d.uz[k] = s.uz[k]; }
+        double r2 = 0.25 * ((double)s.leny * s.leny + (double)s.lenz * s.lenz);
+        d.bound2 = (float)(r2 * 1.002 + 0.01);
     }
     gpet_status st = upload(c, c->d_panels, dp.data(), n);
     if (st) return st;

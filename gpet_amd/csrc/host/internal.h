@@ -47,6 +47,8 @@ struct DevPanel {
     float lenx, leny, lenz, mody, modz, mspy, mspz, cry, crz, cspy, cspz;
     float ox, oy, oz, dirx;
     float ux[3], uy[3], uz[3];
+    float bound2;              /* squared radius (with margin) of a sphere about
+                                  the face centre that contains the face      */
 };
 
 struct DevDetector {

@@ -14,6 +14,7 @@
 #include "../host/internal.h"
 
 #define D_C_LIGHT 29979.2458       /* cm/us, gPET_kernals.cu:264 */
+#define D_INV_C (1.0 / 29979.2458)
 #define D_MC2F 510999.1f           /* constants.h:34 */
 #define D_IMC2F 1.95695061e-06f    /* constants.h:35 */
 #define D_SZERO 1.0e-4f

@@ -102,8 +102,29 @@ __device__ void d_sample_shape(const DevSource &s, gp_rng &r, float &x, float &y
     }
 }
 
-/* Positron range walk; voxel crossings gather only the density half of the
- * voxel record. */
+/* Positron range: 3-D DDA voxel walk (same rule as the oracle, see
+ * setPositronRange, gPET_kernals.cu:347-418); crossings gather only the
+ * density half of the voxel record. */
+__device__ __forceinline__ void d_dda_axis(float x, float off, float d, int i, float u, float &tmax, float &tdelta,
+                                           int &step)
+{
+    if (u > 0.0f) {
+        float inv = 1.0f / u;
+        tmax = (off + (float)(i + 1) * d - x) * inv;
+        tdelta = d * inv;
+        step = 1;
+    } else if (u < 0.0f) {
+        float inv = 1.0f / u;
+        tmax = (off + (float)i * d - x) * inv;
+        tdelta = -(d * inv);
+        step = -1;
+    } else {
+        tmax = INFINITY;
+        tdelta = INFINITY;
+        step = 0;
+    }
+}
+
 __device__ void d_positron_range(const DevPhantom &p, float &x, float &y, float &z, float ek_ev, int usedir,
                                  float vx, float vy, float vz, gp_rng &r, unsigned long long &nsteps)
 {
@@ -121,51 +142,59 @@ __device__ void d_positron_range(const DevPhantom &p, float &x, float &y, float 
         dz = rr * vz / tmp;
     }
     if (!(rr > 0.0f)) return;
+    float inv_rr = 1.0f / rr;
+    float ux = dx * inv_rr, uy = dy * inv_rr, uz = dz * inv_rr;
     int ix = 0, iy = 0, iz = 0;
-    bool inside = p.present && d_vox(p, x, y, z, ix, iy, iz);
-    float s = 0.0f;
-    const float ex = p.offx + (float)p.nx * p.dx, ey = p.offy + (float)p.ny * p.dy, ez = p.offz + (float)p.nz * p.dz;
-    for (int it = 0; it < D_STEP_CAP && s < rr; ++it) {
-        float step;
-        if (inside) {
-            step = 1000.0f;
-            int axis = 0;
-            if (dx != 0.0f) {
-                float b = (p.offx + (float)(ix + (dx > 0.0f)) * p.dx - x) / dx;
-                if (step > b) { step = b; axis = 1; }
+    float t;
+    if (!(p.present && d_vox(p, x, y, z, ix, iy, iz))) {
+        t = rr / D_RHO_OUTSIDE;
+    } else {
+        float tmx, tmy, tmz, tdx, tdy, tdz;
+        int sx, sy, sz;
+        d_dda_axis(x, p.offx, p.dx, ix, ux, tmx, tdx, sx);
+        d_dda_axis(y, p.offy, p.dy, iy, uy, tmy, tdy, sy);
+        d_dda_axis(z, p.offz, p.dz, iz, uz, tmz, tdz, sz);
+        /* The crossing sequence does not depend on the densities: advance the
+         * DDA four crossings at a time and issue their gathers together, then
+         * accumulate in order (identical arithmetic, 4x memory parallelism). */
+        const int *dens = reinterpret_cast<const int *>(p.vox);
+        float tcur = 0.0f, s = 0.0f;
+        t = 0.0f;
+        bool done = false;
+        for (int it = 0; it < D_STEP_CAP && !done; it += 4) {
+            float tn[4];
+            bool in[4];
+            int g[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (tmx <= tmy && tmx <= tmz) { tn[k] = tmx; tmx = tmx + tdx; ix += sx; }
+                else if (tmy <= tmz) { tn[k] = tmy; tmy = tmy + tdy; iy += sy; }
+                else { tn[k] = tmz; tmz = tmz + tdz; iz += sz; }
+                in[k] = ix >= 0 && ix < p.nx && iy >= 0 && iy < p.ny && iz >= 0 && iz < p.nz;
+                g[k] = in[k] ? dens[2 * (((size_t)iz * p.ny + iy) * p.nx + ix)] : 0;
             }
-            if (dy != 0.0f) {
-                float b = (p.offy + (float)(iy + (dy > 0.0f)) * p.dy - y) / dy;
-                if (step > b) { step = b; axis = 2; }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (!done) {
+                    float rho = in[k] ? __int_as_float(g[k]) : D_RHO_OUTSIDE;
+                    if (in[k]) nsteps++;
+                    s = s + (tn[k] - tcur) * rho;
+                    if (s >= rr) {
+                        t = (s > rr) ? tn[k] + (rr - s) / rho : tn[k];
+                        done = true;
+                    } else if (!in[k]) {
+                        t = tn[k] + (rr - s) / D_RHO_OUTSIDE;
+                        done = true;
+                    } else {
+                        tcur = tn[k];
+                    }
+                }
             }
-            if (dz != 0.0f) {
-                float b = (p.offz + (float)(iz + (dz > 0.0f)) * p.dz - z) / dz;
-                if (step > b) { step = b; axis = 3; }
-            }
-            if (axis == 1) ix += (dx > 0.0f) ? 1 : -1;
-            else if (axis == 2) iy += (dy > 0.0f) ? 1 : -1;
-            else iz += (dz > 0.0f) ? 1 : -1;
-            float rho;
-            if (ix >= 0 && ix < p.nx && iy >= 0 && iy < p.ny && iz >= 0 && iz < p.nz) {
-                rho = __int_as_float(p.vox[((size_t)iz * p.ny + iy) * p.nx + ix].x);
-                nsteps++;
-            } else {
-                rho = D_RHO_OUTSIDE;
-            }
-            step = step * rr;
-            s = s + step * rho;
-            if (s > rr) step = step + (rr - s) / rho;
-        } else {
-            step = (rr - s) / D_RHO_OUTSIDE;
-            s = rr + 100.0f;
-        }
-        x = x + (step * dx) / rr;
-        y = y + (step * dy) / rr;
-        z = z + (step * dz) / rr;
-        if (inside) {
-            if (x < p.offx || x > ex || y < p.offy || y > ey || z < p.offz || z > ez) inside = false;
         }
     }
+    x = x + ux * t;
+    y = y + uy * t;
+    z = z + uz * t;
 }
 
 __device__ void d_emit_pair(float sigma, gp_rng &r, float v[2][3], float e[2])
@@ -200,7 +229,7 @@ __device__ bool d_phantom_walk(const DevTables &tab, const DevPhantom &p, float 
         x = x + s * vx;
         y = y + s * vy;
         z = z + s * vz;
-        tof = tof + (double)s / D_C_LIGHT;
+        tof = tof + (double)s * D_INV_C;
         int ix, iy, iz;
         if (!d_vox(p, x, y, z, ix, iy, iz)) return true;
         nsteps++;
@@ -323,8 +352,19 @@ __global__ __launch_bounds__(256) void k_phantom(RunParams rp, DevTables tab, De
             int pa = -1;
             float y2 = 0.f, z2 = 0.f, vx2 = 0.f, vy2 = 0.f, vz2 = 0.f;
             if (alive && det.present) {
+                const float vv = vx * vx + vy * vy + vz * vz;
                 for (int pi = 0; pi < det.npanels; ++pi) {
                     const DevPanel &P = det.p[pi];
+                    /* conservative prefilter: the line through the photon must
+                     * pass within the face's bounding sphere (covers the
+                     * reference's backward entries too); skipping a panel here
+                     * never changes which panel the exact test picks */
+                    {
+                        float wx = P.ox - x, wy = P.oy - y, wz = P.oz - z;
+                        float a = vx * wx + vy * wy + vz * wz;
+                        float w2 = wx * wx + wy * wy + wz * wz;
+                        if (w2 * vv - a * a > P.bound2 * vv) continue;
+                    }
                     float rx = x - P.ox, ry = y - P.oy, rz = z - P.oz;
                     float tx = rx * P.ux[0] + ry * P.ux[1] + rz * P.ux[2];
                     float ty = rx * P.uy[0] + ry * P.uy[1] + rz * P.uy[2];
@@ -411,55 +451,39 @@ __device__ __forceinline__ void d_adder(EvList &L, int &n, int site, int mod, in
     n++;
 }
 
-/* readout (gPET_kernals.cu:756-813) on the register list; O gets the output. */
-__device__ __forceinline__ int d_readout(const EvList &L, int n, int depth, int policy, int pann, int moduleN, EvList &O)
+/* readout (gPET_kernals.cu:756-813) in place on the register list: merged
+ * entries are flagged dead; survivors keep their order. */
+__device__ __forceinline__ void d_readout(EvList &L, int n, int depth, int policy, int pann, int moduleN,
+                                          bool (&alive)[GPET_MAX_PH_EVENTS])
 {
-    if (depth == 3) {
-        O = L;
-        return n;
-    }
+#pragma unroll
+    for (int i = 0; i < GPET_MAX_PH_EVENTS; ++i) alive[i] = i < n;
+    if (depth == 3) return;
     if (policy == 1) depth = 2;
-    int s[GPET_MAX_PH_EVENTS];
-    bool dead[GPET_MAX_PH_EVENTS];
+#pragma unroll
+    for (int i = 0; i < GPET_MAX_PH_EVENTS; ++i)
+        L.site[i] = (depth == 0) ? 0 : (depth == 1 ? pann : pann * moduleN + L.mod[i]);
 #pragma unroll
     for (int i = 0; i < GPET_MAX_PH_EVENTS; ++i) {
-        s[i] = (depth == 0) ? 0 : (depth == 1 ? pann : pann * moduleN + L.mod[i]);
-        dead[i] = false;
-    }
-    int out = 0;
-#pragma unroll
-    for (int i = 0; i < GPET_MAX_PH_EVENTS; ++i) {
-        if (i < n && !dead[i]) {
-            int cmod = L.mod[i], ccry = L.cry[i];
-            double ct = L.t[i];
-            float cE = L.E[i], cx = L.x[i], cy = L.y[i], cz = L.z[i];
+        if (alive[i]) {
 #pragma unroll
             for (int j = i + 1; j < GPET_MAX_PH_EVENTS; ++j) {
-                if (j < n && s[j] == s[i]) {
+                if (alive[j] && L.site[j] == L.site[i]) {
                     if (policy == 1) {
-                        float sE = cE + L.E[j];
-                        cx = (cx * cE + L.x[j] * L.E[j]) / sE;
-                        cy = (cy * cE + L.y[j] * L.E[j]) / sE;
-                        cz = (cz * cE + L.z[j] * L.E[j]) / sE;
-                        cE = sE;
-                    } else if (!(cE > L.E[j])) {
-                        cmod = L.mod[j]; ccry = L.cry[j]; ct = L.t[j];
-                        cE = L.E[j]; cx = L.x[j]; cy = L.y[j]; cz = L.z[j];
+                        float sE = L.E[i] + L.E[j];
+                        L.x[i] = (L.x[i] * L.E[i] + L.x[j] * L.E[j]) / sE;
+                        L.y[i] = (L.y[i] * L.E[i] + L.y[j] * L.E[j]) / sE;
+                        L.z[i] = (L.z[i] * L.E[i] + L.z[j] * L.E[j]) / sE;
+                        L.E[i] = sE;
+                    } else if (!(L.E[i] > L.E[j])) {
+                        L.mod[i] = L.mod[j]; L.cry[i] = L.cry[j]; L.t[i] = L.t[j];
+                        L.E[i] = L.E[j]; L.x[i] = L.x[j]; L.y[i] = L.y[j]; L.z[i] = L.z[j];
                     }
-                    dead[j] = true;
+                    alive[j] = false;
                 }
             }
-#pragma unroll
-            for (int o = 0; o < GPET_MAX_PH_EVENTS; ++o) {
-                if (o == out) {
-                    O.site[o] = s[i]; O.mod[o] = cmod; O.cry[o] = ccry; O.t[o] = ct;
-                    O.E[o] = cE; O.x[o] = cx; O.y[o] = cy; O.z[o] = cz;
-                }
-            }
-            out++;
         }
     }
-    return out;
 }
 
 __global__ __launch_bounds__(256) void k_panel(PanelParams pp, DevTables tab, DevDetector det, DevQueue q, int nq)
@@ -489,7 +513,7 @@ __global__ __launch_bounds__(256) void k_panel(PanelParams pp, DevTables tab, De
             x2 = x2 + s * vx2;
             y2 = y2 + s * vy2;
             z2 = z2 + s * vz2;
-            tof = tof + (double)s / D_C_LIGHT;
+            tof = tof + (double)s * D_INV_C;
             if (fabsf(y2) > hy || fabsf(z2) > hz || x2 * P.dirx < 0.0f || x2 * P.dirx > hx) break;
             nsteps++;
             int M_id = -1, L_id = -1;
@@ -540,7 +564,9 @@ __global__ __launch_bounds__(256) void k_panel(PanelParams pp, DevTables tab, De
             }
             if (m_id == 0 && type != 0) {
                 const int site = pann * det.moduleN * det.crystalN + M_id * det.crystalN + L_id;
-                for (int rec = 0; rec < (second ? 2 : 1); ++rec) {
+#pragma unroll
+                for (int rec = 0; rec < 2; ++rec) {
+                    if (rec == 1 && !second) break;
                     const int ty = rec ? 2 : type;
                     const float ed = rec ? de2 : (type == 3 ? 0.0f : edep);
                     c_hits++;
@@ -561,9 +587,11 @@ __global__ __launch_bounds__(256) void k_panel(PanelParams pp, DevTables tab, De
             if (stop) break;
             if (rot) d_rotate(vx2, vy2, vz2, cth, uphi);
         }
-        EvList O;
+        bool alive[GPET_MAX_PH_EVENTS];
+        d_readout(L, nev, pp.readout_depth, pp.readout_policy, pann, det.moduleN, alive);
         int nout = 0;
-        if (nev > 0) nout = d_readout(L, nev, pp.readout_depth, pp.readout_policy, pann, det.moduleN, O);
+#pragma unroll
+        for (int j = 0; j < GPET_MAX_PH_EVENTS; ++j) nout += alive[j] ? 1 : 0;
         /* wave-aggregated append of nout (0..4) events per lane */
         unsigned long long lt = lanemask_lt();
         int off = 0, tot = 0;
@@ -578,17 +606,19 @@ __global__ __launch_bounds__(256) void k_panel(PanelParams pp, DevTables tab, De
             if (__lane_id() == 0) base = (int)atomicAdd(pp.n_events, (unsigned)tot);
             base = __shfl(base, 0);
         }
+        int o = 0;
 #pragma unroll
         for (int j = 0; j < GPET_MAX_PH_EVENTS; ++j) {
-            if (j < nout) {
-                int64_t dst = (int64_t)base + off + j;
+            if (alive[j]) {
+                int64_t dst = (int64_t)base + off + o;
                 if (dst < pp.events_cap) {
                     gpet_event ev;
-                    ev.parn = parn; ev.pann = pann; ev.modn = O.mod[j]; ev.cryn = O.cry[j];
-                    ev.siten = O.site[j]; ev.eventid = eid; ev.t = O.t[j];
-                    ev.E = O.E[j]; ev.x = O.x[j]; ev.y = O.y[j]; ev.z = O.z[j];
+                    ev.parn = parn; ev.pann = pann; ev.modn = L.mod[j]; ev.cryn = L.cry[j];
+                    ev.siten = L.site[j]; ev.eventid = eid; ev.t = L.t[j];
+                    ev.E = L.E[j]; ev.x = L.x[j]; ev.y = L.y[j]; ev.z = L.z[j];
                     pp.events[dst] = ev;
                 }
+                o++;
             }
         }
         c_steps += (unsigned long long)nsteps;

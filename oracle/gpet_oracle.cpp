@@ -42,6 +42,7 @@
 namespace {
 
 constexpr double C_LIGHT = 29979.2458;   /* cm/us, gPET_kernals.cu:264        */
+constexpr double INV_C = 1.0 / 29979.2458;
 constexpr float MC2F = 510999.1f;        /* constants.h:34                    */
 constexpr float IMC2F = 1.95695061e-06f; /* constants.h:35                    */
 constexpr float SZERO = 1.0e-4f;         /* constants.h:37                    */
@@ -425,7 +426,7 @@ bool phantom_transport(const gpet_ctx &C, float &x, float &y, float &z, float &v
         x = x + s * vx;
         y = y + s * vy;
         z = z + s * vz;
-        tof = tof + (double)s / C_LIGHT;
+        tof = tof + (double)s * INV_C;
         int ix, iy, iz;
         if (!vox_index(p, x, y, z, ix, iy, iz)) return true; /* escaped (Q9: landing point) */
         if (tr) tr->n_steps++;
@@ -523,7 +524,7 @@ void panel_transport(const gpet_ctx &C, float x, float y, float z, float vx, flo
         x2 = x2 + s * vx2;
         y2 = y2 + s * vy2;
         z2 = z2 + s * vz2;
-        tof = tof + (double)s / C_LIGHT;
+        tof = tof + (double)s * INV_C;
         if (std::fabs(y2) > hy || std::fabs(z2) > hz || x2 * P.dirx < 0.0f || x2 * P.dirx > hx) break;
         if (tr) tr->n_steps_panel++;
         cnt.panel_steps++;
@@ -639,11 +640,36 @@ void sample_shape(const gpet_source &s, gp_rng &r, float &x, float &y, float &z)
     }
 }
 
-/* setPositronRange (gPET_kernals.cu:347-418).  Fixes: outside the phantom the
- * remaining range is covered at RHO_OUTSIDE (the reference adds it to a
- * leftover 1000-unit step); a crossing into a voxel outside the grid uses
- * RHO_OUTSIDE (the reference reads the clamped border voxel).  Kept (Q8): the
- * segment just crossed is charged at the density of the voxel entered. */
+/* setPositronRange (gPET_kernals.cu:347-418).  The range r = |sigma N3| is a
+ * mass thickness; the positron advances until the accumulated sum of
+ * segment length x density reaches r.  The voxel walk is restated as an
+ * incremental 3-D DDA (Amanatides-Woo): per axis the distance to the next
+ * boundary and the boundary spacing along the unit direction, advanced by
+ * the smallest (ties: x, then y, then z, as the reference's sequential
+ * `step > b` tests).  Kept (Q8): a segment is charged at the density of
+ * the voxel it enters.  Fixes: a crossing out of the grid charges
+ * RHO_OUTSIDE and the remaining range is covered at RHO_OUTSIDE (the
+ * reference reads the clamped border voxel and then adds the remainder to
+ * a leftover 1000-unit step). */
+inline void dda_axis(float x, float off, float d, int i, float u, float &tmax, float &tdelta, int &step)
+{
+    if (u > 0.0f) {
+        float inv = 1.0f / u;
+        tmax = (off + (float)(i + 1) * d - x) * inv;
+        tdelta = d * inv;
+        step = 1;
+    } else if (u < 0.0f) {
+        float inv = 1.0f / u;
+        tmax = (off + (float)i * d - x) * inv;
+        tdelta = -(d * inv);
+        step = -1;
+    } else {
+        tmax = INFINITY;
+        tdelta = INFINITY;
+        step = 0;
+    }
+}
+
 void positron_range(const gpet_ctx &C, float &x, float &y, float &z, float ek_ev, int usedir,
                     float vx, float vy, float vz, gp_rng &r, Counters &cnt)
 {
@@ -662,52 +688,46 @@ void positron_range(const gpet_ctx &C, float &x, float &y, float &z, float ek_ev
         dz = rr * vz / tmp;
     }
     if (!(rr > 0.0f)) return;
+    float inv_rr = 1.0f / rr;
+    float ux = dx * inv_rr, uy = dy * inv_rr, uz = dz * inv_rr;
     int ix = 0, iy = 0, iz = 0;
-    bool inside = C.has_ph && vox_index(p, x, y, z, ix, iy, iz);
-    float s = 0.0f;
-    for (int it = 0; it < STEP_CAP && s < rr; ++it) {
-        float step;
-        if (inside) {
-            step = 1000.0f;
-            int axis = 0;
-            if (dx != 0.0f) {
-                float b = (p.offx + (float)(ix + (dx > 0.0f)) * p.dx - x) / dx;
-                if (step > b) { step = b; axis = 1; }
-            }
-            if (dy != 0.0f) {
-                float b = (p.offy + (float)(iy + (dy > 0.0f)) * p.dy - y) / dy;
-                if (step > b) { step = b; axis = 2; }
-            }
-            if (dz != 0.0f) {
-                float b = (p.offz + (float)(iz + (dz > 0.0f)) * p.dz - z) / dz;
-                if (step > b) { step = b; axis = 3; }
-            }
-            if (axis == 1) ix += (dx > 0.0f) ? 1 : -1;
-            else if (axis == 2) iy += (dy > 0.0f) ? 1 : -1;
-            else iz += (dz > 0.0f) ? 1 : -1;
-            float rho;
-            if (ix >= 0 && ix < p.nx && iy >= 0 && iy < p.ny && iz >= 0 && iz < p.nz) {
+    float t;
+    if (!(C.has_ph && vox_index(p, x, y, z, ix, iy, iz))) {
+        t = rr / RHO_OUTSIDE;
+    } else {
+        float tmx, tmy, tmz, tdx, tdy, tdz;
+        int sx, sy, sz;
+        dda_axis(x, p.offx, p.dx, ix, ux, tmx, tdx, sx);
+        dda_axis(y, p.offy, p.dy, iy, uy, tmy, tdy, sy);
+        dda_axis(z, p.offz, p.dz, iz, uz, tmz, tdz, sz);
+        float tcur = 0.0f, s = 0.0f;
+        t = 0.0f;
+        for (int it = 0; it < STEP_CAP; ++it) {
+            float tnext;
+            if (tmx <= tmy && tmx <= tmz) { tnext = tmx; tmx = tmx + tdx; ix += sx; }
+            else if (tmy <= tmz) { tnext = tmy; tmy = tmy + tdy; iy += sy; }
+            else { tnext = tmz; tmz = tmz + tdz; iz += sz; }
+            bool in = ix >= 0 && ix < p.nx && iy >= 0 && iy < p.ny && iz >= 0 && iz < p.nz;
+            float rho = RHO_OUTSIDE;
+            if (in) {
                 rho = p.dens[((size_t)iz * p.ny + iy) * p.nx + ix];
                 cnt.range_steps++;
-            } else {
-                rho = RHO_OUTSIDE;
             }
-            step = step * rr;
-            s = s + step * rho;
-            if (s > rr) step = step + (rr - s) / rho;
-        } else {
-            step = (rr - s) / RHO_OUTSIDE;
-            s = rr + 100.0f;
-        }
-        x = x + (step * dx) / rr;
-        y = y + (step * dy) / rr;
-        z = z + (step * dz) / rr;
-        if (inside) {
-            float ex = p.offx + (float)p.nx * p.dx, ey = p.offy + (float)p.ny * p.dy,
-                  ez = p.offz + (float)p.nz * p.dz;
-            if (x < p.offx || x > ex || y < p.offy || y > ey || z < p.offz || z > ez) inside = false;
+            s = s + (tnext - tcur) * rho;
+            if (s >= rr) {
+                t = (s > rr) ? tnext + (rr - s) / rho : tnext;
+                break;
+            }
+            if (!in) {
+                t = tnext + (rr - s) / RHO_OUTSIDE;
+                break;
+            }
+            tcur = tnext;
         }
     }
+    x = x + ux * t;
+    y = y + uy * t;
+    z = z + uz * t;
 }
 
 /* Pair emission (gPET_kernals.cu:536-555): isotropic direction, Gaussian
