@@ -94,6 +94,23 @@ GP_HD uint32_t gp_rng_u32(gp_rng *s)
     return i == 0u ? s->b0 : (i == 1u ? s->b1 : (i == 2u ? s->b2 : s->b3));
 }
 
+/* Skip to the next 4-word block boundary.  Called at the start of every
+ * sampling stage, so lanes whose earlier rejection loops ran for different
+ * numbers of iterations refill their blocks at the same draw sites (on a
+ * GPU a refill then runs once per wavefront instead of once per divergent
+ * lane group). */
+GP_HD void gp_rng_align(gp_rng *s) { s->n = (s->n + 3u) & ~3u; }
+
+/* The whole next block (after aligning): one generator call for up to four
+ * draws of a stage. */
+GP_HD gp_u4 gp_rng_block(gp_rng *s)
+{
+    gp_rng_align(s);
+    gp_u4 c = {s->n >> 2, s->aux, s->id_lo, s->id_hi};
+    s->n += 4u;
+    return gp_philox4x32_10(c, s->k0, s->k1);
+}
+
 /* Uniform in (0, 1]: 24 random bits, same range as curand_uniform. */
 GP_HD float gp_u01(uint32_t x)
 {

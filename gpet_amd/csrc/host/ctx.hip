@@ -58,6 +58,7 @@ struct gpet_ctx {
     /* phantom */
     bool has_ph = false;
     DBuf<int2> d_vox;
+    DBuf<float> d_dens_b;
     DBuf<float> d_wck_ph;
     std::vector<float> h_wck_ph;
     DevPhantom dph;
@@ -538,7 +539,7 @@ gpet_status gpet_destroy(gpet_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     dfree(c->d_mu4); dfree(c->d_surf[0]); dfree(c->d_surf[1]);
-    dfree(c->d_vox); dfree(c->d_wck_ph); dfree(c->d_panels); dfree(c->d_wck_det);
+    dfree(c->d_vox); dfree(c->d_dens_b); dfree(c->d_wck_ph); dfree(c->d_panels); dfree(c->d_wck_det);
     for (auto &b : c->q_f) dfree(b);
     dfree(c->q_t); dfree(c->q_info); dfree(c->q_slot); dfree(c->d_small); dfree(c->d_cnt);
     dfree(c->d_events); dfree(c->d_trace); dfree(c->d_hit_id); dfree(c->d_hit_f);
@@ -623,7 +624,19 @@ gpet_status gpet_set_phantom(gpet_ctx *c, const int32_t *mat, const float *dens,
     }
     gpet_status st = upload(c, c->d_vox, vox.data(), (int64_t)nv);
     if (st) return st;
+    const int nbx = (dims[0] + 3) / 4, nby = (dims[1] + 3) / 4, nbz = (dims[2] + 3) / 4;
+    {
+        std::vector<float> db((size_t)nbx * nby * nbz * 64, 0.0f);
+        for (int iz = 0; iz < dims[2]; ++iz)
+            for (int iy = 0; iy < dims[1]; ++iy)
+                for (int ix = 0; ix < dims[0]; ++ix)
+                    db[gpet_brick_offset(ix, iy, iz, nbx, nby)] = dens[((size_t)iz * dims[1] + iy) * dims[0] + ix];
+        if ((st = upload(c, c->d_dens_b, db.data(), (int64_t)db.size()))) return st;
+    }
     DevPhantom &p = c->dph;
+    p.dens_b = c->d_dens_b.p;
+    p.nbx = nbx;
+    p.nby = nby;
     build_majorant(c, rmax, c->h_wck_ph, p.wck.e0, p.wck.idl);
     if ((st = upload(c, c->d_wck_ph, c->h_wck_ph.data(), GPET_NWCK))) return st;
     p.wck.lam = c->d_wck_ph.p;

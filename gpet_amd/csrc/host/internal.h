@@ -38,9 +38,20 @@ struct DevPhantom {
     int nx, ny, nz;
     float offx, offy, offz, dx, dy, dz, idx, idy, idz;
     const int2 *vox;           /* {float bits of density, material}           */
+    const float *dens_b;       /* density in 4x4x4 bricks (positron walk)     */
+    int nbx, nby;              /* bricks along x and y                        */
     DevWck wck;
     int present;
 };
+
+/* Offset of voxel (ix, iy, iz) in the bricked density array: 64 voxels per
+ * brick, x fastest inside a brick, bricks x fastest.  A 128-byte line holds
+ * a 4 x 4 x 2 block, so the short random walks of positrons reuse lines. */
+__host__ __device__ __forceinline__ size_t gpet_brick_offset(int ix, int iy, int iz, int nbx, int nby)
+{
+    size_t b = ((size_t)(iz >> 2) * nby + (size_t)(iy >> 2)) * nbx + (size_t)(ix >> 2);
+    return (b << 6) | (size_t)(((iz & 3) << 4) | ((iy & 3) << 2) | (ix & 3));
+}
 
 struct DevPanel {
     int panel;

@@ -157,7 +157,6 @@ __device__ void d_positron_range(const DevPhantom &p, float &x, float &y, float 
         /* The crossing sequence does not depend on the densities: advance the
          * DDA four crossings at a time and issue their gathers together, then
          * accumulate in order (identical arithmetic, 4x memory parallelism). */
-        const int *dens = reinterpret_cast<const int *>(p.vox);
         float tcur = 0.0f, s = 0.0f;
         t = 0.0f;
         bool done = false;
@@ -171,7 +170,7 @@ __device__ void d_positron_range(const DevPhantom &p, float &x, float &y, float 
                 else if (tmy <= tmz) { tn[k] = tmy; tmy = tmy + tdy; iy += sy; }
                 else { tn[k] = tmz; tmz = tmz + tdz; iz += sz; }
                 in[k] = ix >= 0 && ix < p.nx && iy >= 0 && iy < p.ny && iz >= 0 && iz < p.nz;
-                g[k] = in[k] ? dens[2 * (((size_t)iz * p.ny + iy) * p.nx + ix)] : 0;
+                g[k] = in[k] ? __float_as_int(p.dens_b[gpet_brick_offset(ix, iy, iz, p.nbx, p.nby)]) : 0;
             }
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
@@ -224,8 +223,10 @@ __device__ bool d_phantom_walk(const DevTables &tab, const DevPhantom &p, float 
                                int &ncomp, int &nrayl, unsigned long long &nmaj)
 {
     for (int it = 0; it < D_STEP_CAP; ++it) {
+        /* one 4-word block per step: length, selector, scattering CDF, azimuth */
+        gp_u4 b = gp_rng_block(&r);
         float lammin = d_lam(p.wck, E);
-        float s = -lammin * gp_logf(gp_rng_uniform(&r));
+        float s = -lammin * gp_logf(gp_u01(b.x));
         x = x + s * vx;
         y = y + s * vy;
         z = z + s * vz;
@@ -240,13 +241,13 @@ __device__ bool d_phantom_walk(const DevTables &tab, const DevPhantom &p, float 
         float lamden = lammin * den;
         float prob = 1.0f - lamden * mu.x;
         if (prob < 0.0f) { prob = 0.0f; nmaj++; }
-        float rn = gp_rng_uniform(&r);
+        float rn = gp_u01(b.y);
         if (rn < prob) continue;
         prob = prob + lamden * mu.y;
         if (rn < prob) {
-            float cth = d_surf(tab, GPET_SURF_COMPTON, mat, E, gp_rng_uniform(&r));
+            float cth = d_surf(tab, GPET_SURF_COMPTON, mat, E, gp_u01(b.z));
             float efrac = 1.0f / (1.0f + (E * D_IMC2F) * (1.0f - cth));
-            float uphi = gp_rng_uniform(&r);
+            float uphi = gp_u01(b.w);
             ncomp++;
             E = E * efrac;
             if (E < eabs) return false;
@@ -255,8 +256,8 @@ __device__ bool d_phantom_walk(const DevTables &tab, const DevPhantom &p, float 
         }
         prob = prob + lamden * mu.z;
         if (rn < prob) {
-            float cth = d_surf(tab, GPET_SURF_RAYLEIGH, mat, E, gp_rng_uniform(&r));
-            float uphi = gp_rng_uniform(&r);
+            float cth = d_surf(tab, GPET_SURF_RAYLEIGH, mat, E, gp_u01(b.z));
+            float uphi = gp_u01(b.w);
             nrayl++;
             d_rotate(vx, vy, vz, cth, uphi);
             continue;
@@ -293,11 +294,15 @@ __global__ __launch_bounds__(256) void k_phantom(RunParams rp, DevTables tab, De
             tstart = ts * 1.0e6;
             float rb = gp_rng_uniform(&r);
             if (rb < I.ratio) {
+                gp_rng_align(&r);
                 d_sample_shape(S, r, px, py, pz);
                 if (rp.prange) {
+                    gp_rng_align(&r);
                     float ek = d_sample_ek(I, r);
+                    gp_rng_align(&r);
                     d_positron_range(ph, px, py, pz, ek, 0, 0.f, 0.f, 0.f, r, c_rs);
                 }
+                gp_rng_align(&r);
                 d_emit_pair(rp.sigma, r, v, e);
                 nph = 2;
             }
@@ -315,6 +320,7 @@ __global__ __launch_bounds__(256) void k_phantom(RunParams rp, DevTables tab, De
                 gp_rng r;
                 gp_rng_init(&r, rp.seed, (uint64_t)h, GP_TAG_DECAY, 0u);
                 if (rp.prange) d_positron_range(ph, px, py, pz, rE, 1, rvx, rvy, rvz, r, c_rs);
+                gp_rng_align(&r);
                 d_emit_pair(rp.sigma, r, v, e);
                 nph = 2;
             }
@@ -508,8 +514,11 @@ __global__ __launch_bounds__(256) void k_panel(PanelParams pp, DevTables tab, De
         int nev = 0;
         int nsteps = 0, nint = 0;
         for (int it = 0; it < D_STEP_CAP; ++it) {
+            /* block per step: length, selector, Rayleigh CDF, azimuth; the KN
+             * rejection loop draws after the block */
+            gp_u4 b = gp_rng_block(&r);
             float lammin = d_lam(det.wck, E);
-            float s = -lammin * gp_logf(gp_rng_uniform(&r));
+            float s = -lammin * gp_logf(gp_u01(b.x));
             x2 = x2 + s * vx2;
             y2 = y2 + s * vy2;
             z2 = z2 + s * vz2;
@@ -524,7 +533,7 @@ __global__ __launch_bounds__(256) void k_panel(PanelParams pp, DevTables tab, De
             float lamden = lammin * den;
             float prob = 1.0f - lamden * mu.x;
             if (prob < 0.0f) { prob = 0.0f; c_maj++; }
-            float rn = gp_rng_uniform(&r);
+            float rn = gp_u01(b.y);
             if (rn < prob) continue;
             nint++;
             prob = prob + lamden * mu.y;
@@ -538,7 +547,7 @@ __global__ __launch_bounds__(256) void k_panel(PanelParams pp, DevTables tab, De
                 float efrac;
                 d_kn(E, r, efrac, cth);
                 float de = E * (1.0f - efrac);
-                uphi = gp_rng_uniform(&r);
+                uphi = gp_u01(b.w);
                 type = 1;
                 edep = de;
                 E = E - de;
@@ -552,8 +561,8 @@ __global__ __launch_bounds__(256) void k_panel(PanelParams pp, DevTables tab, De
             } else {
                 prob = prob + lamden * mu.z;
                 if (rn < prob) {
-                    cth = d_surf(tab, GPET_SURF_RAYLEIGH, mat, E, gp_rng_uniform(&r));
-                    uphi = gp_rng_uniform(&r);
+                    cth = d_surf(tab, GPET_SURF_RAYLEIGH, mat, E, gp_u01(b.z));
+                    uphi = gp_u01(b.w);
                     type = pp.record_rayleigh ? 3 : 0;
                     rot = true;
                 } else {

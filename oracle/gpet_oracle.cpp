@@ -421,8 +421,10 @@ bool phantom_transport(const gpet_ctx &C, float &x, float &y, float &z, float &v
     const Phantom &p = C.ph;
     const Tables &t = C.tab;
     for (int it = 0; it < STEP_CAP; ++it) {
+        /* one 4-word block per step: length, selector, scattering CDF, azimuth */
+        gp_u4 b = gp_rng_block(&r);
         float lammin = lam_wck(p.wck, E);
-        float s = -lammin * gp_logf(gp_rng_uniform(&r));
+        float s = -lammin * gp_logf(gp_u01(b.x));
         x = x + s * vx;
         y = y + s * vy;
         z = z + s * vz;
@@ -437,13 +439,13 @@ bool phantom_transport(const gpet_ctx &C, float &x, float &y, float &z, float &v
         float lamden = lammin * den;
         float prob = 1.0f - lamden * mu_tab(t, GPET_TAB_LAMPH, mat, E);
         if (prob < 0.0f) { prob = 0.0f; cnt.majorant++; }
-        float rn = gp_rng_uniform(&r);
+        float rn = gp_u01(b.y);
         if (rn < prob) continue;
         prob = prob + lamden * mu_tab(t, GPET_TAB_COMPT, mat, E);
         if (rn < prob) {
-            float cth = surf_read(t, GPET_SURF_COMPTON, mat, E, gp_rng_uniform(&r));
+            float cth = surf_read(t, GPET_SURF_COMPTON, mat, E, gp_u01(b.z));
             float efrac = 1.0f / (1.0f + (E * IMC2F) * (1.0f - cth));
-            float uphi = gp_rng_uniform(&r);
+            float uphi = gp_u01(b.w);
             if (tr) tr->n_compton++;
             E = E * efrac;
             if (E < C.opt.eabs) return false;
@@ -452,8 +454,8 @@ bool phantom_transport(const gpet_ctx &C, float &x, float &y, float &z, float &v
         }
         prob = prob + lamden * mu_tab(t, GPET_TAB_RAYLE, mat, E);
         if (rn < prob) {
-            float cth = surf_read(t, GPET_SURF_RAYLEIGH, mat, E, gp_rng_uniform(&r));
-            float uphi = gp_rng_uniform(&r);
+            float cth = surf_read(t, GPET_SURF_RAYLEIGH, mat, E, gp_u01(b.z));
+            float uphi = gp_u01(b.w);
             if (tr) tr->n_rayleigh++;
             rotate_dir(vx, vy, vz, cth, uphi);
             continue;
@@ -519,8 +521,11 @@ void panel_transport(const gpet_ctx &C, float x, float y, float z, float vx, flo
         adder(ev, nev, e, cnt);
     };
     for (int it = 0; it < STEP_CAP; ++it) {
+        /* block per step: length, selector, Rayleigh CDF, azimuth; the KN
+         * rejection loop draws after the block */
+        gp_u4 b = gp_rng_block(&r);
         float lammin = lam_wck(d.wck, E);
-        float s = -lammin * gp_logf(gp_rng_uniform(&r));
+        float s = -lammin * gp_logf(gp_u01(b.x));
         x2 = x2 + s * vx2;
         y2 = y2 + s * vy2;
         z2 = z2 + s * vz2;
@@ -535,7 +540,7 @@ void panel_transport(const gpet_ctx &C, float x, float y, float z, float vx, flo
         float lamden = lammin * den;
         float prob = 1.0f - lamden * mu_tab(t, GPET_TAB_LAMPH, mat, E);
         if (prob < 0.0f) { prob = 0.0f; cnt.majorant++; }
-        float rn = gp_rng_uniform(&r);
+        float rn = gp_u01(b.y);
         if (rn < prob) continue;
         if (tr) tr->n_int_panel++;
         prob = prob + lamden * mu_tab(t, GPET_TAB_COMPT, mat, E);
@@ -543,7 +548,7 @@ void panel_transport(const gpet_ctx &C, float x, float y, float z, float vx, flo
             float efrac, cth;
             kn_sample(E, r, efrac, cth);
             float de = E * (1.0f - efrac);
-            float uphi = gp_rng_uniform(&r);
+            float uphi = gp_u01(b.w);
             if (m_id == 0) record(1, de, M_id, L_id, true);
             E = E - de;
             if (E < C.opt.eabs) {
@@ -555,8 +560,8 @@ void panel_transport(const gpet_ctx &C, float x, float y, float z, float vx, flo
         }
         prob = prob + lamden * mu_tab(t, GPET_TAB_RAYLE, mat, E);
         if (rn < prob) {
-            float cth = surf_read(t, GPET_SURF_RAYLEIGH, mat, E, gp_rng_uniform(&r));
-            float uphi = gp_rng_uniform(&r);
+            float cth = surf_read(t, GPET_SURF_RAYLEIGH, mat, E, gp_u01(b.z));
+            float uphi = gp_u01(b.w);
             if (C.dig.record_rayleigh && m_id == 0) record(3, 0.0f, M_id, L_id, false);
             rotate_dir(vx2, vy2, vz2, cth, uphi);
             continue;
@@ -849,12 +854,16 @@ void decay_history(const gpet_ctx &C, const SourcePlan &pl, const gpet_source *s
     }
     if (!(rb < I.beta_ratio)) return;
     float x, y, z;
+    gp_rng_align(&r);
     sample_shape(S, r, x, y, z);
     if (C.opt.positron_range) {
+        gp_rng_align(&r);
         float ek = sample_ek(I, r);
+        gp_rng_align(&r);
         positron_range(C, x, y, z, ek, 0, 0.0f, 0.0f, 0.0f, r, cnt);
     }
     float v[2][3], e[2];
+    gp_rng_align(&r);
     emit_pair(C.opt.noncol_sigma, r, v, e);
     for (int k = 0; k < 2; ++k)
         photon_full(C, h, k, x, y, z, v[k][0], v[k][1], v[k][2], e[k], tus, (int32_t)(2 * h + k),
@@ -884,6 +893,7 @@ void psf_history(const gpet_ctx &C, const double *rec, int ptype, int64_t h, gpe
     gp_rng_init(&r, C.opt.seed, (uint64_t)h, GP_TAG_DECAY, 0u);
     if (C.opt.positron_range) positron_range(C, x, y, z, E, 1, vx, vy, vz, r, cnt);
     float v[2][3], e[2];
+    gp_rng_align(&r);
     emit_pair(C.opt.noncol_sigma, r, v, e);
     for (int k = 0; k < 2; ++k)
         photon_full(C, h, k, x, y, z, v[k][0], v[k][1], v[k][2], e[k], t, (int32_t)(2 * h + k),
