@@ -782,6 +782,19 @@ gpet_status gpet_get_tallies(gpet_ctx *c, gpet_tallies *t)
         dfree(dc);
         dfree(dh);
     }
+    if (sg && nsb > 0 && nph > 0 && smax > 0.0f && c->has_det) {
+        DBuf<unsigned long long> ds;
+        gpet_status st;
+        const int64_t nb = (int64_t)nsb * nph;
+        if ((st = dalloc(c, ds, nb))) return st;
+        HIPOK(c, hipMemsetAsync(ds.p, 0, sizeof(unsigned long long) * nb, c->stream));
+        int e = gpet_digi_sinogram(c->dout.coinc, c->dout.n_coinc, c->ddet.p, c->ddet.npanels, c->ddet.moduleNy,
+                                   c->ddet.crystalNy, ds.p, nsb, nph, smax, c->stream);
+        if (e) return fail(c, GPET_ERR_DEVICE, "sinogram kernel");
+        HIPOK(c, hipMemcpyAsync(sg, ds.p, sizeof(int64_t) * nb, hipMemcpyDeviceToHost, c->stream));
+        HIPOK(c, hipStreamSynchronize(c->stream));
+        dfree(ds);
+    }
     return GPET_OK;
 }
 

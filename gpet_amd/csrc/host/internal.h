@@ -160,6 +160,8 @@ DigiWork *gpet_digi_work_create();
 void gpet_digi_work_destroy(DigiWork *w);
 int gpet_digitize(DigiWork *w, const DigiParams &p, const gpet_event *events, int64_t n,
                   DigiOut *out, hipStream_t s, char *err, int errlen);
+int gpet_digi_sinogram(const gpet_event *pairs, int64_t npairs, const DevPanel *panels, int npanels, int moduleNy,
+                       int crystalNy, unsigned long long *sino, int n_s, int n_phi, float s_max, hipStream_t s);
 int gpet_digi_tallies(DigiWork *w, const gpet_event *singles, int64_t n, int moduleN, int crystalN,
                       unsigned long long *crystal, int ncrystal, unsigned long long *ehist, int nebins,
                       float elo, float ehi, hipStream_t s);

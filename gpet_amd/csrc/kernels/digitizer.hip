@@ -369,6 +369,47 @@ __global__ __launch_bounds__(256) void k_tally(const gpet_event *s, int64_t n, i
     }
 }
 
+/* Sinogram of coincidence pairs: same rule as the oracle (crystal centres,
+ * transaxial line as (phi, s), uniform bins). */
+__device__ __forceinline__ void d_crystal_centre(const DevPanel &P, int modn, int cryn, int moduleNy, int crystalNy,
+                                                 float &gx, float &gy)
+{
+    int my = modn % moduleNy, mz = modn / moduleNy;
+    int ly = cryn % crystalNy, lz = cryn / crystalNy;
+    float xl = (P.dirx * P.lenx) * 0.5f;
+    float yl = (-P.leny * 0.5f + (float)my * (P.mody + P.mspy)) + ((float)ly * (P.cry + P.cspy) + P.cry * 0.5f);
+    float zl = (-P.lenz * 0.5f + (float)mz * (P.modz + P.mspz)) + ((float)lz * (P.crz + P.cspz) + P.crz * 0.5f);
+    gx = P.ox + xl * P.ux[0] + yl * P.uy[0] + zl * P.uz[0];
+    gy = P.oy + xl * P.ux[1] + yl * P.uy[1] + zl * P.uz[1];
+}
+
+__global__ __launch_bounds__(256) void k_sinogram(const gpet_event *pairs, int64_t npairs, const DevPanel *panels,
+                                                  int npanels, int moduleNy, int crystalNy, unsigned long long *sino,
+                                                  int n_s, int n_phi, float s_max)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npairs) return;
+    const gpet_event &a = pairs[2 * i], &b = pairs[2 * i + 1];
+    if (a.pann < 0 || a.pann >= npanels || b.pann < 0 || b.pann >= npanels) return;
+    float x1, y1, x2, y2;
+    d_crystal_centre(panels[a.pann], a.modn, a.cryn, moduleNy, crystalNy, x1, y1);
+    d_crystal_centre(panels[b.pann], b.modn, b.cryn, moduleNy, crystalNy, x2, y2);
+    float dx = x2 - x1, dy = y2 - y1;
+    if (dx == 0.0f && dy == 0.0f) return;
+    const float PIF = 3.14159274f;
+    float phi = gp_atan2f(dy, dx);
+    if (phi < 0.0f) phi = phi + PIF;
+    if (phi >= PIF) phi = phi - PIF;
+    float sp, cp;
+    gp_sincosf(phi, &sp, &cp);
+    float sd = x1 * sp - y1 * cp;
+    int ip = (int)(phi * ((float)n_phi / PIF));
+    if (ip > n_phi - 1) ip = n_phi - 1;
+    float fs = (sd + s_max) * ((float)n_s / (2.0f * s_max));
+    if (!(fs >= 0.0f && fs < (float)n_s)) return;
+    atomicAdd(&sino[(int64_t)ip * n_s + (int)fs], 1ull);
+}
+
 inline int grid_for(int64_t n, int block) { return (int)((n + block - 1) / block); }
 
 } // namespace
@@ -578,6 +619,15 @@ int gpet_digitize(DigiWork *w, const DigiParams &p, const gpet_event *events, in
     out->n_coinc_true = (int64_t)nt;
     out->coinc = w->coinc;
     return 0;
+}
+
+int gpet_digi_sinogram(const gpet_event *pairs, int64_t npairs, const DevPanel *panels, int npanels, int moduleNy,
+                       int crystalNy, unsigned long long *sino, int n_s, int n_phi, float s_max, hipStream_t s)
+{
+    if (npairs <= 0) return 0;
+    hipLaunchKernelGGL(k_sinogram, dim3(grid_for(npairs, 256)), dim3(256), 0, s, pairs, npairs, panels, npanels,
+                       moduleNy, crystalNy, sino, n_s, n_phi, s_max);
+    return (int)hipGetLastError();
 }
 
 int gpet_digi_tallies(DigiWork *w, const gpet_event *singles, int64_t n, int moduleN, int crystalN,

@@ -329,10 +329,16 @@ class Engine:
                  "run_psf")
         return self._result(s, bufs)
 
-    def tallies(self, n_crystals=0, n_ebins=0, e_lo=0.0, e_hi=1.0):
+    def tallies(self, n_crystals=0, n_ebins=0, e_lo=0.0, e_hi=1.0, n_s=0, n_phi=0, s_max=0.0, with_sinogram=False):
+        """Run statistics, and on request per-crystal singles, the singles
+        energy histogram and the coincidence sinogram [n_phi, n_s]."""
         t = capi.Tallies()
         cs = np.zeros(n_crystals, np.int64)
         eh = np.zeros(n_ebins, np.int64)
+        sg = np.zeros((n_phi, n_s), np.int64)
+        if n_s and n_phi and s_max > 0:
+            t.sinogram = sg.ctypes.data_as(C.c_void_p)
+            t.n_s, t.n_phi, t.s_max = n_s, n_phi, s_max
         t.crystal_singles = cs.ctypes.data_as(C.c_void_p) if n_crystals else None
         t.n_crystals = n_crystals
         t.energy_hist = eh.ctypes.data_as(C.c_void_p) if n_ebins else None
@@ -340,4 +346,6 @@ class Engine:
         t.e_lo = e_lo
         t.e_hi = e_hi
         self._ok(self.f["get_tallies"](self.ctx, C.byref(t)), "get_tallies")
+        if with_sinogram:
+            return t, cs, eh, sg
         return t, cs, eh

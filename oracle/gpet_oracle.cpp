@@ -39,6 +39,40 @@
 #include <omp.h>
 #endif
 
+/* Working precision of the transport.  The default build is FP32 (the GPU
+ * path's precision, bit-compatible with it); -DGPET_ORACLE_FP64 builds the
+ * same physics in FP64 with libm transcendentals, used as the reference of
+ * the FP32 error report.  Tables, densities and output records stay FP32. */
+#ifndef GPET_ORACLE_FP64
+typedef float real;
+#define R_LOG gp_logf
+#define R_EXP gp_expf
+#define R_SINCOS2PI gp_sincos2pi
+#define R_SINCOS gp_sincosf
+#define R_CBRT gp_cbrtf
+#define R_NORMAL gp_rng_normal
+#else
+typedef double real;
+#define R_LOG std::log
+#define R_EXP std::exp
+#define R_CBRT std::cbrt
+static inline void R_SINCOS2PI(double u, double *s, double *c)
+{
+    *s = std::sin(6.283185307179586 * u);
+    *c = std::cos(6.283185307179586 * u);
+}
+static inline void R_SINCOS(double x, double *s, double *c)
+{
+    *s = std::sin(x);
+    *c = std::cos(x);
+}
+static inline double R_NORMAL(gp_rng *r)
+{
+    double u1 = gp_rng_uniform(r), u2 = gp_rng_uniform(r);
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+}
+#endif
+
 namespace {
 
 constexpr double C_LIGHT = 29979.2458;   /* cm/us, gPET_kernals.cu:264        */
@@ -140,46 +174,46 @@ gpet_status fail(gpet_ctx *c, gpet_status s, const char *msg)
  * The reference binds the tables to linear-filtered textures and reads them at
  * i = idl*(E-e0)+0.5 (gPET_kernals.cu:31-64); here the same lerp is explicit
  * in FP32 (texture filtering's 9-bit weights are not reproduced). */
-inline float lerp1(const float *T, int n, float e0, float idl, float E)
+inline real lerp1(const float *T, int n, real e0, real idl, real E)
 {
-    float x = (E - e0) * idl;
-    x = std::fmin(std::fmax(x, 0.0f), (float)(n - 1));
+    real x = (E - e0) * idl;
+    x = std::fmin(std::fmax(x, 0.0f), (real)(n - 1));
     int j = (int)x;
     if (j > n - 2) j = n - 2;
-    float f = x - (float)j;
+    real f = x - (real)j;
     return T[j] + f * (T[j + 1] - T[j]);
 }
 
-inline float mu_tab(const Tables &t, int k, int mat, float E)
+inline real mu_tab(const Tables &t, int k, int mat, real E)
 {
     return lerp1(t.mu[k].data() + (size_t)mat * t.n1d, t.n1d, t.e0[k], t.idl[k], E);
 }
 
-inline float lam_wck(const Wck &w, float E)
+inline real lam_wck(const Wck &w, real E)
 {
     return lerp1(w.lam.data(), GPET_NWCK, w.e0, w.idl, E);
 }
 
 /* Bilinear read of an inverse-CDF surface: tex3D(s_tex, inde+0.5, indcp+0.5,
  * mat+0.5) of gPET_kernals.cu:66-88,128-146 with clamp addressing. */
-inline float surf_read(const Tables &t, int k, int mat, float E, float u)
+inline real surf_read(const Tables &t, int k, int mat, real E, real u)
 {
     const int ne = t.ne, ncp = t.ncp;
-    float x = E * t.ide[k];
-    x = std::fmin(std::fmax(x, 0.0f), (float)(ne - 1));
-    float y = u * t.idcp[k];
-    y = std::fmin(std::fmax(y, 0.0f), (float)(ncp - 1));
+    real x = E * t.ide[k];
+    x = std::fmin(std::fmax(x, 0.0f), (real)(ne - 1));
+    real y = u * t.idcp[k];
+    y = std::fmin(std::fmax(y, 0.0f), (real)(ncp - 1));
     int ix = (int)x;
     if (ix > ne - 2) ix = ne - 2;
     int iy = (int)y;
     if (iy > ncp - 2) iy = ncp - 2;
-    float fx = x - (float)ix, fy = y - (float)iy;
+    real fx = x - (real)ix, fy = y - (real)iy;
     const float *S = t.surf[k].data() + (size_t)mat * ncp * ne;
-    float v00 = S[iy * ne + ix], v01 = S[iy * ne + ix + 1];
-    float v10 = S[(iy + 1) * ne + ix], v11 = S[(iy + 1) * ne + ix + 1];
-    float a = v00 + fx * (v01 - v00);
-    float b = v10 + fx * (v11 - v10);
-    float v = a + fy * (b - a);
+    real v00 = S[iy * ne + ix], v01 = S[iy * ne + ix + 1];
+    real v10 = S[(iy + 1) * ne + ix], v11 = S[(iy + 1) * ne + ix + 1];
+    real a = v00 + fx * (v01 - v00);
+    real b = v10 + fx * (v11 - v10);
+    real v = a + fy * (b - a);
     return std::fmin(std::fmax(v, -1.0f), 1.0f);
 }
 
@@ -195,7 +229,7 @@ double mu_lin_max(const Tables &t, const float *rho_max, double E)
     double best = 0.0;
     for (int m = 0; m < t.nmat; ++m) {
         if (rho_max[m] <= 0.0f) continue;
-        double v = (double)mu_tab(t, GPET_TAB_LAMPH, m, (float)E) * rho_max[m];
+        double v = (double)(float)mu_tab(t, GPET_TAB_LAMPH, m, (float)E) * rho_max[m];
         if (v > best) best = v;
     }
     return best;
@@ -204,7 +238,7 @@ double mu_lin_max(const Tables &t, const float *rho_max, double E)
 void build_wck(const Tables &t, const float *rho_max, Wck &w)
 {
     w.e0 = t.eminph;
-    float de = (t.emax * (1.0f - 1.0e-10f) - w.e0) / (float)GPET_NWCK;
+    real de = (t.emax * (1.0f - 1.0e-10f) - w.e0) / (real)GPET_NWCK;
     w.idl = 1.0f / de;
     w.lam.assign(GPET_NWCK, 0.0f);
     const double d_node = 1.0 / (double)t.idl[GPET_TAB_LAMPH];
@@ -222,7 +256,7 @@ void build_wck(const Tables &t, const float *rho_max, Wck &w)
     for (int i = 0; i < GPET_NWCK; ++i) {
         double m = imax[i];
         if (i > 0) m = std::max(m, imax[i - 1]);
-        w.lam[i] = (m > 0.0) ? (float)(1.0 / (m * (1.0 + 1.0e-5))) : 1.0e20f;
+        w.lam[i] = (m > 0.0) ? (real)(1.0 / (m * (1.0 + 1.0e-5))) : 1.0e20f;
     }
 }
 
@@ -230,13 +264,13 @@ void build_wck(const Tables &t, const float *rho_max, Wck &w)
 
 /* getAbsVox (gPET_kernals.cu:19-29), Q1 fixed: voxel 0 is inside and
  * truncation toward zero no longer maps (-1,0) into the grid. */
-inline bool vox_index(const Phantom &p, float x, float y, float z, int &ix, int &iy, int &iz)
+inline bool vox_index(const Phantom &p, real x, real y, real z, int &ix, int &iy, int &iz)
 {
-    float fx = (x - p.offx) * p.idx;
-    float fy = (y - p.offy) * p.idy;
-    float fz = (z - p.offz) * p.idz;
-    if (!(fx >= 0.0f && fx < (float)p.nx && fy >= 0.0f && fy < (float)p.ny && fz >= 0.0f &&
-          fz < (float)p.nz))
+    real fx = (x - p.offx) * p.idx;
+    real fy = (y - p.offy) * p.idy;
+    real fz = (z - p.offz) * p.idz;
+    if (!(fx >= 0.0f && fx < (real)p.nx && fy >= 0.0f && fy < (real)p.ny && fz >= 0.0f &&
+          fz < (real)p.nz))
         return false;
     ix = (int)fx; iy = (int)fy; iz = (int)fz;
     return true;
@@ -245,28 +279,28 @@ inline bool vox_index(const Phantom &p, float x, float y, float z, int &ix, int 
 /* rotate (gPET_kernals.cu:172-254).  The azimuth arrives as a uniform u with
  * phi = 2 pi u.  rho2 is recomputed after a renormalisation (the reference
  * keeps the stale value). */
-inline void rotate_dir(float &u, float &v, float &w, float costh, float uphi)
+inline void rotate_dir(real &u, real &v, real &w, real costh, real uphi)
 {
-    float rho2 = u * u + v * v;
-    float norm = rho2 + w * w;
+    real rho2 = u * u + v * v;
+    real norm = rho2 + w * w;
     if (std::fabs(norm - 1.0f) > SZERO) {
         norm = 1.0f / std::sqrt(norm);
         u = u * norm; v = v * norm; w = w * norm;
         rho2 = u * u + v * v;
     }
-    float sinphi, cosphi;
-    gp_sincos2pi(uphi, &sinphi, &cosphi);
-    float temp = costh * costh;
+    real sinphi, cosphi;
+    R_SINCOS2PI(uphi, &sinphi, &cosphi);
+    real temp = costh * costh;
     if (rho2 > ZEROF) {
-        float sthrho = (temp < 1.0f) ? std::sqrt((1.0f - temp) / rho2) : 0.0f;
-        float urho = u * sthrho;
-        float vrho = v * sthrho;
-        float nu = u * costh - vrho * sinphi + (w * urho) * cosphi;
-        float nv = v * costh + urho * sinphi + (w * vrho) * cosphi;
-        float nw = w * costh - (rho2 * sthrho) * cosphi;
+        real sthrho = (temp < 1.0f) ? std::sqrt((1.0f - temp) / rho2) : 0.0f;
+        real urho = u * sthrho;
+        real vrho = v * sthrho;
+        real nu = u * costh - vrho * sinphi + (w * urho) * cosphi;
+        real nv = v * costh + urho * sinphi + (w * vrho) * cosphi;
+        real nw = w * costh - (rho2 * sthrho) * cosphi;
         u = nu; v = nv; w = nw;
     } else {
-        float sinth = (temp < 1.0f) ? std::sqrt(1.0f - temp) : 0.0f;
+        real sinth = (temp < 1.0f) ? std::sqrt(1.0f - temp) : 0.0f;
         v = sinth * sinphi;
         if (w > 0.0f) { u = sinth * cosphi; w = costh; }
         else { u = -sinth * cosphi; w = -costh; }
@@ -276,24 +310,24 @@ inline void rotate_dir(float &u, float &v, float &w, float costh, float uphi)
 /* rotate with sin(theta) given: used for the pair's noncollinearity, where
  * sqrt(1 - cos^2) in FP32 would round every |delta| < 3.5e-4 rad to a
  * collinear pair (the reference computes it that way). */
-inline void rotate_dir_cs(float &u, float &v, float &w, float costh, float sinth, float uphi)
+inline void rotate_dir_cs(real &u, real &v, real &w, real costh, real sinth, real uphi)
 {
-    float rho2 = u * u + v * v;
-    float norm = rho2 + w * w;
+    real rho2 = u * u + v * v;
+    real norm = rho2 + w * w;
     if (std::fabs(norm - 1.0f) > SZERO) {
         norm = 1.0f / std::sqrt(norm);
         u = u * norm; v = v * norm; w = w * norm;
         rho2 = u * u + v * v;
     }
-    float sinphi, cosphi;
-    gp_sincos2pi(uphi, &sinphi, &cosphi);
+    real sinphi, cosphi;
+    R_SINCOS2PI(uphi, &sinphi, &cosphi);
     if (rho2 > ZEROF) {
-        float sthrho = sinth / std::sqrt(rho2);
-        float urho = u * sthrho;
-        float vrho = v * sthrho;
-        float nu = u * costh - vrho * sinphi + (w * urho) * cosphi;
-        float nv = v * costh + urho * sinphi + (w * vrho) * cosphi;
-        float nw = w * costh - (rho2 * sthrho) * cosphi;
+        real sthrho = sinth / std::sqrt(rho2);
+        real urho = u * sthrho;
+        real vrho = v * sthrho;
+        real nu = u * costh - vrho * sinphi + (w * urho) * cosphi;
+        real nv = v * costh + urho * sinphi + (w * vrho) * cosphi;
+        real nw = w * costh - (rho2 * sthrho) * cosphi;
         u = nu; v = nv; w = nw;
     } else {
         v = sinth * sinphi;
@@ -303,50 +337,50 @@ inline void rotate_dir_cs(float &u, float &v, float &w, float costh, float sinth
 }
 
 /* Analytic Klein-Nishina sampling (gPET_kernals.cu:90-126), FP32. */
-inline void kn_sample(float E, gp_rng &r, float &efrac, float &costhe)
+inline void kn_sample(real E, gp_rng &r, real &efrac, real &costhe)
 {
-    float e0 = E * IMC2F;
-    float twoe = 2.0f * e0;
-    float kmin2 = 1.0f / ((1.0f + twoe) * (1.0f + twoe));
-    float loge = gp_logf(1.0f + twoe);
-    float a1 = loge + (twoe * (1.0f + e0)) * kmin2;
+    real e0 = E * IMC2F;
+    real twoe = 2.0f * e0;
+    real kmin2 = 1.0f / ((1.0f + twoe) * (1.0f + twoe));
+    real loge = R_LOG(1.0f + twoe);
+    real a1 = loge + (twoe * (1.0f + e0)) * kmin2;
     for (int it = 0; it < STEP_CAP; ++it) {
-        float u1 = gp_rng_uniform(&r);
-        float u2 = gp_rng_uniform(&r);
-        if (u1 * a1 < loge) efrac = gp_expf(-u2 * loge);
+        real u1 = gp_rng_uniform(&r);
+        real u2 = gp_rng_uniform(&r);
+        if (u1 * a1 < loge) efrac = R_EXP(-u2 * loge);
         else efrac = std::sqrt(kmin2 + u2 * (1.0f - kmin2));
-        float mess = ((e0 * e0) * efrac) * (1.0f + efrac * efrac);
-        float u3 = gp_rng_uniform(&r);
+        real mess = ((e0 * e0) * efrac) * (1.0f + efrac * efrac);
+        real u3 = gp_rng_uniform(&r);
         if (u3 * mess <= mess - (1.0f - efrac) * ((1.0f + twoe) * efrac - 1.0f)) break;
     }
     costhe = 1.0f - (1.0f - efrac) / (efrac * e0);
 }
 
 /* crystalSearch (gPET_kernals.cu:1236-1279): m_id 0 crystal, 1 gap/excluded. */
-inline int crystal_search(const Detector &d, const gpet_panel &P, float x, float y, float z,
+inline int crystal_search(const Detector &d, const gpet_panel &P, real x, real y, real z,
                           int &M_id, int &L_id)
 {
     for (int q = 0; q < d.nquad; ++q) {
         const float *s = &d.quad[q * 10];
-        float v = s[0] * x * x + s[1] * y * y + s[2] * z * z + s[3] * x * y + s[4] * x * z +
+        real v = s[0] * x * x + s[1] * y * y + s[2] * z * z + s[3] * x * y + s[4] * x * z +
                   s[5] * y * z + s[6] * x + s[7] * y + s[8] * z + s[9];
         if (v < 0.0f) return 1;
     }
-    float yy = P.leny / 2.0f + y;
-    float zz = P.lenz / 2.0f + z;
-    float mpy = P.mody + P.mspy, mpz = P.modz + P.mspz;
+    real yy = P.leny / 2.0f + y;
+    real zz = P.lenz / 2.0f + z;
+    real mpy = P.mody + P.mspy, mpz = P.modz + P.mspz;
     int my = std::floor(yy / mpy) > 0.0f ? (int)(yy / mpy) : 0;
     int mz = std::floor(zz / mpz) > 0.0f ? (int)(zz / mpz) : 0;
     M_id = mz * d.moduleNy + my;
-    yy = yy - (float)my * mpy;
-    zz = zz - (float)mz * mpz;
+    yy = yy - (real)my * mpy;
+    zz = zz - (real)mz * mpz;
     if (yy > P.mody || zz > P.modz) return 1;
-    float cpy = P.cry + P.cspy, cpz = P.crz + P.cspz;
+    real cpy = P.cry + P.cspy, cpz = P.crz + P.cspz;
     int ly = std::floor(yy / cpy) > 0.0f ? (int)(yy / cpy) : 0;
     int lz = std::floor(zz / cpz) > 0.0f ? (int)(zz / cpz) : 0;
     L_id = lz * d.crystalNy + ly;
-    yy = yy - (float)ly * cpy;
-    zz = zz - (float)lz * cpz;
+    yy = yy - (real)ly * cpy;
+    zz = zz - (real)lz * cpz;
     if (yy > P.cry || zz > P.crz) return 1;
     return 0;
 }
@@ -356,7 +390,7 @@ inline void adder(gpet_event *ev, int &n, const gpet_event &e, Counters &c)
 {
     for (int i = 0; i < n; ++i) {
         if (ev[i].siten == e.siten) {
-            float sE = ev[i].E + e.E;
+            real sE = ev[i].E + e.E;
             ev[i].x = (ev[i].x * ev[i].E + e.x * e.E) / sE;
             ev[i].y = (ev[i].y * ev[i].E + e.y * e.E) / sE;
             ev[i].z = (ev[i].z * ev[i].E + e.z * e.E) / sE;
@@ -387,7 +421,7 @@ inline int readout(gpet_event *ev, int n, int depth, int policy, int moduleN)
             gpet_event e = ev[j];
             if (e.parn == e0.parn && e.siten == e0.siten) {
                 if (policy == 1) {
-                    float sE = e0.E + e.E;
+                    real sE = e0.E + e.E;
                     e0.x = (e0.x * e0.E + e.x * e.E) / sE;
                     e0.y = (e0.y * e0.E + e.y * e.E) / sE;
                     e0.z = (e0.z * e0.E + e.z * e.E) / sE;
@@ -415,16 +449,16 @@ struct PhotonOut {
 /* photon (gPET_kernals.cu:256-345): Woodcock tracking in the phantom.
  * Returns false if the photon is absorbed.  Q10 fixed: a Compton event that
  * leaves less than eabs absorbs the photon. */
-bool phantom_transport(const gpet_ctx &C, float &x, float &y, float &z, float &vx, float &vy,
-                       float &vz, float &E, double &tof, gp_rng &r, gpet_trace *tr, Counters &cnt)
+bool phantom_transport(const gpet_ctx &C, real &x, real &y, real &z, real &vx, real &vy,
+                       real &vz, real &E, double &tof, gp_rng &r, gpet_trace *tr, Counters &cnt)
 {
     const Phantom &p = C.ph;
     const Tables &t = C.tab;
     for (int it = 0; it < STEP_CAP; ++it) {
         /* one 4-word block per step: length, selector, scattering CDF, azimuth */
         gp_u4 b = gp_rng_block(&r);
-        float lammin = lam_wck(p.wck, E);
-        float s = -lammin * gp_logf(gp_u01(b.x));
+        real lammin = lam_wck(p.wck, E);
+        real s = -lammin * R_LOG(gp_u01(b.x));
         x = x + s * vx;
         y = y + s * vy;
         z = z + s * vz;
@@ -434,18 +468,18 @@ bool phantom_transport(const gpet_ctx &C, float &x, float &y, float &z, float &v
         if (tr) tr->n_steps++;
         cnt.phantom_steps++;
         size_t vid = ((size_t)iz * p.ny + iy) * p.nx + ix;
-        float den = p.dens[vid];
+        real den = p.dens[vid];
         int mat = p.mat[vid];
-        float lamden = lammin * den;
-        float prob = 1.0f - lamden * mu_tab(t, GPET_TAB_LAMPH, mat, E);
+        real lamden = lammin * den;
+        real prob = 1.0f - lamden * mu_tab(t, GPET_TAB_LAMPH, mat, E);
         if (prob < 0.0f) { prob = 0.0f; cnt.majorant++; }
-        float rn = gp_u01(b.y);
+        real rn = gp_u01(b.y);
         if (rn < prob) continue;
         prob = prob + lamden * mu_tab(t, GPET_TAB_COMPT, mat, E);
         if (rn < prob) {
-            float cth = surf_read(t, GPET_SURF_COMPTON, mat, E, gp_u01(b.z));
-            float efrac = 1.0f / (1.0f + (E * IMC2F) * (1.0f - cth));
-            float uphi = gp_u01(b.w);
+            real cth = surf_read(t, GPET_SURF_COMPTON, mat, E, gp_u01(b.z));
+            real efrac = 1.0f / (1.0f + (E * IMC2F) * (1.0f - cth));
+            real uphi = gp_u01(b.w);
             if (tr) tr->n_compton++;
             E = E * efrac;
             if (E < C.opt.eabs) return false;
@@ -454,8 +488,8 @@ bool phantom_transport(const gpet_ctx &C, float &x, float &y, float &z, float &v
         }
         prob = prob + lamden * mu_tab(t, GPET_TAB_RAYLE, mat, E);
         if (rn < prob) {
-            float cth = surf_read(t, GPET_SURF_RAYLEIGH, mat, E, gp_u01(b.z));
-            float uphi = gp_u01(b.w);
+            real cth = surf_read(t, GPET_SURF_RAYLEIGH, mat, E, gp_u01(b.z));
+            real uphi = gp_u01(b.w);
             if (tr) tr->n_rayleigh++;
             rotate_dir(vx, vy, vz, cth, uphi);
             continue;
@@ -467,27 +501,27 @@ bool phantom_transport(const gpet_ctx &C, float &x, float &y, float &z, float &v
 
 /* photonde (gPET_kernals.cu:839-1233) for one photon: panel entry, Woodcock
  * in the panel, hit records, adder, readout. */
-void panel_transport(const gpet_ctx &C, float x, float y, float z, float vx, float vy, float vz,
-                     float E, double tof, int32_t parn, int32_t eid, gp_rng &r, gpet_trace *tr,
+void panel_transport(const gpet_ctx &C, real x, real y, real z, real vx, real vy, real vz,
+                     real E, double tof, int32_t parn, int32_t eid, gp_rng &r, gpet_trace *tr,
                      PhotonOut &out, Counters &cnt, bool want_hits)
 {
     const Detector &d = C.det;
     const Tables &t = C.tab;
     int pa = -1;
-    float x2 = 0, y2 = 0, z2 = 0, vx2 = 0, vy2 = 0, vz2 = 0;
+    real x2 = 0, y2 = 0, z2 = 0, vx2 = 0, vy2 = 0, vz2 = 0;
     const int np = (int)d.p.size();
     for (int i = 0; i < np; ++i) {
         const gpet_panel &P = d.p[i];
-        float rx = x - P.ox, ry = y - P.oy, rz = z - P.oz;
-        float tx = rx * P.ux[0] + ry * P.ux[1] + rz * P.ux[2];
-        float ty = rx * P.uy[0] + ry * P.uy[1] + rz * P.uy[2];
-        float tz = rx * P.uz[0] + ry * P.uz[1] + rz * P.uz[2];
-        float tvx = vx * P.ux[0] + vy * P.ux[1] + vz * P.ux[2];
-        float tvy = vx * P.uy[0] + vy * P.uy[1] + vz * P.uy[2];
-        float tvz = vx * P.uz[0] + vy * P.uz[1] + vz * P.uz[2];
+        real rx = x - P.ox, ry = y - P.oy, rz = z - P.oz;
+        real tx = rx * P.ux[0] + ry * P.ux[1] + rz * P.ux[2];
+        real ty = rx * P.uy[0] + ry * P.uy[1] + rz * P.uy[2];
+        real tz = rx * P.uz[0] + ry * P.uz[1] + rz * P.uz[2];
+        real tvx = vx * P.ux[0] + vy * P.ux[1] + vz * P.ux[2];
+        real tvy = vx * P.uy[0] + vy * P.uy[1] + vz * P.uy[2];
+        real tvz = vx * P.uz[0] + vy * P.uz[1] + vz * P.uz[2];
         if (tvx * P.dirx >= 0.0f) {
-            float ty2 = ty - (tx / tvx) * tvy;
-            float tz2 = tz - (tx / tvx) * tvz;
+            real ty2 = ty - (tx / tvx) * tvy;
+            real tz2 = tz - (tx / tvx) * tvz;
             if (std::fabs(ty2) < P.leny / 2.0f && std::fabs(tz2) < P.lenz / 2.0f) {
                 x2 = 0.0f; y2 = ty2; z2 = tz2;
                 tof = tof + (double)(-tx) / (C_LIGHT * (double)tvx);
@@ -502,15 +536,15 @@ void panel_transport(const gpet_ctx &C, float x, float y, float z, float vx, flo
     cnt.panel_entries++;
     const gpet_panel &P = d.p[pa];
     const int pann = P.panel;
-    const float hx = P.lenx, hy = P.leny * 0.5f, hz = P.lenz * 0.5f;
+    const real hx = P.lenx, hy = P.leny * 0.5f, hz = P.lenz * 0.5f;
     int nev = 0;
     gpet_event *ev = out.ev;
-    auto record = [&](int type, float edep, int M_id, int L_id, bool to_adder) {
+    auto record = [&](int type, real edep, int M_id, int L_id, bool to_adder) {
         cnt.hits++;
         if (want_hits) {
             HitRec h;
             h.id[0] = parn; h.id[1] = pann; h.id[2] = M_id; h.id[3] = L_id; h.id[4] = type;
-            h.f[0] = edep; h.f[1] = (float)tof; h.f[2] = x2; h.f[3] = y2; h.f[4] = z2;
+            h.f[0] = edep; h.f[1] = (real)tof; h.f[2] = x2; h.f[3] = y2; h.f[4] = z2;
             out.hits.push_back(h);
         }
         if (!to_adder) return;
@@ -524,8 +558,8 @@ void panel_transport(const gpet_ctx &C, float x, float y, float z, float vx, flo
         /* block per step: length, selector, Rayleigh CDF, azimuth; the KN
          * rejection loop draws after the block */
         gp_u4 b = gp_rng_block(&r);
-        float lammin = lam_wck(d.wck, E);
-        float s = -lammin * gp_logf(gp_u01(b.x));
+        real lammin = lam_wck(d.wck, E);
+        real s = -lammin * R_LOG(gp_u01(b.x));
         x2 = x2 + s * vx2;
         y2 = y2 + s * vy2;
         z2 = z2 + s * vz2;
@@ -535,20 +569,20 @@ void panel_transport(const gpet_ctx &C, float x, float y, float z, float vx, flo
         cnt.panel_steps++;
         int M_id = -1, L_id = -1;
         int m_id = crystal_search(d, P, x2, y2, z2, M_id, L_id);
-        float den = d.mats[m_id].density;
+        real den = d.mats[m_id].density;
         int mat = d.mats[m_id].material;
-        float lamden = lammin * den;
-        float prob = 1.0f - lamden * mu_tab(t, GPET_TAB_LAMPH, mat, E);
+        real lamden = lammin * den;
+        real prob = 1.0f - lamden * mu_tab(t, GPET_TAB_LAMPH, mat, E);
         if (prob < 0.0f) { prob = 0.0f; cnt.majorant++; }
-        float rn = gp_u01(b.y);
+        real rn = gp_u01(b.y);
         if (rn < prob) continue;
         if (tr) tr->n_int_panel++;
         prob = prob + lamden * mu_tab(t, GPET_TAB_COMPT, mat, E);
         if (rn < prob) {
-            float efrac, cth;
+            real efrac, cth;
             kn_sample(E, r, efrac, cth);
-            float de = E * (1.0f - efrac);
-            float uphi = gp_u01(b.w);
+            real de = E * (1.0f - efrac);
+            real uphi = gp_u01(b.w);
             if (m_id == 0) record(1, de, M_id, L_id, true);
             E = E - de;
             if (E < C.opt.eabs) {
@@ -560,8 +594,8 @@ void panel_transport(const gpet_ctx &C, float x, float y, float z, float vx, flo
         }
         prob = prob + lamden * mu_tab(t, GPET_TAB_RAYLE, mat, E);
         if (rn < prob) {
-            float cth = surf_read(t, GPET_SURF_RAYLEIGH, mat, E, gp_u01(b.z));
-            float uphi = gp_u01(b.w);
+            real cth = surf_read(t, GPET_SURF_RAYLEIGH, mat, E, gp_u01(b.z));
+            real uphi = gp_u01(b.w);
             if (C.dig.record_rayleigh && m_id == 0) record(3, 0.0f, M_id, L_id, false);
             rotate_dir(vx2, vy2, vz2, cth, uphi);
             continue;
@@ -575,8 +609,8 @@ void panel_transport(const gpet_ctx &C, float x, float y, float z, float vx, flo
 }
 
 /* Phantom + panel for one photon, filling the trace record. */
-void photon_full(const gpet_ctx &C, int64_t hist, int k, float x, float y, float z, float vx,
-                 float vy, float vz, float E, double t, int32_t parn, gpet_trace *tr,
+void photon_full(const gpet_ctx &C, int64_t hist, int k, real x, real y, real z, real vx,
+                 real vy, real vz, real E, double t, int32_t parn, gpet_trace *tr,
                  PhotonOut &out, Counters &cnt, bool want_hits)
 {
     gp_rng r;
@@ -604,13 +638,13 @@ void photon_full(const gpet_ctx &C, int64_t hist, int k, float x, float y, float
 
 /* sampleEkPositron (gPET_kernals.cu:420-443): rejection from the fitted
  * spectrum; Horner form of the degree-5 polynomial; returns kinetic eV. */
-float sample_ek(const gpet_isotope &iso, gp_rng &r)
+real sample_ek(const gpet_isotope &iso, gp_rng &r)
 {
-    float E = 0.511f;
+    real E = 0.511f;
     for (int it = 0; it < 100000; ++it) {
         E = gp_rng_uniform(&r) * (iso.coef[0] - 0.511f) + 0.511f;
-        float u = iso.coef[1] * gp_rng_uniform(&r);
-        float p = iso.coef[2];
+        real u = iso.coef[1] * gp_rng_uniform(&r);
+        real p = iso.coef[2];
         for (int i = 3; i < 8; ++i) p = p * E + iso.coef[i];
         if (u <= p) break;
     }
@@ -618,7 +652,7 @@ float sample_ek(const gpet_isotope &iso, gp_rng &r)
 }
 
 /* getPositionFromShape (gPET_kernals.cu:445-482). */
-void sample_shape(const gpet_source &s, gp_rng &r, float &x, float &y, float &z)
+void sample_shape(const gpet_source &s, gp_rng &r, real &x, real &y, real &z)
 {
     const float *c = s.coef;
     int shape = (s.shape < 0 || s.shape > 2) ? 0 : s.shape;
@@ -627,18 +661,18 @@ void sample_shape(const gpet_source &s, gp_rng &r, float &x, float &y, float &z)
         y = c[1] + (c[4] * (2.0f * gp_rng_uniform(&r) - 1.0f)) * 0.5f;
         z = c[2] + (c[5] * (2.0f * gp_rng_uniform(&r) - 1.0f)) * 0.5f;
     } else if (shape == 1) {
-        float sp, cp;
-        gp_sincos2pi(gp_rng_uniform(&r), &sp, &cp);
-        float rr = c[3] * std::sqrt(gp_rng_uniform(&r));
+        real sp, cp;
+        R_SINCOS2PI(gp_rng_uniform(&r), &sp, &cp);
+        real rr = c[3] * std::sqrt(gp_rng_uniform(&r));
         x = c[0] + rr * cp;
         y = c[1] + rr * sp;
         z = c[2] + (c[4] * (2.0f * gp_rng_uniform(&r) - 1.0f)) * 0.5f;
     } else {
-        float sp, cp;
-        gp_sincos2pi(gp_rng_uniform(&r), &sp, &cp);
-        float ct = 2.0f * gp_rng_uniform(&r) - 1.0f;
-        float rr = c[3] * gp_cbrtf(gp_rng_uniform(&r));
-        float st = std::sqrt(1.0f - ct * ct);
+        real sp, cp;
+        R_SINCOS2PI(gp_rng_uniform(&r), &sp, &cp);
+        real ct = 2.0f * gp_rng_uniform(&r) - 1.0f;
+        real rr = c[3] * R_CBRT(gp_rng_uniform(&r));
+        real st = std::sqrt(1.0f - ct * ct);
         x = c[0] + (rr * st) * cp;
         y = c[1] + (rr * st) * sp;
         z = c[2] + rr * ct;
@@ -656,16 +690,16 @@ void sample_shape(const gpet_source &s, gp_rng &r, float &x, float &y, float &z)
  * RHO_OUTSIDE and the remaining range is covered at RHO_OUTSIDE (the
  * reference reads the clamped border voxel and then adds the remainder to
  * a leftover 1000-unit step). */
-inline void dda_axis(float x, float off, float d, int i, float u, float &tmax, float &tdelta, int &step)
+inline void dda_axis(real x, real off, real d, int i, real u, real &tmax, real &tdelta, int &step)
 {
     if (u > 0.0f) {
-        float inv = 1.0f / u;
-        tmax = (off + (float)(i + 1) * d - x) * inv;
+        real inv = 1.0f / u;
+        tmax = (off + (real)(i + 1) * d - x) * inv;
         tdelta = d * inv;
         step = 1;
     } else if (u < 0.0f) {
-        float inv = 1.0f / u;
-        tmax = (off + (float)i * d - x) * inv;
+        real inv = 1.0f / u;
+        tmax = (off + (real)i * d - x) * inv;
         tdelta = -(d * inv);
         step = -1;
     } else {
@@ -675,45 +709,45 @@ inline void dda_axis(float x, float off, float d, int i, float u, float &tmax, f
     }
 }
 
-void positron_range(const gpet_ctx &C, float &x, float &y, float &z, float ek_ev, int usedir,
-                    float vx, float vy, float vz, gp_rng &r, Counters &cnt)
+void positron_range(const gpet_ctx &C, real &x, real &y, real &z, real ek_ev, int usedir,
+                    real vx, real vy, real vz, gp_rng &r, Counters &cnt)
 {
     const Phantom &p = C.ph;
-    float ekin = ek_ev / 1.0e6f;
-    float rex = ((0.1f * 5.44040782f) * ekin * ekin) / (0.369516529f + ekin);
-    float sigma = rex / 2.0f;
-    float dx = sigma * gp_rng_normal(&r);
-    float dy = sigma * gp_rng_normal(&r);
-    float dz = sigma * gp_rng_normal(&r);
-    float rr = std::sqrt(dx * dx + dy * dy + dz * dz);
+    real ekin = ek_ev / 1.0e6f;
+    real rex = ((0.1f * 5.44040782f) * ekin * ekin) / (0.369516529f + ekin);
+    real sigma = rex / 2.0f;
+    real dx = sigma * R_NORMAL(&r);
+    real dy = sigma * R_NORMAL(&r);
+    real dz = sigma * R_NORMAL(&r);
+    real rr = std::sqrt(dx * dx + dy * dy + dz * dz);
     if (usedir) {
-        float tmp = std::sqrt(vx * vx + vy * vy + vz * vz);
+        real tmp = std::sqrt(vx * vx + vy * vy + vz * vz);
         dx = rr * vx / tmp;
         dy = rr * vy / tmp;
         dz = rr * vz / tmp;
     }
     if (!(rr > 0.0f)) return;
-    float inv_rr = 1.0f / rr;
-    float ux = dx * inv_rr, uy = dy * inv_rr, uz = dz * inv_rr;
+    real inv_rr = 1.0f / rr;
+    real ux = dx * inv_rr, uy = dy * inv_rr, uz = dz * inv_rr;
     int ix = 0, iy = 0, iz = 0;
-    float t;
+    real t;
     if (!(C.has_ph && vox_index(p, x, y, z, ix, iy, iz))) {
         t = rr / RHO_OUTSIDE;
     } else {
-        float tmx, tmy, tmz, tdx, tdy, tdz;
+        real tmx, tmy, tmz, tdx, tdy, tdz;
         int sx, sy, sz;
         dda_axis(x, p.offx, p.dx, ix, ux, tmx, tdx, sx);
         dda_axis(y, p.offy, p.dy, iy, uy, tmy, tdy, sy);
         dda_axis(z, p.offz, p.dz, iz, uz, tmz, tdz, sz);
-        float tcur = 0.0f, s = 0.0f;
+        real tcur = 0.0f, s = 0.0f;
         t = 0.0f;
         for (int it = 0; it < STEP_CAP; ++it) {
-            float tnext;
+            real tnext;
             if (tmx <= tmy && tmx <= tmz) { tnext = tmx; tmx = tmx + tdx; ix += sx; }
             else if (tmy <= tmz) { tnext = tmy; tmy = tmy + tdy; iy += sy; }
             else { tnext = tmz; tmz = tmz + tdz; iz += sz; }
             bool in = ix >= 0 && ix < p.nx && iy >= 0 && iy < p.ny && iz >= 0 && iz < p.nz;
-            float rho = RHO_OUTSIDE;
+            real rho = RHO_OUTSIDE;
             if (in) {
                 rho = p.dens[((size_t)iz * p.ny + iy) * p.nx + ix];
                 cnt.range_steps++;
@@ -738,20 +772,20 @@ void positron_range(const gpet_ctx &C, float &x, float &y, float &z, float ek_ev
 /* Pair emission (gPET_kernals.cu:536-555): isotropic direction, Gaussian
  * noncollinearity delta; photon 1 along d with E = mc2 + delta mc2/2,
  * photon 2 rotated by (cos = -cos delta, phi) with E = mc2 - delta mc2/2. */
-void emit_pair(float sigma, gp_rng &r, float v[2][3], float e[2])
+void emit_pair(real sigma, gp_rng &r, real v[2][3], real e[2])
 {
-    float ct = 2.0f * gp_rng_uniform(&r) - 1.0f;
-    float sp, cp;
-    gp_sincos2pi(gp_rng_uniform(&r), &sp, &cp);
-    float st = std::sqrt(1.0f - ct * ct);
+    real ct = 2.0f * gp_rng_uniform(&r) - 1.0f;
+    real sp, cp;
+    R_SINCOS2PI(gp_rng_uniform(&r), &sp, &cp);
+    real st = std::sqrt(1.0f - ct * ct);
     v[0][0] = st * cp; v[0][1] = st * sp; v[0][2] = ct;
-    float uphi = gp_rng_uniform(&r);
-    float delta = gp_rng_normal(&r) * sigma;
+    real uphi = gp_rng_uniform(&r);
+    real delta = R_NORMAL(&r) * sigma;
     e[0] = MC2F + (delta * MC2F) / 2.0f;
     e[1] = MC2F - (delta * MC2F) / 2.0f;
     v[1][0] = v[0][0]; v[1][1] = v[0][1]; v[1][2] = v[0][2];
-    float sd, cd;
-    gp_sincosf(delta, &sd, &cd);
+    real sd, cd;
+    R_SINCOS(delta, &sd, &cd);
     rotate_dir_cs(v[1][0], v[1][1], v[1][2], -cd, std::fabs(sd), uphi);
 }
 
@@ -846,23 +880,23 @@ void decay_history(const gpet_ctx &C, const SourcePlan &pl, const gpet_source *s
     double u = gp_rng_uniform_d(&r);
     double ts = t0 + pl.tau[si] * (-gp_log1p_d(-u * pl.pdec[si]));
     double tus = ts * 1.0e6;
-    float rb = gp_rng_uniform(&r);
+    real rb = gp_rng_uniform(&r);
     if (tr2) {
         for (int k = 0; k < 2; ++k) {
             tr2[k].history = h; tr2[k].photon = k; tr2[k].panel = -1; tr2[k].t0 = tus;
         }
     }
     if (!(rb < I.beta_ratio)) return;
-    float x, y, z;
+    real x, y, z;
     gp_rng_align(&r);
     sample_shape(S, r, x, y, z);
     if (C.opt.positron_range) {
         gp_rng_align(&r);
-        float ek = sample_ek(I, r);
+        real ek = sample_ek(I, r);
         gp_rng_align(&r);
         positron_range(C, x, y, z, ek, 0, 0.0f, 0.0f, 0.0f, r, cnt);
     }
-    float v[2][3], e[2];
+    real v[2][3], e[2];
     gp_rng_align(&r);
     emit_pair(C.opt.noncol_sigma, r, v, e);
     for (int k = 0; k < 2; ++k)
@@ -876,9 +910,9 @@ void decay_history(const gpet_ctx &C, const SourcePlan &pl, const gpet_source *s
 void psf_history(const gpet_ctx &C, const double *rec, int ptype, int64_t h, gpet_trace *tr2,
                  PhotonOut out[2], Counters &cnt, bool want_hits)
 {
-    float x = (float)rec[0], y = (float)rec[1], z = (float)rec[2];
+    real x = (real)rec[0], y = (real)rec[1], z = (real)rec[2];
     double t = rec[3];
-    float vx = (float)rec[4], vy = (float)rec[5], vz = (float)rec[6], E = (float)rec[7];
+    real vx = (real)rec[4], vy = (real)rec[5], vz = (real)rec[6], E = (real)rec[7];
     if (tr2) {
         for (int k = 0; k < 2; ++k) {
             tr2[k].history = h; tr2[k].photon = k; tr2[k].panel = -1; tr2[k].t0 = t;
@@ -892,7 +926,7 @@ void psf_history(const gpet_ctx &C, const double *rec, int ptype, int64_t h, gpe
     gp_rng r;
     gp_rng_init(&r, C.opt.seed, (uint64_t)h, GP_TAG_DECAY, 0u);
     if (C.opt.positron_range) positron_range(C, x, y, z, E, 1, vx, vy, vz, r, cnt);
-    float v[2][3], e[2];
+    real v[2][3], e[2];
     gp_rng_align(&r);
     emit_pair(C.opt.noncol_sigma, r, v, e);
     for (int k = 0; k < 2; ++k)
@@ -924,19 +958,19 @@ void blur_event(const gpet_ctx &C, gpet_event &e)
     const gpet_digitizer &g = C.dig;
     gp_rng r;
     gp_rng_init(&r, C.opt.seed, (uint64_t)(uint32_t)e.parn, (uint32_t)GP_TAG_DIGI | ((uint32_t)e.siten << 4), 0u);
-    float R = 0.0f;
+    real R = 0.0f;
     if (g.blur_policy == 0) R = std::sqrt(g.e_ref / e.E) * g.r_ref;
     if (g.blur_policy == 1) R = g.r_ref + g.slope * (e.E - g.e_ref) / 1.0e6f;
     if (R < 0.0f) R = 0.0f;
-    float n = gp_rng_normal(&r);
+    real n = R_NORMAL(&r);
     e.E = e.E + ((n * R) * e.E) / 2.35482f;
     if (g.space_blur > 0.0f) {
-        e.x = e.x + g.space_blur * gp_rng_normal(&r);
-        e.y = e.y + g.space_blur * gp_rng_normal(&r);
-        e.z = e.z + g.space_blur * gp_rng_normal(&r);
+        e.x = e.x + g.space_blur * R_NORMAL(&r);
+        e.y = e.y + g.space_blur * R_NORMAL(&r);
+        e.z = e.z + g.space_blur * R_NORMAL(&r);
     }
     if (g.time_blur_fwhm_us > 0.0) {
-        float nt = gp_rng_normal(&r);
+        real nt = R_NORMAL(&r);
         e.t = e.t + (double)nt * (g.time_blur_fwhm_us / 2.35482);
     }
 }
@@ -1025,6 +1059,44 @@ void run_digitizer(gpet_ctx &C, Result &R)
             i = j;
         }
     }
+}
+
+/* Sinogram bin of a coincidence (new; the reference has no coincidences):
+ * crystal centres of both singles in global coordinates, the line through
+ * them in the transaxial (x, y) plane as (phi in [0, pi), signed distance s
+ * = x1 sin(phi) - y1 cos(phi)), binned uniformly; -1 outside the grid. */
+inline void crystal_centre(const Detector &d, const gpet_event &e, float &gx, float &gy)
+{
+    const gpet_panel &P = d.p[e.pann];
+    int my = e.modn % d.moduleNy, mz = e.modn / d.moduleNy;
+    int ly = e.cryn % d.crystalNy, lz = e.cryn / d.crystalNy;
+    float xl = (P.dirx * P.lenx) * 0.5f;
+    float yl = (-P.leny * 0.5f + (float)my * (P.mody + P.mspy)) + ((float)ly * (P.cry + P.cspy) + P.cry * 0.5f);
+    float zl = (-P.lenz * 0.5f + (float)mz * (P.modz + P.mspz)) + ((float)lz * (P.crz + P.cspz) + P.crz * 0.5f);
+    gx = P.ox + xl * P.ux[0] + yl * P.uy[0] + zl * P.uz[0];
+    gy = P.oy + xl * P.ux[1] + yl * P.uy[1] + zl * P.uz[1];
+}
+
+inline int sino_bin(const Detector &d, const gpet_event &a, const gpet_event &b, int n_s, int n_phi, float s_max)
+{
+    if (a.pann < 0 || a.pann >= (int)d.p.size() || b.pann < 0 || b.pann >= (int)d.p.size()) return -1;
+    float x1, y1, x2, y2;
+    crystal_centre(d, a, x1, y1);
+    crystal_centre(d, b, x2, y2);
+    float dx = x2 - x1, dy = y2 - y1;
+    if (dx == 0.0f && dy == 0.0f) return -1;
+    const float PIF = 3.14159274f;
+    float phi = gp_atan2f(dy, dx);
+    if (phi < 0.0f) phi = phi + PIF;
+    if (phi >= PIF) phi = phi - PIF;
+    float sp, cp;
+    gp_sincosf(phi, &sp, &cp);
+    float sd = x1 * sp - y1 * cp;
+    int ip = (int)(phi * ((float)n_phi / PIF));
+    if (ip > n_phi - 1) ip = n_phi - 1;
+    float fs = (sd + s_max) * ((float)n_s / (2.0f * s_max));
+    if (!(fs >= 0.0f && fs < (float)n_s)) return -1;
+    return ip * n_s + (int)fs;
 }
 
 bool ev_less_id(const gpet_event &a, const gpet_event &b)
@@ -1301,6 +1373,13 @@ gpet_status gpet_cpu_get_tallies(gpet_ctx *c, gpet_tallies *t)
             if (g >= 0 && g < t->n_crystals) t->crystal_singles[g]++;
         }
     }
+    if (t->sinogram && t->n_s > 0 && t->n_phi > 0 && t->s_max > 0.0f && c->has_det) {
+        memset(t->sinogram, 0, sizeof(int64_t) * (size_t)t->n_s * t->n_phi);
+        for (size_t i = 0; i + 1 < R.coinc.size(); i += 2) {
+            int b = sino_bin(d, R.coinc[i], R.coinc[i + 1], t->n_s, t->n_phi, t->s_max);
+            if (b >= 0) t->sinogram[b]++;
+        }
+    }
     if (t->energy_hist && t->n_ebins > 0 && t->e_hi > t->e_lo) {
         memset(t->energy_hist, 0, sizeof(int64_t) * t->n_ebins);
         float inv = (float)t->n_ebins / (t->e_hi - t->e_lo);
@@ -1325,6 +1404,7 @@ gpet_status gpet_cpu_get_majorant(gpet_ctx *c, int32_t which, float *out, float 
     return GPET_OK;
 }
 
+#ifndef GPET_ORACLE_FP64
 /* Single-function probes used by the CPU physics tests. */
 float gpet_cpu_lerp_mu(gpet_ctx *c, int32_t table, int32_t mat, float E)
 {
@@ -1401,6 +1481,8 @@ int32_t gpet_cpu_crystal_search(gpet_ctx *c, int32_t panel, float x, float y, fl
     return m;
 }
 
+#endif /* GPET_ORACLE_FP64 */
+
 /* Digitizer stages on a caller list (already adder/readout output). */
 gpet_status gpet_cpu_digitize(gpet_ctx *c, const gpet_event *ev, int64_t n, gpet_sink *sink)
 {
@@ -1439,6 +1521,7 @@ void gpet_cpu_mathf(int32_t kind, const float *in, float *out, int64_t n)
         case 4: gp_sincosf(x, &s, &c); out[i] = s; break;
         case 5: gp_sincosf(x, &s, &c); out[i] = c; break;
         case 6: out[i] = gp_cbrtf(x); break;
+        case 7: out[i] = gp_atan2f(x, in[n + i]); break;
         default: out[i] = 0.0f;
         }
     }

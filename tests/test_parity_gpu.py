@@ -19,9 +19,16 @@ def run_both(tables, opt, setup, call):
         try:
             setup(eng)
             res[name] = call(eng)
+            res[name].arrays = tallies_of(eng)
         finally:
             eng.close()
     return res["gpu"], res["oracle"]
+
+
+def tallies_of(eng):
+    t, cs, eh, sg = eng.tallies(n_crystals=8 * 400, n_ebins=100, e_lo=0.0, e_hi=1e6, n_s=64, n_phi=48, s_max=8.0,
+                                with_sinogram=True)
+    return cs, eh, sg
 
 
 def assert_parity(a, b, hits=False):
@@ -30,6 +37,9 @@ def assert_parity(a, b, hits=False):
     assert same_bytes(a.adder, b.adder), "adder events: " + first_diff(a.adder, b.adder)
     assert same_bytes(a.singles, b.singles), "singles: " + first_diff(a.singles, b.singles)
     assert same_bytes(a.coinc, b.coinc), "coincidences: " + first_diff(a.coinc.ravel(), b.coinc.ravel())
+    for x, y in zip(a.arrays, b.arrays):
+        assert np.array_equal(x, y), "tally arrays differ"
+    assert a.arrays[0].sum() == a.tallies.n_singles
     for f in ("n_photons", "n_panel_entries", "n_hits", "n_adder", "n_threshold", "n_deadtime", "n_singles",
               "n_coinc", "n_coinc_true", "dropped_events", "majorant_violations", "phantom_steps", "range_steps",
               "panel_steps"):

@@ -138,3 +138,16 @@ def test_log_double_error_bound(lib):
     y = np.zeros_like(t)
     lib.gpet_cpu_mathd(1, t.ctypes.data_as(C.c_void_p), y.ctypes.data_as(C.c_void_p), len(t))
     assert (np.abs(y - np.log1p(t)) / np.abs(np.log1p(t))).max() < 8e-16
+
+
+def test_atan2_error_bound(lib):
+    rng = np.random.default_rng(9)
+    y = rng.normal(size=200000).astype(np.float32)
+    x = rng.normal(size=200000).astype(np.float32)
+    x[:100] = 0.0
+    y[100:200] = 0.0
+    inp = np.concatenate([y, x])
+    out = np.zeros(len(y), np.float32)
+    lib.gpet_cpu_mathf(7, inp.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p), len(y))
+    ref = np.arctan2(y.astype(np.float64), x.astype(np.float64))
+    assert np.abs(out - ref).max() < 4e-7
