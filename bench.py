@@ -1,6 +1,6 @@
 """Benchmark of the photon transport and detection path on one MI355X.
 
-Workload (the configuration the metric is quoted on): cylinder phantom of
+Default workload (the configuration the metric is quoted on): cylinder phantom of
 200^3 voxels over a 1 cm cube (water cylinder r = 0.4 cm with a cortical
 bone rod r = 0.1 cm, in air), uniform F-18 activity in the cylinder with
 positron range and annihilation, 1e8 decays with emission times over 10 s,
@@ -8,7 +8,11 @@ positron range and annihilation, 1e8 decays with emission times over 10 s,
 8 cm, 12% FWHM energy blur, 350-650 keV window, 300 ps time blur, 4 ns
 coincidence window.  FP32 transport, FP64 time.
 
-One step = one full run of the path over the 1e8 decays (decay sampling,
+--workload body runs the large case instead: a 256 x 256 x 512 body
+phantom (ellipsoidal water body, lungs, bone spine, 20 hot spheres at 8:1)
+in 4 rings x 12 LSO panels, 1e9 F-18 decays.
+
+One step = one full run of the path over the workload's decays (decay sampling,
 phantom and panel transport, digitizer, coincidence sorter), inputs resident
 in HBM, outputs kept in HBM.  Prints one JSON line.
 """
@@ -28,7 +32,19 @@ HBM_PEAK_GBS = 8000.0
 QUEUE_BYTES = 6 * 4 + 8 + 16   # one queued photon: 6 floats, a double, an int4
 
 
-def cpu_baseline(n_sample: int, threads: int):
+WORKLOADS = {
+    "cylinder": dict(config=2, decays=1e8, seed=2,
+                     text="cylinder phantom 200^3 over 1 cm (water r=0.4, bone rod r=0.1, air), uniform F-18 "
+                          "with positron range, %.0e decays over 10 s, 8-panel LSO octagon at 8 cm, "
+                          "plain MC (no surrogate)"),
+    "body": dict(config=5, decays=1e9, seed=5,
+                 text="body phantom 256x256x512 at 1 mm (water ellipsoid 23x17x48 cm, lungs 0.3 g/cc, bone spine, "
+                      "20 hot spheres 8:1), F-18 with positron range, %.0e decays over 60 s, 4 rings x 12 LSO "
+                      "panels (20x20 crystals, 40 cm ring), plain MC (no surrogate)"),
+}
+
+
+def cpu_baseline(n_sample: int, threads: int, wl: dict):
     """The oracle (host scalar build, OpenMP) timed on a bounded sample of the
     same workload on this machine's cores."""
     from gpet_amd import engine
@@ -36,13 +52,13 @@ def cpu_baseline(n_sample: int, threads: int):
     import ctypes as C
 
     tabs = engine.load_tables()
-    opt = engine.make_options(seed=2, positron_range=1)
+    opt = engine.make_options(seed=wl["seed"], positron_range=1)
     eng = engine.Engine(opt, lib_path=os.path.join(ROOT, "oracle", "liboracle.so"), prefix="gpet_cpu_")
     f = eng.lib.gpet_cpu_set_threads
     f.argtypes = [C.c_void_p, C.c_int32]
     f(eng.ctx, threads)
-    W.setup_engine(eng, tabs, config=2)
-    src, iso, t0, t1 = W.workload_sources(2)
+    W.setup_engine(eng, tabs, config=wl["config"])
+    src, iso, t0, t1 = W.workload_sources(wl["config"])
     eng.run_source(src, iso, t0, t1, n_decays=max(n_sample // 20, 1000))   # warm-up
     t = time.perf_counter()
     r = eng.run_source(src, iso, t0, t1, n_decays=n_sample)
@@ -56,7 +72,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--decays", type=float, default=1e8)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="cylinder")
+    ap.add_argument("--decays", type=float, default=0.0, help="decays per step (default: the workload's)")
     ap.add_argument("--batch", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -69,12 +86,13 @@ def main():
     from gpet_amd import engine
     from gpet_amd import workloads as W
 
-    n_dec = int(args.decays)
+    wl = WORKLOADS[args.workload]
+    n_dec = int(args.decays or wl["decays"])
     tabs = engine.load_tables()
-    opt = engine.make_options(seed=2, positron_range=1, batch=args.batch)
+    opt = engine.make_options(seed=wl["seed"], positron_range=1, batch=args.batch)
     eng = engine.Engine(opt)
-    W.setup_engine(eng, tabs, config=2)
-    src, iso, t0, t1 = W.workload_sources(2)
+    W.setup_engine(eng, tabs, config=wl["config"])
+    src, iso, t0, t1 = W.workload_sources(wl["config"])
 
     for _ in range(args.warmup):
         eng.run_source(src, iso, t0, t1, n_decays=n_dec)
@@ -114,9 +132,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "cylinder phantom 200^3 over 1 cm (water r=0.4, bone rod r=0.1, air), uniform F-18 "
-                               "with positron range, %.0e decays over 10 s, 8-panel LSO octagon at 8 cm, "
-                               "plain MC (no surrogate)" % n_dec,
+        "config": {"workload": wl["text"] % n_dec,
                    "decays": n_dec, "batch": args.batch or (1 << 22), "precision": "fp32 transport, fp64 time"},
         "coincidences_per_s": coinc / wall,
         "singles_per_step": last.n_singles,
@@ -129,9 +145,9 @@ def main():
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         probe = 200000
-        rate, _, dt = cpu_baseline(probe, threads)
+        rate, _, dt = cpu_baseline(probe, threads, wl)
         n_sample = int(min(max(rate * args.cpu_seconds, probe), 4e8))
-        rate, crate, dt = cpu_baseline(n_sample, threads)
+        rate, crate, dt = cpu_baseline(n_sample, threads, wl)
         out["cpu_baseline"] = {"value": rate, "unit": "histories/s", "cores": threads, "kind": "port",
                                "sample": "%d decays of the same workload (%.1f s), oracle build -O3 OpenMP" % (n_sample, dt),
                                "coincidences_per_s": crate}

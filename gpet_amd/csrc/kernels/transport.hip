@@ -78,7 +78,7 @@ __device__ float d_sample_ek(const DevIsotope &iso, gp_rng &r)
 __device__ void d_sample_shape(const DevSource &s, gp_rng &r, float &x, float &y, float &z)
 {
     const float *c = s.coef;
-    int shape = (s.shape < 0 || s.shape > 2) ? 0 : s.shape;
+    int shape = (s.shape < 0 || s.shape > 3) ? 0 : s.shape;
     if (shape == 0) {
         x = c[0] + (c[3] * (2.0f * gp_rng_uniform(&r) - 1.0f)) * 0.5f;
         y = c[1] + (c[4] * (2.0f * gp_rng_uniform(&r) - 1.0f)) * 0.5f;
@@ -94,11 +94,18 @@ __device__ void d_sample_shape(const DevSource &s, gp_rng &r, float &x, float &y
         float sp, cp;
         gp_sincos2pi(gp_rng_uniform(&r), &sp, &cp);
         float ct = 2.0f * gp_rng_uniform(&r) - 1.0f;
-        float rr = c[3] * gp_cbrtf(gp_rng_uniform(&r));
+        float rr = gp_cbrtf(gp_rng_uniform(&r));
         float st = sqrtf(1.0f - ct * ct);
-        x = c[0] + (rr * st) * cp;
-        y = c[1] + (rr * st) * sp;
-        z = c[2] + rr * ct;
+        if (shape == 2) {
+            rr = c[3] * rr;
+            x = c[0] + (rr * st) * cp;
+            y = c[1] + (rr * st) * sp;
+            z = c[2] + rr * ct;
+        } else { /* 3: ellipsoid with semi-axes c3, c4, c5 (extension) */
+            x = c[0] + ((c[3] * rr) * st) * cp;
+            y = c[1] + ((c[4] * rr) * st) * sp;
+            z = c[2] + (c[5] * rr) * ct;
+        }
     }
 }
 

@@ -96,6 +96,104 @@ def octagon_scanner(**kw):
         os.unlink(path)
 
 
+# Body workload: 256 x 256 x 512 voxels inside a 4-ring scanner of 12 panels
+# per ring.  The ring's inscribed diameter is 40 cm; the voxel grid must lie
+# inside the panels (photons are carried to the grid boundary and only then
+# projected onto the panels), and a square grid of half width h clears flat
+# panels at 30-degree steps when 1.366 h < 20 cm, so the voxels are 0.1 cm
+# (25.6 x 25.6 x 51.2 cm grid).
+BODY_DIMS = (256, 256, 512)
+BODY_VOXEL = 0.1
+BODY_AXES = (11.5, 8.5, 24.0)      # water ellipsoid semi-axes (cm)
+LUNG_AXES = (3.5, 4.5, 7.0)
+LUNG_CENTRES = ((-5.0, -0.5, 6.0), (5.0, -0.5, 6.0))
+LUNG_DENSITY = 0.3                 # TissueICRU at lung density
+SPINE_CENTRE = (0.0, 5.5)          # cylinder along z
+SPINE_RADIUS = 1.2
+BONE_DENSITY = 1.85
+N_HOT = 20
+HOT_RATIO = 8.0                    # sphere : background activity concentration
+
+
+def _inside_ellipsoid(x, y, z, c, a):
+    return ((x - c[0]) / a[0]) ** 2 + ((y - c[1]) / a[1]) ** 2 + ((z - c[2]) / a[2]) ** 2 <= 1.0
+
+
+def hot_spheres(seed: int = 5):
+    """Centres and radii (cm) of the hot spheres: radii 0.5-1.5 cm, placed by
+    rejection inside the soft tissue of the body (clear of lungs and spine)
+    with a deterministic generator."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < N_HOT:
+        r = float(rng.uniform(0.5, 1.5))
+        c = rng.uniform(-1.0, 1.0, 3) * np.array(BODY_AXES) * 0.8
+        # sphere fully inside the body: shrink the axes by r
+        if not _inside_ellipsoid(c[0], c[1], c[2], (0, 0, 0), tuple(a - r for a in BODY_AXES)):
+            continue
+        if any(_inside_ellipsoid(c[0], c[1], c[2], lc, tuple(a + r for a in LUNG_AXES)) for lc in LUNG_CENTRES):
+            continue
+        if np.hypot(c[0] - SPINE_CENTRE[0], c[1] - SPINE_CENTRE[1]) < SPINE_RADIUS + r:
+            continue
+        if any(np.linalg.norm(c - np.array(o[0])) < r + o[1] + 0.2 for o in out):
+            continue
+        out.append((tuple(float(v) for v in c), r))
+    return out
+
+
+def body_phantom(dims=BODY_DIMS, voxel: float = BODY_VOXEL, air_density: float = 0.0012):
+    """Ellipsoidal water body with two lung ellipsoids, a cortical-bone spine
+    cylinder and (for transport) water hot spheres, in air, centred on the
+    origin; arrays indexed [z, y, x].  The grid covers dims * voxel cm; a
+    coarse grid of the same extent (e.g. 32 x 32 x 64 at 0.8 cm) gives the
+    same body at low resolution."""
+    nx, ny, nz = dims
+    cx = (np.arange(nx, dtype=np.float32) + 0.5) * voxel - nx * voxel / 2
+    cy = (np.arange(ny, dtype=np.float32) + 0.5) * voxel - ny * voxel / 2
+    cz = (np.arange(nz, dtype=np.float32) + 0.5) * voxel - nz * voxel / 2
+    mat = np.full((nz, ny, nx), AIR, np.int32)
+    dens = np.full((nz, ny, nx), air_density, np.float32)
+    yy, xx = np.meshgrid(cy, cx, indexing="ij")
+    for k in range(nz):
+        z = cz[k]
+        m = mat[k]
+        d = dens[k]
+        body = _inside_ellipsoid(xx, yy, z, (0, 0, 0), BODY_AXES)
+        m[body] = WATER
+        d[body] = 1.0
+        for lc in LUNG_CENTRES:
+            lung = body & _inside_ellipsoid(xx, yy, z, lc, LUNG_AXES)
+            m[lung] = TISSUE
+            d[lung] = LUNG_DENSITY
+        spine = body & ((xx - SPINE_CENTRE[0]) ** 2 + (yy - SPINE_CENTRE[1]) ** 2 <= SPINE_RADIUS ** 2)
+        m[spine] = BONE
+        d[spine] = BONE_DENSITY
+    size = (nx * voxel, ny * voxel, nz * voxel)
+    return mat, dens, dims, tuple(-v / 2 for v in size), size
+
+
+def body_scanner(rings: int = 4, **kw):
+    """4 rings x 12 flat LSO panels of 20 x 20 crystals (0.53 cm pitch,
+    2 cm deep) with a 40 cm inscribed ring diameter."""
+    args = dict(n_panels=12, radius=20.0, ncry=20, pitch=0.53, gap=0.005, depth=2.0, rings=rings,
+                ring_spacing=10.7)
+    args.update(kw)
+    return octagon_scanner(**args)
+
+
+def body_sources(seed: int = 5, concentration: float = 1.0e7):
+    """Background F-18 in the body ellipsoid (shape 3) plus the hot spheres
+    at HOT_RATIO : 1 (each sphere adds HOT_RATIO - 1 times the background
+    concentration on top of the background that already covers it);
+    natom in atoms per cm^3 * volume."""
+    a, b, c = BODY_AXES
+    srcs = [source(3, c3=a, c4=b, c5=c, natom=concentration * 4.0 / 3.0 * np.pi * a * b * c)]
+    for centre, r in hot_spheres(seed):
+        srcs.append(source(2, centre=centre, c3=r,
+                           natom=(HOT_RATIO - 1.0) * concentration * 4.0 / 3.0 * np.pi * r ** 3))
+    return srcs
+
+
 def isotope_f18() -> capi.Isotope:
     """Row 0 of data/isotopes.txt (F-18)."""
     iso = capi.Isotope()
@@ -155,22 +253,34 @@ def digitizer(readout_depth=2, readout_policy=1, e_threshold=50e3, blur_policy=1
 
 def setup_engine(eng, tables, config: int = 2, phantom_n=None, scanner_kw=None, dig=None):
     """Load tables, phantom, scanner and digitizer of configuration 1 (water
-    cube) or 2 (cylinder with bone rod)."""
+    cube), 2 (cylinder with bone rod) or 5 (body in the 48-panel scanner;
+    phantom_n, if given, is the x/y voxel count of a coarser grid of the same
+    extent, z twice that)."""
     eng.set_tables(tables)
     if config == 1:
         ph = water_cube(phantom_n or 64)
+    elif config == 5:
+        if phantom_n:
+            ph = body_phantom((phantom_n, phantom_n, 2 * phantom_n), BODY_VOXEL * BODY_DIMS[0] / phantom_n)
+        else:
+            ph = body_phantom()
     else:
         ph = cylinder_phantom(phantom_n or 200)
     eng.set_phantom(*ph)
-    panels, mats = octagon_scanner(**(scanner_kw or {}))
+    if config == 5:
+        panels, mats = body_scanner(**(scanner_kw or {}))
+    else:
+        panels, mats = octagon_scanner(**(scanner_kw or {}))
     eng.set_detector(panels, mats, np.array([0, 0, 0, 0, 0, 0, 0, 0, 0, 1], np.float32))
     eng.set_digitizer(dig or digitizer())
     return panels, mats
 
 
 def workload_sources(config: int):
-    """(sources, isotopes, t0, t1) of configuration 1 or 2."""
+    """(sources, isotopes, t0, t1) of configuration 1, 2 or 5."""
     if config == 1:
         return [source(2, natom=1.0e15)], [isotope_pure_pairs()], 0.0, 1.0
+    if config == 5:
+        return body_sources(), [isotope_f18()], 0.0, 60.0
     # F-18 uniform in the r = 0.4 cm water cylinder (full 1 cm height)
     return [source(1, c3=0.4, c4=1.0, natom=1.0e11)], [isotope_f18()], 0.0, 10.0

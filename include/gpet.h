@@ -134,8 +134,10 @@ typedef struct gpet_options {
 typedef struct gpet_source {
     double natom;
     int32_t isotope;           /* row of the isotope table                      */
-    int32_t shape;             /* 0 box, 1 cylinder (axis z), 2 sphere          */
-    float coef[6];             /* centre xyz + box sizes / cyl r,h / sphere r   */
+    int32_t shape;             /* 0 box, 1 cylinder (axis z), 2 sphere,
+                                  3 ellipsoid (extension)                       */
+    float coef[6];             /* centre xyz + box sizes / cyl r,h / sphere r /
+                                  ellipsoid semi-axes a,b,c                     */
 } gpet_source;
 
 /* A row of data/isotopes.txt (initialize.cu:10-31). */

@@ -655,7 +655,7 @@ real sample_ek(const gpet_isotope &iso, gp_rng &r)
 void sample_shape(const gpet_source &s, gp_rng &r, real &x, real &y, real &z)
 {
     const float *c = s.coef;
-    int shape = (s.shape < 0 || s.shape > 2) ? 0 : s.shape;
+    int shape = (s.shape < 0 || s.shape > 3) ? 0 : s.shape;
     if (shape == 0) {
         x = c[0] + (c[3] * (2.0f * gp_rng_uniform(&r) - 1.0f)) * 0.5f;
         y = c[1] + (c[4] * (2.0f * gp_rng_uniform(&r) - 1.0f)) * 0.5f;
@@ -671,11 +671,18 @@ void sample_shape(const gpet_source &s, gp_rng &r, real &x, real &y, real &z)
         real sp, cp;
         R_SINCOS2PI(gp_rng_uniform(&r), &sp, &cp);
         real ct = 2.0f * gp_rng_uniform(&r) - 1.0f;
-        real rr = c[3] * R_CBRT(gp_rng_uniform(&r));
+        real rr = R_CBRT(gp_rng_uniform(&r));
         real st = std::sqrt(1.0f - ct * ct);
-        x = c[0] + (rr * st) * cp;
-        y = c[1] + (rr * st) * sp;
-        z = c[2] + rr * ct;
+        if (shape == 2) {
+            rr = c[3] * rr;
+            x = c[0] + (rr * st) * cp;
+            y = c[1] + (rr * st) * sp;
+            z = c[2] + rr * ct;
+        } else { /* 3: ellipsoid with semi-axes c3, c4, c5 (extension) */
+            x = c[0] + ((c[3] * rr) * st) * cp;
+            y = c[1] + ((c[4] * rr) * st) * sp;
+            z = c[2] + (c[5] * rr) * ct;
+        }
     }
 }
 

@@ -99,7 +99,7 @@ def test_positron_energy_spectrum(lib):
     assert 0.24 < (E - 0.511).mean() < 0.30
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3])
 def test_source_shapes_uniform(lib, shape):
     from gpet_amd import workloads as W
     s = W.source(shape, centre=(0.3, -0.2, 0.1), c3=0.8 if shape else 0.6, c4=0.5 if shape != 2 else 0.0, c5=0.4)
@@ -116,6 +116,16 @@ def test_source_shapes_uniform(lib, shape):
         assert stats.kstest(q[:, 2] / 0.5 + 0.5, "uniform").pvalue > 0.01
         phi = (np.arctan2(q[:, 1], q[:, 0]) / (2 * np.pi)) % 1.0
         assert stats.kstest(phi, "uniform").pvalue > 0.01
+    elif shape == 3:
+        # uniform in the ellipsoid: the scaled point is uniform in the unit
+        # ball, and each coordinate's second moment is a^2/5
+        u = q / [0.8, 0.5, 0.4]
+        rn = np.linalg.norm(u, axis=1)
+        assert rn.max() <= 1.0 + 1e-6
+        assert stats.kstest(rn ** 3, "uniform").pvalue > 0.01
+        assert stats.kstest((u[:, 2] / rn + 1) / 2, "uniform").pvalue > 0.01
+        for k, a in enumerate((0.8, 0.5, 0.4)):
+            assert abs((q[:, k] ** 2).mean() / (a * a / 5) - 1) < 0.01
     else:
         r3 = (np.linalg.norm(q, axis=1) / 0.8) ** 3
         assert stats.kstest(np.minimum(r3, 1), "uniform").pvalue > 0.01

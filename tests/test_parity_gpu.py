@@ -26,7 +26,7 @@ def run_both(tables, opt, setup, call):
 
 
 def tallies_of(eng):
-    t, cs, eh, sg = eng.tallies(n_crystals=8 * 400, n_ebins=100, e_lo=0.0, e_hi=1e6, n_s=64, n_phi=48, s_max=8.0,
+    t, cs, eh, sg = eng.tallies(n_crystals=48 * 400, n_ebins=100, e_lo=0.0, e_hi=1e6, n_s=64, n_phi=48, s_max=8.0,
                                 with_sinogram=True)
     return cs, eh, sg
 
@@ -64,6 +64,23 @@ def test_source_parity(tables, config, n, prange):
                     lambda e: e.run_source(src, iso, t0, t1, n_decays=n, singles_cap=n, coinc_cap=n, adder_cap=4 * n,
                                            hits_cap=20 * n, trace_cap=2 * n))
     assert a.tallies.n_singles > 100 and a.tallies.n_coinc > 10
+    assert_parity(a, b, hits=True)
+
+
+def test_body_parity(tables):
+    """Body workload (ellipsoid background + hot spheres, lungs, spine) in
+    the 4-ring 48-panel scanner, on a 32 x 32 x 64 grid of the same extent."""
+    from gpet_amd import engine
+    from gpet_amd import workloads as W
+
+    opt = engine.make_options(seed=5, positron_range=1, record_hits=1, batch=5000)
+    src, iso, t0, t1 = W.workload_sources(5)
+    n = 12000
+    a, b = run_both(tables, opt, lambda e: W.setup_engine(e, tables, config=5, phantom_n=32),
+                    lambda e: e.run_source(src, iso, t0, t1, n_decays=n, singles_cap=n, coinc_cap=n, adder_cap=4 * n,
+                                           hits_cap=20 * n, trace_cap=2 * n))
+    assert a.tallies.n_singles > 100 and a.tallies.n_coinc > 10
+    assert a.arrays[0][8 * 400:].sum() > 0      # panels beyond the first ring see singles
     assert_parity(a, b, hits=True)
 
 
