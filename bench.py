@@ -10,7 +10,11 @@ coincidence window.  FP32 transport, FP64 time.
 
 --workload body runs the large case instead: a 256 x 256 x 512 body
 phantom (ellipsoidal water body, lungs, bone spine, 20 hot spheres at 8:1)
-in 4 rings x 12 LSO panels, 1e9 F-18 decays.
+in 4 rings x 12 LSO panels, 1e9 F-18 decays.  --workload batch runs 64
+independent blob phantoms of 32^3..256^3 voxels (seed 3) with 1e5..1e7
+decays each (log-uniform, 1.47e8 in total) as one batched run.
+
+Launch shapes are picked by the library's tuner during the warm-up run.
 
 One step = one full run of the path over the workload's decays (decay sampling,
 phantom and panel transport, digitizer, coincidence sorter), inputs resident
@@ -37,6 +41,10 @@ WORKLOADS = {
                      text="cylinder phantom 200^3 over 1 cm (water r=0.4, bone rod r=0.1, air), uniform F-18 "
                           "with positron range, %.0e decays over 10 s, 8-panel LSO octagon at 8 cm, "
                           "plain MC (no surrogate)"),
+    "batch": dict(config=3, decays=0, seed=3,
+                  text="64 blob phantoms 32^3..256^3 over 10 cm (water/bone/lung/air, seed 3), F-18 cylinder "
+                       "sources, 1e5..1e7 decays each (%.3g in total) over 10 s, one batched run on the 8-panel "
+                       "octagon, plain MC (no surrogate)"),
     "body": dict(config=5, decays=1e9, seed=5,
                  text="body phantom 256x256x512 at 1 mm (water ellipsoid 23x17x48 cm, lungs 0.3 g/cc, bone spine, "
                       "20 hot spheres 8:1), F-18 with positron range, %.0e decays over 60 s, 4 rings x 12 LSO "
@@ -44,11 +52,31 @@ WORKLOADS = {
 }
 
 
-def cpu_baseline(n_sample: int, threads: int, wl: dict):
-    """The oracle (host scalar build, OpenMP) timed on a bounded sample of the
-    same workload on this machine's cores."""
-    from gpet_amd import engine
+def setup(eng, wl, tabs, batch_data=None):
+    """Load the workload into an engine; returns run(n_decays) -> RunResult."""
     from gpet_amd import workloads as W
+
+    if wl["config"] == 3:
+        phantoms, jobs, src, iso = batch_data
+        W.setup_batch(eng, tabs, phantoms)
+
+        def run(scale=1.0):
+            js = [dict(j, n_decays=max(1, int(round(j["n_decays"] * scale)))) for j in jobs]
+            return eng.run_batch(js, src, iso)
+        return run
+    W.setup_engine(eng, tabs, config=wl["config"])
+    src, iso, t0, t1 = W.workload_sources(wl["config"])
+
+    def run(n):
+        return eng.run_source(src, iso, t0, t1, n_decays=int(n))
+    return run
+
+
+def cpu_baseline(n_sample: int, n_full: int, threads: int, wl: dict, batch_data=None):
+    """The oracle (host scalar build, OpenMP) timed on a bounded sample of the
+    same workload on this machine's cores (batched workload: every job's
+    decay count scaled by n_sample / n_full)."""
+    from gpet_amd import engine
     import ctypes as C
 
     tabs = engine.load_tables()
@@ -57,14 +85,14 @@ def cpu_baseline(n_sample: int, threads: int, wl: dict):
     f = eng.lib.gpet_cpu_set_threads
     f.argtypes = [C.c_void_p, C.c_int32]
     f(eng.ctx, threads)
-    W.setup_engine(eng, tabs, config=wl["config"])
-    src, iso, t0, t1 = W.workload_sources(wl["config"])
-    eng.run_source(src, iso, t0, t1, n_decays=max(n_sample // 20, 1000))   # warm-up
+    run = setup(eng, wl, tabs, batch_data)
+    arg = (lambda n: n / n_full) if wl["config"] == 3 else (lambda n: n)
+    run(arg(max(n_sample // 20, 1000)))   # warm-up
     t = time.perf_counter()
-    r = eng.run_source(src, iso, t0, t1, n_decays=n_sample)
+    r = run(arg(n_sample))
     dt = time.perf_counter() - t
     eng.close()
-    return n_sample / dt, r.tallies.n_coinc / dt, dt
+    return r.tallies.n_histories / dt, r.tallies.n_coinc / dt, dt, r.tallies.n_histories
 
 
 def main():
@@ -83,41 +111,56 @@ def main():
               "(histories shard by index; see the design notes).", file=sys.stderr)
         sys.exit(2)
 
-    from gpet_amd import engine
+    from gpet_amd import capi, engine
     from gpet_amd import workloads as W
 
     wl = WORKLOADS[args.workload]
-    n_dec = int(args.decays or wl["decays"])
     tabs = engine.load_tables()
-    opt = engine.make_options(seed=wl["seed"], positron_range=1, batch=args.batch)
+    opt = engine.make_options(seed=wl["seed"], positron_range=1, batch=args.batch, flags=capi.FLAG_AUTOTUNE)
     eng = engine.Engine(opt)
-    W.setup_engine(eng, tabs, config=wl["config"])
-    src, iso, t0, t1 = W.workload_sources(wl["config"])
+    batch_data = W.batch_workload() if wl["config"] == 3 else None
+    if batch_data is not None:
+        n_full = sum(j["n_decays"] for j in batch_data[1])
+        n_dec = int(args.decays or n_full)
+        step_arg = n_dec / n_full
+    else:
+        n_dec = int(args.decays or wl["decays"])
+        n_full = n_dec
+        step_arg = n_dec
+    run = setup(eng, wl, tabs, batch_data)
 
     for _ in range(args.warmup):
-        eng.run_source(src, iso, t0, t1, n_decays=n_dec)
+        run(step_arg)
     t_k = [0.0, 0.0]
     l_k = [0, 0]
+    h_timed = 0
     hist = coinc = 0
     b_alg = 0.0
     t_start = time.perf_counter()
     for _ in range(args.steps):
-        r = eng.run_source(src, iso, t0, t1, n_decays=n_dec)   # returns after device sync
+        r = run(step_arg)   # returns after device sync
         T = r.tallies
         hist += T.n_histories
         coinc += T.n_coinc
+        # HIP-event durations of the launches the library timed (the eager
+        # first batch of a graph run, every batch otherwise)
         for k in range(2):
             t_k[k] += T.kernel_seconds[k]
             l_k[k] += T.kernel_launches[k]
+        h_timed += T.kernel_histories
         b_alg += 4.0 * T.range_steps + 8.0 * T.phantom_steps + QUEUE_BYTES * T.n_panel_entries
     wall = time.perf_counter() - t_start
     last = T
+    launch = eng.get_launch()
     eng.close()
 
     value = hist / wall
-    # dominant kernel: the decay + positron range + phantom kernel
+    # dominant kernel: the decay + positron range + phantom kernel.
+    # Algorithmic bytes per history (4 B per positron-walk voxel, 8 B per
+    # Woodcock step in the phantom, one queue record per panel entry) times
+    # the histories of an average timed launch, over its average duration.
     avg_launch = t_k[0] / max(l_k[0], 1)
-    bytes_per_launch = b_alg / max(l_k[0], 1)
+    bytes_per_launch = b_alg / max(hist, 1) * h_timed / max(l_k[0], 1)
     achieved = bytes_per_launch / avg_launch / 1e9 if avg_launch > 0 else 0.0
     out = {
         "metric": METRIC,
@@ -133,11 +176,14 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": wl["text"] % n_dec,
-                   "decays": n_dec, "batch": args.batch or (1 << 22), "precision": "fp32 transport, fp64 time"},
+                   "decays": n_dec, "launch": launch, "graph": bool(last.graph), "jobs": last.n_jobs,
+                   "precision": "fp32 transport, fp64 time"},
         "coincidences_per_s": coinc / wall,
         "singles_per_step": last.n_singles,
         "coincidences_per_step": last.n_coinc,
-        "kernel_ms": {"phantom": 1e3 * t_k[0] / args.steps, "panel": 1e3 * t_k[1] / args.steps},
+        "kernel_avg_ms": {"phantom": 1e3 * avg_launch, "panel": 1e3 * t_k[1] / max(l_k[1], 1),
+                          "timed_launches": l_k[0], "histories_per_timed_launch": h_timed / max(l_k[0], 1),
+                          "launches_per_step": last.launches},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_phantom", "bytes_per_launch": bytes_per_launch, "avg_launch_ms": 1e3 * avg_launch},
@@ -145,11 +191,11 @@ def main():
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         probe = 200000
-        rate, _, dt = cpu_baseline(probe, threads, wl)
-        n_sample = int(min(max(rate * args.cpu_seconds, probe), 4e8))
-        rate, crate, dt = cpu_baseline(n_sample, threads, wl)
+        rate, _, dt, _ = cpu_baseline(probe, n_full, threads, wl, batch_data)
+        n_sample = int(min(max(rate * args.cpu_seconds, probe), 4e8, n_full))
+        rate, crate, dt, n_done = cpu_baseline(n_sample, n_full, threads, wl, batch_data)
         out["cpu_baseline"] = {"value": rate, "unit": "histories/s", "cores": threads, "kind": "port",
-                               "sample": "%d decays of the same workload (%.1f s), oracle build -O3 OpenMP" % (n_sample, dt),
+                               "sample": "%d decays of the same workload (%.1f s), oracle build -O3 OpenMP" % (n_done, dt),
                                "coincidences_per_s": crate}
         out["gpu_over_cpu"] = value / rate
     print(json.dumps(out))

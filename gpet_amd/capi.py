@@ -90,7 +90,21 @@ class Sink(C.Structure):
                 ("adder", C.c_void_p), ("adder_cap", C.c_int64), ("n_adder", C.c_int64),
                 ("hit_id", C.c_void_p), ("hit_f", C.c_void_p), ("hits_cap", C.c_int64), ("n_hits", C.c_int64),
                 ("trace", C.c_void_p), ("trace_cap", C.c_int64), ("n_trace", C.c_int64),
-                ("out_dir", C.c_char_p)]
+                ("out_dir", C.c_char_p), ("job_counts", C.c_void_p)]
+
+
+class Launch(C.Structure):
+    _fields_ = [("block", C.c_int32 * 2), ("wpe", C.c_int32 * 2), ("blocks_per_cu", C.c_int32 * 2),
+                ("batch", C.c_int64)]
+
+    def as_dict(self):
+        return {"block": list(self.block), "wpe": list(self.wpe), "blocks_per_cu": list(self.blocks_per_cu),
+                "batch": int(self.batch)}
+
+
+class Job(C.Structure):
+    _fields_ = [("phantom", C.c_int32), ("first_source", C.c_int32), ("n_sources", C.c_int32),
+                ("reserved", C.c_int32), ("t0", C.c_double), ("t1", C.c_double), ("n_decays", C.c_int64)]
 
 
 class Tallies(C.Structure):
@@ -102,7 +116,12 @@ class Tallies(C.Structure):
         ("energy_hist", C.c_void_p), ("n_ebins", C.c_int32), ("e_lo", C.c_float), ("e_hi", C.c_float),
         ("sinogram", C.c_void_p), ("n_s", C.c_int32), ("n_phi", C.c_int32), ("s_max", C.c_float),
         ("seconds_transport", C.c_double), ("seconds_digitizer", C.c_double),
-        ("kernel_seconds", C.c_double * 2), ("kernel_launches", C.c_int64 * 2)]
+        ("kernel_seconds", C.c_double * 2), ("kernel_launches", C.c_int64 * 2), ("kernel_histories", C.c_int64),
+        ("launches", C.c_int64),
+        ("launch", Launch), ("graph", C.c_int32), ("n_jobs", C.c_int32), ("seconds_tune", C.c_double)]
+
+FLAG_NO_GRAPH = 1
+FLAG_AUTOTUNE = 2
 
 
 class Input(C.Structure):
@@ -130,11 +149,15 @@ CTX_FUNCS = {
     "last_error": (C.c_char_p, [_VP]),
     "set_tables": (C.c_int32, [_VP, P(Tables)]),
     "set_phantom": (C.c_int32, [_VP, _VP, _VP, P(C.c_int32), P(C.c_float), P(C.c_float)]),
+    "set_phantom_slot": (C.c_int32, [_VP, C.c_int32, _VP, _VP, P(C.c_int32), P(C.c_float), P(C.c_float)]),
     "set_detector": (C.c_int32, [_VP, P(Panel), C.c_int32, P(PanelMaterial), _VP, C.c_int32]),
     "set_digitizer": (C.c_int32, [_VP, P(Digitizer)]),
     "run_source": (C.c_int32, [_VP, P(Source), C.c_int32, P(Isotope), C.c_int32, C.c_double, C.c_double,
                                C.c_int64, P(Sink)]),
     "run_psf": (C.c_int32, [_VP, _VP, C.c_int64, C.c_int32, P(Sink)]),
+    "run_batch": (C.c_int32, [_VP, P(Job), C.c_int32, P(Source), C.c_int32, P(Isotope), C.c_int32, P(Sink)]),
+    "set_launch": (C.c_int32, [_VP, P(Launch)]),
+    "get_launch": (C.c_int32, [_VP, P(Launch)]),
     "get_tallies": (C.c_int32, [_VP, P(Tallies)]),
     "get_majorant": (C.c_int32, [_VP, C.c_int32, _VP, P(C.c_float), P(C.c_float)]),
 }

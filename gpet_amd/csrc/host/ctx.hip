@@ -35,6 +35,23 @@ struct DBuf {
     int64_t cap = 0;
 };
 
+/* One phantom slot: voxels, bricked densities and its own majorant. */
+struct PhSlot {
+    DBuf<int2> vox;
+    DBuf<float> dens_b;
+    DBuf<float> wck;
+    std::vector<float> h_wck;
+    DevPhantom dp;
+    bool present = false;
+};
+
+/* Captured transport batches of the last graph run and the launch
+ * arguments they were captured with. */
+struct GraphCache {
+    hipGraphExec_t exec = nullptr;
+    std::vector<unsigned char> key;
+};
+
 } // namespace
 
 struct gpet_ctx {
@@ -55,13 +72,23 @@ struct gpet_ctx {
     DBuf<float> d_surf[2];
     DevTables dtab;
 
-    /* phantom */
+    /* phantom slots; slot 0 is the phantom of gpet_set_phantom */
     bool has_ph = false;
-    DBuf<int2> d_vox;
-    DBuf<float> d_dens_b;
-    DBuf<float> d_wck_ph;
-    std::vector<float> h_wck_ph;
-    DevPhantom dph;
+    std::vector<PhSlot> slots;
+    DevPhantom dph;              /* slot 0's descriptor (or absent)          */
+    DBuf<DevPhantom> d_phs;      /* descriptors of all slots, for batches    */
+    bool phs_dirty = true;
+
+    /* launch shape, tuner state, graph cache */
+    gpet_launch launch;
+    int n_cu = 256;
+    bool tuned[2] = {false, false}; /* single-phantom runs, batched runs    */
+    double tune_seconds = 0;
+    GraphCache graph;
+    double ev_per_hist = -1.0;   /* adder events per history of the last run */
+    DBuf<DevSegment> d_seg;
+    DBuf<int64_t> d_job_first;
+    DBuf<unsigned long long> d_job_cnt;
 
     /* detector */
     bool has_det = false;
@@ -242,20 +269,26 @@ int64_t binomial_draw(double n, double p, uint64_t seed, int src)
     return (int64_t)k;
 }
 
-gpet_status plan_sources(gpet_ctx *c, const gpet_source *src, int ns, const gpet_isotope *iso, int niso, double t0,
-                         double t1, int64_t n_decays, std::vector<int64_t> &first, std::vector<DevSource> &ds)
+/* Decay counts of one acquisition's sources (appended to cnt and ds).
+ * src_base is the index of src[0] in the run's source array: it keys the
+ * binomial stream, so a source draws the same count alone or in a batch. */
+gpet_status plan_job(gpet_ctx *c, const gpet_source *src, int ns, const gpet_isotope *iso, int niso, double t0,
+                     double t1, int64_t n_decays, int src_base, int phantom, std::vector<int64_t> &cnt_out,
+                     std::vector<DevSource> &ds_out)
 {
-    first.assign(ns + 1, 0);
-    ds.assign(ns, DevSource());
+    std::vector<DevSource> ds(ns);
     std::vector<double> expect(ns, 0.0), navail(ns, 0.0);
     for (int i = 0; i < ns; ++i) {
         if (src[i].isotope < 0 || src[i].isotope >= niso) return fail(c, GPET_ERR_ARG, "source isotope out of range");
         double th = iso[src[i].isotope].half_life_s;
         if (!(th > 0.0)) return fail(c, GPET_ERR_ARG, "half life must be positive");
+        memset(&ds[i], 0, sizeof(DevSource));
         ds[i].tau = th / std::log(2.0);
         ds[i].pdec = -std::expm1(-(t1 - t0) / ds[i].tau);
+        ds[i].t0 = t0;
         ds[i].shape = src[i].shape;
         ds[i].iso = src[i].isotope;
+        ds[i].phantom = phantom;
         for (int k = 0; k < 6; ++k) ds[i].coef[k] = src[i].coef[k];
         navail[i] = std::floor(src[i].natom * std::exp2(-t0 / th));
         expect[i] = navail[i] * ds[i].pdec;
@@ -277,9 +310,10 @@ gpet_status plan_sources(gpet_ctx *c, const gpet_source *src, int ns, const gpet
                          [](const std::pair<double, int> &a, const std::pair<double, int> &b) { return a.first > b.first; });
         for (int k = 0; assigned < n_decays; ++k, ++assigned) cnt[rem[k % ns].second]++;
     } else {
-        for (int i = 0; i < ns; ++i) cnt[i] = binomial_draw(navail[i], ds[i].pdec, c->opt.seed, i);
+        for (int i = 0; i < ns; ++i) cnt[i] = binomial_draw(navail[i], ds[i].pdec, c->opt.seed, src_base + i);
     }
-    for (int i = 0; i < ns; ++i) first[i + 1] = first[i] + cnt[i];
+    cnt_out.insert(cnt_out.end(), cnt.begin(), cnt.end());
+    ds_out.insert(ds_out.end(), ds.begin(), ds.end());
     return GPET_OK;
 }
 
@@ -312,34 +346,106 @@ gpet_status write_outputs(gpet_ctx *c, const std::string &dir, const std::vector
 }
 
 /* The batch loop shared by source and PSF runs. */
-gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sink)
+gpet_status upload_run_sources(gpet_ctx *c, const std::vector<int64_t> &first, const std::vector<DevSource> &ds,
+                               const gpet_isotope *iso, int niso)
+{
+    std::vector<DevIsotope> di(niso);
+    for (int i = 0; i < niso; ++i) {
+        di[i].ratio = iso[i].beta_ratio;
+        for (int k = 0; k < 8; ++k) di[i].coef[k] = iso[i].coef[k];
+    }
+    gpet_status st;
+    if ((st = upload(c, c->d_first, first.data(), (int64_t)first.size())) ||
+        (st = upload(c, c->d_src, ds.data(), (int64_t)ds.size())) || (st = upload(c, c->d_iso, di.data(), niso)))
+        return st;
+    return GPET_OK;
+}
+
+/* What a batched run adds: history ranges per source, their phantom slots
+ * and the first history of each job. */
+struct BatchInfo {
+    std::vector<int64_t> first;      /* [nsrc + 1]  */
+    std::vector<int> src_phantom;    /* [nsrc]      */
+    std::vector<int64_t> job_first;  /* [njobs + 1] */
+};
+
+/* Segments of the launch covering histories [h0, h0 + nh): maximal runs of
+ * consecutive sources on one phantom slot, each starting a new virtual
+ * block of `block` histories. */
+void build_segments(const BatchInfo &bi, int64_t h0, int64_t nh, int block, std::vector<DevSegment> &out,
+                    int64_t &nvb)
+{
+    const int nsrc = (int)bi.src_phantom.size();
+    nvb = 0;
+    int64_t h = h0;
+    const int64_t end = h0 + nh;
+    int s = (int)(std::upper_bound(bi.first.begin(), bi.first.end(), h) - bi.first.begin()) - 1;
+    while (h < end) {
+        while (s < nsrc - 1 && bi.first[s + 1] <= h) ++s;
+        const int ph = bi.src_phantom[s];
+        int t = s + 1;
+        while (t < nsrc && bi.src_phantom[t] == ph) ++t;
+        const int64_t e = std::min(end, bi.first[t]);
+        DevSegment g;
+        g.h0 = h;
+        g.nh = e - h;
+        g.vb0 = nvb;
+        g.phantom = ph;
+        g.pad = 0;
+        nvb += (g.nh + block - 1) / block;
+        out.push_back(g);
+        h = e;
+        s = t;
+    }
+}
+
+LaunchCfg cfg_of(const gpet_launch &L, int k)
+{
+    LaunchCfg c;
+    c.block = L.block[k];
+    c.wpe = L.wpe[k];
+    c.blocks_per_cu = L.blocks_per_cu[k];
+    return c;
+}
+
+gpet_launch default_launch()
+{
+    gpet_launch L;
+    memset(&L, 0, sizeof(L));
+    L.block[0] = 256; L.block[1] = 256;
+    L.wpe[0] = 0; L.wpe[1] = 0;
+    L.blocks_per_cu[0] = 0; L.blocks_per_cu[1] = 8;
+    L.batch = DEFAULT_BATCH;
+    return L;
+}
+
+gpet_status check_launch(gpet_ctx *c, const gpet_launch &L)
+{
+    for (int k = 0; k < 2; ++k) {
+        if (L.block[k] < 64 || L.block[k] > 256 || L.block[k] % 64) return fail(c, GPET_ERR_ARG, "launch block size");
+        bool ok = false;
+        for (int v : gpet_wpe_variants) ok = ok || v == L.wpe[k];
+        if (!ok) return fail(c, GPET_ERR_ARG, "launch register variant (wpe) must be 0, 6 or 8");
+        if (L.blocks_per_cu[k] < 0 || L.blocks_per_cu[k] > 64) return fail(c, GPET_ERR_ARG, "launch blocks per CU");
+    }
+    if (L.batch < 256 || L.batch > ((int64_t)1 << 26)) return fail(c, GPET_ERR_ARG, "launch batch size");
+    return GPET_OK;
+}
+
+gpet_status alloc_queue(gpet_ctx *c, int64_t qcap)
 {
     gpet_status st;
-    const int64_t batch = c->opt.batch > 0 ? c->opt.batch : DEFAULT_BATCH;
-    const int64_t qcap = 2 * std::min<int64_t>(batch, std::max<int64_t>(n_hist, 1));
     if (qcap > c->q_cap) {
         for (int k = 0; k < 6; ++k)
             if ((st = dalloc(c, c->q_f[k], qcap))) return st;
         if ((st = dalloc(c, c->q_t, qcap)) || (st = dalloc(c, c->q_info, qcap))) return st;
         c->q_cap = qcap;
     }
-    const bool want_trace = sink && sink->trace && sink->trace_cap > 0;
-    int64_t ntr = want_trace ? std::min<int64_t>(2 * n_hist, sink->trace_cap) : 0;
-    if (want_trace) {
-        if ((st = dalloc(c, c->d_trace, ntr)) || (st = dalloc(c, c->q_slot, qcap))) return st;
-        HIPOK(c, hipMemsetAsync(c->d_trace.p, 0, sizeof(gpet_trace) * (size_t)ntr, c->stream));
-    }
-    const bool want_hits = c->opt.record_hits != 0;
-    int64_t hits_cap = 0;
-    if (want_hits) {
-        hits_cap = (sink && sink->hit_id && sink->hits_cap > 0) ? sink->hits_cap : (int64_t)1 << 24;
-        if ((st = dalloc(c, c->d_hit_id, 5 * hits_cap)) || (st = dalloc(c, c->d_hit_f, 5 * hits_cap))) return st;
-    }
-    if ((st = dalloc(c, c->d_small, 4)) || (st = dalloc(c, c->d_cnt, CNT_NUM))) return st;
-    HIPOK(c, hipMemsetAsync(c->d_small.p, 0, sizeof(unsigned) * 4, c->stream));
-    HIPOK(c, hipMemsetAsync(c->d_cnt.p, 0, sizeof(unsigned long long) * CNT_NUM, c->stream));
-    if ((st = dgrow(c, c->d_events, std::max<int64_t>(1 << 20, n_hist / 4), 0))) return st;
+    return GPET_OK;
+}
 
+DevQueue make_queue(gpet_ctx *c, int64_t qcap, bool want_trace)
+{
     DevQueue q;
     q.y = c->q_f[0].p; q.z = c->q_f[1].p; q.vx = c->q_f[2].p; q.vy = c->q_f[3].p; q.vz = c->q_f[4].p; q.E = c->q_f[5].p;
     q.t = c->q_t.p;
@@ -347,73 +453,351 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
     q.trace_slot = want_trace ? c->q_slot.p : nullptr;
     q.count = (int *)c->d_small.p;
     q.cap = qcap;
+    return q;
+}
+
+/* Launch arguments of one transport batch. */
+struct BatchLaunch {
+    RunParams rp;
+    int64_t nh;
+};
+
+gpet_status launch_batch(gpet_ctx *c, const BatchLaunch &B, const DevQueue &q, const PanelParams &pp,
+                         const gpet_launch &L, bool batched, hipEvent_t *ev)
+{
+    int e = gpet_launch_zero((unsigned *)q.count, 1, c->stream);
+    if (e) return fail(c, GPET_ERR_DEVICE, std::string("zero kernel: ") + hipGetErrorString((hipError_t)e));
+    if (ev) HIPOK(c, hipEventRecord(ev[0], c->stream));
+    e = gpet_launch_phantom(B.rp, c->dtab, c->dph, c->ddet, q, batched, cfg_of(L, 0), c->n_cu, c->stream);
+    if (e) return fail(c, GPET_ERR_DEVICE, std::string("phantom kernel: ") + hipGetErrorString((hipError_t)e));
+    if (ev) HIPOK(c, hipEventRecord(ev[1], c->stream));
+    if (ev) HIPOK(c, hipEventRecord(ev[2], c->stream));
+    e = gpet_launch_panel(pp, c->dtab, c->ddet, q, std::min<int64_t>(2 * B.nh, q.cap), cfg_of(L, 1), c->n_cu, c->stream);
+    if (e) return fail(c, GPET_ERR_DEVICE, std::string("panel kernel: ") + hipGetErrorString((hipError_t)e));
+    if (ev) HIPOK(c, hipEventRecord(ev[3], c->stream));
+    return GPET_OK;
+}
+
+template <class T>
+void key_add(std::vector<unsigned char> &k, const T &v)
+{
+    const unsigned char *p = reinterpret_cast<const unsigned char *>(&v);
+    k.insert(k.end(), p, p + sizeof(T));
+}
+
+void drop_graph(gpet_ctx *c)
+{
+    if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);
+    c->graph.exec = nullptr;
+    c->graph.key.clear();
+}
+
+/* Replays batches 1..n-1 as one hipGraph (captured on first use, reused
+ * while every launch argument is unchanged).  Returns false, leaving the
+ * stream out of capture mode, if capture is not possible; the caller then
+ * launches the batches one by one. */
+bool run_graph(gpet_ctx *c, const std::vector<BatchLaunch> &B, const DevQueue &q, const PanelParams &pp,
+               const gpet_launch &L, bool batched)
+{
+    std::vector<unsigned char> key;
+    for (size_t b = 1; b < B.size(); ++b) {
+        key_add(key, B[b].rp);
+        key_add(key, B[b].nh);
+    }
+    key_add(key, pp);
+    key_add(key, q);
+    key_add(key, c->dtab);
+    key_add(key, c->dph);
+    key_add(key, c->ddet);
+    key_add(key, L);
+    key_add(key, batched);
+    if (!(c->graph.exec && key == c->graph.key)) {
+        drop_graph(c);
+        if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
+        bool ok = true;
+        for (size_t b = 1; b < B.size() && ok; ++b)
+            ok = launch_batch(c, B[b], q, pp, L, batched, &c->kev[4 * b]) == GPET_OK;
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (!ok || e != hipSuccess || !g) {
+            if (g) (void)hipGraphDestroy(g);
+            (void)hipGetLastError();
+            return false;
+        }
+        hipGraphExec_t x = nullptr;
+        e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        c->graph.exec = x;
+        c->graph.key.swap(key);
+    }
+    return hipGraphLaunch(c->graph.exec, c->stream) == hipSuccess;
+}
+
+/* Launch-shape tuner: times the variants of each kernel on a prefix of the
+ * run's own histories (coordinate search: register variant, then grid
+ * shape, then block size) and keeps the fastest.  Every variant computes
+ * the same histories, so the choice cannot change results; the prefix's
+ * outputs are discarded. */
+gpet_status autotune(gpet_ctx *c, const RunParams &rp0, int64_t n_hist, const PanelParams &pp0, const BatchInfo *bi,
+                     double *seconds)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    gpet_status st;
+    const int64_t ns = std::min<int64_t>(n_hist, (int64_t)1 << 21);
+    const int64_t qcap = 2 * ns;
+    if ((st = alloc_queue(c, qcap))) return st;
+    if ((st = dgrow(c, c->d_events, GPET_MAX_PH_EVENTS * qcap, 0))) return st;
+    DevQueue q = make_queue(c, qcap, false);
+    PanelParams pp = pp0;
+    pp.hit_id = nullptr;
+    pp.hit_f = nullptr;
+    pp.trace = nullptr;
+    pp.events = c->d_events.p;
+    pp.events_cap = c->d_events.cap;
+    hipEvent_t e0 = c->ev0, e1 = c->ev1;
+    gpet_launch L = c->launch;
+    std::vector<DevSegment> segs;
+
+    auto time_phantom = [&](const gpet_launch &T, float &ms) -> gpet_status {
+        RunParams rp = rp0;
+        rp.h0 = 0;
+        rp.nh = ns;
+        rp.trace = nullptr;
+        rp.trace_cap = 0;
+        if (bi) {
+            segs.clear();
+            int64_t nvb = 0;
+            build_segments(*bi, 0, ns, T.block[0], segs, nvb);
+            gpet_status s2 = upload(c, c->d_seg, segs.data(), (int64_t)segs.size());
+            if (s2) return s2;
+            rp.seg = c->d_seg.p;
+            rp.nseg = (int)segs.size();
+            rp.n_vblocks = nvb;
+        }
+        float best = 1e30f;
+        for (int rep = 0; rep < 2; ++rep) {
+            int e = gpet_launch_zero((unsigned *)q.count, 1, c->stream);
+            if (e) return fail(c, GPET_ERR_DEVICE, "zero kernel");
+            HIPOK(c, hipEventRecord(e0, c->stream));
+            e = gpet_launch_phantom(rp, c->dtab, c->dph, c->ddet, q, bi != nullptr, cfg_of(T, 0), c->n_cu, c->stream);
+            if (e) return fail(c, GPET_ERR_DEVICE, std::string("phantom kernel: ") + hipGetErrorString((hipError_t)e));
+            HIPOK(c, hipEventRecord(e1, c->stream));
+            HIPOK(c, hipEventSynchronize(e1));
+            float m = 0;
+            HIPOK(c, hipEventElapsedTime(&m, e0, e1));
+            best = std::min(best, m);
+        }
+        ms = best;
+        return GPET_OK;
+    };
+    auto time_panel = [&](const gpet_launch &T, float &ms) -> gpet_status {
+        float best = 1e30f;
+        for (int rep = 0; rep < 2; ++rep) {
+            int e = gpet_launch_zero(c->d_small.p + 1, 1, c->stream);
+            if (e) return fail(c, GPET_ERR_DEVICE, "zero kernel");
+            HIPOK(c, hipEventRecord(e0, c->stream));
+            e = gpet_launch_panel(pp, c->dtab, c->ddet, q, qcap, cfg_of(T, 1), c->n_cu, c->stream);
+            if (e) return fail(c, GPET_ERR_DEVICE, std::string("panel kernel: ") + hipGetErrorString((hipError_t)e));
+            HIPOK(c, hipEventRecord(e1, c->stream));
+            HIPOK(c, hipEventSynchronize(e1));
+            float m = 0;
+            HIPOK(c, hipEventElapsedTime(&m, e0, e1));
+            best = std::min(best, m);
+        }
+        ms = best;
+        return GPET_OK;
+    };
+    /* coordinate search over one kernel's knobs */
+    auto search = [&](int k, auto timer) -> gpet_status {
+        float best_ms = 0;
+        if ((st = timer(L, best_ms))) return st;
+        const int wpes[] = {0, 6, 8};
+        const int bpcs[] = {0, 2, 4, 8, 16};
+        const int blocks[] = {128, 256};
+        for (int stage = 0; stage < 3; ++stage) {
+            gpet_launch base = L;
+            const int *vals = stage == 0 ? wpes : (stage == 1 ? bpcs : blocks);
+            int nv = stage == 0 ? 3 : (stage == 1 ? 5 : 2);
+            for (int v = 0; v < nv; ++v) {
+                gpet_launch T = base;
+                if (stage == 0) T.wpe[k] = vals[v];
+                else if (stage == 1) T.blocks_per_cu[k] = vals[v];
+                else T.block[k] = vals[v];
+                if (k == 1 && stage == 1 && T.blocks_per_cu[k] == 0) continue; /* panel grid needs a device count */
+                if (memcmp(&T, &base, sizeof(T)) == 0) continue;
+                float ms = 0;
+                if ((st = timer(T, ms))) return st;
+                if (ms < best_ms * 0.98f) {
+                    best_ms = ms;
+                    L = T;
+                }
+            }
+        }
+        return GPET_OK;
+    };
+    if ((st = search(0, time_phantom))) return st;
+    /* leave a queue from the chosen phantom variant for the panel trials */
+    float dummy = 0;
+    if ((st = time_phantom(L, dummy))) return st;
+    if ((st = search(1, time_panel))) return st;
+    c->launch = L;
+    *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return GPET_OK;
+}
+
+gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sink, const BatchInfo *bi)
+{
+    gpet_status st;
+    if (n_hist > GPET_MAX_HISTORIES) return fail(c, GPET_ERR_CAPACITY, "more than 2^30 histories in one run");
+    const bool batched = bi != nullptr;
+    if ((st = dalloc(c, c->d_small, 4)) || (st = dalloc(c, c->d_cnt, CNT_NUM))) return st;
 
     rp.seed = c->opt.seed;
     rp.eabs = c->opt.eabs;
     rp.sigma = c->opt.noncol_sigma;
     rp.prange = c->opt.positron_range;
-    rp.trace = want_trace ? c->d_trace.p : nullptr;
-    rp.trace_cap = ntr;
     rp.cnt = c->d_cnt.p;
+    rp.phs = batched ? c->d_phs.p : nullptr;
 
     PanelParams pp;
+    memset(&pp, 0, sizeof(pp));
     pp.seed = c->opt.seed;
     pp.eabs = c->opt.eabs;
     pp.readout_depth = c->dig.readout_depth;
     pp.readout_policy = c->dig.readout_policy;
     pp.record_rayleigh = c->dig.record_rayleigh;
     pp.n_events = c->d_small.p + 1;
-    pp.hit_id = want_hits ? c->d_hit_id.p : nullptr;
-    pp.hit_f = want_hits ? c->d_hit_f.p : nullptr;
     pp.n_hits = c->d_small.p + 2;
-    pp.hits_cap = hits_cap;
-    pp.trace = rp.trace;
     pp.cnt = c->d_cnt.p;
 
-    const double *psf_base = rp.psf;
+    double tune_s = 0;
+    if ((c->opt.flags & GPET_FLAG_AUTOTUNE) && !c->tuned[batched ? 1 : 0] && n_hist > 0) {
+        if ((st = autotune(c, rp, n_hist, pp, bi, &tune_s))) return st;
+        c->tuned[batched ? 1 : 0] = true;
+    }
+    const gpet_launch L = c->launch;
+    const int64_t batch = L.batch;
+    const int64_t qcap = 2 * std::min<int64_t>(batch, std::max<int64_t>(n_hist, 1));
+    if ((st = alloc_queue(c, qcap))) return st;
+    const bool want_trace = sink && sink->trace && sink->trace_cap > 0;
+    int64_t ntr = want_trace ? std::min<int64_t>(2 * n_hist, sink->trace_cap) : 0;
+    if (want_trace)
+        if ((st = dalloc(c, c->d_trace, ntr)) || (st = dalloc(c, c->q_slot, qcap))) return st;
+    const bool want_hits = c->opt.record_hits != 0;
+    int64_t hits_cap = 0;
+    if (want_hits) {
+        hits_cap = (sink && sink->hit_id && sink->hits_cap > 0) ? sink->hits_cap : (int64_t)1 << 24;
+        if ((st = dalloc(c, c->d_hit_id, 5 * hits_cap)) || (st = dalloc(c, c->d_hit_f, 5 * hits_cap))) return st;
+    }
+    DevQueue q = make_queue(c, qcap, want_trace);
+    rp.trace = want_trace ? c->d_trace.p : nullptr;
+    rp.trace_cap = ntr;
+    pp.hit_id = want_hits ? c->d_hit_id.p : nullptr;
+    pp.hit_f = want_hits ? c->d_hit_f.p : nullptr;
+    pp.hits_cap = hits_cap;
+    pp.trace = rp.trace;
+
+    /* per-batch launch arguments (and the segment tables of a batched run) */
     const int64_t nbatch = (n_hist + batch - 1) / batch;
-    while ((int64_t)c->kev.size() < 4 * nbatch) {
+    std::vector<BatchLaunch> B((size_t)nbatch);
+    {
+        std::vector<DevSegment> segs;
+        std::vector<int64_t> seg_off((size_t)nbatch, 0), nvb((size_t)nbatch, 0);
+        std::vector<int> nseg((size_t)nbatch, 0);
+        for (int64_t b = 0; b < nbatch; ++b) {
+            const int64_t h0 = b * batch, nh = std::min(batch, n_hist - h0);
+            if (batched) {
+                seg_off[b] = (int64_t)segs.size();
+                build_segments(*bi, h0, nh, L.block[0], segs, nvb[b]);
+                nseg[b] = (int)(segs.size() - seg_off[b]);
+            }
+        }
+        if (batched && (st = upload(c, c->d_seg, segs.data(), (int64_t)segs.size()))) return st;
+        for (int64_t b = 0; b < nbatch; ++b) {
+            BatchLaunch &X = B[b];
+            X.rp = rp;
+            X.rp.h0 = b * batch;
+            X.nh = X.rp.nh = std::min(batch, n_hist - X.rp.h0);
+            if (rp.psf) X.rp.psf = rp.psf + 8 * X.rp.h0;
+            if (batched) {
+                X.rp.seg = c->d_seg.p + seg_off[b];
+                X.rp.nseg = nseg[b];
+                X.rp.n_vblocks = nvb[b];
+            }
+        }
+    }
+    while ((int64_t)c->kev.size() < 4 * std::max<int64_t>(nbatch, 1)) {
         hipEvent_t e;
         HIPOK(c, hipEventCreate(&e));
         c->kev.push_back(e);
     }
-    int64_t ib = 0;
-    HIPOK(c, hipEventRecord(c->ev0, c->stream));
+
+    /* Event capacity: from the last run's rate when known, else measured on
+     * batch 0.  A run whose events overflow the estimate is run again with
+     * the exact capacity (same histories, same results). */
+    const int64_t per_batch_max = (int64_t)GPET_MAX_PH_EVENTS * qcap;
+    int64_t ev_need = per_batch_max;
+    if (c->ev_per_hist >= 0) ev_need = (int64_t)(c->ev_per_hist * 1.05 * (double)n_hist) + per_batch_max;
+    if ((st = dgrow(c, c->d_events, std::max<int64_t>(ev_need, 1 << 20), 0))) return st;
+
     unsigned small[4] = {0, 0, 0, 0};
-    for (int64_t b0 = 0; b0 < n_hist; b0 += batch) {
-        rp.h0 = b0;
-        rp.nh = std::min(batch, n_hist - b0);
-        if (psf_base) rp.psf = psf_base + 8 * b0;
-        HIPOK(c, hipMemsetAsync(c->d_small.p, 0, sizeof(unsigned), c->stream));
-        HIPOK(c, hipEventRecord(c->kev[4 * ib], c->stream));
-        int e = gpet_launch_phantom(rp, c->dtab, c->dph, c->ddet, q, c->stream);
-        if (e) return fail(c, GPET_ERR_DEVICE, std::string("phantom kernel: ") + hipGetErrorString((hipError_t)e));
-        HIPOK(c, hipEventRecord(c->kev[4 * ib + 1], c->stream));
-        HIPOK(c, hipMemcpyAsync(small, c->d_small.p, sizeof(small), hipMemcpyDeviceToHost, c->stream));
-        HIPOK(c, hipStreamSynchronize(c->stream));
-        int64_t nq = std::min<int64_t>(small[0], qcap);
-        int64_t ne = small[1];
-        if ((st = dgrow(c, c->d_events, ne + GPET_MAX_PH_EVENTS * nq, ne))) return st;
+    unsigned long long cnt[CNT_NUM];
+    bool used_graph = false;
+    for (int attempt = 0;; ++attempt) {
+        HIPOK(c, hipMemsetAsync(c->d_small.p, 0, sizeof(unsigned) * 4, c->stream));
+        HIPOK(c, hipMemsetAsync(c->d_cnt.p, 0, sizeof(unsigned long long) * CNT_NUM, c->stream));
+        if (want_trace) HIPOK(c, hipMemsetAsync(c->d_trace.p, 0, sizeof(gpet_trace) * (size_t)ntr, c->stream));
         pp.events = c->d_events.p;
         pp.events_cap = c->d_events.cap;
-        HIPOK(c, hipEventRecord(c->kev[4 * ib + 2], c->stream));
-        e = gpet_launch_panel(pp, c->dtab, c->ddet, q, nq, c->stream);
-        if (e) return fail(c, GPET_ERR_DEVICE, std::string("panel kernel: ") + hipGetErrorString((hipError_t)e));
-        HIPOK(c, hipEventRecord(c->kev[4 * ib + 3], c->stream));
-        ++ib;
+        HIPOK(c, hipEventRecord(c->ev0, c->stream));
+        used_graph = false;
+        if (nbatch > 0) {
+            if ((st = launch_batch(c, B[0], q, pp, L, batched, &c->kev[0]))) return st;
+            if (nbatch > 1 && c->ev_per_hist < 0 && attempt == 0) {
+                HIPOK(c, hipMemcpyAsync(small, c->d_small.p, sizeof(small), hipMemcpyDeviceToHost, c->stream));
+                HIPOK(c, hipStreamSynchronize(c->stream));
+                const double rate = (double)small[1] / (double)B[0].nh;
+                const int64_t need = (int64_t)small[1] + (int64_t)(rate * 1.1 * (double)(n_hist - B[0].nh)) + per_batch_max;
+                if ((st = dgrow(c, c->d_events, need, (int64_t)small[1]))) return st;
+                pp.events = c->d_events.p;
+                pp.events_cap = c->d_events.cap;
+            }
+            if (nbatch > 1) {
+                if (!(c->opt.flags & GPET_FLAG_NO_GRAPH)) used_graph = run_graph(c, B, q, pp, L, batched);
+                if (!used_graph)
+                    for (int64_t b = 1; b < nbatch; ++b)
+                        if ((st = launch_batch(c, B[b], q, pp, L, batched, &c->kev[4 * b]))) return st;
+            }
+        }
+        HIPOK(c, hipEventRecord(c->ev1, c->stream));
+        HIPOK(c, hipMemcpyAsync(small, c->d_small.p, sizeof(small), hipMemcpyDeviceToHost, c->stream));
+        HIPOK(c, hipMemcpyAsync(cnt, c->d_cnt.p, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+        HIPOK(c, hipStreamSynchronize(c->stream));
+        if ((int64_t)small[1] <= c->d_events.cap) break;
+        if (attempt >= 1) return fail(c, GPET_ERR_CAPACITY, "event buffer overflow after resize");
+        if ((st = dgrow(c, c->d_events, (int64_t)small[1] + per_batch_max, 0))) return st;
     }
-    HIPOK(c, hipEventRecord(c->ev1, c->stream));
-    HIPOK(c, hipMemcpyAsync(small, c->d_small.p, sizeof(small), hipMemcpyDeviceToHost, c->stream));
-    unsigned long long cnt[CNT_NUM];
-    HIPOK(c, hipMemcpyAsync(cnt, c->d_cnt.p, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
-    HIPOK(c, hipStreamSynchronize(c->stream));
-    c->n_events = std::min<int64_t>(small[1], c->d_events.cap);
+    c->n_events = small[1];
+    if (n_hist > 0) c->ev_per_hist = (double)small[1] / (double)n_hist;
 
     DigiParams dp;
+    memset(&dp, 0, sizeof(dp));
     dp.seed = c->opt.seed;
     dp.d = c->dig;
     dp.moduleN = c->ddet.moduleN;
     dp.crystalN = c->ddet.crystalN;
+    const int njobs = batched ? (int)bi->job_first.size() - 1 : 1;
+    if (batched) {
+        if ((st = upload(c, c->d_job_first, bi->job_first.data(), (int64_t)bi->job_first.size()))) return st;
+        if ((st = dalloc(c, c->d_job_cnt, 2 * (int64_t)(njobs + 1)))) return st;
+        dp.job_first = c->d_job_first.p;
+        dp.njobs = njobs;
+        dp.job_bounds = c->d_job_cnt.p;
+    }
     char err[256] = {0};
     if (gpet_digitize(c->dw, dp, c->d_events.p, c->n_events, &c->dout, c->stream, err, sizeof(err)))
         return fail(c, GPET_ERR_DEVICE, std::string("digitizer: ") + err);
@@ -443,17 +827,34 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
     T.panel_steps = (int64_t)cnt[CNT_PANEL_STEPS];
     T.seconds_transport = ms_t * 1e-3;
     T.seconds_digitizer = ms_d * 1e-3;
-    for (int64_t i = 0; i < ib; ++i) {
+    /* eager launches carry event timing; graph replays do not */
+    const int64_t n_timed = used_graph ? std::min<int64_t>(nbatch, 1) : nbatch;
+    for (int64_t i = 0; i < n_timed; ++i) {
         float a = 0, b = 0;
         HIPOK(c, hipEventElapsedTime(&a, c->kev[4 * i], c->kev[4 * i + 1]));
         HIPOK(c, hipEventElapsedTime(&b, c->kev[4 * i + 2], c->kev[4 * i + 3]));
         T.kernel_seconds[0] += a * 1e-3;
         T.kernel_seconds[1] += b * 1e-3;
+        T.kernel_histories += B[i].nh;
     }
-    T.kernel_launches[0] = ib;
-    T.kernel_launches[1] = ib;
+    T.kernel_launches[0] = n_timed;
+    T.kernel_launches[1] = n_timed;
+    T.launches = nbatch;
+    T.launch = L;
+    T.graph = used_graph ? 1 : 0;
+    T.n_jobs = njobs;
+    T.seconds_tune = tune_s;
 
     if (!sink) return GPET_OK;
+    if (batched && sink->job_counts) {
+        std::vector<unsigned long long> jb(2 * (size_t)(njobs + 1));
+        HIPOK(c, hipMemcpy(jb.data(), c->d_job_cnt.p, sizeof(unsigned long long) * jb.size(), hipMemcpyDeviceToHost));
+        for (int j = 0; j < njobs; ++j) {
+            sink->job_counts[3 * j] = bi->job_first[j + 1] - bi->job_first[j];
+            sink->job_counts[3 * j + 1] = (int64_t)(jb[j + 1] - jb[j]);
+            sink->job_counts[3 * j + 2] = (int64_t)(jb[njobs + 1 + j + 1] - jb[njobs + 1 + j]);
+        }
+    }
     const bool to_files = sink->out_dir && sink->out_dir[0];
     std::vector<gpet_event> adder, singles, coinc;
     std::vector<int32_t> hid;
@@ -522,12 +923,19 @@ gpet_status gpet_create(gpet_ctx **out, const gpet_options *opt)
     memset(&c->dtab, 0, sizeof(c->dtab));
     memset(&c->dph, 0, sizeof(c->dph));
     memset(&c->ddet, 0, sizeof(c->ddet));
+    c->launch = default_launch();
+    if (opt->batch > 0) c->launch.batch = std::min<int64_t>(std::max<int64_t>(opt->batch, 256), (int64_t)1 << 26);
+    c->slots.resize(1);
+    memset(&c->slots[0].dp, 0, sizeof(DevPhantom));
     if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreate(&c->ev2) != hipSuccess) {
         delete c;
         return GPET_ERR_DEVICE;
     }
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess && ncu > 0)
+        c->n_cu = ncu;
     c->dw = gpet_digi_work_create();
     *out = c;
     return GPET_OK;
@@ -539,7 +947,12 @@ gpet_status gpet_destroy(gpet_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     dfree(c->d_mu4); dfree(c->d_surf[0]); dfree(c->d_surf[1]);
-    dfree(c->d_vox); dfree(c->d_dens_b); dfree(c->d_wck_ph); dfree(c->d_panels); dfree(c->d_wck_det);
+    for (PhSlot &sl : c->slots) {
+        dfree(sl.vox); dfree(sl.dens_b); dfree(sl.wck);
+    }
+    dfree(c->d_phs); dfree(c->d_seg); dfree(c->d_job_first); dfree(c->d_job_cnt);
+    drop_graph(c);
+    dfree(c->d_panels); dfree(c->d_wck_det);
     for (auto &b : c->q_f) dfree(b);
     dfree(c->q_t); dfree(c->q_info); dfree(c->q_slot); dfree(c->d_small); dfree(c->d_cnt);
     dfree(c->d_events); dfree(c->d_trace); dfree(c->d_hit_id); dfree(c->d_hit_f);
@@ -604,13 +1017,20 @@ gpet_status gpet_load_tables(gpet_ctx *c, const char *dir)
     return st;
 }
 
-gpet_status gpet_set_phantom(gpet_ctx *c, const int32_t *mat, const float *dens, const int32_t dims[3],
-                             const float off[3], const float size[3])
+gpet_status gpet_set_phantom_slot(gpet_ctx *c, int32_t slot, const int32_t *mat, const float *dens,
+                                  const int32_t dims[3], const float off[3], const float size[3])
 {
     if (!c || !mat || !dens || !dims || !off || !size) return GPET_ERR_ARG;
+    if (slot < 0 || slot >= GPET_MAXPHANTOMS) return fail(c, GPET_ERR_ARG, "phantom slot out of range");
     if (!c->has_tab) return fail(c, GPET_ERR_STATE, "tables must be set before the phantom");
     if (dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0) return fail(c, GPET_ERR_ARG, "phantom dims");
     HIPOK(c, hipSetDevice(c->device));
+    if ((int)c->slots.size() <= slot) {
+        size_t n0 = c->slots.size();
+        c->slots.resize((size_t)slot + 1);
+        for (size_t k = n0; k < c->slots.size(); ++k) memset(&c->slots[k].dp, 0, sizeof(DevPhantom));
+    }
+    PhSlot &S = c->slots[slot];
     const size_t nv = (size_t)dims[0] * dims[1] * dims[2];
     std::vector<int2> vox(nv);
     float rmax[GPET_MAXMAT] = {0};
@@ -622,7 +1042,7 @@ gpet_status gpet_set_phantom(gpet_ctx *c, const int32_t *mat, const float *dens,
         memcpy(&b, &dens[i], 4);
         vox[i] = make_int2(b, m);
     }
-    gpet_status st = upload(c, c->d_vox, vox.data(), (int64_t)nv);
+    gpet_status st = upload(c, S.vox, vox.data(), (int64_t)nv);
     if (st) return st;
     const int nbx = (dims[0] + 3) / 4, nby = (dims[1] + 3) / 4, nbz = (dims[2] + 3) / 4;
     {
@@ -631,23 +1051,35 @@ gpet_status gpet_set_phantom(gpet_ctx *c, const int32_t *mat, const float *dens,
             for (int iy = 0; iy < dims[1]; ++iy)
                 for (int ix = 0; ix < dims[0]; ++ix)
                     db[gpet_brick_offset(ix, iy, iz, nbx, nby)] = dens[((size_t)iz * dims[1] + iy) * dims[0] + ix];
-        if ((st = upload(c, c->d_dens_b, db.data(), (int64_t)db.size()))) return st;
+        if ((st = upload(c, S.dens_b, db.data(), (int64_t)db.size()))) return st;
     }
-    DevPhantom &p = c->dph;
-    p.dens_b = c->d_dens_b.p;
+    DevPhantom &p = S.dp;
+    memset(&p, 0, sizeof(p));
+    p.dens_b = S.dens_b.p;
     p.nbx = nbx;
     p.nby = nby;
-    build_majorant(c, rmax, c->h_wck_ph, p.wck.e0, p.wck.idl);
-    if ((st = upload(c, c->d_wck_ph, c->h_wck_ph.data(), GPET_NWCK))) return st;
-    p.wck.lam = c->d_wck_ph.p;
+    build_majorant(c, rmax, S.h_wck, p.wck.e0, p.wck.idl);
+    if ((st = upload(c, S.wck, S.h_wck.data(), GPET_NWCK))) return st;
+    p.wck.lam = S.wck.p;
     p.nx = dims[0]; p.ny = dims[1]; p.nz = dims[2];
     p.offx = off[0]; p.offy = off[1]; p.offz = off[2];
     p.dx = size[0] / (float)p.nx; p.dy = size[1] / (float)p.ny; p.dz = size[2] / (float)p.nz;
     p.idx = 1.0f / p.dx; p.idy = 1.0f / p.dy; p.idz = 1.0f / p.dz;
-    p.vox = c->d_vox.p;
+    p.vox = S.vox.p;
     p.present = 1;
-    c->has_ph = true;
+    S.present = true;
+    c->phs_dirty = true;
+    if (slot == 0) {
+        c->dph = p;
+        c->has_ph = true;
+    }
     return GPET_OK;
+}
+
+gpet_status gpet_set_phantom(gpet_ctx *c, const int32_t *mat, const float *dens, const int32_t dims[3],
+                             const float off[3], const float size[3])
+{
+    return gpet_set_phantom_slot(c, 0, mat, dens, dims, off, size);
 }
 
 gpet_status gpet_set_detector(gpet_ctx *c, const gpet_panel *panels, int32_t n, const gpet_panel_material mats[2],
@@ -714,27 +1146,86 @@ gpet_status gpet_run_source(gpet_ctx *c, const gpet_source *src, int32_t ns, con
     if (st) return st;
     if (!src || ns <= 0 || !iso || niso <= 0 || !(t1 >= t0)) return fail(c, GPET_ERR_ARG, "source arguments");
     HIPOK(c, hipSetDevice(c->device));
-    std::vector<int64_t> first;
+    std::vector<int64_t> cnt;
     std::vector<DevSource> ds;
-    if ((st = plan_sources(c, src, ns, iso, niso, t0, t1, n_decays, first, ds))) return st;
-    std::vector<DevIsotope> di(niso);
-    for (int i = 0; i < niso; ++i) {
-        di[i].ratio = iso[i].beta_ratio;
-        for (int k = 0; k < 8; ++k) di[i].coef[k] = iso[i].coef[k];
-    }
-    if ((st = upload(c, c->d_first, first.data(), ns + 1)) || (st = upload(c, c->d_src, ds.data(), ns)) ||
-        (st = upload(c, c->d_iso, di.data(), niso)))
-        return st;
+    if ((st = plan_job(c, src, ns, iso, niso, t0, t1, n_decays, 0, 0, cnt, ds))) return st;
+    std::vector<int64_t> first(ns + 1, 0);
+    for (int i = 0; i < ns; ++i) first[i + 1] = first[i] + cnt[i];
+    if ((st = upload_run_sources(c, first, ds, iso, niso))) return st;
     RunParams rp;
     memset(&rp, 0, sizeof(rp));
     rp.mode = RUN_DECAY;
-    rp.t0 = t0;
     rp.nsrc = ns;
     rp.first = c->d_first.p;
     rp.src = c->d_src.p;
     rp.iso = c->d_iso.p;
     rp.psf = nullptr;
-    return run_common(c, rp, first[ns], sink);
+    return run_common(c, rp, first[ns], sink, nullptr);
+}
+
+gpet_status gpet_run_batch(gpet_ctx *c, const gpet_job *jobs, int32_t njobs, const gpet_source *src, int32_t nsrc,
+                           const gpet_isotope *iso, int32_t niso, gpet_sink *sink)
+{
+    gpet_status st = check_ready(c);
+    if (st) return st;
+    if (!jobs || njobs <= 0 || !src || nsrc <= 0 || !iso || niso <= 0) return fail(c, GPET_ERR_ARG, "batch arguments");
+    HIPOK(c, hipSetDevice(c->device));
+    BatchInfo bi;
+    std::vector<int64_t> cnt;
+    std::vector<DevSource> ds;
+    bi.job_first.assign(1, 0);
+    for (int j = 0; j < njobs; ++j) {
+        const gpet_job &J = jobs[j];
+        if (J.first_source < 0 || J.n_sources <= 0 || J.first_source + J.n_sources > nsrc || !(J.t1 >= J.t0))
+            return fail(c, GPET_ERR_ARG, "job source range or time window");
+        if (J.phantom < 0 || J.phantom >= (int)c->slots.size() || !c->slots[J.phantom].present)
+            return fail(c, GPET_ERR_STATE, "job phantom slot not set");
+        const size_t before = cnt.size();
+        if ((st = plan_job(c, src + J.first_source, J.n_sources, iso, niso, J.t0, J.t1, J.n_decays, J.first_source,
+                           J.phantom, cnt, ds)))
+            return st;
+        int64_t tot = 0;
+        for (size_t k = before; k < cnt.size(); ++k) {
+            tot += cnt[k];
+            bi.src_phantom.push_back(J.phantom);
+        }
+        bi.job_first.push_back(bi.job_first.back() + tot);
+    }
+    const int ns = (int)cnt.size();
+    bi.first.assign(ns + 1, 0);
+    for (int i = 0; i < ns; ++i) bi.first[i + 1] = bi.first[i] + cnt[i];
+    if ((st = upload_run_sources(c, bi.first, ds, iso, niso))) return st;
+    if (c->phs_dirty) {
+        std::vector<DevPhantom> dp(c->slots.size());
+        for (size_t k = 0; k < dp.size(); ++k) dp[k] = c->slots[k].dp;
+        if ((st = upload(c, c->d_phs, dp.data(), (int64_t)dp.size()))) return st;
+        c->phs_dirty = false;
+    }
+    RunParams rp;
+    memset(&rp, 0, sizeof(rp));
+    rp.mode = RUN_DECAY;
+    rp.nsrc = ns;
+    rp.first = c->d_first.p;
+    rp.src = c->d_src.p;
+    rp.iso = c->d_iso.p;
+    rp.psf = nullptr;
+    return run_common(c, rp, bi.first[ns], sink, &bi);
+}
+
+gpet_status gpet_set_launch(gpet_ctx *c, const gpet_launch *l)
+{
+    if (!c || !l) return GPET_ERR_ARG;
+    gpet_status st = check_launch(c, *l);
+    if (st) return st;
+    c->launch = *l;
+    return GPET_OK;
+}
+
+gpet_status gpet_get_launch(gpet_ctx *c, gpet_launch *l)
+{
+    if (!c || !l) return GPET_ERR_ARG;
+    *l = c->launch;
+    return GPET_OK;
 }
 
 gpet_status gpet_run_psf(gpet_ctx *c, const double *rec8, int64_t n, int32_t ptype, gpet_sink *sink)
@@ -748,7 +1239,7 @@ gpet_status gpet_run_psf(gpet_ctx *c, const double *rec8, int64_t n, int32_t pty
     memset(&rp, 0, sizeof(rp));
     rp.mode = ptype == 0 ? RUN_PSF_POSITRON : RUN_PSF_PHOTON;
     rp.psf = c->d_psf.p;
-    return run_common(c, rp, n, sink);
+    return run_common(c, rp, n, sink, nullptr);
 }
 
 gpet_status gpet_get_tallies(gpet_ctx *c, gpet_tallies *t)
@@ -823,10 +1314,12 @@ gpet_status gpet_build_majorant(const gpet_tables *t, const float *rho_max, floa
 gpet_status gpet_get_majorant(gpet_ctx *c, int32_t which, float *out, float *e0, float *idl)
 {
     if (!c || !out) return GPET_ERR_ARG;
-    const std::vector<float> &v = which == 0 ? c->h_wck_ph : c->h_wck_det;
+    if (which < 0 || (which >= 2 && which - 2 >= (int)c->slots.size())) return GPET_ERR_ARG;
+    const int slot = which == 0 ? 0 : which - 2;
+    const std::vector<float> &v = which == 1 ? c->h_wck_det : c->slots[slot].h_wck;
     if (v.empty()) return GPET_ERR_STATE;
     memcpy(out, v.data(), sizeof(float) * GPET_NWCK);
-    const DevWck &w = which == 0 ? c->dph.wck : c->ddet.wck;
+    const DevWck &w = which == 1 ? c->ddet.wck : c->slots[slot].dp.wck;
     if (e0) *e0 = w.e0;
     if (idl) *idl = w.idl;
     return GPET_OK;

@@ -76,13 +76,22 @@ struct DevDetector {
 
 struct DevSource {
     double tau, pdec;          /* T1/2 / ln 2 (s); 1 - 2^(-(t1-t0)/T1/2)      */
+    double t0;                 /* start of the source's acquisition (s)       */
     int shape, iso;
+    int phantom;               /* phantom slot (batched runs)                 */
     float coef[6];
 };
 
 struct DevIsotope {
     float ratio;
     float coef[8];
+};
+
+/* A history range of one phantom slot inside a batched launch; vb0 is its
+ * first virtual block (blocks of the launch's block size). */
+struct DevSegment {
+    int64_t h0, nh, vb0;
+    int phantom, pad;
 };
 
 /* What the phantom kernel produces for the panel kernel. */
@@ -109,11 +118,14 @@ struct RunParams {
     uint64_t seed;
     float eabs, sigma;
     int prange;
-    double t0;
     int nsrc;
     const int64_t *first;      /* [nsrc+1] first history per source            */
     const DevSource *src;
     const DevIsotope *iso;
+    const DevPhantom *phs;     /* batched runs: phantom slots                 */
+    const struct DevSegment *seg; /* batched runs: segments of this launch    */
+    int nseg;
+    int64_t n_vblocks;         /* batched runs: virtual blocks of this launch */
     const double *psf;         /* mode 1/2: records of 8 doubles for [h0,...)  */
     gpet_trace *trace;         /* device trace buffer or null                  */
     int64_t trace_cap;         /* slots: 2*(h - 0) < trace_cap are written     */
@@ -135,11 +147,28 @@ struct PanelParams {
     unsigned long long *cnt;
 };
 
-/* Launchers (transport.hip). */
+/* Launch shape of the two transport kernels: block size, a register
+ * budget variant (minimum waves per SIMD, 0 = compiler default) and the
+ * number of resident blocks per CU the grid-stride grid is sized for. */
+struct LaunchCfg {
+    int block = 256;
+    int wpe = 0;
+    int blocks_per_cu = 0;     /* 0: one lane per item (no grid-stride reuse) */
+};
+#define GPET_N_WPE 3
+extern const int gpet_wpe_variants[GPET_N_WPE];
+
+/* Launchers (transport.hip).  batch: sources carry phantom slots (rp.phs).
+ * The panel kernel reads its item count from q.count on the device, so the
+ * pair can be captured in a graph without a host round trip. */
 int gpet_launch_phantom(const RunParams &rp, const DevTables &tab, const DevPhantom &ph,
-                        const DevDetector &det, const DevQueue &q, hipStream_t s);
+                        const DevDetector &det, const DevQueue &q, bool batch, const LaunchCfg &cfg,
+                        int n_cu, hipStream_t s);
 int gpet_launch_panel(const PanelParams &pp, const DevTables &tab, const DevDetector &det,
-                      const DevQueue &q, int64_t max_items, hipStream_t s);
+                      const DevQueue &q, int64_t max_items, const LaunchCfg &cfg, int n_cu, hipStream_t s);
+/* Sets the first `n` entries of `p` to zero (graph-safe: a kernel, not a
+ * memset node). */
+int gpet_launch_zero(unsigned *p, int n, hipStream_t s);
 
 /* Digitizer (digitizer.hip).  Works on the device event list; returns
  * counts through the pointers. */
@@ -147,6 +176,13 @@ struct DigiParams {
     uint64_t seed;
     gpet_digitizer d;
     int moduleN, crystalN;
+    /* batched runs: job j owns histories [job_first[j], job_first[j+1]);
+     * deadtime and coincidences stay inside a job, singles come out in
+     * (job, t) order and job_bounds receives 2 x (njobs + 1) prefix offsets
+     * {singles..., pairs...}.  njobs 0 or 1: one acquisition. */
+    const int64_t *job_first;
+    int njobs;
+    unsigned long long *job_bounds;
 };
 struct DigiOut {
     gpet_event *singles;       /* device, owned by the caller of digitize     */

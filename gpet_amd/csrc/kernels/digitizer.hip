@@ -236,6 +236,49 @@ __global__ __launch_bounds__(256) void k_compact_events(const gpet_event *in, co
     if (i < n && flag[i]) out[pos[i]] = in[i];
 }
 
+/* Job of a history in a batched run: the largest j with jf[j] <= h. */
+__device__ __forceinline__ int d_job_of(const int64_t *jf, int nj, int64_t h)
+{
+    if (nj <= 1) return 0;
+    int lo = 0, hi = nj;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (jf[mid] <= h) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+/* Job of the event at perm[j] as the key of a stable pass after the time
+ * sort: the order becomes (job, t). */
+__global__ __launch_bounds__(256) void k_job_keys(const gpet_event *ev, const unsigned *perm, unsigned long long *key,
+                                                  int64_t n, const int64_t *jf, int nj)
+{
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) key[j] = (unsigned long long)d_job_of(jf, nj, ev[perm[j]].eventid);
+}
+
+/* First single and first pair of every job (lower bounds over the
+ * (job, t)-ordered lists); entry nj is the list length. */
+__global__ void k_job_bounds(const gpet_event *singles, int64_t ns, const gpet_event *pairs, int64_t np,
+                             const int64_t *jf, int nj, unsigned long long *out)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > nj) return;
+    int64_t lo = 0, hi = ns;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (d_job_of(jf, nj, singles[mid].eventid) < j) lo = mid + 1; else hi = mid;
+    }
+    out[j] = (unsigned long long)lo;
+    lo = 0;
+    hi = np;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (d_job_of(jf, nj, pairs[2 * mid].eventid) < j) lo = mid + 1; else hi = mid;
+    }
+    out[nj + 1 + j] = (unsigned long long)lo;
+}
+
 __global__ __launch_bounds__(256) void k_time_keys(const gpet_event *ev, unsigned long long *key, unsigned *val, int64_t n)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -261,12 +304,12 @@ __global__ __launch_bounds__(256) void k_set_site(gpet_event *ev, int64_t n, int
 }
 
 __global__ __launch_bounds__(256) void k_site_keys(const gpet_event *ev, const unsigned *perm, unsigned long long *key,
-                                                   unsigned *val, int64_t n)
+                                                   unsigned *val, int64_t n, const int64_t *jf, int nj)
 {
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     unsigned idx = perm[j];
-    key[j] = (unsigned long long)(unsigned)ev[idx].siten;
+    key[j] = ((unsigned long long)d_job_of(jf, nj, ev[idx].eventid) << 32) | (unsigned long long)(unsigned)ev[idx].siten;
     val[j] = idx;
 }
 
@@ -318,11 +361,16 @@ __global__ __launch_bounds__(256) void k_gather_singles(const gpet_event *ev, co
  * singles with t - t_open <= W; exactly two -> a pair, more -> policy. */
 template <bool WRITE>
 __global__ __launch_bounds__(256) void k_coinc(const gpet_event *s, int64_t n, double W, int policy, unsigned *cnt,
-                                               const unsigned *pos, gpet_event *out, unsigned long long *ntrue)
+                                               const unsigned *pos, gpet_event *out, unsigned long long *ntrue,
+                                               const int64_t *jf, int nj)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    bool start = (i == 0) || (s[i].t - s[i - 1].t > W);
+    /* singles are in (job, t) order; a job change closes every window */
+    auto same_job = [&](int64_t a, int64_t b) {
+        return nj <= 1 || d_job_of(jf, nj, s[a].eventid) == d_job_of(jf, nj, s[b].eventid);
+    };
+    bool start = (i == 0) || (s[i].t - s[i - 1].t > W) || !same_job(i, i - 1);
     if (!start) {
         if (!WRITE) cnt[i] = 0;
         return;
@@ -332,7 +380,7 @@ __global__ __launch_bounds__(256) void k_coinc(const gpet_event *s, int64_t n, d
     int64_t a = i;
     while (a < n) {
         int64_t b = a + 1;
-        while (b < n && s[b].t - s[a].t <= W) ++b;
+        while (b < n && s[b].t - s[a].t <= W && same_job(b, a)) ++b;
         int64_t m = b - a;
         if (m == 2 || (m > 2 && policy == GPET_COINC_TAKE_ALL_GOODS)) {
             for (int64_t p = a; p < b; ++p)
@@ -346,7 +394,7 @@ __global__ __launch_bounds__(256) void k_coinc(const gpet_event *s, int64_t n, d
                 }
         }
         a = b;
-        if (a >= n || s[a].t - s[a - 1].t > W) break;
+        if (a >= n || s[a].t - s[a - 1].t > W || !same_job(a, a - 1)) break;
     }
     if (!WRITE) cnt[i] = np;
     else if (nt) atomicAdd(ntrue, (unsigned long long)nt);
@@ -503,6 +551,9 @@ int gpet_digitize(DigiWork *w, const DigiParams &p, const gpet_event *events, in
 {
     const gpet_digitizer &g = p.d;
     *out = DigiOut();
+    const int64_t *jf = p.job_first;
+    const int nj = (p.job_first && p.njobs > 1) ? p.njobs : 0;
+    const int job_bits = nj > 65536 ? 24 : (nj > 256 ? 16 : 8);
     int64_t c1 = w->cap, c2 = w->cap, c3 = w->cap, c4 = w->cap, c5 = w->cap, c6 = w->cap, c7 = w->cap, c8 = w->cap;
     int64_t need = std::max<int64_t>(n, 1);
     if (need > w->cap) {
@@ -543,6 +594,11 @@ int gpet_digitize(DigiWork *w, const DigiParams &p, const gpet_event *events, in
         hipLaunchKernelGGL(k_time_keys, dim3(grid_for(m, 256)), dim3(256), 0, s, ev, w->key_a, w->val_a, m);
         HIPCHK(hipGetLastError());
         if (radix_sort(w, m, 64, s, err, errlen)) return -1;
+        if (nj > 1) {
+            hipLaunchKernelGGL(k_job_keys, dim3(grid_for(m, 256)), dim3(256), 0, s, ev, w->val_a, w->key_a, m, jf, nj);
+            HIPCHK(hipGetLastError());
+            if (radix_sort(w, m, job_bits, s, err, errlen)) return -1;
+        }
     }
     /* val_a = perm_t.  Keep it in `pos`-independent storage: copy to val_b? we
      * sort sites on (key_b,val_b) after moving perm_t aside into `alive` ... */
@@ -553,9 +609,10 @@ int gpet_digitize(DigiWork *w, const DigiParams &p, const gpet_event *events, in
         /* move perm_t out of the sort buffers */
         HIPCHK(hipMemcpyAsync(w->pos, perm_t, sizeof(unsigned) * m, hipMemcpyDeviceToDevice, s));
         perm_t = w->pos;
-        hipLaunchKernelGGL(k_site_keys, dim3(grid_for(m, 256)), dim3(256), 0, s, ev, perm_t, w->key_a, w->val_a, m);
+        hipLaunchKernelGGL(k_site_keys, dim3(grid_for(m, 256)), dim3(256), 0, s, ev, perm_t, w->key_a, w->val_a, m,
+                           jf, nj);
         HIPCHK(hipGetLastError());
-        if (radix_sort(w, m, 32, s, err, errlen)) return -1;
+        if (radix_sort(w, m, nj > 1 ? 32 + job_bits : 32, s, err, errlen)) return -1;
         HIPCHK(hipMemsetAsync(w->alive, 0, sizeof(unsigned) * m, s));
         hipLaunchKernelGGL(k_deadtime, dim3(grid_for(m, 256)), dim3(256), 0, s, ev, w->val_a, w->key_a, w->alive, m,
                            (double)g.dead_time_us, g.dead_type);
@@ -604,12 +661,17 @@ int gpet_digitize(DigiWork *w, const DigiParams &p, const gpet_event *events, in
         unsigned *cpos = w->pos;
         hipLaunchKernelGGL(k_coinc<false>, dim3(grid_for(ns, 256)), dim3(256), 0, s, w->singles, (int64_t)ns,
                            g.coinc_window_us, g.coinc_policy, cnt, (const unsigned *)nullptr, (gpet_event *)nullptr,
-                           (unsigned long long *)nullptr);
+                           (unsigned long long *)nullptr, jf, nj);
         HIPCHK(hipGetLastError());
         if (scan(w, cnt, cpos, ns, &npairs, s, err, errlen)) return -1;
         if (ensure(w->coinc, w->coinc_cap, std::max<int64_t>(2 * (int64_t)npairs, 2), err, errlen)) return -1;
         hipLaunchKernelGGL(k_coinc<true>, dim3(grid_for(ns, 256)), dim3(256), 0, s, w->singles, (int64_t)ns,
-                           g.coinc_window_us, g.coinc_policy, cnt, cpos, w->coinc, w->ntrue);
+                           g.coinc_window_us, g.coinc_policy, cnt, cpos, w->coinc, w->ntrue, jf, nj);
+        HIPCHK(hipGetLastError());
+    }
+    if (nj > 1 && p.job_bounds) {
+        hipLaunchKernelGGL(k_job_bounds, dim3(grid_for(nj + 1, 64)), dim3(64), 0, s, w->singles, (int64_t)ns, w->coinc,
+                           (int64_t)npairs, jf, nj, p.job_bounds);
         HIPCHK(hipGetLastError());
     }
     unsigned long long nt = 0;

@@ -24,6 +24,7 @@
  * does not change any history's numbers.
  */
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include "dev_physics.h"
 
 namespace {
@@ -274,156 +275,205 @@ __device__ bool d_phantom_walk(const DevTables &tab, const DevPhantom &p, float 
     return false;
 }
 
-template <int MODE>
-__global__ __launch_bounds__(256) void k_phantom(RunParams rp, DevTables tab, DevPhantom ph, DevDetector det, DevQueue q)
+struct PhCounters {
+    unsigned long long ph = 0, ent = 0, maj = 0, ps = 0, rs = 0;
+};
+
+/* Source of history h: the largest si with first[si] <= h. */
+__device__ __forceinline__ int d_source_of(const RunParams &rp, int64_t h)
 {
-    unsigned long long c_ph = 0, c_ent = 0, c_maj = 0, c_ps = 0, c_rs = 0;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < rp.nh; i += stride) {
-        const int64_t h = rp.h0 + i;
-        float px = 0.f, py = 0.f, pz = 0.f;
-        float v[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
-        float e[2] = {0.f, 0.f};
-        double tstart = 0.0;
-        int nph = 0;
-        if (MODE == RUN_DECAY) {
-            int lo = 0, hi = rp.nsrc; /* largest si with first[si] <= h */
-            while (hi - lo > 1) {
-                int mid = (lo + hi) >> 1;
-                if (rp.first[mid] <= h) lo = mid; else hi = mid;
+    int lo = 0, hi = rp.nsrc;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (rp.first[mid] <= h) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+/* One history: decay (or PSF record i), positron range, pair, phantom walk
+ * of both photons, panel entry and queue append. */
+template <int MODE>
+__device__ __forceinline__ void d_history(const RunParams &rp, const DevTables &tab, const DevPhantom &ph,
+                                          const DevDetector &det, const DevQueue &q, int64_t i, int64_t h, int si,
+                                          PhCounters &C)
+{
+    unsigned long long &c_ph = C.ph, &c_ent = C.ent, &c_maj = C.maj, &c_ps = C.ps, &c_rs = C.rs;
+    float px = 0.f, py = 0.f, pz = 0.f;
+    float v[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+    float e[2] = {0.f, 0.f};
+    double tstart = 0.0;
+    int nph = 0;
+    if (MODE == RUN_DECAY) {
+        const DevSource S = rp.src[si];
+        const DevIsotope I = rp.iso[S.iso];
+        gp_rng r;
+        gp_rng_init(&r, rp.seed, (uint64_t)h, GP_TAG_DECAY, 0u);
+        double u = gp_rng_uniform_d(&r);
+        double ts = S.t0 + S.tau * (-gp_log1p_d(-u * S.pdec));
+        tstart = ts * 1.0e6;
+        float rb = gp_rng_uniform(&r);
+        if (rb < I.ratio) {
+            gp_rng_align(&r);
+            d_sample_shape(S, r, px, py, pz);
+            if (rp.prange) {
+                gp_rng_align(&r);
+                float ek = d_sample_ek(I, r);
+                gp_rng_align(&r);
+                d_positron_range(ph, px, py, pz, ek, 0, 0.f, 0.f, 0.f, r, c_rs);
             }
-            const DevSource S = rp.src[lo];
-            const DevIsotope I = rp.iso[S.iso];
+            gp_rng_align(&r);
+            d_emit_pair(rp.sigma, r, v, e);
+            nph = 2;
+        }
+    } else {
+        const double *rec = rp.psf + 8 * i;
+        px = (float)rec[0];
+        py = (float)rec[1];
+        pz = (float)rec[2];
+        tstart = rec[3];
+        float rvx = (float)rec[4], rvy = (float)rec[5], rvz = (float)rec[6], rE = (float)rec[7];
+        if (MODE == RUN_PSF_PHOTON) {
+            v[0][0] = rvx; v[0][1] = rvy; v[0][2] = rvz; e[0] = rE;
+            nph = 1;
+        } else {
             gp_rng r;
             gp_rng_init(&r, rp.seed, (uint64_t)h, GP_TAG_DECAY, 0u);
-            double u = gp_rng_uniform_d(&r);
-            double ts = rp.t0 + S.tau * (-gp_log1p_d(-u * S.pdec));
-            tstart = ts * 1.0e6;
-            float rb = gp_rng_uniform(&r);
-            if (rb < I.ratio) {
-                gp_rng_align(&r);
-                d_sample_shape(S, r, px, py, pz);
-                if (rp.prange) {
-                    gp_rng_align(&r);
-                    float ek = d_sample_ek(I, r);
-                    gp_rng_align(&r);
-                    d_positron_range(ph, px, py, pz, ek, 0, 0.f, 0.f, 0.f, r, c_rs);
-                }
-                gp_rng_align(&r);
-                d_emit_pair(rp.sigma, r, v, e);
-                nph = 2;
-            }
-        } else {
-            const double *rec = rp.psf + 8 * i;
-            px = (float)rec[0];
-            py = (float)rec[1];
-            pz = (float)rec[2];
-            tstart = rec[3];
-            float rvx = (float)rec[4], rvy = (float)rec[5], rvz = (float)rec[6], rE = (float)rec[7];
-            if (MODE == RUN_PSF_PHOTON) {
-                v[0][0] = rvx; v[0][1] = rvy; v[0][2] = rvz; e[0] = rE;
-                nph = 1;
-            } else {
-                gp_rng r;
-                gp_rng_init(&r, rp.seed, (uint64_t)h, GP_TAG_DECAY, 0u);
-                if (rp.prange) d_positron_range(ph, px, py, pz, rE, 1, rvx, rvy, rvz, r, c_rs);
-                gp_rng_align(&r);
-                d_emit_pair(rp.sigma, r, v, e);
-                nph = 2;
+            if (rp.prange) d_positron_range(ph, px, py, pz, rE, 1, rvx, rvy, rvz, r, c_rs);
+            gp_rng_align(&r);
+            d_emit_pair(rp.sigma, r, v, e);
+            nph = 2;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const bool active = k < nph;
+        const int64_t slot = 2 * h + k;
+        gpet_trace *tr = (rp.trace && slot < rp.trace_cap) ? rp.trace + slot : nullptr;
+        float x = px, y = py, z = pz, vx = v[k][0], vy = v[k][1], vz = v[k][2], E = e[k];
+        double t = tstart;
+        gp_rng r;
+        gp_rng_init(&r, rp.seed, (uint64_t)h, (uint32_t)(GP_TAG_PHOTON + k), 0u);
+        int ns = 0, nc = 0, nr = 0;
+        bool alive = active;
+        if (active) {
+            c_ph++;
+            if (ph.present) alive = d_phantom_walk(tab, ph, rp.eabs, x, y, z, vx, vy, vz, E, t, r, ns, nc, nr, c_maj);
+        }
+        c_ps += (unsigned long long)ns;
+        if (tr) {
+            tr->history = h;
+            tr->photon = k;
+            tr->panel = -1;
+            tr->t0 = tstart;
+            if (active) {
+                tr->status = GPET_TR_ALIVE | (alive ? GPET_TR_ESCAPED : GPET_TR_ABSORBED);
+                tr->n_steps = ns; tr->n_compton = nc; tr->n_rayleigh = nr;
+                tr->x0 = px; tr->y0 = py; tr->z0 = pz; tr->E0 = e[k];
+                tr->x = x; tr->y = y; tr->z = z; tr->vx = vx; tr->vy = vy; tr->vz = vz; tr->E = E;
+                tr->t = t;
+                tr->draws = r.n;
             }
         }
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            const bool active = k < nph;
-            const int64_t slot = 2 * h + k;
-            gpet_trace *tr = (rp.trace && slot < rp.trace_cap) ? rp.trace + slot : nullptr;
-            float x = px, y = py, z = pz, vx = v[k][0], vy = v[k][1], vz = v[k][2], E = e[k];
-            double t = tstart;
-            gp_rng r;
-            gp_rng_init(&r, rp.seed, (uint64_t)h, (uint32_t)(GP_TAG_PHOTON + k), 0u);
-            int ns = 0, nc = 0, nr = 0;
-            bool alive = active;
-            if (active) {
-                c_ph++;
-                if (ph.present) alive = d_phantom_walk(tab, ph, rp.eabs, x, y, z, vx, vy, vz, E, t, r, ns, nc, nr, c_maj);
-            }
-            c_ps += (unsigned long long)ns;
-            if (tr) {
-                tr->history = h;
-                tr->photon = k;
-                tr->panel = -1;
-                tr->t0 = tstart;
-                if (active) {
-                    tr->status = GPET_TR_ALIVE | (alive ? GPET_TR_ESCAPED : GPET_TR_ABSORBED);
-                    tr->n_steps = ns; tr->n_compton = nc; tr->n_rayleigh = nr;
-                    tr->x0 = px; tr->y0 = py; tr->z0 = pz; tr->E0 = e[k];
-                    tr->x = x; tr->y = y; tr->z = z; tr->vx = vx; tr->vy = vy; tr->vz = vz; tr->E = E;
-                    tr->t = t;
-                    tr->draws = r.n;
+        int pa = -1;
+        float y2 = 0.f, z2 = 0.f, vx2 = 0.f, vy2 = 0.f, vz2 = 0.f;
+        if (alive && det.present) {
+            const float vv = vx * vx + vy * vy + vz * vz;
+            for (int pi = 0; pi < det.npanels; ++pi) {
+                const DevPanel &P = det.p[pi];
+                /* conservative prefilter: the line through the photon must
+                 * pass within the face's bounding sphere (covers the
+                 * reference's backward entries too); skipping a panel here
+                 * never changes which panel the exact test picks */
+                {
+                    float wx = P.ox - x, wy = P.oy - y, wz = P.oz - z;
+                    float a = vx * wx + vy * wy + vz * wz;
+                    float w2 = wx * wx + wy * wy + wz * wz;
+                    if (w2 * vv - a * a > P.bound2 * vv) continue;
                 }
-            }
-            int pa = -1;
-            float y2 = 0.f, z2 = 0.f, vx2 = 0.f, vy2 = 0.f, vz2 = 0.f;
-            if (alive && det.present) {
-                const float vv = vx * vx + vy * vy + vz * vz;
-                for (int pi = 0; pi < det.npanels; ++pi) {
-                    const DevPanel &P = det.p[pi];
-                    /* conservative prefilter: the line through the photon must
-                     * pass within the face's bounding sphere (covers the
-                     * reference's backward entries too); skipping a panel here
-                     * never changes which panel the exact test picks */
-                    {
-                        float wx = P.ox - x, wy = P.oy - y, wz = P.oz - z;
-                        float a = vx * wx + vy * wy + vz * wz;
-                        float w2 = wx * wx + wy * wy + wz * wz;
-                        if (w2 * vv - a * a > P.bound2 * vv) continue;
-                    }
-                    float rx = x - P.ox, ry = y - P.oy, rz = z - P.oz;
-                    float tx = rx * P.ux[0] + ry * P.ux[1] + rz * P.ux[2];
-                    float ty = rx * P.uy[0] + ry * P.uy[1] + rz * P.uy[2];
-                    float tz = rx * P.uz[0] + ry * P.uz[1] + rz * P.uz[2];
-                    float tvx = vx * P.ux[0] + vy * P.ux[1] + vz * P.ux[2];
-                    float tvy = vx * P.uy[0] + vy * P.uy[1] + vz * P.uy[2];
-                    float tvz = vx * P.uz[0] + vy * P.uz[1] + vz * P.uz[2];
-                    if (tvx * P.dirx >= 0.0f) {
-                        float ty2 = ty - (tx / tvx) * tvy;
-                        float tz2 = tz - (tx / tvx) * tvz;
-                        if (fabsf(ty2) < P.leny / 2.0f && fabsf(tz2) < P.lenz / 2.0f) {
-                            y2 = ty2;
-                            z2 = tz2;
-                            t = t + (double)(-tx) / (D_C_LIGHT * (double)tvx);
-                            vx2 = tvx; vy2 = tvy; vz2 = tvz;
-                            pa = pi;
-                            break;
-                        }
+                float rx = x - P.ox, ry = y - P.oy, rz = z - P.oz;
+                float tx = rx * P.ux[0] + ry * P.ux[1] + rz * P.ux[2];
+                float ty = rx * P.uy[0] + ry * P.uy[1] + rz * P.uy[2];
+                float tz = rx * P.uz[0] + ry * P.uz[1] + rz * P.uz[2];
+                float tvx = vx * P.ux[0] + vy * P.ux[1] + vz * P.ux[2];
+                float tvy = vx * P.uy[0] + vy * P.uy[1] + vz * P.uy[2];
+                float tvz = vx * P.uz[0] + vy * P.uz[1] + vz * P.uz[2];
+                if (tvx * P.dirx >= 0.0f) {
+                    float ty2 = ty - (tx / tvx) * tvy;
+                    float tz2 = tz - (tx / tvx) * tvz;
+                    if (fabsf(ty2) < P.leny / 2.0f && fabsf(tz2) < P.lenz / 2.0f) {
+                        y2 = ty2;
+                        z2 = tz2;
+                        t = t + (double)(-tx) / (D_C_LIGHT * (double)tvx);
+                        vx2 = tvx; vy2 = tvy; vz2 = tvz;
+                        pa = pi;
+                        break;
                     }
                 }
             }
-            if (tr && pa >= 0) {
-                tr->panel = pa;
-                tr->status |= GPET_TR_PANEL;
-            }
-            const bool want = pa >= 0;
-            int idx = wave_append(q.count, want);
-            if (want) {
-                c_ent++;
-                if (idx < q.cap) {
-                    q.y[idx] = y2; q.z[idx] = z2;
-                    q.vx[idx] = vx2; q.vy[idx] = vy2; q.vz[idx] = vz2;
-                    q.E[idx] = E;
-                    q.t[idx] = t;
-                    int parn = (MODE == RUN_PSF_PHOTON) ? (int)h : (int)(2 * h + k);
-                    q.info[idx] = make_int4(pa, parn, (int)h, (int)(r.n | ((unsigned)k << 31)));
-                    if (q.trace_slot) q.trace_slot[idx] = tr ? (int)slot : -1;
-                }
+        }
+        if (tr && pa >= 0) {
+            tr->panel = pa;
+            tr->status |= GPET_TR_PANEL;
+        }
+        const bool want = pa >= 0;
+        int idx = wave_append(q.count, want);
+        if (want) {
+            c_ent++;
+            if (idx < q.cap) {
+                q.y[idx] = y2; q.z[idx] = z2;
+                q.vx[idx] = vx2; q.vy[idx] = vy2; q.vz[idx] = vz2;
+                q.E[idx] = E;
+                q.t[idx] = t;
+                int parn = (MODE == RUN_PSF_PHOTON) ? (int)h : (int)(2 * h + k);
+                q.info[idx] = make_int4(pa, parn, (int)h, (int)(r.n | ((unsigned)k << 31)));
+                if (q.trace_slot) q.trace_slot[idx] = tr ? (int)slot : -1;
             }
         }
     }
-    flush_counter(rp.cnt + CNT_PHOTONS, c_ph);
-    flush_counter(rp.cnt + CNT_PANEL_ENTRIES, c_ent);
-    flush_counter(rp.cnt + CNT_MAJORANT, c_maj);
-    flush_counter(rp.cnt + CNT_PHANTOM_STEPS, c_ps);
-    flush_counter(rp.cnt + CNT_RANGE_STEPS, c_rs);
+}
+
+/* The phantom kernel, grid-stride over histories.
+ *
+ * BATCH: the launch covers a list of segments, each a history range of one
+ * phantom slot.  Work is dealt in virtual blocks of blockDim histories that
+ * never straddle a segment, so the slot is block-uniform: its descriptor is
+ * read with scalar loads and held in SGPRs, and no wave mixes phantoms.
+ * WPE > 0 asks the compiler for at least WPE waves per SIMD (a register
+ * budget), one of the launch variants the tuner picks from. */
+template <int MODE, bool BATCH, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
+void k_phantom(RunParams rp, DevTables tab, DevPhantom ph, DevDetector det, DevQueue q)
+{
+    PhCounters C;
+    if (!BATCH) {
+        const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < rp.nh; i += stride) {
+            const int64_t h = rp.h0 + i;
+            const int si = (MODE == RUN_DECAY) ? d_source_of(rp, h) : 0;
+            d_history<MODE>(rp, tab, ph, det, q, i, h, si, C);
+        }
+    } else {
+        for (int64_t vb = blockIdx.x; vb < rp.n_vblocks; vb += gridDim.x) {
+            int lo = 0, hi = rp.nseg; /* largest segment with vb0 <= vb */
+            while (hi - lo > 1) {
+                int mid = (lo + hi) >> 1;
+                if (rp.seg[mid].vb0 <= vb) lo = mid; else hi = mid;
+            }
+            const DevSegment S = rp.seg[lo];
+            const int64_t off = (vb - S.vb0) * blockDim.x + threadIdx.x;
+            if (off < S.nh) {
+                const int64_t h = S.h0 + off;
+                const int si = d_source_of(rp, h);
+                const DevPhantom P = rp.phs[S.phantom];
+                d_history<MODE>(rp, tab, P, det, q, h - rp.h0, h, si, C);
+            }
+        }
+    }
+    flush_counter(rp.cnt + CNT_PHOTONS, C.ph);
+    flush_counter(rp.cnt + CNT_PANEL_ENTRIES, C.ent);
+    flush_counter(rp.cnt + CNT_MAJORANT, C.maj);
+    flush_counter(rp.cnt + CNT_PHANTOM_STEPS, C.ps);
+    flush_counter(rp.cnt + CNT_RANGE_STEPS, C.rs);
 }
 
 /* ---- panel -------------------------------------------------------------- */
@@ -499,9 +549,12 @@ __device__ __forceinline__ void d_readout(EvList &L, int n, int depth, int polic
     }
 }
 
-__global__ __launch_bounds__(256) void k_panel(PanelParams pp, DevTables tab, DevDetector det, DevQueue q, int nq)
+template <int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
+void k_panel(PanelParams pp, DevTables tab, DevDetector det, DevQueue q)
 {
     unsigned long long c_hits = 0, c_drop = 0, c_droph = 0, c_maj = 0, c_steps = 0;
+    const int nq = (int)min((int64_t)*q.count, q.cap);
     const int stride = gridDim.x * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += stride) {
         float x2 = 0.0f, y2 = q.y[i], z2 = q.z[i];
@@ -658,29 +711,72 @@ __global__ __launch_bounds__(256) void k_panel(PanelParams pp, DevTables tab, De
 
 } // namespace
 
+const int gpet_wpe_variants[GPET_N_WPE] = {0, 6, 8};
+
+namespace {
+
+int grid_of(int64_t n, const LaunchCfg &cfg, int n_cu)
+{
+    int64_t nb = (n + cfg.block - 1) / cfg.block;
+    int64_t cap = cfg.blocks_per_cu > 0 ? (int64_t)n_cu * cfg.blocks_per_cu : 1048576;
+    return (int)std::max<int64_t>(1, std::min(nb, cap));
+}
+
+template <int MODE, bool BATCH>
+void launch_phantom_wpe(const RunParams &rp, const DevTables &tab, const DevPhantom &ph, const DevDetector &det,
+                        const DevQueue &q, int grid, int block, int wpe, hipStream_t s)
+{
+    if (wpe == 8)
+        hipLaunchKernelGGL((k_phantom<MODE, BATCH, 8>), dim3(grid), dim3(block), 0, s, rp, tab, ph, det, q);
+    else if (wpe == 6)
+        hipLaunchKernelGGL((k_phantom<MODE, BATCH, 6>), dim3(grid), dim3(block), 0, s, rp, tab, ph, det, q);
+    else
+        hipLaunchKernelGGL((k_phantom<MODE, BATCH, 0>), dim3(grid), dim3(block), 0, s, rp, tab, ph, det, q);
+}
+
+__global__ void k_zero(unsigned *p, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+} // namespace
+
+int gpet_launch_zero(unsigned *p, int n, hipStream_t s)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_zero, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+    return (int)hipGetLastError();
+}
+
 int gpet_launch_phantom(const RunParams &rp, const DevTables &tab, const DevPhantom &ph, const DevDetector &det,
-                        const DevQueue &q, hipStream_t s)
+                        const DevQueue &q, bool batch, const LaunchCfg &cfg, int n_cu, hipStream_t s)
 {
     if (rp.nh <= 0) return 0;
-    const int block = 256;
-    int64_t nb = (rp.nh + block - 1) / block;
-    int grid = (int)(nb < 1048576 ? nb : 1048576);
-    if (rp.mode == RUN_DECAY)
-        hipLaunchKernelGGL(k_phantom<RUN_DECAY>, dim3(grid), dim3(block), 0, s, rp, tab, ph, det, q);
-    else if (rp.mode == RUN_PSF_POSITRON)
-        hipLaunchKernelGGL(k_phantom<RUN_PSF_POSITRON>, dim3(grid), dim3(block), 0, s, rp, tab, ph, det, q);
-    else
-        hipLaunchKernelGGL(k_phantom<RUN_PSF_PHOTON>, dim3(grid), dim3(block), 0, s, rp, tab, ph, det, q);
+    if (cfg.block <= 0 || cfg.block > 256 || cfg.block % 64) return (int)hipErrorInvalidValue;
+    const int grid = grid_of(batch ? rp.n_vblocks * cfg.block : rp.nh, cfg, n_cu);
+    if (rp.mode == RUN_DECAY) {
+        if (batch) launch_phantom_wpe<RUN_DECAY, true>(rp, tab, ph, det, q, grid, cfg.block, cfg.wpe, s);
+        else launch_phantom_wpe<RUN_DECAY, false>(rp, tab, ph, det, q, grid, cfg.block, cfg.wpe, s);
+    } else if (rp.mode == RUN_PSF_POSITRON) {
+        launch_phantom_wpe<RUN_PSF_POSITRON, false>(rp, tab, ph, det, q, grid, cfg.block, cfg.wpe, s);
+    } else {
+        launch_phantom_wpe<RUN_PSF_PHOTON, false>(rp, tab, ph, det, q, grid, cfg.block, cfg.wpe, s);
+    }
     return (int)hipGetLastError();
 }
 
 int gpet_launch_panel(const PanelParams &pp, const DevTables &tab, const DevDetector &det, const DevQueue &q,
-                      int64_t n_items, hipStream_t s)
+                      int64_t max_items, const LaunchCfg &cfg, int n_cu, hipStream_t s)
 {
-    if (n_items <= 0) return 0;
-    const int block = 256;
-    int64_t nb = (n_items + block - 1) / block;
-    int grid = (int)(nb < 1048576 ? nb : 1048576);
-    hipLaunchKernelGGL(k_panel, dim3(grid), dim3(block), 0, s, pp, tab, det, q, (int)n_items);
+    if (max_items <= 0) return 0;
+    if (cfg.block <= 0 || cfg.block > 256 || cfg.block % 64) return (int)hipErrorInvalidValue;
+    const int grid = grid_of(max_items, cfg, n_cu);
+    if (cfg.wpe == 8)
+        hipLaunchKernelGGL(k_panel<8>, dim3(grid), dim3(cfg.block), 0, s, pp, tab, det, q);
+    else if (cfg.wpe == 6)
+        hipLaunchKernelGGL(k_panel<6>, dim3(grid), dim3(cfg.block), 0, s, pp, tab, det, q);
+    else
+        hipLaunchKernelGGL(k_panel<0>, dim3(grid), dim3(cfg.block), 0, s, pp, tab, det, q);
     return (int)hipGetLastError();
 }

@@ -189,10 +189,11 @@ class RunResult:
     n_coinc: int = 0
     n_adder: int = 0
     n_hits: int = 0
+    job_counts: np.ndarray = None   # batched runs: [n_jobs, 3] histories, singles, coincidences
 
 
 def make_options(seed=1, eabs=1.0e3, noncol_sigma=0.0037056, positron_range=0, record_hits=0, batch=0,
-                 device=0) -> capi.Options:
+                 device=0, flags=0) -> capi.Options:
     o = capi.Options()
     o.device = device
     o.seed = seed
@@ -201,7 +202,7 @@ def make_options(seed=1, eabs=1.0e3, noncol_sigma=0.0037056, positron_range=0, r
     o.positron_range = positron_range
     o.record_hits = record_hits
     o.batch = batch
-    o.flags = 0
+    o.flags = flags
     return o
 
 
@@ -247,14 +248,32 @@ class Engine:
         t = tabs.as_c()
         self._ok(self.f["set_tables"](self.ctx, C.byref(t)), "set_tables")
 
-    def set_phantom(self, mat: np.ndarray, dens: np.ndarray, dims, offset, size):
+    def set_phantom(self, mat: np.ndarray, dens: np.ndarray, dims, offset, size, slot=None):
+        """Phantom of single runs, or of batch slot `slot`."""
         mat = np.ascontiguousarray(mat, dtype=np.int32).ravel()
         dens = np.ascontiguousarray(dens, dtype=np.float32).ravel()
         d = (C.c_int32 * 3)(*[int(x) for x in dims])
         o = (C.c_float * 3)(*[float(x) for x in offset])
         s = (C.c_float * 3)(*[float(x) for x in size])
-        self._ok(self.f["set_phantom"](self.ctx, mat.ctypes.data_as(C.c_void_p), dens.ctypes.data_as(C.c_void_p),
-                                       d, o, s), "set_phantom")
+        if slot is None:
+            self._ok(self.f["set_phantom"](self.ctx, mat.ctypes.data_as(C.c_void_p),
+                                           dens.ctypes.data_as(C.c_void_p), d, o, s), "set_phantom")
+        else:
+            self._ok(self.f["set_phantom_slot"](self.ctx, int(slot), mat.ctypes.data_as(C.c_void_p),
+                                                dens.ctypes.data_as(C.c_void_p), d, o, s), "set_phantom_slot")
+
+    def set_launch(self, block=(256, 256), wpe=(0, 0), blocks_per_cu=(0, 8), batch=1 << 22):
+        L = capi.Launch()
+        L.block[0], L.block[1] = block
+        L.wpe[0], L.wpe[1] = wpe
+        L.blocks_per_cu[0], L.blocks_per_cu[1] = blocks_per_cu
+        L.batch = batch
+        self._ok(self.f["set_launch"](self.ctx, C.byref(L)), "set_launch")
+
+    def get_launch(self) -> dict:
+        L = capi.Launch()
+        self._ok(self.f["get_launch"](self.ctx, C.byref(L)), "get_launch")
+        return L.as_dict()
 
     def set_detector(self, panels, mats, quadrics=None):
         arr = (capi.Panel * len(panels))(*panels)
@@ -320,6 +339,28 @@ class Engine:
         self._ok(self.f["run_source"](self.ctx, src, len(sources), iso, len(isotopes), float(t0), float(t1),
                                       int(n_decays), C.byref(s)), "run_source")
         return self._result(s, bufs)
+
+    def run_batch(self, jobs, sources, isotopes, *, singles_cap=0, coinc_cap=0, adder_cap=0, hits_cap=0,
+                  trace_cap=0, out_dir=None) -> RunResult:
+        """jobs: sequence of dicts {phantom, first_source, n_sources, t0, t1, n_decays}."""
+        J = (capi.Job * len(jobs))()
+        for k, j in enumerate(jobs):
+            J[k].phantom = j["phantom"]
+            J[k].first_source = j["first_source"]
+            J[k].n_sources = j["n_sources"]
+            J[k].t0 = j["t0"]
+            J[k].t1 = j["t1"]
+            J[k].n_decays = j.get("n_decays", 0)
+        src = (capi.Source * len(sources))(*sources)
+        iso = (capi.Isotope * len(isotopes))(*isotopes)
+        s, bufs = self._sink(singles_cap, coinc_cap, adder_cap, hits_cap, trace_cap, out_dir)
+        jc = np.zeros((len(jobs), 3), np.int64)
+        s.job_counts = jc.ctypes.data_as(C.c_void_p)
+        self._ok(self.f["run_batch"](self.ctx, J, len(jobs), src, len(sources), iso, len(isotopes), C.byref(s)),
+                 "run_batch")
+        r = self._result(s, bufs)
+        r.job_counts = jc
+        return r
 
     def run_psf(self, rec: np.ndarray, ptype: int, *, singles_cap=0, coinc_cap=0, adder_cap=0, hits_cap=0,
                 trace_cap=0, out_dir=None) -> RunResult:

@@ -194,6 +194,78 @@ def body_sources(seed: int = 5, concentration: float = 1.0e7):
     return srcs
 
 
+# Batched workload: many independent phantoms of uneven size on the octagon
+# scanner.  Every grid spans the same 10 cm cube (inside the 8 cm panels:
+# 1.307 x 5 cm < 8 cm), so voxel size is 10/n cm.
+BATCH_EXTENT = 10.0
+BATCH_BODY_RADIUS = 4.5
+LABEL_DENSITY = {AIR: 0.0012, WATER: 1.0, BONE: 1.85, TISSUE: 0.3}   # TissueICRU as lung
+
+
+def blob_phantom(n: int, rng: np.random.Generator, extent: float = BATCH_EXTENT):
+    """n^3 labels from a smoothed Gaussian random field: a coarse 8^3 white
+    noise field smoothed and upsampled linearly, cut at its 15th/85th
+    percentiles into lung / water / bone inside a cylinder of radius 4.5 cm
+    (air outside); densities are the label density times U(0.95, 1.05)."""
+    from scipy import ndimage
+
+    coarse = ndimage.gaussian_filter(rng.normal(size=(8, 8, 8)), 1.0, mode="wrap").astype(np.float32)
+    # separable linear upsampling 8 -> n: f = A (x) A (x) A applied to coarse
+    u = np.clip((np.arange(n) + 0.5) * (8.0 / n) - 0.5, 0.0, 7.0)
+    i0 = np.minimum(u.astype(int), 6)
+    w = (u - i0).astype(np.float32)
+    A = np.zeros((n, 8), np.float32)
+    A[np.arange(n), i0] = 1.0 - w
+    A[np.arange(n), i0 + 1] += w
+    f = np.tensordot(A, coarse, axes=(1, 0))              # [n, 8, 8]
+    f = np.tensordot(f, A, axes=(2, 1))                   # [n, 8, n]
+    f = np.einsum("jb,ibk->ijk", A, f, optimize=True)     # [n, n, n]
+    f = np.ascontiguousarray(f, dtype=np.float32)
+    lo, hi = np.quantile(f, [0.15, 0.85])
+    mat = np.full((n, n, n), WATER, np.int32)
+    mat[f < lo] = TISSUE
+    mat[f > hi] = BONE
+    d = extent / n
+    c = (np.arange(n, dtype=np.float32) + 0.5) * d - extent / 2
+    yy, xx = np.meshgrid(c, c, indexing="ij")
+    outside = (xx * xx + yy * yy) > BATCH_BODY_RADIUS ** 2
+    mat[:, outside] = AIR
+    lut = np.zeros(max(LABEL_DENSITY) + 1, np.float32)
+    for lab, rho in LABEL_DENSITY.items():
+        lut[lab] = rho
+    dens = rng.random((n, n, n), dtype=np.float32)
+    dens *= np.float32(0.1)
+    dens += np.float32(0.95)
+    dens *= lut[mat]
+    return mat, dens, (n, n, n), (-extent / 2,) * 3, (extent,) * 3
+
+
+def batch_workload(n_phantoms: int = 64, seed: int = 3, size_range=(32, 256), decay_range=(1e5, 1e7)):
+    """(phantoms, jobs, sources, isotopes): phantom k has n_k ~ U{lo..hi}
+    voxels per axis and one job of F-18 uniform in a cylinder (r 4 cm,
+    h 8 cm) over 10 s with a log-uniform decay count."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(size_range[0], size_range[1] + 1, n_phantoms)
+    counts = np.floor(10.0 ** rng.uniform(np.log10(decay_range[0]), np.log10(decay_range[1]), n_phantoms))
+    phantoms = [blob_phantom(int(n), rng) for n in sizes]
+    sources, jobs = [], []
+    for k in range(n_phantoms):
+        sources.append(source(1, c3=4.0, c4=8.0, natom=1.0e12))
+        jobs.append(dict(phantom=k, first_source=k, n_sources=1, t0=0.0, t1=10.0, n_decays=int(counts[k])))
+    return phantoms, jobs, sources, [isotope_f18()]
+
+
+def setup_batch(eng, tables, phantoms, dig=None):
+    """Tables, the phantom slots, the octagon scanner and the digitizer."""
+    eng.set_tables(tables)
+    for k, ph in enumerate(phantoms):
+        eng.set_phantom(*ph, slot=k)
+    panels, mats = octagon_scanner()
+    eng.set_detector(panels, mats, np.array([0, 0, 0, 0, 0, 0, 0, 0, 0, 1], np.float32))
+    eng.set_digitizer(dig or digitizer())
+    return panels, mats
+
+
 def isotope_f18() -> capi.Isotope:
     """Row 0 of data/isotopes.txt (F-18)."""
     iso = capi.Isotope()

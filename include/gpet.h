@@ -46,6 +46,8 @@ enum {
 #define GPET_NE 151            /* constants.h:23,27 NECM/NERL                   */
 #define GPET_MAXSURFACE 8      /* constants.h:19 MAXSURFACE (reference: 5)      */
 #define GPET_MAXPANELS 256
+#define GPET_MAXPHANTOMS 256   /* phantom slots of a batched run               */
+#define GPET_MAX_HISTORIES (1ll << 30) /* photon ids 2h+k must fit int32 (Event.parn) */
 
 /* 48-byte digitizer record; byte-identical to Event, gPET.h:87-92
  * (t at offset 24, E at offset 32).  siten is the site index at the current
@@ -126,8 +128,39 @@ typedef struct gpet_options {
     int32_t positron_range;    /* input_PET.in:21                               */
     int32_t record_hits;       /* constants.h:6 OUTPUTHIT                        */
     int64_t batch;             /* histories per transport launch; 0 = automatic */
-    int32_t flags;             /* reserved, 0                                   */
+    int32_t flags;             /* GPET_FLAG_* bits                              */
 } gpet_options;
+
+enum {
+    GPET_FLAG_NO_GRAPH = 1,    /* launch transport batches one by one instead of
+                                  replaying one captured hipGraph per run       */
+    GPET_FLAG_AUTOTUNE = 2     /* time launch-shape variants on the first run of
+                                  each kind and keep the fastest                */
+};
+
+/* Launch shape of the two transport kernels (index 0 phantom, 1 panel).
+ * Results do not depend on it: every history draws from its own streams. */
+typedef struct gpet_launch {
+    int32_t block[2];          /* threads per block: 64, 128, 192 or 256         */
+    int32_t wpe[2];            /* register budget as minimum waves per SIMD:
+                                  0 (compiler default), 6 or 8                  */
+    int32_t blocks_per_cu[2];  /* grid = CUs x this, grid-stride; 0 = one lane
+                                  per item                                      */
+    int64_t batch;             /* histories per phantom-kernel launch           */
+} gpet_launch;
+
+/* One acquisition of a batched run (new; the reference runs one phantom per
+ * process): phantom slot, its sources (a range of the run's source array),
+ * time window and decay count.  Jobs are independent acquisitions on the same
+ * scanner: deadtime and coincidences never mix events of two jobs. */
+typedef struct gpet_job {
+    int32_t phantom;           /* slot set with gpet_set_phantom_slot           */
+    int32_t first_source;      /* sources [first_source, first_source+n_sources) */
+    int32_t n_sources;
+    int32_t reserved;
+    double t0, t1;             /* seconds                                       */
+    int64_t n_decays;          /* > 0: fixed total; 0: Binomial per source      */
+} gpet_job;
 
 /* A source line of source.txt (initialize.cu:116-144).  natom is a double so
  * that activities beyond uint32 (reference gPET.h:50) are expressible. */
@@ -182,6 +215,7 @@ typedef struct gpet_sink {
     int32_t *hit_id;     float *hit_f;        int64_t hits_cap; int64_t n_hits; /* 5+5 per hit */
     gpet_trace *trace;   int64_t trace_cap;   int64_t n_trace;
     const char *out_dir; /* non-NULL: write singles.dat, adder.dat, coincidences.dat, Hits*.dat */
+    int64_t *job_counts; /* gpet_run_batch: n_jobs x 3 {histories, singles, coincidences}, or NULL */
 } gpet_sink;
 
 /* Run statistics and tallies.  Arrays are caller-provided (may be NULL). */
@@ -206,8 +240,16 @@ typedef struct gpet_tallies {
     int64_t *energy_hist;      int32_t n_ebins; float e_lo, e_hi;
     int64_t *sinogram;         int32_t n_s, n_phi; float s_max;
     double seconds_transport, seconds_digitizer;
-    double kernel_seconds[2];  /* summed launch durations: 0 phantom, 1 panel   */
-    int64_t kernel_launches[2];
+    double kernel_seconds[2];  /* summed durations of the timed launches (HIP
+                                  events; launches replayed from a graph carry no
+                                  event timing): 0 phantom, 1 panel             */
+    int64_t kernel_launches[2];/* launches timed                                */
+    int64_t kernel_histories;  /* histories covered by the timed launches       */
+    int64_t launches;          /* phantom-kernel launches of the run            */
+    gpet_launch launch;        /* launch shape the run used                     */
+    int32_t graph;             /* 1 if the batches ran as one hipGraph          */
+    int32_t n_jobs;
+    double seconds_tune;       /* autotuning time spent inside this run         */
 } gpet_tallies;
 
 typedef struct gpet_ctx gpet_ctx;
@@ -300,6 +342,11 @@ gpet_status gpet_load_tables(gpet_ctx *ctx, const char *data_dir);
 gpet_status gpet_set_phantom(gpet_ctx *ctx, const int32_t *mat, const float *dens,
                              const int32_t dims[3], const float offset[3], const float size[3]);
 
+/* The same for phantom slot `slot` of a batched run (slot 0 is the phantom of
+ * gpet_set_phantom).  Each slot keeps its own majorant. */
+gpet_status gpet_set_phantom_slot(gpet_ctx *ctx, int32_t slot, const int32_t *mat, const float *dens,
+                                  const int32_t dims[3], const float offset[3], const float size[3]);
+
 /* iniPanel + iniwck(panel materials) (initialize.cu:919-1224) and the
  * quadric exclusion surfaces (main.cu:149-161; 10 coefficients each). */
 gpet_status gpet_set_detector(gpet_ctx *ctx, const gpet_panel *panels, int32_t n_panels,
@@ -323,11 +370,26 @@ gpet_status gpet_run_source(gpet_ctx *ctx, const gpet_source *src, int32_t n_src
 gpet_status gpet_run_psf(gpet_ctx *ctx, const double *rec8, int64_t n, int32_t ptype,
                          gpet_sink *sink);
 
+/* Batched sampleParticle (new): n_jobs acquisitions of different phantoms
+ * in one run.  Histories are numbered job after job, each job's sources in
+ * order, so a run of one job equals gpet_run_source on its sources.  Every
+ * transport launch covers histories of several jobs; singles and
+ * coincidences come out ordered by (job, time), and sink->job_counts gets
+ * the per-job counts. */
+gpet_status gpet_run_batch(gpet_ctx *ctx, const gpet_job *jobs, int32_t n_jobs,
+                           const gpet_source *src, int32_t n_src,
+                           const gpet_isotope *iso, int32_t n_iso, gpet_sink *sink);
+
+/* Launch shape used by the following runs (validated; results unchanged). */
+gpet_status gpet_set_launch(gpet_ctx *ctx, const gpet_launch *l);
+gpet_status gpet_get_launch(gpet_ctx *ctx, gpet_launch *l);
+
 /* Tallies of the last run. */
 gpet_status gpet_get_tallies(gpet_ctx *ctx, gpet_tallies *t);
 
 /* Woodcock majorant nodes (GPET_NWCK floats) built by gpet_set_phantom
- * (which = 0) or gpet_set_detector (which = 1); iniwck, initialize.cu:773-829. */
+ * (which = 0) or gpet_set_detector (which = 1), or for phantom slot k
+ * (which = 2 + k); iniwck, initialize.cu:773-829. */
 gpet_status gpet_get_majorant(gpet_ctx *ctx, int32_t which, float *out, float *e0, float *idl);
 
 /* Host-only: the majorant the two calls above would build for tables t and

@@ -137,6 +137,8 @@ struct HitRec {
 };
 
 struct Result {
+    std::vector<int64_t> job_first;   /* batched runs: first history of each job */
+    std::vector<int64_t> job_counts;  /* batched runs: per job {histories, singles, pairs} */
     std::vector<gpet_event> adder, singles, coinc;
     std::vector<HitRec> hits;
     std::vector<gpet_trace> trace;
@@ -151,7 +153,8 @@ struct gpet_ctx {
     gpet_options opt;
     Tables tab;
     bool has_tab = false;
-    Phantom ph;
+    std::vector<Phantom> slots = std::vector<Phantom>(1); /* slot 0: gpet_cpu_set_phantom */
+    std::vector<char> slot_set = std::vector<char>(1, 0);
     bool has_ph = false;
     Detector det;
     bool has_det = false;
@@ -160,6 +163,7 @@ struct gpet_ctx {
     Result last;
     std::string err;
     int threads = 0;
+    gpet_launch launch = gpet_launch();
 };
 
 namespace {
@@ -449,10 +453,9 @@ struct PhotonOut {
 /* photon (gPET_kernals.cu:256-345): Woodcock tracking in the phantom.
  * Returns false if the photon is absorbed.  Q10 fixed: a Compton event that
  * leaves less than eabs absorbs the photon. */
-bool phantom_transport(const gpet_ctx &C, real &x, real &y, real &z, real &vx, real &vy,
+bool phantom_transport(const gpet_ctx &C, const Phantom &p, real &x, real &y, real &z, real &vx, real &vy,
                        real &vz, real &E, double &tof, gp_rng &r, gpet_trace *tr, Counters &cnt)
 {
-    const Phantom &p = C.ph;
     const Tables &t = C.tab;
     for (int it = 0; it < STEP_CAP; ++it) {
         /* one 4-word block per step: length, selector, scattering CDF, azimuth */
@@ -609,7 +612,7 @@ void panel_transport(const gpet_ctx &C, real x, real y, real z, real vx, real vy
 }
 
 /* Phantom + panel for one photon, filling the trace record. */
-void photon_full(const gpet_ctx &C, int64_t hist, int k, real x, real y, real z, real vx,
+void photon_full(const gpet_ctx &C, const Phantom *ph, int64_t hist, int k, real x, real y, real z, real vx,
                  real vy, real vz, real E, double t, int32_t parn, gpet_trace *tr,
                  PhotonOut &out, Counters &cnt, bool want_hits)
 {
@@ -622,7 +625,7 @@ void photon_full(const gpet_ctx &C, int64_t hist, int k, real x, real y, real z,
         tr->panel = -1;
     }
     bool alive = true;
-    if (C.has_ph) alive = phantom_transport(C, x, y, z, vx, vy, vz, E, t, r, tr, cnt);
+    if (ph) alive = phantom_transport(C, *ph, x, y, z, vx, vy, vz, E, t, r, tr, cnt);
     if (tr) {
         tr->status |= alive ? GPET_TR_ESCAPED : GPET_TR_ABSORBED;
         tr->x = x; tr->y = y; tr->z = z; tr->vx = vx; tr->vy = vy; tr->vz = vz; tr->E = E; tr->t = t;
@@ -716,10 +719,9 @@ inline void dda_axis(real x, real off, real d, int i, real u, real &tmax, real &
     }
 }
 
-void positron_range(const gpet_ctx &C, real &x, real &y, real &z, real ek_ev, int usedir,
+void positron_range(const Phantom &p, real &x, real &y, real &z, real ek_ev, int usedir,
                     real vx, real vy, real vz, gp_rng &r, Counters &cnt)
 {
-    const Phantom &p = C.ph;
     real ekin = ek_ev / 1.0e6f;
     real rex = ((0.1f * 5.44040782f) * ekin * ekin) / (0.369516529f + ekin);
     real sigma = rex / 2.0f;
@@ -738,7 +740,7 @@ void positron_range(const gpet_ctx &C, real &x, real &y, real &z, real ek_ev, in
     real ux = dx * inv_rr, uy = dy * inv_rr, uz = dz * inv_rr;
     int ix = 0, iy = 0, iz = 0;
     real t;
-    if (!(C.has_ph && vox_index(p, x, y, z, ix, iy, iz))) {
+    if (!(p.nx > 0 && vox_index(p, x, y, z, ix, iy, iz))) {
         t = rr / RHO_OUTSIDE;
     } else {
         real tmx, tmy, tmz, tdx, tdy, tdz;
@@ -800,6 +802,10 @@ void emit_pair(real sigma, gp_rng &r, real v[2][3], real e[2])
 struct SourcePlan {
     std::vector<int64_t> first;   /* first history of each source (prefix) */
     std::vector<double> tau, pdec; /* T1/2/ln2 (s) and 1-2^(-(t1-t0)/T1/2) */
+    std::vector<double> t0;       /* acquisition start of each source (s)  */
+    std::vector<int> phantom;     /* phantom slot of each source           */
+    std::vector<const gpet_source *> src;
+    std::vector<int64_t> job_first; /* batched runs: first history per job */
     int64_t total = 0;
 };
 
@@ -833,20 +839,22 @@ int64_t binomial_draw(double n, double p, uint64_t seed, int src)
     return (int64_t)k;
 }
 
-gpet_status plan_sources(const gpet_ctx &C, const gpet_source *src, int ns, const gpet_isotope *iso,
-                         int niso, double t0, double t1, int64_t n_decays, SourcePlan &pl)
+/* Decay counts of one acquisition's sources, appended to the plan.
+ * src_base is the index of src[0] in the run's source array (it keys the
+ * binomial stream). */
+gpet_status plan_job(const gpet_ctx &C, const gpet_source *src, int ns, const gpet_isotope *iso, int niso, double t0,
+                     double t1, int64_t n_decays, int src_base, int phantom, SourcePlan &pl)
 {
-    pl.first.assign(ns + 1, 0);
-    pl.tau.assign(ns, 0.0);
-    pl.pdec.assign(ns, 0.0);
-    std::vector<double> expect(ns, 0.0), navail(ns, 0.0);
+    if (pl.first.empty()) pl.first.assign(1, 0);
+    std::vector<double> expect(ns, 0.0), navail(ns, 0.0), tau(ns), pdec(ns);
     for (int i = 0; i < ns; ++i) {
         if (src[i].isotope < 0 || src[i].isotope >= niso) return GPET_ERR_ARG;
         double th = iso[src[i].isotope].half_life_s;
-        pl.tau[i] = th / std::log(2.0);
-        pl.pdec[i] = -std::expm1(-(t1 - t0) / pl.tau[i]);
+        if (!(th > 0.0)) return GPET_ERR_ARG;
+        tau[i] = th / std::log(2.0);
+        pdec[i] = -std::expm1(-(t1 - t0) / tau[i]);
         navail[i] = std::floor(src[i].natom * std::exp2(-t0 / th));
-        expect[i] = navail[i] * pl.pdec[i];
+        expect[i] = navail[i] * pdec[i];
     }
     std::vector<int64_t> cnt(ns, 0);
     if (n_decays > 0) {
@@ -866,26 +874,35 @@ gpet_status plan_sources(const gpet_ctx &C, const gpet_source *src, int ns, cons
         });
         for (int k = 0; assigned < n_decays; ++k, ++assigned) cnt[rem[k % ns].second]++;
     } else {
-        for (int i = 0; i < ns; ++i) cnt[i] = binomial_draw(navail[i], pl.pdec[i], C.opt.seed, i);
+        for (int i = 0; i < ns; ++i) cnt[i] = binomial_draw(navail[i], pdec[i], C.opt.seed, src_base + i);
     }
-    for (int i = 0; i < ns; ++i) pl.first[i + 1] = pl.first[i] + cnt[i];
-    pl.total = pl.first[ns];
+    for (int i = 0; i < ns; ++i) {
+        pl.first.push_back(pl.first.back() + cnt[i]);
+        pl.tau.push_back(tau[i]);
+        pl.pdec.push_back(pdec[i]);
+        pl.t0.push_back(t0);
+        pl.phantom.push_back(phantom);
+        pl.src.push_back(&src[i]);
+    }
+    pl.total = pl.first.back();
     return GPET_OK;
 }
 
 /* One decay history (setPosition, gPET_kernals.cu:483-561, sampled per
  * decay instead of per atom, F8/Q17). */
-void decay_history(const gpet_ctx &C, const SourcePlan &pl, const gpet_source *src,
-                   const gpet_isotope *iso, double t0, int64_t h, gpet_trace *tr2,
+void decay_history(const gpet_ctx &C, const SourcePlan &pl, const gpet_isotope *iso, int64_t h, gpet_trace *tr2,
                    PhotonOut out[2], Counters &cnt, bool want_hits)
 {
     int si = (int)(std::upper_bound(pl.first.begin(), pl.first.end(), h) - pl.first.begin()) - 1;
-    const gpet_source &S = src[si];
+    const gpet_source &S = *pl.src[si];
     const gpet_isotope &I = iso[S.isotope];
+    const int slot = pl.phantom[si];
+    const Phantom *ph = C.slot_set[slot] ? &C.slots[slot] : nullptr;
+    static const Phantom none;
     gp_rng r;
     gp_rng_init(&r, C.opt.seed, (uint64_t)h, GP_TAG_DECAY, 0u);
     double u = gp_rng_uniform_d(&r);
-    double ts = t0 + pl.tau[si] * (-gp_log1p_d(-u * pl.pdec[si]));
+    double ts = pl.t0[si] + pl.tau[si] * (-gp_log1p_d(-u * pl.pdec[si]));
     double tus = ts * 1.0e6;
     real rb = gp_rng_uniform(&r);
     if (tr2) {
@@ -901,13 +918,13 @@ void decay_history(const gpet_ctx &C, const SourcePlan &pl, const gpet_source *s
         gp_rng_align(&r);
         real ek = sample_ek(I, r);
         gp_rng_align(&r);
-        positron_range(C, x, y, z, ek, 0, 0.0f, 0.0f, 0.0f, r, cnt);
+        positron_range(ph ? *ph : none, x, y, z, ek, 0, 0.0f, 0.0f, 0.0f, r, cnt);
     }
     real v[2][3], e[2];
     gp_rng_align(&r);
     emit_pair(C.opt.noncol_sigma, r, v, e);
     for (int k = 0; k < 2; ++k)
-        photon_full(C, h, k, x, y, z, v[k][0], v[k][1], v[k][2], e[k], tus, (int32_t)(2 * h + k),
+        photon_full(C, ph, h, k, x, y, z, v[k][0], v[k][1], v[k][2], e[k], tus, (int32_t)(2 * h + k),
                     tr2 ? &tr2[k] : nullptr, out[k], cnt, want_hits);
 }
 
@@ -925,19 +942,21 @@ void psf_history(const gpet_ctx &C, const double *rec, int ptype, int64_t h, gpe
             tr2[k].history = h; tr2[k].photon = k; tr2[k].panel = -1; tr2[k].t0 = t;
         }
     }
+    const Phantom *ph = C.slot_set[0] ? &C.slots[0] : nullptr;
+    static const Phantom none;
     if (ptype == 1) {
-        photon_full(C, h, 0, x, y, z, vx, vy, vz, E, t, (int32_t)h, tr2 ? &tr2[0] : nullptr, out[0],
+        photon_full(C, ph, h, 0, x, y, z, vx, vy, vz, E, t, (int32_t)h, tr2 ? &tr2[0] : nullptr, out[0],
                     cnt, want_hits);
         return;
     }
     gp_rng r;
     gp_rng_init(&r, C.opt.seed, (uint64_t)h, GP_TAG_DECAY, 0u);
-    if (C.opt.positron_range) positron_range(C, x, y, z, E, 1, vx, vy, vz, r, cnt);
+    if (C.opt.positron_range) positron_range(ph ? *ph : none, x, y, z, E, 1, vx, vy, vz, r, cnt);
     real v[2][3], e[2];
     gp_rng_align(&r);
     emit_pair(C.opt.noncol_sigma, r, v, e);
     for (int k = 0; k < 2; ++k)
-        photon_full(C, h, k, x, y, z, v[k][0], v[k][1], v[k][2], e[k], t, (int32_t)(2 * h + k),
+        photon_full(C, ph, h, k, x, y, z, v[k][0], v[k][1], v[k][2], e[k], t, (int32_t)(2 * h + k),
                     tr2 ? &tr2[k] : nullptr, out[k], cnt, want_hits);
 }
 
@@ -985,7 +1004,7 @@ void blur_event(const gpet_ctx &C, gpet_event &e)
 /* deadtime (gPET_kernals.cu:657-698) on (site,t)-sorted events.  Chain
  * starts are taken from the unmodified times (the reference reads times
  * other threads may already have overwritten).  Returns alive flags. */
-void deadtime(std::vector<gpet_event> &ev, const std::vector<int> &site, double tau, int type,
+void deadtime(std::vector<gpet_event> &ev, const std::vector<int64_t> &site, double tau, int type,
               std::vector<char> &alive)
 {
     size_t n = ev.size();
@@ -1007,10 +1026,18 @@ void deadtime(std::vector<gpet_event> &ev, const std::vector<int> &site, double 
     }
 }
 
+/* Job of a history: the largest j with job_first[j] <= h (0 for one job). */
+inline int64_t job_of(const std::vector<int64_t> &jf, int64_t h)
+{
+    if (jf.size() <= 2) return 0;
+    return (int64_t)(std::upper_bound(jf.begin(), jf.end(), h) - jf.begin()) - 1;
+}
+
 void run_digitizer(gpet_ctx &C, Result &R)
 {
     const gpet_digitizer &g = C.dig;
     const Detector &d = C.det;
+    const std::vector<int64_t> &jf = R.job_first;
     std::vector<gpet_event> ev;
     ev.reserve(R.adder.size());
     for (gpet_event e : R.adder) {
@@ -1026,13 +1053,17 @@ void run_digitizer(gpet_ctx &C, Result &R)
     if (g.dead_time_us > 0.0f && !ev.empty()) {
         std::vector<size_t> idx(ev.size());
         for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+        /* deadtime sites are per job: key (job, siten) */
+        std::vector<int64_t> key(ev.size());
+        for (size_t i = 0; i < ev.size(); ++i)
+            key[i] = (job_of(jf, ev[i].eventid) << 32) | (int64_t)(uint32_t)ev[i].siten;
         std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
-            if (ev[a].siten != ev[b].siten) return ev[a].siten < ev[b].siten;
+            if (key[a] != key[b]) return key[a] < key[b];
             return ev_less_t(ev[a], ev[b]);
         });
         std::vector<gpet_event> s(ev.size());
-        std::vector<int> site(ev.size());
-        for (size_t i = 0; i < idx.size(); ++i) { s[i] = ev[idx[i]]; site[i] = s[i].siten; }
+        std::vector<int64_t> site(ev.size());
+        for (size_t i = 0; i < idx.size(); ++i) { s[i] = ev[idx[i]]; site[i] = key[idx[i]]; }
         std::vector<char> alive;
         deadtime(s, site, (double)g.dead_time_us, g.dead_type, alive);
         ev.clear();
@@ -1044,7 +1075,11 @@ void run_digitizer(gpet_ctx &C, Result &R)
     sg.reserve(ev.size());
     for (auto &e : ev)
         if (!(e.E < g.win_min || e.E > g.win_max)) sg.push_back(e);
-    std::sort(sg.begin(), sg.end(), ev_less_t);
+    std::sort(sg.begin(), sg.end(), [&](const gpet_event &a, const gpet_event &b) {
+        int64_t ja = job_of(jf, a.eventid), jb = job_of(jf, b.eventid);
+        if (ja != jb) return ja < jb;
+        return ev_less_t(a, b);
+    });
     R.singles = sg;
     R.coinc.clear();
     R.n_coinc_true = 0;
@@ -1053,7 +1088,8 @@ void run_digitizer(gpet_ctx &C, Result &R)
         size_t n = sg.size(), i = 0;
         while (i < n) {
             size_t j = i + 1;
-            while (j < n && sg[j].t - sg[i].t <= W) ++j;
+            const int64_t ji = job_of(jf, sg[i].eventid);
+            while (j < n && sg[j].t - sg[i].t <= W && job_of(jf, sg[j].eventid) == ji) ++j;
             size_t m = j - i;
             if (m == 2 || (m > 2 && g.coinc_policy == GPET_COINC_TAKE_ALL_GOODS)) {
                 for (size_t a = i; a < j; ++a)
@@ -1065,6 +1101,13 @@ void run_digitizer(gpet_ctx &C, Result &R)
             }
             i = j;
         }
+    }
+    if (jf.size() >= 2) {
+        const size_t nj = jf.size() - 1;
+        R.job_counts.assign(3 * nj, 0);
+        for (size_t j = 0; j < nj; ++j) R.job_counts[3 * j] = jf[j + 1] - jf[j];
+        for (const auto &e : R.singles) R.job_counts[3 * job_of(jf, e.eventid) + 1]++;
+        for (size_t k = 0; k + 1 < R.coinc.size(); k += 2) R.job_counts[3 * job_of(jf, R.coinc[k].eventid) + 2]++;
     }
 }
 
@@ -1114,9 +1157,11 @@ bool ev_less_id(const gpet_event &a, const gpet_event &b)
 
 /* Run the transport over n histories with OpenMP, collect, digitize. */
 template <class F>
-void run_histories(gpet_ctx &C, int64_t n, F hist_fn, gpet_sink *sink, Result &R)
+void run_histories(gpet_ctx &C, int64_t n, F hist_fn, gpet_sink *sink, Result &R,
+                   const std::vector<int64_t> *job_first = nullptr)
 {
     R = Result();
+    if (job_first) R.job_first = *job_first;
     R.n_histories = n;
     const bool want_trace = sink && sink->trace && sink->trace_cap > 0;
     const bool want_hits = C.opt.record_hits != 0;
@@ -1194,6 +1239,8 @@ void fill_sink(const Result &R, gpet_sink *s)
         }
     }
     copy_out(R.trace, s->trace, s->trace_cap, s->n_trace);
+    if (s->job_counts && !R.job_counts.empty())
+        memcpy(s->job_counts, R.job_counts.data(), sizeof(int64_t) * R.job_counts.size());
 }
 
 gpet_status check_ready(gpet_ctx *c)
@@ -1254,14 +1301,19 @@ gpet_status gpet_cpu_set_tables(gpet_ctx *c, const gpet_tables *t)
     return GPET_OK;
 }
 
-gpet_status gpet_cpu_set_phantom(gpet_ctx *c, const int32_t *mat, const float *dens,
-                                 const int32_t dims[3], const float off[3], const float size[3])
+gpet_status gpet_cpu_set_phantom_slot(gpet_ctx *c, int32_t slot, const int32_t *mat, const float *dens,
+                                      const int32_t dims[3], const float off[3], const float size[3])
 {
     if (!c || !mat || !dens || !dims || !off || !size) return GPET_ERR_ARG;
+    if (slot < 0 || slot >= GPET_MAXPHANTOMS) return fail(c, GPET_ERR_ARG, "phantom slot out of range");
     if (!c->has_tab) return fail(c, GPET_ERR_STATE, "tables must be set before the phantom");
-    Phantom &p = c->ph;
+    if (dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0) return fail(c, GPET_ERR_ARG, "phantom dims");
+    if ((int)c->slots.size() <= slot) {
+        c->slots.resize((size_t)slot + 1);
+        c->slot_set.resize((size_t)slot + 1, 0);
+    }
+    Phantom p;
     p.nx = dims[0]; p.ny = dims[1]; p.nz = dims[2];
-    if (p.nx <= 0 || p.ny <= 0 || p.nz <= 0) return fail(c, GPET_ERR_ARG, "phantom dims");
     p.offx = off[0]; p.offy = off[1]; p.offz = off[2];
     p.dx = size[0] / (float)p.nx; p.dy = size[1] / (float)p.ny; p.dz = size[2] / (float)p.nz;
     p.idx = 1.0f / p.dx; p.idy = 1.0f / p.dy; p.idz = 1.0f / p.dz;
@@ -1275,8 +1327,16 @@ gpet_status gpet_cpu_set_phantom(gpet_ctx *c, const int32_t *mat, const float *d
         if (p.dens[i] > rmax[m]) rmax[m] = p.dens[i];
     }
     build_wck(c->tab, rmax, p.wck);
-    c->has_ph = true;
+    c->slots[slot] = std::move(p);
+    c->slot_set[slot] = 1;
+    if (slot == 0) c->has_ph = true;
     return GPET_OK;
+}
+
+gpet_status gpet_cpu_set_phantom(gpet_ctx *c, const int32_t *mat, const float *dens,
+                                 const int32_t dims[3], const float off[3], const float size[3])
+{
+    return gpet_cpu_set_phantom_slot(c, 0, mat, dens, dims, off, size);
 }
 
 gpet_status gpet_cpu_set_detector(gpet_ctx *c, const gpet_panel *panels, int32_t n,
@@ -1323,15 +1383,61 @@ gpet_status gpet_cpu_run_source(gpet_ctx *c, const gpet_source *src, int32_t ns,
     if (st) return st;
     if (!src || ns <= 0 || !iso || niso <= 0 || !(t1 >= t0)) return fail(c, GPET_ERR_ARG, "source arguments");
     SourcePlan pl;
-    st = plan_sources(*c, src, ns, iso, niso, t0, t1, n_decays, pl);
+    st = plan_job(*c, src, ns, iso, niso, t0, t1, n_decays, 0, 0, pl);
     if (st) return fail(c, st, "bad source/isotope");
+    if (pl.total > GPET_MAX_HISTORIES) return fail(c, GPET_ERR_CAPACITY, "more than 2^30 histories in one run");
     const gpet_ctx &C = *c;
     run_histories(*c, pl.total,
                   [&](int64_t h, gpet_trace *tr, PhotonOut *out, Counters &cnt, bool wh) {
-                      decay_history(C, pl, src, iso, t0, h, tr, out, cnt, wh);
+                      decay_history(C, pl, iso, h, tr, out, cnt, wh);
                   },
                   sink, c->last);
     fill_sink(c->last, sink);
+    return GPET_OK;
+}
+
+gpet_status gpet_cpu_run_batch(gpet_ctx *c, const gpet_job *jobs, int32_t njobs, const gpet_source *src,
+                               int32_t nsrc, const gpet_isotope *iso, int32_t niso, gpet_sink *sink)
+{
+    gpet_status st = check_ready(c);
+    if (st) return st;
+    if (!jobs || njobs <= 0 || !src || nsrc <= 0 || !iso || niso <= 0) return fail(c, GPET_ERR_ARG, "batch arguments");
+    SourcePlan pl;
+    pl.job_first.assign(1, 0);
+    for (int j = 0; j < njobs; ++j) {
+        const gpet_job &J = jobs[j];
+        if (J.first_source < 0 || J.n_sources <= 0 || J.first_source + J.n_sources > nsrc || !(J.t1 >= J.t0))
+            return fail(c, GPET_ERR_ARG, "job source range or time window");
+        if (J.phantom < 0 || J.phantom >= (int)c->slots.size() || !c->slot_set[J.phantom])
+            return fail(c, GPET_ERR_STATE, "job phantom slot not set");
+        st = plan_job(*c, src + J.first_source, J.n_sources, iso, niso, J.t0, J.t1, J.n_decays, J.first_source,
+                      J.phantom, pl);
+        if (st) return fail(c, st, "bad source/isotope");
+        pl.job_first.push_back(pl.total);
+    }
+    if (pl.total > GPET_MAX_HISTORIES) return fail(c, GPET_ERR_CAPACITY, "more than 2^30 histories in one run");
+    const gpet_ctx &C = *c;
+    run_histories(*c, pl.total,
+                  [&](int64_t h, gpet_trace *tr, PhotonOut *out, Counters &cnt, bool wh) {
+                      decay_history(C, pl, iso, h, tr, out, cnt, wh);
+                  },
+                  sink, c->last, &pl.job_first);
+    fill_sink(c->last, sink);
+    return GPET_OK;
+}
+
+/* The CPU build has no launch shape; these keep the ABI symmetric. */
+gpet_status gpet_cpu_set_launch(gpet_ctx *c, const gpet_launch *l)
+{
+    if (!c || !l) return GPET_ERR_ARG;
+    c->launch = *l;
+    return GPET_OK;
+}
+
+gpet_status gpet_cpu_get_launch(gpet_ctx *c, gpet_launch *l)
+{
+    if (!c || !l) return GPET_ERR_ARG;
+    *l = c->launch;
     return GPET_OK;
 }
 
@@ -1403,7 +1509,8 @@ gpet_status gpet_cpu_get_tallies(gpet_ctx *c, gpet_tallies *t)
 gpet_status gpet_cpu_get_majorant(gpet_ctx *c, int32_t which, float *out, float *e0, float *idl)
 {
     if (!c || !out) return GPET_ERR_ARG;
-    const Wck &w = which == 0 ? c->ph.wck : c->det.wck;
+    if (which < 0 || (which >= 2 && which - 2 >= (int)c->slots.size())) return GPET_ERR_ARG;
+    const Wck &w = which == 1 ? c->det.wck : c->slots[which == 0 ? 0 : which - 2].wck;
     if (w.lam.empty()) return GPET_ERR_STATE;
     memcpy(out, w.lam.data(), sizeof(float) * GPET_NWCK);
     if (e0) *e0 = w.e0;
@@ -1473,7 +1580,7 @@ gpet_status gpet_cpu_positron_range(gpet_ctx *c, const float *xyz0, float ek, ui
     Counters cnt;
     for (int64_t i = 0; i < n; ++i) {
         float x = xyz0[0], y = xyz0[1], z = xyz0[2];
-        positron_range(*c, x, y, z, ek, 0, 0, 0, 0, r, cnt);
+        positron_range(c->slots[0], x, y, z, ek, 0, 0, 0, 0, r, cnt);
         xyz[3 * i] = x; xyz[3 * i + 1] = y; xyz[3 * i + 2] = z;
     }
     return GPET_OK;
