@@ -41,6 +41,8 @@ HIP_SOURCES = [
     os.path.join(CSRC, "host", "ctx.hip"),
     os.path.join(CSRC, "kernels", "transport.hip"),
     os.path.join(CSRC, "kernels", "digitizer.hip"),
+    os.path.join(CSRC, "kernels", "mlp.hip"),
+    os.path.join(CSRC, "host", "surrogate.hip"),
 ]
 CPP_SOURCES = [os.path.join(CSRC, "host", "io.cpp")]
 

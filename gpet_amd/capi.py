@@ -102,6 +102,21 @@ class Launch(C.Structure):
                 "batch": int(self.batch)}
 
 
+class SurrogateConfig(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("batch", C.c_int32), ("lr", C.c_float), ("seed", C.c_uint64),
+                ("reinit", C.c_int32)]
+
+
+class SurrogateReport(C.Structure):
+    _fields_ = [("records", C.c_int64), ("steps", C.c_int64), ("seconds", C.c_double), ("flops", C.c_double),
+                ("loss_first", C.c_float), ("loss_last", C.c_float)]
+
+
+SURR_NX = 6
+SURR_NY = 5
+SURR_NH = 145
+
+
 class Job(C.Structure):
     _fields_ = [("phantom", C.c_int32), ("first_source", C.c_int32), ("n_sources", C.c_int32),
                 ("reserved", C.c_int32), ("t0", C.c_double), ("t1", C.c_double), ("n_decays", C.c_int64)]
@@ -118,7 +133,8 @@ class Tallies(C.Structure):
         ("seconds_transport", C.c_double), ("seconds_digitizer", C.c_double),
         ("kernel_seconds", C.c_double * 2), ("kernel_launches", C.c_int64 * 2), ("kernel_histories", C.c_int64),
         ("launches", C.c_int64),
-        ("launch", Launch), ("graph", C.c_int32), ("n_jobs", C.c_int32), ("seconds_tune", C.c_double)]
+        ("launch", Launch), ("graph", C.c_int32), ("n_jobs", C.c_int32), ("seconds_tune", C.c_double),
+        ("n_surrogate", C.c_int64), ("seconds_surrogate", C.c_double)]
 
 FLAG_NO_GRAPH = 1
 FLAG_AUTOTUNE = 2
@@ -158,6 +174,17 @@ CTX_FUNCS = {
     "run_batch": (C.c_int32, [_VP, P(Job), C.c_int32, P(Source), C.c_int32, P(Isotope), C.c_int32, P(Sink)]),
     "set_launch": (C.c_int32, [_VP, P(Launch)]),
     "get_launch": (C.c_int32, [_VP, P(Launch)]),
+    "surrogate_harvest": (C.c_int32, [_VP, P(Source), C.c_int32, P(Isotope), C.c_int32, C.c_double, C.c_double,
+                                      C.c_int64, C.c_int64]),
+    "surrogate_train": (C.c_int32, [_VP, P(SurrogateConfig), P(SurrogateReport)]),
+    "surrogate_enable": (C.c_int32, [_VP, C.c_int32]),
+    "surrogate_save": (C.c_int32, [_VP, C.c_char_p]),
+    "surrogate_load": (C.c_int32, [_VP, C.c_char_p]),
+    "surrogate_get_params": (C.c_int32, [_VP, _VP, C.c_int64, P(C.c_int64)]),
+    "surrogate_set_params": (C.c_int32, [_VP, _VP, C.c_int64]),
+    "surrogate_forward": (C.c_int32, [_VP, _VP, C.c_int64, _VP]),
+    "surrogate_records": (C.c_int32, [_VP, _VP, C.c_int64, P(C.c_int64)]),
+    "surrogate_grad": (C.c_int32, [_VP, _VP, _VP, C.c_int64, P(C.c_float), _VP]),
     "get_tallies": (C.c_int32, [_VP, P(Tallies)]),
     "get_majorant": (C.c_int32, [_VP, C.c_int32, _VP, P(C.c_float), P(C.c_float)]),
 }

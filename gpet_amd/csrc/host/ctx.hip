@@ -25,168 +25,9 @@
 #include "../common/gp_math.h"
 #include "internal.h"
 
-namespace {
-
-constexpr int64_t DEFAULT_BATCH = 1 << 22;
-
-template <class T>
-struct DBuf {
-    T *p = nullptr;
-    int64_t cap = 0;
-};
-
-/* One phantom slot: voxels, bricked densities and its own majorant. */
-struct PhSlot {
-    DBuf<int2> vox;
-    DBuf<float> dens_b;
-    DBuf<float> wck;
-    std::vector<float> h_wck;
-    DevPhantom dp;
-    bool present = false;
-};
-
-/* Captured transport batches of the last graph run and the launch
- * arguments they were captured with. */
-struct GraphCache {
-    hipGraphExec_t exec = nullptr;
-    std::vector<unsigned char> key;
-};
-
-} // namespace
-
-struct gpet_ctx {
-    gpet_options opt;
-    std::string err;
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
-    std::vector<hipEvent_t> kev; /* per-launch timing events */
-
-    /* tables */
-    bool has_tab = false;
-    int nmat = 0;
-    float eminph = 0, emax = 0;
-    float e0 = 0, idl = 0, lam_idl = 0;
-    std::vector<float> h_lamph; /* host copy for majorant construction */
-    DBuf<float4> d_mu4;
-    DBuf<float> d_surf[2];
-    DevTables dtab;
-
-    /* phantom slots; slot 0 is the phantom of gpet_set_phantom */
-    bool has_ph = false;
-    std::vector<PhSlot> slots;
-    DevPhantom dph;              /* slot 0's descriptor (or absent)          */
-    DBuf<DevPhantom> d_phs;      /* descriptors of all slots, for batches    */
-    bool phs_dirty = true;
-
-    /* launch shape, tuner state, graph cache */
-    gpet_launch launch;
-    int n_cu = 256;
-    bool tuned[2] = {false, false}; /* single-phantom runs, batched runs    */
-    double tune_seconds = 0;
-    GraphCache graph;
-    double ev_per_hist = -1.0;   /* adder events per history of the last run */
-    DBuf<DevSegment> d_seg;
-    DBuf<int64_t> d_job_first;
-    DBuf<unsigned long long> d_job_cnt;
-
-    /* detector */
-    bool has_det = false;
-    DBuf<DevPanel> d_panels;
-    DBuf<float> d_wck_det;
-    std::vector<float> h_wck_det;
-    std::vector<gpet_panel> panels;
-    DevDetector ddet;
-
-    /* digitizer */
-    bool has_dig = false;
-    gpet_digitizer dig;
-
-    /* run buffers */
-    DBuf<float> q_f[6];
-    DBuf<double> q_t;
-    DBuf<int4> q_info;
-    DBuf<int> q_slot;
-    int64_t q_cap = 0;
-    DBuf<unsigned> d_small;      /* [0] queue count, [1] events, [2] hits */
-    DBuf<unsigned long long> d_cnt;
-    DBuf<gpet_event> d_events;
-    DBuf<gpet_trace> d_trace;
-    DBuf<int32_t> d_hit_id;
-    DBuf<float> d_hit_f;
-    DBuf<int64_t> d_first;
-    DBuf<DevSource> d_src;
-    DBuf<DevIsotope> d_iso;
-    DBuf<double> d_psf;
-    DigiWork *dw = nullptr;
-
-    /* last run */
-    gpet_tallies last;
-    DigiOut dout;
-    int64_t n_events = 0;
-};
+#include "context.h"
 
 namespace {
-
-gpet_status fail(gpet_ctx *c, gpet_status s, const std::string &msg)
-{
-    if (c) c->err = msg;
-    return s;
-}
-
-#define HIPOK(c, x)                                                                                    \
-    do {                                                                                               \
-        hipError_t e_ = (x);                                                                           \
-        if (e_ != hipSuccess)                                                                          \
-            return fail((c), GPET_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_));         \
-    } while (0)
-
-template <class T>
-gpet_status dalloc(gpet_ctx *c, DBuf<T> &b, int64_t n)
-{
-    if (n <= b.cap && b.p) return GPET_OK;
-    if (b.p) (void)hipFree(b.p);
-    b.p = nullptr;
-    b.cap = 0;
-    int64_t m = std::max<int64_t>(n, 1);
-    hipError_t e = hipMalloc((void **)&b.p, sizeof(T) * (size_t)m);
-    if (e != hipSuccess) return fail(c, GPET_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
-    b.cap = m;
-    return GPET_OK;
-}
-
-template <class T>
-void dfree(DBuf<T> &b)
-{
-    if (b.p) (void)hipFree(b.p);
-    b.p = nullptr;
-    b.cap = 0;
-}
-
-/* Grow keeping the first `keep` elements (stream-ordered copy). */
-template <class T>
-gpet_status dgrow(gpet_ctx *c, DBuf<T> &b, int64_t need, int64_t keep)
-{
-    if (need <= b.cap && b.p) return GPET_OK;
-    DBuf<T> nb;
-    gpet_status st = dalloc(c, nb, std::max<int64_t>(need, b.cap + b.cap / 2));
-    if (st) return st;
-    if (b.p && keep > 0) HIPOK(c, hipMemcpyAsync(nb.p, b.p, sizeof(T) * (size_t)keep, hipMemcpyDeviceToDevice, c->stream));
-    HIPOK(c, hipStreamSynchronize(c->stream));
-    dfree(b);
-    b = nb;
-    return GPET_OK;
-}
-
-template <class T>
-gpet_status upload(gpet_ctx *c, DBuf<T> &b, const T *h, int64_t n)
-{
-    gpet_status st = dalloc(c, b, n);
-    if (st) return st;
-    if (n > 0) HIPOK(c, hipMemcpyAsync(b.p, h, sizeof(T) * (size_t)n, hipMemcpyHostToDevice, c->stream));
-    HIPOK(c, hipStreamSynchronize(c->stream));
-    return GPET_OK;
-}
 
 /* ---- Woodcock majorant (iniwck, initialize.cu:773-829, 919-966) ---------
  * Same node grid as the reference; the node value is 1/max of the linearly
@@ -414,7 +255,7 @@ gpet_launch default_launch()
     memset(&L, 0, sizeof(L));
     L.block[0] = 256; L.block[1] = 256;
     L.wpe[0] = 0; L.wpe[1] = 0;
-    L.blocks_per_cu[0] = 0; L.blocks_per_cu[1] = 8;
+    L.blocks_per_cu[0] = 4; L.blocks_per_cu[1] = 4;   /* the tuner's pick on the cylinder workload */
     L.batch = DEFAULT_BATCH;
     return L;
 }
@@ -463,7 +304,7 @@ struct BatchLaunch {
 };
 
 gpet_status launch_batch(gpet_ctx *c, const BatchLaunch &B, const DevQueue &q, const PanelParams &pp,
-                         const gpet_launch &L, bool batched, hipEvent_t *ev)
+                         const gpet_launch &L, bool batched, hipEvent_t *ev, int pmode)
 {
     int e = gpet_launch_zero((unsigned *)q.count, 1, c->stream);
     if (e) return fail(c, GPET_ERR_DEVICE, std::string("zero kernel: ") + hipGetErrorString((hipError_t)e));
@@ -472,7 +313,8 @@ gpet_status launch_batch(gpet_ctx *c, const BatchLaunch &B, const DevQueue &q, c
     if (e) return fail(c, GPET_ERR_DEVICE, std::string("phantom kernel: ") + hipGetErrorString((hipError_t)e));
     if (ev) HIPOK(c, hipEventRecord(ev[1], c->stream));
     if (ev) HIPOK(c, hipEventRecord(ev[2], c->stream));
-    e = gpet_launch_panel(pp, c->dtab, c->ddet, q, std::min<int64_t>(2 * B.nh, q.cap), cfg_of(L, 1), c->n_cu, c->stream);
+    e = gpet_launch_panel(pp, c->dtab, c->ddet, q, std::min<int64_t>(2 * B.nh, q.cap), cfg_of(L, 1), c->n_cu, c->stream,
+                          pmode);
     if (e) return fail(c, GPET_ERR_DEVICE, std::string("panel kernel: ") + hipGetErrorString((hipError_t)e));
     if (ev) HIPOK(c, hipEventRecord(ev[3], c->stream));
     return GPET_OK;
@@ -497,7 +339,7 @@ void drop_graph(gpet_ctx *c)
  * stream out of capture mode, if capture is not possible; the caller then
  * launches the batches one by one. */
 bool run_graph(gpet_ctx *c, const std::vector<BatchLaunch> &B, const DevQueue &q, const PanelParams &pp,
-               const gpet_launch &L, bool batched)
+               const gpet_launch &L, bool batched, int pmode)
 {
     std::vector<unsigned char> key;
     for (size_t b = 1; b < B.size(); ++b) {
@@ -511,12 +353,13 @@ bool run_graph(gpet_ctx *c, const std::vector<BatchLaunch> &B, const DevQueue &q
     key_add(key, c->ddet);
     key_add(key, L);
     key_add(key, batched);
+    key_add(key, pmode);
     if (!(c->graph.exec && key == c->graph.key)) {
         drop_graph(c);
         if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
         bool ok = true;
         for (size_t b = 1; b < B.size() && ok; ++b)
-            ok = launch_batch(c, B[b], q, pp, L, batched, &c->kev[4 * b]) == GPET_OK;
+            ok = launch_batch(c, B[b], q, pp, L, batched, &c->kev[4 * b], pmode) == GPET_OK;
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (!ok || e != hipSuccess || !g) {
@@ -654,6 +497,8 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
     gpet_status st;
     if (n_hist > GPET_MAX_HISTORIES) return fail(c, GPET_ERR_CAPACITY, "more than 2^30 histories in one run");
     const bool batched = bi != nullptr;
+    c->last_surr_items = 0;
+    c->last_surr_seconds = 0;
     if ((st = dalloc(c, c->d_small, 4)) || (st = dalloc(c, c->d_cnt, CNT_NUM))) return st;
 
     rp.seed = c->opt.seed;
@@ -694,6 +539,16 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
         if ((st = dalloc(c, c->d_hit_id, 5 * hits_cap)) || (st = dalloc(c, c->d_hit_f, 5 * hits_cap))) return st;
     }
     DevQueue q = make_queue(c, qcap, want_trace);
+    const int pmode = c->panel_mode == PANEL_HARVEST ? PANEL_HARVEST : (c->surr.enabled ? PANEL_SURROGATE : PANEL_PLAIN);
+    if (pmode == PANEL_SURROGATE) {
+        if (want_hits) return fail(c, GPET_ERR_UNSUPPORTED, "surrogate mode does not produce hit records");
+        if ((st = surr_run_prepare(c, n_hist, qcap, pp.sq))) return st;
+    }
+    if (pmode == PANEL_HARVEST) {
+        pp.harvest = c->surr.set.p + SURR_REC * c->surr.set_n;
+        pp.harvest_count = c->surr.set_count.p;
+        pp.harvest_cap = c->surr.set.cap / SURR_REC - c->surr.set_n;
+    }
     rp.trace = want_trace ? c->d_trace.p : nullptr;
     rp.trace_cap = ntr;
     pp.hit_id = want_hits ? c->d_hit_id.p : nullptr;
@@ -750,13 +605,15 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
     for (int attempt = 0;; ++attempt) {
         HIPOK(c, hipMemsetAsync(c->d_small.p, 0, sizeof(unsigned) * 4, c->stream));
         HIPOK(c, hipMemsetAsync(c->d_cnt.p, 0, sizeof(unsigned long long) * CNT_NUM, c->stream));
+        if (pmode == PANEL_SURROGATE) HIPOK(c, hipMemsetAsync(pp.sq.count, 0, sizeof(int), c->stream));
+        if (pmode == PANEL_HARVEST) HIPOK(c, hipMemsetAsync(pp.harvest_count, 0, sizeof(unsigned long long), c->stream));
         if (want_trace) HIPOK(c, hipMemsetAsync(c->d_trace.p, 0, sizeof(gpet_trace) * (size_t)ntr, c->stream));
         pp.events = c->d_events.p;
         pp.events_cap = c->d_events.cap;
         HIPOK(c, hipEventRecord(c->ev0, c->stream));
         used_graph = false;
         if (nbatch > 0) {
-            if ((st = launch_batch(c, B[0], q, pp, L, batched, &c->kev[0]))) return st;
+            if ((st = launch_batch(c, B[0], q, pp, L, batched, &c->kev[0], pmode))) return st;
             if (nbatch > 1 && c->ev_per_hist < 0 && attempt == 0) {
                 HIPOK(c, hipMemcpyAsync(small, c->d_small.p, sizeof(small), hipMemcpyDeviceToHost, c->stream));
                 HIPOK(c, hipStreamSynchronize(c->stream));
@@ -767,22 +624,49 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
                 pp.events_cap = c->d_events.cap;
             }
             if (nbatch > 1) {
-                if (!(c->opt.flags & GPET_FLAG_NO_GRAPH)) used_graph = run_graph(c, B, q, pp, L, batched);
+                if (!(c->opt.flags & GPET_FLAG_NO_GRAPH)) used_graph = run_graph(c, B, q, pp, L, batched, pmode);
                 if (!used_graph)
                     for (int64_t b = 1; b < nbatch; ++b)
-                        if ((st = launch_batch(c, B[b], q, pp, L, batched, &c->kev[4 * b]))) return st;
+                        if ((st = launch_batch(c, B[b], q, pp, L, batched, &c->kev[4 * b], pmode))) return st;
             }
         }
         HIPOK(c, hipEventRecord(c->ev1, c->stream));
         HIPOK(c, hipMemcpyAsync(small, c->d_small.p, sizeof(small), hipMemcpyDeviceToHost, c->stream));
         HIPOK(c, hipMemcpyAsync(cnt, c->d_cnt.p, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+        int n_surr = 0;
+        if (pmode == PANEL_SURROGATE)
+            HIPOK(c, hipMemcpyAsync(&n_surr, pp.sq.count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPOK(c, hipStreamSynchronize(c->stream));
-        if ((int64_t)small[1] <= c->d_events.cap) break;
+        const bool ev_ok = (int64_t)small[1] <= c->d_events.cap;
+        const bool sq_ok = (int64_t)n_surr <= pp.sq.cap;
+        if (ev_ok && sq_ok) {
+            if (pmode == PANEL_SURROGATE) {
+                /* the network's events: at most one per listed photon */
+                if ((st = dgrow(c, c->d_events, (int64_t)small[1] + n_surr, (int64_t)small[1]))) return st;
+                pp.events = c->d_events.p;
+                pp.events_cap = c->d_events.cap;
+                if ((st = surr_run_infer(c, n_surr, pp))) return st;
+                HIPOK(c, hipMemcpyAsync(small, c->d_small.p, sizeof(small), hipMemcpyDeviceToHost, c->stream));
+                HIPOK(c, hipStreamSynchronize(c->stream));
+                if (n_hist > 0) c->surr.items_per_hist = (double)n_surr / (double)n_hist;
+                c->last_surr_items = n_surr;
+            }
+            break;
+        }
         if (attempt >= 1) return fail(c, GPET_ERR_CAPACITY, "event buffer overflow after resize");
-        if ((st = dgrow(c, c->d_events, (int64_t)small[1] + per_batch_max, 0))) return st;
+        if (!ev_ok && (st = dgrow(c, c->d_events, (int64_t)small[1] + per_batch_max, 0))) return st;
+        if (!sq_ok) {
+            c->surr.items_per_hist = (double)n_surr / (double)std::max<int64_t>(n_hist, 1);
+            if ((st = surr_run_prepare(c, n_hist, qcap, pp.sq))) return st;
+        }
     }
     c->n_events = small[1];
-    if (n_hist > 0) c->ev_per_hist = (double)small[1] / (double)n_hist;
+    if (n_hist > 0 && pmode != PANEL_SURROGATE) c->ev_per_hist = (double)small[1] / (double)n_hist;
+    if (pmode == PANEL_HARVEST) {
+        unsigned long long nh = 0;
+        HIPOK(c, hipMemcpy(&nh, c->surr.set_count.p, sizeof(nh), hipMemcpyDeviceToHost));
+        c->surr.set_n += std::min<int64_t>((int64_t)nh, pp.harvest_cap);
+    }
 
     DigiParams dp;
     memset(&dp, 0, sizeof(dp));
@@ -844,6 +728,8 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
     T.graph = used_graph ? 1 : 0;
     T.n_jobs = njobs;
     T.seconds_tune = tune_s;
+    T.n_surrogate = c->last_surr_items;
+    T.seconds_surrogate = c->last_surr_seconds;
 
     if (!sink) return GPET_OK;
     if (batched && sink->job_counts) {
@@ -952,6 +838,7 @@ gpet_status gpet_destroy(gpet_ctx *c)
     }
     dfree(c->d_phs); dfree(c->d_seg); dfree(c->d_job_first); dfree(c->d_job_cnt);
     drop_graph(c);
+    surr_free(c);
     dfree(c->d_panels); dfree(c->d_wck_det);
     for (auto &b : c->q_f) dfree(b);
     dfree(c->q_t); dfree(c->q_info); dfree(c->q_slot); dfree(c->d_small); dfree(c->d_cnt);

@@ -145,7 +145,17 @@ struct PanelParams {
     int64_t hits_cap;
     gpet_trace *trace;
     unsigned long long *cnt;
+    /* surrogate: queue entries inside the validity range are copied here
+     * (run-wide) and left to the network instead of being transported */
+    DevQueue sq;
+    /* harvest: one record (features, targets) per in-range queue item */
+    float *harvest;
+    unsigned long long *harvest_count;
+    int64_t harvest_cap;
 };
+
+/* Panel kernel modes. */
+enum { PANEL_PLAIN = 0, PANEL_HARVEST = 1, PANEL_SURROGATE = 2 };
 
 /* Launch shape of the two transport kernels: block size, a register
  * budget variant (minimum waves per SIMD, 0 = compiler default) and the
@@ -165,7 +175,40 @@ int gpet_launch_phantom(const RunParams &rp, const DevTables &tab, const DevPhan
                         const DevDetector &det, const DevQueue &q, bool batch, const LaunchCfg &cfg,
                         int n_cu, hipStream_t s);
 int gpet_launch_panel(const PanelParams &pp, const DevTables &tab, const DevDetector &det,
-                      const DevQueue &q, int64_t max_items, const LaunchCfg &cfg, int n_cu, hipStream_t s);
+                      const DevQueue &q, int64_t max_items, const LaunchCfg &cfg, int n_cu, hipStream_t s,
+                      int mode = PANEL_PLAIN);
+
+/* Surrogate kernels (mlp.hip). */
+struct GemmArgs {
+    const uint16_t *A, *B;
+    int64_t lda, ldb;
+    int64_t M, N, K;
+    int64_t k_chunk;
+    float *C;
+    int64_t ldc;
+    uint16_t *Cb;
+    int64_t ldcb;
+    const float *Z;
+    int64_t ldz;
+};
+/* EPI_F32: C fp32; EPI_SILU: Z = C fp32 and silu(C) bf16; EPI_ACT: silu(C)
+ * bf16 only; EPI_DSILU: C * silu'(Z) bf16; EPI_PARTIAL: split-K partial. */
+enum { EPI_F32 = 0, EPI_SILU = 1, EPI_DSILU = 2, EPI_PARTIAL = 3, EPI_ACT = 4 };
+int gpet_gemm(int kind, bool ta, bool tb, const GemmArgs &g, int splits, hipStream_t s);
+int gpet_sum_partials(const float *part, int nz, int64_t n, float *out, hipStream_t s);
+int gpet_bias_columns(uint16_t *a, int64_t M, int64_t ld, int c0, hipStream_t s);
+int gpet_mdn_loss_grad(const float *H, int64_t ldh, const float *Y, int64_t M, float inv_n, uint16_t *dH,
+                       int64_t lddh, float *part, hipStream_t s);
+int gpet_adam(float *w, float *m, float *v, const float *g, const float *gscale, uint16_t *wb, int64_t n, float lr,
+              float b1, float b2, float c1, float c2, float eps, hipStream_t s);
+/* factor[0] = min(1, clip / |g|) (240 partials in part). */
+int gpet_clip_factor(const float *g, int64_t n, float clip, float *part, float *factor, hipStream_t s);
+int gpet_pack_train(const float *set, int64_t n, uint64_t seed, int64_t step, int64_t M, uint16_t *X, float *Y,
+                    hipStream_t s);
+int gpet_loss_reduce(const float *part, int n, float *out, hipStream_t s);
+int gpet_pack_infer(const DevQueue &sq, int64_t row0, int64_t n, const DevDetector &det, uint16_t *X, hipStream_t s);
+int gpet_surr_sample(const float *H, int64_t ldh, const DevQueue &sq, int64_t row0, int64_t n, const DevDetector &det,
+                     const PanelParams &pp, hipStream_t s);
 /* Sets the first `n` entries of `p` to zero (graph-safe: a kernel, not a
  * memset node). */
 int gpet_launch_zero(unsigned *p, int n, hipStream_t s);

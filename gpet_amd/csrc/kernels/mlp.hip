@@ -1,0 +1,507 @@
+/*
+ * mlp.hip - kernels of the detector-response surrogate: a BF16 MFMA GEMM
+ * with FP32 accumulation and fused epilogues, the mixture-density loss and
+ * its gradient, Adam, and the inference-side input packing and sampling.
+ *
+ * The network (6 -> 256 -> 256 -> 256 -> 256 -> 145) maps a photon's panel
+ * entry (energy, local direction, entry point) to P(event) and a 16-component
+ * diagonal Gaussian mixture over (deposited-energy fraction, depth of the
+ * first deposit, its lateral offsets from the straight entry ray).  It stands
+ * in for the panel transport of photonde (gPET_kernals.cu:1011-1193) plus the
+ * adder/readout (:737-813) inside its validity range; nothing in the reference
+ * corresponds to the network itself.
+ *
+ * Layout conventions (all row-major):
+ *   activations  bf16 [M x Kp], Kp = K + 8: column K holds the constant 1
+ *                that carries the bias, columns K+1.. are zero;
+ *   weights      bf16 [N x Kp] (one output neuron per row, bias in column K);
+ *   pre-activations fp32 [M x N] (kept for the backward pass).
+ *
+ * GEMM: C[M x N] = op(A)[M x K] op(B)[K x N] on 64 x 64 block tiles, four
+ * waves of 32 x 32, v_mfma_f32_32x32x16_bf16.  Operands are staged through
+ * LDS k-contiguous, so the transposed forms (A^T for weight gradients, B^T
+ * for input gradients) are transposed on the way into LDS.  Every operand's
+ * contiguous dimension is a multiple of 8 (16-byte loads); M and the
+ * reduction length are arbitrary (zero-filled tails).
+ */
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <stdint.h>
+
+#include "../common/gp_math.h"
+#include "../host/internal.h"
+#include "surr_common.h"
+
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8_t;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16_t;
+
+constexpr int BM = 64, BN = 64, BK = 32, LDK = BK + 8; /* +8: spread LDS banks */
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+/* round to nearest even */
+__device__ __forceinline__ uint16_t f2bf(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + gp_expf(-z)); }
+__device__ __forceinline__ float silu_grad(float z)
+{
+    float s = 1.0f / (1.0f + gp_expf(-z));
+    return s * (1.0f + z * (1.0f - s));
+}
+
+/* Stage a BM x BK (or BN x BK) tile into LDS as [row][k].
+ * TRANS = false: element (r, k) at src[r * ld + k] (k contiguous);
+ * TRANS = true:  element (r, k) at src[k * ld + r] (r contiguous). */
+template <bool TRANS>
+__device__ __forceinline__ void stage(uint16_t (*dst)[LDK], const uint16_t *src, int64_t ld, int64_t r0, int64_t nr,
+                                      int64_t k0, int64_t nk)
+{
+    const int t = threadIdx.x;
+    if (!TRANS) {
+        /* 64 rows x 32 k = 64 x 4 vectors of 8 */
+        const int r = t >> 2, kv = (t & 3) * 8;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (r0 + r < nr && k0 + kv < nk) v = *reinterpret_cast<const uint4 *>(src + (r0 + r) * ld + k0 + kv);
+        *reinterpret_cast<uint4 *>(&dst[r][kv]) = v;
+    } else {
+        /* 32 k x 64 rows = 32 x 8 vectors of 8 rows */
+        const int k = t >> 3, rv = (t & 7) * 8;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (k0 + k < nk && r0 + rv < nr) v = *reinterpret_cast<const uint4 *>(src + (k0 + k) * ld + r0 + rv);
+        const uint16_t *e = reinterpret_cast<const uint16_t *>(&v);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) dst[rv + j][k] = e[j];
+    }
+}
+
+} // namespace
+
+template <bool TA, bool TB, int EPI>
+__global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t As[BM][LDK];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[BN][LDK];
+    const int64_t m0 = (int64_t)blockIdx.x * BM, n0 = (int64_t)blockIdx.y * BN;
+    const int64_t kb = (int64_t)blockIdx.z * g.k_chunk;
+    const int64_t ke = min(g.K, kb + g.k_chunk);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    f32x16_t acc = {};
+    for (int64_t k0 = kb; k0 < ke; k0 += BK) {
+        stage<TA>(As, g.A, g.lda, m0, g.M, k0, ke);
+        stage<TB>(Bs, g.B, g.ldb, n0, g.N, k0, ke);
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 16) {
+            /* lanes 0-31 hold k 0-7 of the step, lanes 32-63 k 8-15; A and B
+             * use the same k assignment, so the product is the dot product */
+            const int kk = ks + (lane >> 5) * 8;
+            bf16x8_t a = *reinterpret_cast<const bf16x8_t *>(&As[wm + (lane & 31)][kk]);
+            bf16x8_t b = *reinterpret_cast<const bf16x8_t *>(&Bs[wn + (lane & 31)][kk]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    /* accumulator i of lane l: row (i/4)*8 + (l/32)*4 + i%4, column l%32 */
+    const int64_t col = n0 + wn + (lane & 31);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int64_t row = m0 + wm + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
+        if (row >= g.M || col >= g.N) continue;
+        const float v = acc[i];
+        if (EPI == EPI_F32) {
+            g.C[row * g.ldc + col] = v;
+        } else if (EPI == EPI_SILU) {
+            g.C[row * g.ldc + col] = v;
+            g.Cb[row * g.ldcb + col] = f2bf(silu(v));
+        } else if (EPI == EPI_ACT) {
+            g.Cb[row * g.ldcb + col] = f2bf(silu(v));
+        } else if (EPI == EPI_DSILU) {
+            g.Cb[row * g.ldcb + col] = f2bf(v * silu_grad(g.Z[row * g.ldz + col]));
+        } else {
+            g.C[(int64_t)blockIdx.z * g.M * g.N + row * g.N + col] = v;
+        }
+    }
+}
+
+/* Sum of split-K partials: out[i] = sum_z part[z][i] (fixed order). */
+__global__ void k_sum_partials(const float *part, int nz, int64_t n, float *out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.0f;
+    for (int z = 0; z < nz; ++z) s += part[(int64_t)z * n + i];
+    out[i] = s;
+}
+
+/* Columns [c0, ld) of an activation buffer: 1 in column c0, 0 after. */
+__global__ void k_bias_columns(uint16_t *a, int64_t M, int64_t ld, int c0)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int w = (int)(ld - c0);
+    if (i >= M * w) return;
+    int64_t r = i / w;
+    int c = (int)(i % w);
+    a[r * ld + c0 + c] = c == 0 ? (uint16_t)0x3f80 : (uint16_t)0;
+}
+
+/* ---- mixture-density head ------------------------------------------------
+ * h[0]            event logit
+ * h[1 .. 16]      component logits
+ * h[17 + 4k + d]  mean of component k, target d
+ * h[81 + 4k + d]  log sigma, clamped to [SURR_LOGSIG_MIN, SURR_LOGSIG_MAX]
+ * Targets y: {event, fE, depth, dy, dz}. */
+
+__device__ __forceinline__ float softplus(float x) { return x > 0.0f ? x + gp_logf(1.0f + gp_expf(-x)) : gp_logf(1.0f + gp_expf(x)); }
+
+/* Per-sample loss and gradient dL/dh / n_total (bf16, cols >= 145 zero);
+ * loss partial sums per block to part[blockIdx.x] (fixed order). */
+__global__ __launch_bounds__(256) void k_mdn_loss_grad(const float *H, int64_t ldh, const float *Y, int64_t M,
+                                                       float inv_n, uint16_t *dH, int64_t lddh, float *part)
+{
+    __shared__ float red[256];
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float loss = 0.0f;
+    if (i < M) {
+        const float *h = H + i * ldh;
+        const float *y = Y + i * SURR_NY;
+        uint16_t *g = dH + i * lddh;
+        const float ev = y[0];
+        const float h0 = h[0];
+        const float p = 1.0f / (1.0f + gp_expf(-h0));
+        loss = ev > 0.5f ? softplus(-h0) : softplus(h0);
+        g[0] = f2bf((p - ev) * inv_n);
+        float lw[SURR_K], lp[SURR_K];
+        float mx = -1e30f, mw = -1e30f;
+#pragma unroll
+        for (int k = 0; k < SURR_K; ++k) {
+            lw[k] = h[1 + k];
+            mw = fmaxf(mw, lw[k]);
+        }
+        float sw = 0.0f;
+#pragma unroll
+        for (int k = 0; k < SURR_K; ++k) sw += gp_expf(lw[k] - mw);
+        const float lse_w = mw + gp_logf(sw);
+        if (ev > 0.5f) {
+#pragma unroll
+            for (int k = 0; k < SURR_K; ++k) {
+                float s = lw[k] - lse_w;
+#pragma unroll
+                for (int d = 0; d < SURR_D; ++d) {
+                    float mu = h[17 + 4 * k + d];
+                    float ls = fminf(fmaxf(h[81 + 4 * k + d], SURR_LOGSIG_MIN), SURR_LOGSIG_MAX);
+                    float zz = (y[1 + d] - mu) * gp_expf(-ls);
+                    s += -0.5f * zz * zz - ls - 0.918938533f;
+                }
+                lp[k] = s;
+                mx = fmaxf(mx, s);
+            }
+            float se = 0.0f;
+#pragma unroll
+            for (int k = 0; k < SURR_K; ++k) se += gp_expf(lp[k] - mx);
+            const float lse = mx + gp_logf(se);
+            loss += -lse;
+#pragma unroll
+            for (int k = 0; k < SURR_K; ++k) {
+                const float gam = gp_expf(lp[k] - lse);
+                const float pik = gp_expf(lw[k] - lse_w);
+                g[1 + k] = f2bf((pik - gam) * inv_n);
+#pragma unroll
+                for (int d = 0; d < SURR_D; ++d) {
+                    float mu = h[17 + 4 * k + d];
+                    float raw = h[81 + 4 * k + d];
+                    float ls = fminf(fmaxf(raw, SURR_LOGSIG_MIN), SURR_LOGSIG_MAX);
+                    float is = gp_expf(-ls);
+                    float zz = (y[1 + d] - mu) * is;
+                    g[17 + 4 * k + d] = f2bf(-gam * zz * is * inv_n);
+                    bool free_ = raw > SURR_LOGSIG_MIN && raw < SURR_LOGSIG_MAX;
+                    g[81 + 4 * k + d] = f2bf(free_ ? gam * (1.0f - zz * zz) * inv_n : 0.0f);
+                }
+            }
+        } else {
+            for (int c = 1; c < SURR_NH; ++c) g[c] = 0;
+        }
+        for (int c = SURR_NH; c < lddh; ++c) g[c] = 0;
+    }
+    red[threadIdx.x] = loss;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+/* Squared norm of g: per-block partial sums (fixed order), then one block
+ * turns them into the clipping factor min(1, clip / |g|). */
+__global__ __launch_bounds__(256) void k_sqnorm_part(const float *g, int64_t n, float *part)
+{
+    __shared__ float red[256];
+    float s = 0.0f;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        s += g[i] * g[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void k_clip_factor(const float *part, int nb, float clip, float *factor)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float s = 0.0f;
+    for (int i = 0; i < nb; ++i) s += part[i];
+    const float nrm = sqrtf(s);
+    factor[0] = nrm > clip ? clip / nrm : 1.0f;
+}
+
+/* Adam on fp32 master weights (gradient scaled by *gscale); refreshes the
+ * bf16 copy. */
+__global__ void k_adam(float *w, float *m, float *v, const float *g, const float *gscale, uint16_t *wb, int64_t n,
+                       float lr, float b1, float b2, float c1, float c2, float eps)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float gi = g[i] * gscale[0];
+    float mi = b1 * m[i] + (1.0f - b1) * gi;
+    float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    float wi = w[i] - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+    w[i] = wi;
+    wb[i] = f2bf(wi);
+}
+
+/* Sum of n loss partials into out[0] (one block, fixed order). */
+__global__ void k_loss_reduce(const float *part, int n, float *out)
+{
+    __shared__ float red[256];
+    float s = 0.0f;
+    for (int i = threadIdx.x; i < n; i += 256) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+/* Training minibatch: M rows drawn with replacement from the n harvested
+ * records (Philox stream (step, 14), one draw per row) -> input matrix
+ * (bf16 [M x 8]: 6 features, 1, 0) and targets (fp32 [M x SURR_NY]). */
+__global__ void k_pack_train(const float *set, int64_t n, uint64_t seed, int64_t step, int64_t M, uint16_t *X,
+                             float *Y)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    gp_u4 c = {(uint32_t)i, 14u, (uint32_t)step, (uint32_t)(step >> 32)};
+    gp_u4 rr = gp_philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const uint64_t u = ((uint64_t)rr.x << 32) | rr.y;
+    const int64_t j = (int64_t)(u % (uint64_t)n);
+    const float *r = set + j * SURR_REC;
+#pragma unroll
+    for (int c = 0; c < SURR_NX; ++c) X[i * SURR_LDX + c] = f2bf(r[c]);
+    X[i * SURR_LDX + SURR_NX] = 0x3f80;
+    for (int c = SURR_NX + 1; c < SURR_LDX; ++c) X[i * SURR_LDX + c] = 0;
+#pragma unroll
+    for (int c = 0; c < SURR_NY; ++c) Y[i * SURR_NY + c] = r[SURR_NX + c];
+}
+
+/* Inference inputs of surrogate-queue rows [row0, row0 + n). */
+__global__ void k_pack_infer(DevQueue q, int64_t row0, int64_t n, const DevDetector det, uint16_t *X)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t j = row0 + i;
+    const DevPanel &P = det.p[q.info[j].x];
+    float f[SURR_NX];
+    surr_features(P, q.E[j], q.vx[j], q.vy[j], q.vz[j], q.y[j], q.z[j], f);
+#pragma unroll
+    for (int c = 0; c < SURR_NX; ++c) X[i * SURR_LDX + c] = f2bf(f[c]);
+    X[i * SURR_LDX + SURR_NX] = 0x3f80;
+    for (int c = SURR_NX + 1; c < SURR_LDX; ++c) X[i * SURR_LDX + c] = 0;
+}
+
+/* Sampling from the head: P(event) on the photon's own stream, a component
+ * by its weight, four normals; the event at the first deposit point, with
+ * the readout site of the digitizer settings (one event per photon). */
+__global__ __launch_bounds__(256) void k_surr_sample(const float *H, int64_t ldh, DevQueue q, int64_t row0, int64_t n,
+                                                     DevDetector det, PanelParams pp)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = i < n;
+    bool has = false;
+    gpet_event ev;
+    if (in) {
+        const int64_t j = row0 + i;
+        const float *h = H + i * ldh;
+        const int4 info = q.info[j];
+        const DevPanel &P = det.p[info.x];
+        const unsigned draws = (unsigned)info.w & 0x7fffffffu;
+        const unsigned k = (unsigned)info.w >> 31;
+        gp_rng r;
+        gp_rng_init(&r, pp.seed, (uint64_t)(int64_t)info.z, GP_TAG_PHOTON + k, draws);
+        gp_u4 b = gp_rng_block(&r);
+        const float pe = 1.0f / (1.0f + gp_expf(-h[0]));
+        if (gp_u01(b.x) <= pe) {
+            float mw = -1e30f;
+            for (int c = 0; c < SURR_K; ++c) mw = fmaxf(mw, h[1 + c]);
+            float sw = 0.0f;
+            for (int c = 0; c < SURR_K; ++c) sw += gp_expf(h[1 + c] - mw);
+            float u = gp_u01(b.y) * sw, acc = 0.0f;
+            int kc = SURR_K - 1;
+            for (int c = 0; c < SURR_K; ++c) {
+                acc += gp_expf(h[1 + c] - mw);
+                if (u <= acc) { kc = c; break; }
+            }
+            gp_u4 b2 = gp_rng_block(&r);
+            float z[4];
+            float s0, c0, s1, c1;
+            gp_sincos2pi(gp_u01(b2.y), &s0, &c0);
+            gp_sincos2pi(gp_u01(b2.w), &s1, &c1);
+            float r0 = sqrtf(-2.0f * gp_logf(gp_u01(b2.x))), r1 = sqrtf(-2.0f * gp_logf(gp_u01(b2.z)));
+            z[0] = r0 * c0; z[1] = r0 * s0; z[2] = r1 * c1; z[3] = r1 * s1;
+            float y[4];
+            for (int d = 0; d < 4; ++d) {
+                float ls = fminf(fmaxf(h[81 + 4 * kc + d], SURR_LOGSIG_MIN), SURR_LOGSIG_MAX);
+                y[d] = h[17 + 4 * kc + d] + gp_expf(ls) * z[d];
+            }
+            float lx, ly, lz;
+            const float E = q.E[j];
+            const float fe = fminf(y[0], 1.0f);
+            has = fe > 0.0f && surr_position(P, q.vx[j], q.vy[j], q.vz[j], q.y[j], q.z[j], y[1], y[2], y[3], lx, ly, lz);
+            int M_id = -1, L_id = -1;
+            if (has) has = d_crystal(det, P, lx, ly, lz, M_id, L_id) == 0;
+            if (has) {
+                const int pann = P.panel;
+                float dx = lx, dy = ly - q.y[j], dz = lz - q.z[j];
+                ev.parn = info.y;
+                ev.pann = pann;
+                ev.modn = M_id;
+                ev.cryn = L_id;
+                ev.siten = surr_site(pp.readout_depth, pp.readout_policy, pann, M_id, L_id, det.moduleN, det.crystalN);
+                ev.eventid = info.z;
+                ev.t = q.t[j] + (double)sqrtf(dx * dx + dy * dy + dz * dz) * D_INV_C;
+                ev.E = fe * E;
+                ev.x = lx;
+                ev.y = ly;
+                ev.z = lz;
+            }
+        }
+    }
+    /* wave-aggregated append */
+    unsigned long long m = __ballot(has);
+    unsigned base = 0;
+    if (m) {
+        int leader = __ffsll((unsigned long long)m) - 1;
+        if ((int)__lane_id() == leader) base = atomicAdd(pp.n_events, (unsigned)__popcll(m));
+        base = __shfl(base, leader);
+    }
+    unsigned long long lt = __lane_id() == 0 ? 0ull : (~0ull >> (64 - __lane_id()));
+    if (has) {
+        int64_t dst = (int64_t)base + __popcll(m & lt);
+        if (dst < pp.events_cap) pp.events[dst] = ev;
+    }
+}
+
+/* ---- launchers ------------------------------------------------------------ */
+
+int gpet_gemm(int kind, bool ta, bool tb, const GemmArgs &g, int splits, hipStream_t s)
+{
+    dim3 grid((unsigned)((g.M + BM - 1) / BM), (unsigned)((g.N + BN - 1) / BN), (unsigned)(splits > 0 ? splits : 1));
+    GemmArgs a = g;
+    if (a.k_chunk <= 0) a.k_chunk = a.K;
+#define GEMM_CASE(TA_, TB_, E_)                                                             \
+    if (ta == TA_ && tb == TB_ && kind == E_) {                                             \
+        hipLaunchKernelGGL((k_gemm_bf16<TA_, TB_, E_>), grid, dim3(256), 0, s, a);          \
+        return (int)hipGetLastError();                                                      \
+    }
+    GEMM_CASE(false, false, EPI_F32)
+    GEMM_CASE(false, false, EPI_SILU)
+    GEMM_CASE(false, false, EPI_ACT)
+    GEMM_CASE(false, true, EPI_DSILU)
+    GEMM_CASE(false, true, EPI_F32)
+    GEMM_CASE(true, true, EPI_PARTIAL)
+    GEMM_CASE(true, true, EPI_F32)
+#undef GEMM_CASE
+    return (int)hipErrorInvalidValue;
+}
+
+int gpet_sum_partials(const float *part, int nz, int64_t n, float *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, nz, n, out);
+    return (int)hipGetLastError();
+}
+
+int gpet_bias_columns(uint16_t *a, int64_t M, int64_t ld, int c0, hipStream_t s)
+{
+    int64_t n = M * (ld - c0);
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_bias_columns, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, M, ld, c0);
+    return (int)hipGetLastError();
+}
+
+int gpet_mdn_loss_grad(const float *H, int64_t ldh, const float *Y, int64_t M, float inv_n, uint16_t *dH,
+                       int64_t lddh, float *part, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mdn_loss_grad, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, H, ldh, Y, M, inv_n, dH,
+                       lddh, part);
+    return (int)hipGetLastError();
+}
+
+int gpet_clip_factor(const float *g, int64_t n, float clip, float *part, float *factor, hipStream_t s)
+{
+    const int nb = 240;
+    hipLaunchKernelGGL(k_sqnorm_part, dim3(nb), dim3(256), 0, s, g, n, part);
+    hipLaunchKernelGGL(k_clip_factor, dim3(1), dim3(64), 0, s, part, nb, clip, factor);
+    return (int)hipGetLastError();
+}
+
+int gpet_adam(float *w, float *m, float *v, const float *g, const float *gscale, uint16_t *wb, int64_t n, float lr,
+              float b1, float b2, float c1, float c2, float eps, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, w, m, v, g, gscale, wb, n, lr, b1,
+                       b2, c1, c2, eps);
+    return (int)hipGetLastError();
+}
+
+int gpet_pack_train(const float *set, int64_t n, uint64_t seed, int64_t step, int64_t M, uint16_t *X, float *Y,
+                    hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pack_train, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, set, n, seed, step, M, X, Y);
+    return (int)hipGetLastError();
+}
+
+int gpet_loss_reduce(const float *part, int n, float *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_loss_reduce, dim3(1), dim3(256), 0, s, part, n, out);
+    return (int)hipGetLastError();
+}
+
+int gpet_pack_infer(const DevQueue &sq, int64_t row0, int64_t n, const DevDetector &det, uint16_t *X, hipStream_t s)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_pack_infer, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, sq, row0, n, det, X);
+    return (int)hipGetLastError();
+}
+
+int gpet_surr_sample(const float *H, int64_t ldh, const DevQueue &sq, int64_t row0, int64_t n, const DevDetector &det,
+                     const PanelParams &pp, hipStream_t s)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_surr_sample, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, H, ldh, sq, row0, n, det,
+                       pp);
+    return (int)hipGetLastError();
+}

@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include "dev_physics.h"
+#include "surr_common.h"
 
 namespace {
 
@@ -549,7 +550,11 @@ __device__ __forceinline__ void d_readout(EvList &L, int n, int depth, int polic
     }
 }
 
-template <int WPE>
+/* MODE: PANEL_PLAIN transports every queued photon; PANEL_SURROGATE lists
+ * the photons inside the network's validity range for the surrogate and
+ * transports the rest; PANEL_HARVEST transports every photon and also writes
+ * a training record (features, first-deposit targets) per in-range photon. */
+template <int WPE, int MODE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 void k_panel(PanelParams pp, DevTables tab, DevDetector det, DevQueue q)
 {
@@ -557,6 +562,22 @@ void k_panel(PanelParams pp, DevTables tab, DevDetector det, DevQueue q)
     const int nq = (int)min((int64_t)*q.count, q.cap);
     const int stride = gridDim.x * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += stride) {
+        if (MODE == PANEL_SURROGATE) {
+            const bool sv = surr_valid(q.E[i], q.vx[i]);
+            const int slot = wave_append(pp.sq.count, sv);
+            if (sv) {
+                if (slot < pp.sq.cap) {
+                    pp.sq.y[slot] = q.y[i]; pp.sq.z[slot] = q.z[i];
+                    pp.sq.vx[slot] = q.vx[i]; pp.sq.vy[slot] = q.vy[i]; pp.sq.vz[slot] = q.vz[i];
+                    pp.sq.E[slot] = q.E[i];
+                    pp.sq.t[slot] = q.t[i];
+                    pp.sq.info[slot] = q.info[i];
+                }
+                continue;
+            }
+        }
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+        bool got = false;
         float x2 = 0.0f, y2 = q.y[i], z2 = q.z[i];
         float vx2 = q.vx[i], vy2 = q.vy[i], vz2 = q.vz[i];
         float E = q.E[i];
@@ -650,7 +671,13 @@ void k_panel(PanelParams pp, DevTables tab, DevDetector det, DevQueue q)
                             c_droph++;
                         }
                     }
-                    if (ty != 3) d_adder(L, nev, site, M_id, L_id, tof, ed, x2, y2, z2, c_drop);
+                    if (ty != 3) {
+                        if (MODE == PANEL_HARVEST && !got) {
+                            fx = x2; fy = y2; fz = z2;
+                            got = true;
+                        }
+                        d_adder(L, nev, site, M_id, L_id, tof, ed, x2, y2, z2, c_drop);
+                    }
                 }
             }
             if (stop) break;
@@ -670,10 +697,13 @@ void k_panel(PanelParams pp, DevTables tab, DevDetector det, DevQueue q)
             off += (int)__popcll(b & lt);
             tot += (int)__popcll(b);
         }
-        int base = 0;
+        /* leader = first active lane (lanes that handed their photon to
+         * the surrogate have left this iteration) */
+        unsigned base = 0;
         if (tot) {
-            if (__lane_id() == 0) base = (int)atomicAdd(pp.n_events, (unsigned)tot);
-            base = __shfl(base, 0);
+            const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
+            if ((int)__lane_id() == leader) base = atomicAdd(pp.n_events, (unsigned)tot);
+            base = __shfl(base, leader);
         }
         int o = 0;
 #pragma unroll
@@ -688,6 +718,41 @@ void k_panel(PanelParams pp, DevTables tab, DevDetector det, DevQueue q)
                     pp.events[dst] = ev;
                 }
                 o++;
+            }
+        }
+        if (MODE == PANEL_HARVEST) {
+            const float E0 = q.E[i], vx0 = q.vx[i], vy0 = q.vy[i], vz0 = q.vz[i], y0 = q.y[i], z0 = q.z[i];
+            const bool hv = surr_valid(E0, vx0);
+            unsigned long long m = __ballot(hv);
+            unsigned long long base = 0;
+            if (m) {
+                int leader = __ffsll((unsigned long long)m) - 1;
+                if ((int)__lane_id() == leader) base = atomicAdd(pp.harvest_count, (unsigned long long)__popcll(m));
+                base = __shfl(base, leader);
+            }
+            if (hv) {
+                const unsigned long long dst = base + __popcll(m & lanemask_lt());
+                if ((int64_t)dst < pp.harvest_cap) {
+                    float *rec = pp.harvest + dst * SURR_REC;
+                    surr_features(P, E0, vx0, vy0, vz0, y0, z0, rec);
+                    int j0 = -1;
+#pragma unroll
+                    for (int j = GPET_MAX_PH_EVENTS - 1; j >= 0; --j)
+                        if (alive[j]) j0 = j;
+                    float dep = 0.0f, tdy = 0.0f, tdz = 0.0f, fe = 0.0f;
+                    if (j0 >= 0 && got) {
+#pragma unroll
+                        for (int j = 0; j < GPET_MAX_PH_EVENTS; ++j)
+                            if (j == j0) fe = L.E[j] / E0;
+                        surr_targets(P, vx0, vy0, vz0, y0, z0, fx, fy, fz, dep, tdy, tdz);
+                    }
+                    rec[6] = (j0 >= 0 && got) ? 1.0f : 0.0f;
+                    rec[7] = fe;
+                    rec[8] = dep;
+                    rec[9] = tdy;
+                    rec[10] = tdz;
+                    rec[11] = 0.0f;
+                }
             }
         }
         c_steps += (unsigned long long)nsteps;
@@ -767,16 +832,20 @@ int gpet_launch_phantom(const RunParams &rp, const DevTables &tab, const DevPhan
 }
 
 int gpet_launch_panel(const PanelParams &pp, const DevTables &tab, const DevDetector &det, const DevQueue &q,
-                      int64_t max_items, const LaunchCfg &cfg, int n_cu, hipStream_t s)
+                      int64_t max_items, const LaunchCfg &cfg, int n_cu, hipStream_t s, int mode)
 {
     if (max_items <= 0) return 0;
     if (cfg.block <= 0 || cfg.block > 256 || cfg.block % 64) return (int)hipErrorInvalidValue;
     const int grid = grid_of(max_items, cfg, n_cu);
-    if (cfg.wpe == 8)
-        hipLaunchKernelGGL(k_panel<8>, dim3(grid), dim3(cfg.block), 0, s, pp, tab, det, q);
+    if (mode == PANEL_HARVEST)
+        hipLaunchKernelGGL((k_panel<0, PANEL_HARVEST>), dim3(grid), dim3(cfg.block), 0, s, pp, tab, det, q);
+    else if (mode == PANEL_SURROGATE)
+        hipLaunchKernelGGL((k_panel<0, PANEL_SURROGATE>), dim3(grid), dim3(cfg.block), 0, s, pp, tab, det, q);
+    else if (cfg.wpe == 8)
+        hipLaunchKernelGGL((k_panel<8, PANEL_PLAIN>), dim3(grid), dim3(cfg.block), 0, s, pp, tab, det, q);
     else if (cfg.wpe == 6)
-        hipLaunchKernelGGL(k_panel<6>, dim3(grid), dim3(cfg.block), 0, s, pp, tab, det, q);
+        hipLaunchKernelGGL((k_panel<6, PANEL_PLAIN>), dim3(grid), dim3(cfg.block), 0, s, pp, tab, det, q);
     else
-        hipLaunchKernelGGL(k_panel<0>, dim3(grid), dim3(cfg.block), 0, s, pp, tab, det, q);
+        hipLaunchKernelGGL((k_panel<0, PANEL_PLAIN>), dim3(grid), dim3(cfg.block), 0, s, pp, tab, det, q);
     return (int)hipGetLastError();
 }

@@ -262,7 +262,7 @@ class Engine:
             self._ok(self.f["set_phantom_slot"](self.ctx, int(slot), mat.ctypes.data_as(C.c_void_p),
                                                 dens.ctypes.data_as(C.c_void_p), d, o, s), "set_phantom_slot")
 
-    def set_launch(self, block=(256, 256), wpe=(0, 0), blocks_per_cu=(0, 8), batch=1 << 22):
+    def set_launch(self, block=(256, 256), wpe=(0, 0), blocks_per_cu=(4, 4), batch=1 << 22):
         L = capi.Launch()
         L.block[0], L.block[1] = block
         L.wpe[0], L.wpe[1] = wpe
@@ -361,6 +361,74 @@ class Engine:
         r = self._result(s, bufs)
         r.job_counts = jc
         return r
+
+    # ---- detector-response surrogate ---------------------------------------
+    def surrogate_harvest(self, sources, isotopes, t0, t1, n_decays=0, capacity=10 ** 7):
+        src = (capi.Source * len(sources))(*sources)
+        iso = (capi.Isotope * len(isotopes))(*isotopes)
+        self._ok(self.f["surrogate_harvest"](self.ctx, src, len(sources), iso, len(isotopes), float(t0), float(t1),
+                                             int(n_decays), int(capacity)), "surrogate_harvest")
+
+    def surrogate_train(self, steps, batch=65536, lr=2e-3, seed=1, reinit=False) -> capi.SurrogateReport:
+        cfg = capi.SurrogateConfig()
+        cfg.steps, cfg.batch, cfg.lr, cfg.seed, cfg.reinit = steps, batch, lr, seed, int(reinit)
+        rep = capi.SurrogateReport()
+        self._ok(self.f["surrogate_train"](self.ctx, C.byref(cfg), C.byref(rep)), "surrogate_train")
+        return rep
+
+    def surrogate_enable(self, on=True):
+        self._ok(self.f["surrogate_enable"](self.ctx, int(bool(on))), "surrogate_enable")
+
+    def surrogate_save(self, path):
+        self._ok(self.f["surrogate_save"](self.ctx, path.encode()), "surrogate_save")
+
+    def surrogate_load(self, path):
+        self._ok(self.f["surrogate_load"](self.ctx, path.encode()), "surrogate_load")
+
+    def surrogate_params(self) -> np.ndarray:
+        n = C.c_int64()
+        self._ok(self.f["surrogate_get_params"](self.ctx, None, 0, C.byref(n)), "surrogate_get_params")
+        w = np.zeros(n.value, np.float32)
+        self._ok(self.f["surrogate_get_params"](self.ctx, w.ctypes.data_as(C.c_void_p), n.value, C.byref(n)),
+                 "surrogate_get_params")
+        return w
+
+    def surrogate_set_params(self, w: np.ndarray):
+        w = np.ascontiguousarray(w, np.float32)
+        self._ok(self.f["surrogate_set_params"](self.ctx, w.ctypes.data_as(C.c_void_p), len(w)),
+                 "surrogate_set_params")
+
+    def surrogate_n_params(self) -> int:
+        n = C.c_int64()
+        self._ok(self.f["surrogate_get_params"](self.ctx, None, 0, C.byref(n)), "surrogate_get_params")
+        return n.value
+
+    def surrogate_records(self, limit=None) -> np.ndarray:
+        """Harvested training records [n, 12]: 6 features, event, fE, depth, dy, dz, 0."""
+        n = C.c_int64()
+        self._ok(self.f["surrogate_records"](self.ctx, None, 0, C.byref(n)), "surrogate_records")
+        m = n.value if limit is None else min(n.value, int(limit))
+        out = np.zeros((m, 12), np.float32)
+        if m:
+            self._ok(self.f["surrogate_records"](self.ctx, out.ctypes.data_as(C.c_void_p), m, C.byref(n)),
+                     "surrogate_records")
+        return out
+
+    def surrogate_forward(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, capi.SURR_NX)
+        h = np.zeros((len(x), capi.SURR_NH), np.float32)
+        self._ok(self.f["surrogate_forward"](self.ctx, x.ctypes.data_as(C.c_void_p), len(x),
+                                             h.ctypes.data_as(C.c_void_p)), "surrogate_forward")
+        return h
+
+    def surrogate_grad(self, x: np.ndarray, y: np.ndarray):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, capi.SURR_NX)
+        y = np.ascontiguousarray(y, np.float32).reshape(-1, capi.SURR_NY)
+        g = np.zeros(self.surrogate_n_params(), np.float32)
+        loss = C.c_float()
+        self._ok(self.f["surrogate_grad"](self.ctx, x.ctypes.data_as(C.c_void_p), y.ctypes.data_as(C.c_void_p), len(x),
+                                          C.byref(loss), g.ctypes.data_as(C.c_void_p)), "surrogate_grad")
+        return loss.value, g
 
     def run_psf(self, rec: np.ndarray, ptype: int, *, singles_cap=0, coinc_cap=0, adder_cap=0, hits_cap=0,
                 trace_cap=0, out_dir=None) -> RunResult:

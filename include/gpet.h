@@ -250,6 +250,8 @@ typedef struct gpet_tallies {
     int32_t graph;             /* 1 if the batches ran as one hipGraph          */
     int32_t n_jobs;
     double seconds_tune;       /* autotuning time spent inside this run         */
+    int64_t n_surrogate;       /* panel entries handed to the surrogate         */
+    double seconds_surrogate;  /* network inference and sampling                */
 } gpet_tallies;
 
 typedef struct gpet_ctx gpet_ctx;
@@ -379,6 +381,59 @@ gpet_status gpet_run_psf(gpet_ctx *ctx, const double *rec8, int64_t n, int32_t p
 gpet_status gpet_run_batch(gpet_ctx *ctx, const gpet_job *jobs, int32_t n_jobs,
                            const gpet_source *src, int32_t n_src,
                            const gpet_isotope *iso, int32_t n_iso, gpet_sink *sink);
+
+/* ---- detector-response surrogate (new; no counterpart in the reference) --
+ * A 6 -> 256 -> 256 -> 256 -> 256 -> 145 MLP (SiLU, BF16 weights, FP32
+ * accumulation) maps a photon's panel entry (energy, local direction, entry
+ * point) to P(event) and a 16-component Gaussian mixture over the deposited
+ * energy fraction, the depth of the first deposit and its lateral offsets.
+ * It replaces photonde's panel transport, adder and readout
+ * (gPET_kernals.cu:1011-1193, 737-813) for entries of 100-520 keV within 60
+ * degrees of the panel normal; other entries take the plain transport.  It
+ * models one output event per photon, so it requires the digitizer readout
+ * and module layout it was trained with, and runs without hit records. */
+typedef struct gpet_surrogate_config {
+    int64_t steps;             /* Adam steps                                    */
+    int32_t batch;             /* rows per step                                 */
+    float lr;                  /* peak learning rate (warm-up, cosine decay)     */
+    uint64_t seed;             /* initialisation and minibatch sampling          */
+    int32_t reinit;            /* 1: start from fresh weights                   */
+} gpet_surrogate_config;
+
+typedef struct gpet_surrogate_report {
+    int64_t records;           /* harvested records trained on                  */
+    int64_t steps;
+    double seconds;            /* training wall time (device synchronised)      */
+    double flops;              /* algorithmic FLOPs of forward + backward       */
+    float loss_first, loss_last; /* mean loss of the first / last 2% of steps   */
+} gpet_surrogate_report;
+
+/* Plain run (as gpet_run_source) that also appends one training record per
+ * in-range panel entry to the context's training set (capacity in records;
+ * the set is kept across calls). */
+gpet_status gpet_surrogate_harvest(gpet_ctx *ctx, const gpet_source *src, int32_t n_src,
+                                   const gpet_isotope *iso, int32_t n_iso, double t0, double t1,
+                                   int64_t n_decays, int64_t capacity);
+gpet_status gpet_surrogate_train(gpet_ctx *ctx, const gpet_surrogate_config *cfg,
+                                 gpet_surrogate_report *report);
+/* 1: later runs send in-range panel entries to the network; 0: plain. */
+gpet_status gpet_surrogate_enable(gpet_ctx *ctx, int32_t on);
+gpet_status gpet_surrogate_save(gpet_ctx *ctx, const char *path);
+gpet_status gpet_surrogate_load(gpet_ctx *ctx, const char *path);
+/* Flat FP32 parameter vector (layer by layer, each [outputs x (inputs + 8)]
+ * with the bias in column `inputs`); *n receives its length. */
+gpet_status gpet_surrogate_get_params(gpet_ctx *ctx, float *out, int64_t cap, int64_t *n);
+gpet_status gpet_surrogate_set_params(gpet_ctx *ctx, const float *w, int64_t n);
+/* The network on caller data: x is n x 6 features, h receives n x 145 head
+ * outputs; grad also takes n x 5 targets and returns the mean loss and the
+ * parameter gradient (length as get_params). */
+gpet_status gpet_surrogate_forward(gpet_ctx *ctx, const float *x, int64_t n, float *h);
+/* Copy of the harvested training set: records of 12 floats (6 features,
+ * event flag, deposited-energy fraction, depth, dy, dz, 0); *n receives the
+ * record count. */
+gpet_status gpet_surrogate_records(gpet_ctx *ctx, float *out, int64_t cap, int64_t *n);
+gpet_status gpet_surrogate_grad(gpet_ctx *ctx, const float *x, const float *y, int64_t n, float *loss,
+                                float *grad);
 
 /* Launch shape used by the following runs (validated; results unchanged). */
 gpet_status gpet_set_launch(gpet_ctx *ctx, const gpet_launch *l);

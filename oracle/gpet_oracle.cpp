@@ -164,6 +164,8 @@ struct gpet_ctx {
     std::string err;
     int threads = 0;
     gpet_launch launch = gpet_launch();
+    std::vector<float> mlp_w;  /* surrogate parameters (set_params)        */
+    int mlp_exact = 0;         /* 1: no bf16 rounding (finite differences) */
 };
 
 namespace {
@@ -1661,3 +1663,252 @@ void gpet_cpu_stream(uint64_t seed, uint64_t id, uint32_t aux, int32_t kind, int
 }
 
 } /* extern "C" */
+
+/* ======================= surrogate network (CPU restatement) =============
+ * The detector-response MLP of the HIP library restated in FP64 for the
+ * gradient check: 6 -> 256 -> 256 -> 256 -> 256 -> 145, SiLU, weights
+ * [outputs x (inputs + 8)] with the bias in column `inputs`, head stored
+ * with 192 rows.  By default it rounds what the GPU path stores in BF16
+ * (weights, inputs, activations, dL/dh, dL/dz) to BF16 and accumulates in
+ * FP64; exact mode (gpet_cpu_surrogate_set_exact) rounds nothing, for
+ * finite-difference checks.  Loss: BCE on the event logit plus, for events,
+ * the negative log-likelihood of the 16-component diagonal Gaussian mixture
+ * over the 4 targets (log sigma clamped to [-5, 2]), averaged over rows. */
+namespace {
+
+constexpr int M_NL = 5, M_HID = 256, M_NH = 145, M_K = 16, M_D = 4, M_NX = 6, M_NY = 5;
+constexpr int M_LN[M_NL] = {256, 256, 256, 256, 192};
+constexpr int M_LK[M_NL] = {6, 256, 256, 256, 256};
+constexpr double M_LSMIN = -5.0, M_LSMAX = 2.0;
+
+int m_kp(int l) { return l == 0 ? 8 : M_LK[l] + 8; }
+int64_t m_off(int l)
+{
+    int64_t o = 0;
+    for (int i = 0; i < l; ++i) o += (int64_t)M_LN[i] * m_kp(i);
+    return o;
+}
+
+double bfr(double x, int exact)
+{
+    if (exact) return x;
+    float f = (float)x;
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) != 0x7f800000u) u += 0x7fffu + ((u >> 16) & 1u);
+    u &= 0xffff0000u;
+    memcpy(&f, &u, 4);
+    return (double)f;
+}
+
+double m_sig(double z) { return 1.0 / (1.0 + std::exp(-z)); }
+
+struct MlpRow {
+    std::vector<double> a[M_NL];   /* inputs of layer l (length kp(l)) */
+    std::vector<double> z[4];      /* pre-activations of hidden layers  */
+    double h[M_NH];
+};
+
+void mlp_forward_row(const std::vector<double> &W, const float *x, int exact, MlpRow &R)
+{
+    R.a[0].assign(8, 0.0);
+    for (int k = 0; k < M_NX; ++k) R.a[0][k] = bfr(x[k], exact);
+    R.a[0][M_NX] = 1.0;
+    for (int l = 0; l < 4; ++l) {
+        const int kp = m_kp(l);
+        R.z[l].assign(M_HID, 0.0);
+        R.a[l + 1].assign(M_HID + 8, 0.0);
+        for (int n = 0; n < M_HID; ++n) {
+            double s = 0.0;
+            for (int k = 0; k < kp; ++k) s += W[(size_t)(m_off(l) + (int64_t)n * kp + k)] * R.a[l][k];
+            R.z[l][n] = s;
+            R.a[l + 1][n] = bfr(s * m_sig(s), exact);
+        }
+        R.a[l + 1][M_HID] = 1.0;
+    }
+    for (int n = 0; n < M_NH; ++n) {
+        double s = 0.0;
+        for (int k = 0; k < m_kp(4); ++k) s += W[(size_t)(m_off(4) + (int64_t)n * m_kp(4) + k)] * R.a[4][k];
+        R.h[n] = s;
+    }
+}
+
+double softplus_d(double x) { return x > 0 ? x + std::log1p(std::exp(-x)) : std::log1p(std::exp(x)); }
+
+/* loss of one row and dL/dh (unscaled) */
+double mlp_loss_row(const double *h, const float *y, double *g)
+{
+    const double ev = y[0];
+    double loss = ev > 0.5 ? softplus_d(-h[0]) : softplus_d(h[0]);
+    for (int c = 0; c < M_NH; ++c) g[c] = 0.0;
+    g[0] = m_sig(h[0]) - ev;
+    if (ev > 0.5) {
+        double mw = -1e300;
+        for (int k = 0; k < M_K; ++k) mw = std::max(mw, h[1 + k]);
+        double sw = 0.0;
+        for (int k = 0; k < M_K; ++k) sw += std::exp(h[1 + k] - mw);
+        const double lse_w = mw + std::log(sw);
+        double lp[M_K], mx = -1e300;
+        for (int k = 0; k < M_K; ++k) {
+            double s = h[1 + k] - lse_w;
+            for (int d = 0; d < M_D; ++d) {
+                double ls = std::min(std::max(h[81 + 4 * k + d], M_LSMIN), M_LSMAX);
+                double zz = (y[1 + d] - h[17 + 4 * k + d]) * std::exp(-ls);
+                s += -0.5 * zz * zz - ls - 0.91893853320467274;
+            }
+            lp[k] = s;
+            mx = std::max(mx, s);
+        }
+        double se = 0.0;
+        for (int k = 0; k < M_K; ++k) se += std::exp(lp[k] - mx);
+        const double lse = mx + std::log(se);
+        loss += -lse;
+        for (int k = 0; k < M_K; ++k) {
+            const double gam = std::exp(lp[k] - lse), pik = std::exp(h[1 + k] - lse_w);
+            g[1 + k] = pik - gam;
+            for (int d = 0; d < M_D; ++d) {
+                const double raw = h[81 + 4 * k + d];
+                const double ls = std::min(std::max(raw, M_LSMIN), M_LSMAX);
+                const double is = std::exp(-ls);
+                const double zz = (y[1 + d] - h[17 + 4 * k + d]) * is;
+                g[17 + 4 * k + d] = -gam * zz * is;
+                g[81 + 4 * k + d] = (raw > M_LSMIN && raw < M_LSMAX) ? gam * (1.0 - zz * zz) : 0.0;
+            }
+        }
+    }
+    return loss;
+}
+
+std::vector<double> mlp_weights(const gpet_ctx &c)
+{
+    std::vector<double> W(c.mlp_w.size());
+    for (size_t i = 0; i < W.size(); ++i) W[i] = bfr(c.mlp_w[i], c.mlp_exact);
+    return W;
+}
+
+} // namespace
+
+extern "C" {
+
+gpet_status gpet_cpu_surrogate_set_params(gpet_ctx *c, const float *w, int64_t n)
+{
+    if (!c || !w || n != m_off(M_NL)) return GPET_ERR_ARG;
+    c->mlp_w.assign(w, w + n);
+    return GPET_OK;
+}
+
+gpet_status gpet_cpu_surrogate_get_params(gpet_ctx *c, float *out, int64_t cap, int64_t *n)
+{
+    if (!c) return GPET_ERR_ARG;
+    if (n) *n = m_off(M_NL);
+    if (!out) return GPET_OK;
+    if (c->mlp_w.empty()) return GPET_ERR_STATE;
+    if (cap < m_off(M_NL)) return GPET_ERR_CAPACITY;
+    memcpy(out, c->mlp_w.data(), sizeof(float) * c->mlp_w.size());
+    return GPET_OK;
+}
+
+gpet_status gpet_cpu_surrogate_set_exact(gpet_ctx *c, int32_t exact)
+{
+    if (!c) return GPET_ERR_ARG;
+    c->mlp_exact = exact != 0;
+    return GPET_OK;
+}
+
+gpet_status gpet_cpu_surrogate_forward(gpet_ctx *c, const float *x, int64_t n, float *h)
+{
+    if (!c || !x || !h || n <= 0) return GPET_ERR_ARG;
+    if (c->mlp_w.empty()) return fail(c, GPET_ERR_STATE, "no surrogate parameters");
+    const std::vector<double> W = mlp_weights(*c);
+    for (int64_t i = 0; i < n; ++i) {
+        MlpRow R;
+        mlp_forward_row(W, x + i * M_NX, c->mlp_exact, R);
+        for (int k = 0; k < M_NH; ++k) h[i * M_NH + k] = (float)R.h[k];
+    }
+    return GPET_OK;
+}
+
+/* Mean loss and its gradient w.r.t. the flat parameter vector. */
+gpet_status gpet_cpu_surrogate_grad_d(gpet_ctx *c, const float *x, const float *y, int64_t n, double *loss,
+                                      double *grad)
+{
+    if (!c || !x || !y || !grad || n <= 0) return GPET_ERR_ARG;
+    if (c->mlp_w.empty()) return fail(c, GPET_ERR_STATE, "no surrogate parameters");
+    const int ex = c->mlp_exact;
+    const std::vector<double> W = mlp_weights(*c);
+    const int64_t np = m_off(M_NL);
+    for (int64_t i = 0; i < np; ++i) grad[i] = 0.0;
+    double L = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        MlpRow R;
+        mlp_forward_row(W, x + i * M_NX, ex, R);
+        double gh[M_NH];
+        L += mlp_loss_row(R.h, y + i * M_NY, gh);
+        std::vector<double> dn(M_HID + 8, 0.0);
+        for (int k = 0; k < M_NH; ++k) dn[k] = bfr(gh[k] / (double)n, ex);
+        int nout = M_NH;
+        for (int l = 4; l >= 0; --l) {
+            const int kp = m_kp(l);
+            const int64_t o = m_off(l);
+            for (int r = 0; r < nout; ++r)
+                for (int k = 0; k < kp; ++k) grad[o + (int64_t)r * kp + k] += dn[r] * R.a[l][k];
+            if (l == 0) break;
+            std::vector<double> dz(M_HID, 0.0);
+            for (int k = 0; k < M_HID; ++k) {
+                double s = 0.0;
+                for (int r = 0; r < nout; ++r) s += dn[r] * W[(size_t)(o + (int64_t)r * kp + k)];
+                const double zz = R.z[l - 1][k], sg = m_sig(zz);
+                dz[k] = bfr(s * (sg * (1.0 + zz * (1.0 - sg))), ex);
+            }
+            dn.assign(dz.begin(), dz.end());
+            nout = M_HID;
+        }
+    }
+    if (loss) *loss = L / (double)n;
+    return GPET_OK;
+}
+
+/* Same as gpet_surrogate_grad (float outputs). */
+gpet_status gpet_cpu_surrogate_grad(gpet_ctx *c, const float *x, const float *y, int64_t n, float *loss, float *grad)
+{
+    if (!grad) return GPET_ERR_ARG;
+    std::vector<double> g((size_t)m_off(M_NL));
+    double l = 0;
+    gpet_status st = gpet_cpu_surrogate_grad_d(c, x, y, n, &l, g.data());
+    if (st) return st;
+    for (size_t i = 0; i < g.size(); ++i) grad[i] = (float)g[i];
+    if (loss) *loss = (float)l;
+    return GPET_OK;
+}
+
+/* The CPU build does not harvest, train or run the network in a transport
+ * run; these keep the ABI symmetric. */
+gpet_status gpet_cpu_surrogate_harvest(gpet_ctx *c, const gpet_source *, int32_t, const gpet_isotope *, int32_t, double,
+                                       double, int64_t, int64_t)
+{
+    return fail(c, GPET_ERR_UNSUPPORTED, "the CPU oracle does not harvest");
+}
+gpet_status gpet_cpu_surrogate_train(gpet_ctx *c, const gpet_surrogate_config *, gpet_surrogate_report *)
+{
+    return fail(c, GPET_ERR_UNSUPPORTED, "the CPU oracle does not train");
+}
+gpet_status gpet_cpu_surrogate_enable(gpet_ctx *c, int32_t on)
+{
+    return on ? fail(c, GPET_ERR_UNSUPPORTED, "the CPU oracle runs plain transport only") : GPET_OK;
+}
+gpet_status gpet_cpu_surrogate_save(gpet_ctx *c, const char *)
+{
+    return fail(c, GPET_ERR_UNSUPPORTED, "the CPU oracle does not save networks");
+}
+gpet_status gpet_cpu_surrogate_records(gpet_ctx *c, float *, int64_t, int64_t *n)
+{
+    if (n) *n = 0;
+    return c ? GPET_OK : GPET_ERR_ARG;
+}
+gpet_status gpet_cpu_surrogate_load(gpet_ctx *c, const char *)
+{
+    return fail(c, GPET_ERR_UNSUPPORTED, "the CPU oracle does not load networks");
+}
+
+} /* extern "C" */
+
