@@ -67,13 +67,14 @@ class Digitizer(C.Structure):
                 ("slope", C.c_float), ("space_blur", C.c_float), ("dead_level", C.c_int32),
                 ("dead_type", C.c_int32), ("dead_time_us", C.c_float), ("win_min", C.c_float),
                 ("win_max", C.c_float), ("time_blur_fwhm_us", C.c_double), ("coinc_window_us", C.c_double),
-                ("coinc_policy", C.c_int32), ("record_rayleigh", C.c_int32)]
+                ("coinc_policy", C.c_int32), ("record_rayleigh", C.c_int32), ("noise_rate", C.c_double),
+                ("noise_e_mean", C.c_float), ("noise_e_sigma", C.c_float), ("delay_us", C.c_double)]
 
 
 class Options(C.Structure):
     _fields_ = [("device", C.c_int32), ("seed", C.c_uint64), ("eabs", C.c_float), ("noncol_sigma", C.c_float),
                 ("positron_range", C.c_int32), ("record_hits", C.c_int32), ("batch", C.c_int64),
-                ("flags", C.c_int32)]
+                ("flags", C.c_int32), ("record_sphere", C.c_float * 4)]
 
 
 class Source(C.Structure):
@@ -90,7 +91,16 @@ class Sink(C.Structure):
                 ("adder", C.c_void_p), ("adder_cap", C.c_int64), ("n_adder", C.c_int64),
                 ("hit_id", C.c_void_p), ("hit_f", C.c_void_p), ("hits_cap", C.c_int64), ("n_hits", C.c_int64),
                 ("trace", C.c_void_p), ("trace_cap", C.c_int64), ("n_trace", C.c_int64),
-                ("out_dir", C.c_char_p), ("job_counts", C.c_void_p)]
+                ("out_dir", C.c_char_p), ("job_counts", C.c_void_p),
+                ("psf_source", C.c_void_p), ("psf_source_cap", C.c_int64), ("n_psf_source", C.c_int64),
+                ("psf_phantom", C.c_void_p), ("psf_phantom_cap", C.c_int64), ("n_psf_phantom", C.c_int64),
+                ("out_append", C.c_int32),
+                ("delayed", C.c_void_p), ("delayed_cap", C.c_int64), ("n_delayed", C.c_int64)]
+
+
+PSF_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("vx", "<f4"), ("vy", "<f4"), ("vz", "<f4"),
+                      ("E", "<f4"), ("parn", "<i4"), ("eventid", "<i4"), ("pad", "<i4"), ("t", "<f8")])
+assert PSF_DTYPE.itemsize == 48
 
 
 class Launch(C.Structure):
@@ -134,10 +144,13 @@ class Tallies(C.Structure):
         ("kernel_seconds", C.c_double * 2), ("kernel_launches", C.c_int64 * 2), ("kernel_histories", C.c_int64),
         ("launches", C.c_int64),
         ("launch", Launch), ("graph", C.c_int32), ("n_jobs", C.c_int32), ("seconds_tune", C.c_double),
-        ("n_surrogate", C.c_int64), ("seconds_surrogate", C.c_double)]
+        ("n_surrogate", C.c_int64), ("seconds_surrogate", C.c_double), ("n_noise", C.c_int64),
+        ("n_delayed", C.c_int64), ("sinogram_delayed", C.c_void_p)]
 
 FLAG_NO_GRAPH = 1
 FLAG_AUTOTUNE = 2
+FLAG_PSF_SOURCE = 4
+FLAG_PSF_PHANTOM = 8
 
 
 class Input(C.Structure):
@@ -152,7 +165,9 @@ class Input(C.Structure):
                 ("e_ref", C.c_float), ("r_ref", C.c_float), ("slope", C.c_float), ("space_blur", C.c_float),
                 ("dead_level", C.c_int32), ("dead_type", C.c_int32), ("dead_time", C.c_float),
                 ("win_min", C.c_float), ("win_max", C.c_float), ("time_blur_fwhm_us", C.c_double),
-                ("coinc_window_us", C.c_double), ("coinc_policy", C.c_int32), ("seed", C.c_uint64)]
+                ("coinc_window_us", C.c_double), ("coinc_policy", C.c_int32), ("seed", C.c_uint64),
+                ("noise_rate", C.c_double), ("noise_e_mean", C.c_float), ("noise_e_sigma", C.c_float),
+                ("delay_us", C.c_double)]
 
 
 P = C.POINTER
@@ -202,6 +217,7 @@ HOST_FUNCS = {
     "gpet_input_digitizer": (None, [P(Input), P(Digitizer)]),
     "gpet_write_events": (C.c_int32, [C.c_char_p, _VP, C.c_int64, C.c_int32]),
     "gpet_write_hits": (C.c_int32, [C.c_char_p, C.c_char_p, _VP, _VP, C.c_int64, C.c_int32]),
+    "gpet_write_psf": (C.c_int32, [C.c_char_p, C.c_char_p, _VP, C.c_int64, C.c_int32]),
     "gpet_build_majorant": (C.c_int32, [P(Tables), _VP, _VP, P(C.c_float), P(C.c_float)]),
 }
 

@@ -7,7 +7,11 @@
  * Outputs go to ./output/ (singles.dat, adder.dat, coincidences.dat and,
  * with --hits, HitsID.dat/Hits.dat), truncated per run unless --append.
  *
- * Options: --data DIR (tables), --out DIR, --hits, --append, --batch N.
+ * Options: --data DIR (tables), --out DIR, --hits, --append, --batch N,
+ * --psf-source / --psf-phantom (outsource/idsource/timesource.dat and
+ * outphantom/idphantom/timephantom.dat: OUTPUTPSF 1 / 2, gPET.cu:63-114),
+ * --record-sphere (move photons leaving the phantom onto the input file's
+ * record sphere: RECORDPSF -1), --autotune (pick launch shapes by timing).
  */
 #include "../../../include/gpet.h"
 
@@ -39,18 +43,24 @@ int main(int argc, char **argv)
 {
     const char *input = nullptr;
     std::string data_dir = "data", out_dir = "output";
-    bool hits = false, append = false;
+    bool hits = false, append = false, sphere = false;
     long long batch = 0;
+    int32_t flags = 0;
     for (int i = 1; i < argc; ++i) {
         if (!strcmp(argv[i], "--data") && i + 1 < argc) data_dir = argv[++i];
         else if (!strcmp(argv[i], "--out") && i + 1 < argc) out_dir = argv[++i];
         else if (!strcmp(argv[i], "--hits")) hits = true;
         else if (!strcmp(argv[i], "--append")) append = true;
         else if (!strcmp(argv[i], "--batch") && i + 1 < argc) batch = atoll(argv[++i]);
+        else if (!strcmp(argv[i], "--psf-source")) flags |= GPET_FLAG_PSF_SOURCE;
+        else if (!strcmp(argv[i], "--psf-phantom")) flags |= GPET_FLAG_PSF_PHANTOM;
+        else if (!strcmp(argv[i], "--autotune")) flags |= GPET_FLAG_AUTOTUNE;
+        else if (!strcmp(argv[i], "--record-sphere")) sphere = true;
         else if (!input) input = argv[i];
     }
     if (!input) {
-        printf("Please execute ./gpet input_file [--data DIR] [--out DIR] [--hits] [--append]\n");
+        printf("Please execute ./gpet input_file [--data DIR] [--out DIR] [--hits] [--append] [--batch N] "
+               "[--psf-source] [--psf-phantom] [--record-sphere] [--autotune]\n");
         return EXIT_FAILURE;
     }
     gpet_input in;
@@ -66,6 +76,9 @@ int main(int argc, char **argv)
     opt.positron_range = in.prange;
     opt.record_hits = hits ? 1 : 0;
     opt.batch = batch;
+    opt.flags = flags;
+    if (sphere)
+        for (int k = 0; k < 4; ++k) opt.record_sphere[k] = in.record_sphere[k];
     gpet_ctx *c = nullptr;
     if ((st = gpet_create(&c, &opt))) return die("device init", st, nullptr);
 
@@ -101,8 +114,8 @@ int main(int argc, char **argv)
     gpet_sink sink;
     memset(&sink, 0, sizeof(sink));
     std::string tmp_dir = out_dir;
-    sink.out_dir = append ? nullptr : tmp_dir.c_str();
-    std::vector<gpet_event> singles, coinc, adder;
+    sink.out_dir = tmp_dir.c_str();
+    sink.out_append = append ? 1 : 0;
     auto t0 = std::chrono::steady_clock::now();
     if (in.use_psf) {
         int64_t n = 0;
@@ -110,10 +123,6 @@ int main(int argc, char **argv)
         if (in.n_histories < n) n = in.n_histories;
         std::vector<double> rec(8 * (size_t)n);
         if ((st = gpet_read_psf(in.source_file, rec.data(), n, &n))) return die("reading PSF", st, c);
-        if (append) {
-            singles.resize(1 << 20);
-            sink.singles = singles.data(); sink.singles_cap = (int64_t)singles.size();
-        }
         if ((st = gpet_run_psf(c, rec.data(), n, in.ptype, &sink))) return die("PSF run", st, c);
     } else {
         std::vector<gpet_isotope> iso(64);

@@ -126,6 +126,9 @@ struct gpet_ctx {
     DBuf<unsigned long long> d_cnt;
     DBuf<gpet_event> d_events;
     DBuf<gpet_trace> d_trace;
+    DBuf<unsigned> d_rcount;     /* panel-queue region counts                */
+    DBuf<gpet_psf_record> d_psf_src, d_psf_ph;
+    DBuf<unsigned long long> d_npsf;
     DBuf<int32_t> d_hit_id;
     DBuf<float> d_hit_f;
     DBuf<int64_t> d_first;

@@ -174,14 +174,14 @@ gpet_status check_ready(gpet_ctx *c)
 
 gpet_status write_outputs(gpet_ctx *c, const std::string &dir, const std::vector<gpet_event> &adder,
                           const std::vector<gpet_event> &singles, const std::vector<gpet_event> &coinc,
-                          const std::vector<int32_t> &hid, const std::vector<float> &hf)
+                          const std::vector<int32_t> &hid, const std::vector<float> &hf, int32_t app)
 {
-    gpet_status st = gpet_write_events((dir + "/adder.dat").c_str(), adder.data(), (int64_t)adder.size(), 0);
-    if (!st) st = gpet_write_events((dir + "/singles.dat").c_str(), singles.data(), (int64_t)singles.size(), 0);
-    if (!st) st = gpet_write_events((dir + "/coincidences.dat").c_str(), coinc.data(), (int64_t)coinc.size(), 0);
+    gpet_status st = gpet_write_events((dir + "/adder.dat").c_str(), adder.data(), (int64_t)adder.size(), app);
+    if (!st) st = gpet_write_events((dir + "/singles.dat").c_str(), singles.data(), (int64_t)singles.size(), app);
+    if (!st) st = gpet_write_events((dir + "/coincidences.dat").c_str(), coinc.data(), (int64_t)coinc.size(), app);
     if (!st && c->opt.record_hits)
         st = gpet_write_hits((dir + "/HitsID.dat").c_str(), (dir + "/Hits.dat").c_str(), hid.data(), hf.data(),
-                             (int64_t)hid.size() / 5, 0);
+                             (int64_t)hid.size() / 5, app);
     if (st) return fail(c, st, "cannot write outputs to " + dir);
     return GPET_OK;
 }
@@ -268,6 +268,9 @@ gpet_status check_launch(gpet_ctx *c, const gpet_launch &L)
         for (int v : gpet_wpe_variants) ok = ok || v == L.wpe[k];
         if (!ok) return fail(c, GPET_ERR_ARG, "launch register variant (wpe) must be 0, 6 or 8");
         if (L.blocks_per_cu[k] < 0 || L.blocks_per_cu[k] > 64) return fail(c, GPET_ERR_ARG, "launch blocks per CU");
+        /* the phantom kernel's grid sets the panel queue's regions: 1..16 per CU */
+        if (k == 0 && (L.blocks_per_cu[0] < 1 || L.blocks_per_cu[0] > 16))
+            return fail(c, GPET_ERR_ARG, "phantom-kernel blocks per CU must be 1..16");
     }
     if (L.batch < 256 || L.batch > ((int64_t)1 << 26)) return fail(c, GPET_ERR_ARG, "launch batch size");
     return GPET_OK;
@@ -276,6 +279,7 @@ gpet_status check_launch(gpet_ctx *c, const gpet_launch &L)
 gpet_status alloc_queue(gpet_ctx *c, int64_t qcap)
 {
     gpet_status st;
+    if ((st = dalloc(c, c->d_rcount, 64 * (int64_t)c->n_cu + 1))) return st;
     if (qcap > c->q_cap) {
         for (int k = 0; k < 6; ++k)
             if ((st = dalloc(c, c->q_f[k], qcap))) return st;
@@ -288,6 +292,8 @@ gpet_status alloc_queue(gpet_ctx *c, int64_t qcap)
 DevQueue make_queue(gpet_ctx *c, int64_t qcap, bool want_trace)
 {
     DevQueue q;
+    memset(&q, 0, sizeof(q));
+    q.rcount = c->d_rcount.p;
     q.y = c->q_f[0].p; q.z = c->q_f[1].p; q.vx = c->q_f[2].p; q.vy = c->q_f[3].p; q.vz = c->q_f[4].p; q.E = c->q_f[5].p;
     q.t = c->q_t.p;
     q.info = c->q_info.p;
@@ -297,24 +303,39 @@ DevQueue make_queue(gpet_ctx *c, int64_t qcap, bool want_trace)
     return q;
 }
 
-/* Launch arguments of one transport batch. */
+/* Launch arguments of one transport batch; the panel queue has G regions
+ * of `region` slots (one per phantom-kernel block). */
 struct BatchLaunch {
     RunParams rp;
     int64_t nh;
+    int G;
+    int64_t region;
 };
 
-gpet_status launch_batch(gpet_ctx *c, const BatchLaunch &B, const DevQueue &q, const PanelParams &pp,
+/* Queue shape of a phantom launch over `items` histories (virtual blocks x
+ * block size for batched runs): a block handles at most
+ * ceil(items / (G * block)) rounds of `block` histories, 2 photons each. */
+void queue_shape(int64_t items, const gpet_launch &L, int n_cu, int &G, int64_t &region)
+{
+    const LaunchCfg cfg = cfg_of(L, 0);
+    G = gpet_phantom_grid(std::max<int64_t>(items, 1), cfg, n_cu);
+    const int64_t rounds = (std::max<int64_t>(items, 1) + (int64_t)G * cfg.block - 1) / ((int64_t)G * cfg.block);
+    region = 2 * rounds * cfg.block;
+}
+
+gpet_status launch_batch(gpet_ctx *c, const BatchLaunch &B, const DevQueue &q0, const PanelParams &pp,
                          const gpet_launch &L, bool batched, hipEvent_t *ev, int pmode)
 {
-    int e = gpet_launch_zero((unsigned *)q.count, 1, c->stream);
-    if (e) return fail(c, GPET_ERR_DEVICE, std::string("zero kernel: ") + hipGetErrorString((hipError_t)e));
+    DevQueue q = q0;
+    q.nregions = B.G;
+    q.region = B.region;
+    if ((int64_t)B.G * B.region > q.cap) return fail(c, GPET_ERR_CAPACITY, "panel queue shape exceeds its buffer");
     if (ev) HIPOK(c, hipEventRecord(ev[0], c->stream));
-    e = gpet_launch_phantom(B.rp, c->dtab, c->dph, c->ddet, q, batched, cfg_of(L, 0), c->n_cu, c->stream);
+    int e = gpet_launch_phantom(B.rp, c->dtab, c->dph, c->ddet, q, batched, cfg_of(L, 0), c->n_cu, c->stream);
     if (e) return fail(c, GPET_ERR_DEVICE, std::string("phantom kernel: ") + hipGetErrorString((hipError_t)e));
     if (ev) HIPOK(c, hipEventRecord(ev[1], c->stream));
     if (ev) HIPOK(c, hipEventRecord(ev[2], c->stream));
-    e = gpet_launch_panel(pp, c->dtab, c->ddet, q, std::min<int64_t>(2 * B.nh, q.cap), cfg_of(L, 1), c->n_cu, c->stream,
-                          pmode);
+    e = gpet_launch_panel(pp, c->dtab, c->ddet, q, 2 * B.nh, cfg_of(L, 1), c->n_cu, c->stream, pmode);
     if (e) return fail(c, GPET_ERR_DEVICE, std::string("panel kernel: ") + hipGetErrorString((hipError_t)e));
     if (ev) HIPOK(c, hipEventRecord(ev[3], c->stream));
     return GPET_OK;
@@ -391,7 +412,9 @@ gpet_status autotune(gpet_ctx *c, const RunParams &rp0, int64_t n_hist, const Pa
     auto t0 = std::chrono::steady_clock::now();
     gpet_status st;
     const int64_t ns = std::min<int64_t>(n_hist, (int64_t)1 << 21);
-    const int64_t qcap = 2 * ns;
+    /* queue large enough for every grid the search tries (>= 1 block per CU,
+     * <= 256 threads per block) */
+    const int64_t qcap = 2 * ns + 2 * (int64_t)64 * c->n_cu * 256;
     if ((st = alloc_queue(c, qcap))) return st;
     if ((st = dgrow(c, c->d_events, GPET_MAX_PH_EVENTS * qcap, 0))) return st;
     DevQueue q = make_queue(c, qcap, false);
@@ -411,6 +434,7 @@ gpet_status autotune(gpet_ctx *c, const RunParams &rp0, int64_t n_hist, const Pa
         rp.nh = ns;
         rp.trace = nullptr;
         rp.trace_cap = 0;
+        rp.psf_src = rp.psf_ph = nullptr;
         if (bi) {
             segs.clear();
             int64_t nvb = 0;
@@ -421,12 +445,12 @@ gpet_status autotune(gpet_ctx *c, const RunParams &rp0, int64_t n_hist, const Pa
             rp.nseg = (int)segs.size();
             rp.n_vblocks = nvb;
         }
+        queue_shape(bi ? rp.n_vblocks * T.block[0] : ns, T, c->n_cu, q.nregions, q.region);
+        if ((int64_t)q.nregions * q.region > q.cap) return fail(c, GPET_ERR_CAPACITY, "tuner queue shape");
         float best = 1e30f;
         for (int rep = 0; rep < 2; ++rep) {
-            int e = gpet_launch_zero((unsigned *)q.count, 1, c->stream);
-            if (e) return fail(c, GPET_ERR_DEVICE, "zero kernel");
             HIPOK(c, hipEventRecord(e0, c->stream));
-            e = gpet_launch_phantom(rp, c->dtab, c->dph, c->ddet, q, bi != nullptr, cfg_of(T, 0), c->n_cu, c->stream);
+            int e = gpet_launch_phantom(rp, c->dtab, c->dph, c->ddet, q, bi != nullptr, cfg_of(T, 0), c->n_cu, c->stream);
             if (e) return fail(c, GPET_ERR_DEVICE, std::string("phantom kernel: ") + hipGetErrorString((hipError_t)e));
             HIPOK(c, hipEventRecord(e1, c->stream));
             HIPOK(c, hipEventSynchronize(e1));
@@ -443,7 +467,7 @@ gpet_status autotune(gpet_ctx *c, const RunParams &rp0, int64_t n_hist, const Pa
             int e = gpet_launch_zero(c->d_small.p + 1, 1, c->stream);
             if (e) return fail(c, GPET_ERR_DEVICE, "zero kernel");
             HIPOK(c, hipEventRecord(e0, c->stream));
-            e = gpet_launch_panel(pp, c->dtab, c->ddet, q, qcap, cfg_of(T, 1), c->n_cu, c->stream);
+            e = gpet_launch_panel(pp, c->dtab, c->ddet, q, 2 * ns, cfg_of(T, 1), c->n_cu, c->stream);
             if (e) return fail(c, GPET_ERR_DEVICE, std::string("panel kernel: ") + hipGetErrorString((hipError_t)e));
             HIPOK(c, hipEventRecord(e1, c->stream));
             HIPOK(c, hipEventSynchronize(e1));
@@ -459,7 +483,7 @@ gpet_status autotune(gpet_ctx *c, const RunParams &rp0, int64_t n_hist, const Pa
         float best_ms = 0;
         if ((st = timer(L, best_ms))) return st;
         const int wpes[] = {0, 6, 8};
-        const int bpcs[] = {0, 2, 4, 8, 16};
+        const int bpcs[] = {1, 2, 4, 8, 16};
         const int blocks[] = {128, 256};
         for (int stage = 0; stage < 3; ++stage) {
             gpet_launch base = L;
@@ -470,7 +494,6 @@ gpet_status autotune(gpet_ctx *c, const RunParams &rp0, int64_t n_hist, const Pa
                 if (stage == 0) T.wpe[k] = vals[v];
                 else if (stage == 1) T.blocks_per_cu[k] = vals[v];
                 else T.block[k] = vals[v];
-                if (k == 1 && stage == 1 && T.blocks_per_cu[k] == 0) continue; /* panel grid needs a device count */
                 if (memcmp(&T, &base, sizeof(T)) == 0) continue;
                 float ms = 0;
                 if ((st = timer(T, ms))) return st;
@@ -507,6 +530,19 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
     rp.prange = c->opt.positron_range;
     rp.cnt = c->d_cnt.p;
     rp.phs = batched ? c->d_phs.p : nullptr;
+    for (int k = 0; k < 4; ++k) rp.sphere[k] = c->opt.record_sphere[k];
+    const bool want_psf_src = (c->opt.flags & GPET_FLAG_PSF_SOURCE) != 0;
+    const bool want_psf_ph = (c->opt.flags & GPET_FLAG_PSF_PHANTOM) != 0;
+    const int64_t psf_cap = 2 * std::max<int64_t>(n_hist, 1);
+    if (want_psf_src || want_psf_ph) {
+        if ((st = dalloc(c, c->d_npsf, 2))) return st;
+        if (want_psf_src && (st = dalloc(c, c->d_psf_src, psf_cap))) return st;
+        if (want_psf_ph && (st = dalloc(c, c->d_psf_ph, psf_cap))) return st;
+    }
+    rp.psf_src = want_psf_src ? c->d_psf_src.p : nullptr;
+    rp.psf_ph = want_psf_ph ? c->d_psf_ph.p : nullptr;
+    rp.n_psf = (want_psf_src || want_psf_ph) ? c->d_npsf.p : nullptr;
+    rp.psf_cap = psf_cap;
 
     PanelParams pp;
     memset(&pp, 0, sizeof(pp));
@@ -526,39 +562,11 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
     }
     const gpet_launch L = c->launch;
     const int64_t batch = L.batch;
-    const int64_t qcap = 2 * std::min<int64_t>(batch, std::max<int64_t>(n_hist, 1));
-    if ((st = alloc_queue(c, qcap))) return st;
-    const bool want_trace = sink && sink->trace && sink->trace_cap > 0;
-    int64_t ntr = want_trace ? std::min<int64_t>(2 * n_hist, sink->trace_cap) : 0;
-    if (want_trace)
-        if ((st = dalloc(c, c->d_trace, ntr)) || (st = dalloc(c, c->q_slot, qcap))) return st;
-    const bool want_hits = c->opt.record_hits != 0;
-    int64_t hits_cap = 0;
-    if (want_hits) {
-        hits_cap = (sink && sink->hit_id && sink->hits_cap > 0) ? sink->hits_cap : (int64_t)1 << 24;
-        if ((st = dalloc(c, c->d_hit_id, 5 * hits_cap)) || (st = dalloc(c, c->d_hit_f, 5 * hits_cap))) return st;
-    }
-    DevQueue q = make_queue(c, qcap, want_trace);
-    const int pmode = c->panel_mode == PANEL_HARVEST ? PANEL_HARVEST : (c->surr.enabled ? PANEL_SURROGATE : PANEL_PLAIN);
-    if (pmode == PANEL_SURROGATE) {
-        if (want_hits) return fail(c, GPET_ERR_UNSUPPORTED, "surrogate mode does not produce hit records");
-        if ((st = surr_run_prepare(c, n_hist, qcap, pp.sq))) return st;
-    }
-    if (pmode == PANEL_HARVEST) {
-        pp.harvest = c->surr.set.p + SURR_REC * c->surr.set_n;
-        pp.harvest_count = c->surr.set_count.p;
-        pp.harvest_cap = c->surr.set.cap / SURR_REC - c->surr.set_n;
-    }
-    rp.trace = want_trace ? c->d_trace.p : nullptr;
-    rp.trace_cap = ntr;
-    pp.hit_id = want_hits ? c->d_hit_id.p : nullptr;
-    pp.hit_f = want_hits ? c->d_hit_f.p : nullptr;
-    pp.hits_cap = hits_cap;
-    pp.trace = rp.trace;
-
-    /* per-batch launch arguments (and the segment tables of a batched run) */
+    /* per-batch launch arguments (and the segment tables of a batched run),
+     * and the panel-queue size their shapes need */
     const int64_t nbatch = (n_hist + batch - 1) / batch;
     std::vector<BatchLaunch> B((size_t)nbatch);
+    int64_t qcap = 2;
     {
         std::vector<DevSegment> segs;
         std::vector<int64_t> seg_off((size_t)nbatch, 0), nvb((size_t)nbatch, 0);
@@ -583,8 +591,43 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
                 X.rp.nseg = nseg[b];
                 X.rp.n_vblocks = nvb[b];
             }
+            queue_shape(batched ? nvb[b] * L.block[0] : X.nh, L, c->n_cu, X.G, X.region);
+            qcap = std::max<int64_t>(qcap, (int64_t)X.G * X.region);
         }
     }
+    if ((st = alloc_queue(c, qcap))) return st;
+    const bool want_trace = sink && sink->trace && sink->trace_cap > 0;
+    int64_t ntr = want_trace ? std::min<int64_t>(2 * n_hist, sink->trace_cap) : 0;
+    if (want_trace)
+        if ((st = dalloc(c, c->d_trace, ntr)) || (st = dalloc(c, c->q_slot, qcap))) return st;
+    const bool want_hits = c->opt.record_hits != 0;
+    int64_t hits_cap = 0;
+    if (want_hits) {
+        hits_cap = (sink && sink->hit_id && sink->hits_cap > 0) ? sink->hits_cap : (int64_t)1 << 24;
+        if ((st = dalloc(c, c->d_hit_id, 5 * hits_cap)) || (st = dalloc(c, c->d_hit_f, 5 * hits_cap))) return st;
+    }
+    DevQueue q = make_queue(c, qcap, want_trace);
+    const int pmode = c->panel_mode == PANEL_HARVEST ? PANEL_HARVEST : (c->surr.enabled ? PANEL_SURROGATE : PANEL_PLAIN);
+    if (pmode == PANEL_SURROGATE) {
+        if (want_hits) return fail(c, GPET_ERR_UNSUPPORTED, "surrogate mode does not produce hit records");
+        if ((st = surr_run_prepare(c, n_hist, qcap, pp.sq))) return st;
+    }
+    if (pmode == PANEL_HARVEST) {
+        pp.harvest = c->surr.set.p + SURR_REC * c->surr.set_n;
+        pp.harvest_count = c->surr.set_count.p;
+        pp.harvest_cap = c->surr.set.cap / SURR_REC - c->surr.set_n;
+    }
+    rp.trace = want_trace ? c->d_trace.p : nullptr;
+    rp.trace_cap = ntr;
+    for (BatchLaunch &X : B) {
+        X.rp.trace = rp.trace;
+        X.rp.trace_cap = ntr;
+    }
+    pp.hit_id = want_hits ? c->d_hit_id.p : nullptr;
+    pp.hit_f = want_hits ? c->d_hit_f.p : nullptr;
+    pp.hits_cap = hits_cap;
+    pp.trace = rp.trace;
+
     while ((int64_t)c->kev.size() < 4 * std::max<int64_t>(nbatch, 1)) {
         hipEvent_t e;
         HIPOK(c, hipEventCreate(&e));
@@ -607,6 +650,7 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
         HIPOK(c, hipMemsetAsync(c->d_cnt.p, 0, sizeof(unsigned long long) * CNT_NUM, c->stream));
         if (pmode == PANEL_SURROGATE) HIPOK(c, hipMemsetAsync(pp.sq.count, 0, sizeof(int), c->stream));
         if (pmode == PANEL_HARVEST) HIPOK(c, hipMemsetAsync(pp.harvest_count, 0, sizeof(unsigned long long), c->stream));
+        if (rp.n_psf) HIPOK(c, hipMemsetAsync(rp.n_psf, 0, 2 * sizeof(unsigned long long), c->stream));
         if (want_trace) HIPOK(c, hipMemsetAsync(c->d_trace.p, 0, sizeof(gpet_trace) * (size_t)ntr, c->stream));
         pp.events = c->d_events.p;
         pp.events_cap = c->d_events.cap;
@@ -786,7 +830,32 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
         sink->n_trace = ntr;
         HIPOK(c, hipMemcpy(sink->trace, c->d_trace.p, sizeof(gpet_trace) * (size_t)ntr, hipMemcpyDeviceToHost));
     }
-    if (to_files) return write_outputs(c, sink->out_dir, adder, singles, coinc, hid, hf);
+    /* phase-space records, ordered by photon id */
+    if (rp.n_psf) {
+        unsigned long long np[2] = {0, 0};
+        HIPOK(c, hipMemcpy(np, rp.n_psf, sizeof(np), hipMemcpyDeviceToHost));
+        gpet_psf_record *dev[2] = {rp.psf_src, rp.psf_ph};
+        gpet_psf_record *dst[2] = {sink->psf_source, sink->psf_phantom};
+        const int64_t caps[2] = {sink->psf_source_cap, sink->psf_phantom_cap};
+        int64_t *ns[2] = {&sink->n_psf_source, &sink->n_psf_phantom};
+        const char *names[2] = {"source", "phantom"};
+        for (int k = 0; k < 2; ++k) {
+            if (!dev[k]) continue;
+            const int64_t m = std::min<int64_t>((int64_t)np[k], psf_cap);
+            *ns[k] = m;
+            if (!((dst[k] && caps[k] > 0) || to_files)) continue;
+            std::vector<gpet_psf_record> v((size_t)m);
+            if (m) HIPOK(c, hipMemcpy(v.data(), dev[k], sizeof(gpet_psf_record) * (size_t)m, hipMemcpyDeviceToHost));
+            std::sort(v.begin(), v.end(),
+                      [](const gpet_psf_record &a, const gpet_psf_record &b) { return a.parn < b.parn; });
+            if (dst[k]) memcpy(dst[k], v.data(), sizeof(gpet_psf_record) * (size_t)std::min<int64_t>(m, caps[k]));
+            if (to_files) {
+                gpet_status s2 = gpet_write_psf(sink->out_dir, names[k], v.data(), m, sink->out_append);
+                if (s2) return fail(c, s2, std::string("cannot write PSF files in ") + sink->out_dir);
+            }
+        }
+    }
+    if (to_files) return write_outputs(c, sink->out_dir, adder, singles, coinc, hid, hf, sink->out_append);
     return GPET_OK;
 }
 
@@ -843,6 +912,7 @@ gpet_status gpet_destroy(gpet_ctx *c)
     for (auto &b : c->q_f) dfree(b);
     dfree(c->q_t); dfree(c->q_info); dfree(c->q_slot); dfree(c->d_small); dfree(c->d_cnt);
     dfree(c->d_events); dfree(c->d_trace); dfree(c->d_hit_id); dfree(c->d_hit_f);
+    dfree(c->d_psf_src); dfree(c->d_psf_ph); dfree(c->d_npsf);
     dfree(c->d_first); dfree(c->d_src); dfree(c->d_iso); dfree(c->d_psf);
     gpet_digi_work_destroy(c->dw);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

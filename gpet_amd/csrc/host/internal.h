@@ -94,7 +94,10 @@ struct DevSegment {
     int phantom, pad;
 };
 
-/* What the phantom kernel produces for the panel kernel. */
+/* What the phantom kernel produces for the panel kernel.  The panel queue
+ * is split into one region per phantom-kernel block (no global atomics):
+ * region r holds slots [r * region, r * region + rcount[r]).  The run-wide
+ * surrogate queue uses the single counter `count` instead (region = 0). */
 struct DevQueue {
     float *y, *z, *vx, *vy, *vz, *E;
     double *t;
@@ -102,6 +105,9 @@ struct DevQueue {
     int *trace_slot;           /* trace slot or -1                            */
     int *count;
     int64_t cap;
+    unsigned *rcount;          /* [nregions] entries per region               */
+    int64_t region;            /* slots per region                            */
+    int nregions;
 };
 
 /* Device-side counters, 64-bit, accumulated with atomics. */
@@ -130,6 +136,10 @@ struct RunParams {
     gpet_trace *trace;         /* device trace buffer or null                  */
     int64_t trace_cap;         /* slots: 2*(h - 0) < trace_cap are written     */
     unsigned long long *cnt;   /* [CNT_NUM]                                    */
+    float sphere[4];           /* recording sphere (r = sphere[3] > 0: on)     */
+    gpet_psf_record *psf_src, *psf_ph; /* phase-space outputs or null         */
+    unsigned long long *n_psf;  /* [2] appended records                         */
+    int64_t psf_cap;
 };
 
 struct PanelParams {
@@ -167,6 +177,10 @@ struct LaunchCfg {
 };
 #define GPET_N_WPE 3
 extern const int gpet_wpe_variants[GPET_N_WPE];
+
+/* Grid of a phantom-kernel launch over `items` histories (virtual blocks x
+ * block for batched runs): the queue has one region per block. */
+int gpet_phantom_grid(int64_t items, const LaunchCfg &cfg, int n_cu);
 
 /* Launchers (transport.hip).  batch: sources carry phantom slots (rp.phs).
  * The panel kernel reads its item count from q.count on the device, so the

@@ -532,6 +532,12 @@ gpet_status gpet_read_input(const char *path, gpet_input *in)
         in->coinc_policy = (int32_t)i64(v[1]);
     }
     if (field(1)) in->seed = (uint64_t)strtoull(v[0].c_str(), nullptr, 10);
+    if (field(3)) {
+        in->noise_rate = f64(v[0]);
+        in->noise_e_mean = f32(v[1]);
+        in->noise_e_sigma = f32(v[2]);
+    }
+    if (field(1)) in->delay_us = f64(v[0]);
     return GPET_OK;
 }
 
@@ -556,6 +562,10 @@ void gpet_input_digitizer(const gpet_input *in, gpet_digitizer *d)
     d->coinc_window_us = in->coinc_window_us;
     d->coinc_policy = in->coinc_policy;
     d->record_rayleigh = 0;
+    d->noise_rate = in->noise_rate;
+    d->noise_e_mean = in->noise_e_mean;
+    d->noise_e_sigma = in->noise_e_sigma;
+    d->delay_us = in->delay_us;
 }
 
 gpet_status gpet_write_events(const char *path, const gpet_event *ev, int64_t n, int32_t append)
@@ -586,6 +596,24 @@ gpet_status gpet_write_hits(const char *id_path, const char *f_path, const int32
     fclose(a);
     fclose(b);
     return GPET_OK;
+}
+
+gpet_status gpet_write_psf(const char *dir, const char *name, const gpet_psf_record *rec, int64_t n,
+                           int32_t append)
+{
+    if (!dir || !name || (n > 0 && !rec)) return GPET_ERR_ARG;
+    const std::string d(dir), nm(name);
+    const char *mode = append ? "ab" : "wb";
+    FILE *f[3] = {fopen((d + "/out" + nm + ".dat").c_str(), mode), fopen((d + "/id" + nm + ".dat").c_str(), mode),
+                  fopen((d + "/time" + nm + ".dat").c_str(), mode)};
+    bool ok = f[0] && f[1] && f[2];
+    for (int64_t i = 0; ok && i < n; ++i) {
+        ok = fwrite(&rec[i].x, sizeof(float), 7, f[0]) == 7 && fwrite(&rec[i].eventid, sizeof(int32_t), 1, f[1]) == 1 &&
+             fwrite(&rec[i].t, sizeof(double), 1, f[2]) == 1;
+    }
+    for (FILE *x : f)
+        if (x && fclose(x) != 0) ok = false;
+    return ok ? GPET_OK : GPET_ERR_IO;
 }
 
 } /* extern "C" */

@@ -259,6 +259,7 @@ gpet_status surr_run_prepare(gpet_ctx *c, int64_t n_hist, int64_t qcap, DevQueue
         S.sq_cap = need;
     }
     if ((st = dalloc(c, S.sq_count, 1))) return st;
+    memset(&sq, 0, sizeof(sq));
     sq.y = S.sq_f[0].p; sq.z = S.sq_f[1].p; sq.vx = S.sq_f[2].p; sq.vy = S.sq_f[3].p; sq.vz = S.sq_f[4].p;
     sq.E = S.sq_f[5].p;
     sq.t = S.sq_t.p;

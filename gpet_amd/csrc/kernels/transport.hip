@@ -56,10 +56,23 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
     return v;
 }
 
-__device__ __forceinline__ void flush_counter(unsigned long long *dst, unsigned long long v)
+/* Block-wide sums of five per-lane counters, one atomic per counter and
+ * block (called by every thread of the block, after its work loop). */
+__device__ __forceinline__ void flush_counters5(unsigned long long *const dst[5], const unsigned long long v[5])
 {
-    unsigned long long s = wave_sum(v);
-    if (__lane_id() == 0 && s) atomicAdd(dst, s);
+    __shared__ unsigned long long s_red[4][5];
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const unsigned long long s = wave_sum(v[k]);
+        if (__lane_id() == 0) s_red[w][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        unsigned long long s = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += s_red[i][threadIdx.x];
+        if (s) atomicAdd(dst[threadIdx.x], s);
+    }
 }
 
 /* ---- source ------------------------------------------------------------ */
@@ -276,6 +289,37 @@ __device__ bool d_phantom_walk(const DevTables &tab, const DevPhantom &p, float 
     return false;
 }
 
+/* Distance along the direction to the recording sphere (getDistance,
+ * gPET_kernals.cu:148-171): the far root from inside; from outside the near
+ * root when approaching, the far (negative) one when receding; 0 when the
+ * line misses. */
+__device__ __forceinline__ float d_sphere_dist(const float *sp, float x, float y, float z, float vx, float vy,
+                                               float vz)
+{
+    const float cx = x - sp[0], cy = y - sp[1], cz = z - sp[2];
+    const float a = vx * vx + vy * vy + vz * vz;
+    const float b = 2.0f * (vx * cx + vy * cy + vz * cz);
+    const float c = (cx * cx + cy * cy + cz * cz) - sp[3] * sp[3];
+    const float disc = b * b - 4.0f * a * c;
+    if (disc < 0.0f) return 0.0f;
+    if (c < 0.0f) return (-b + sqrtf(disc)) / (2.0f * a);
+    if (b < 0.0f) return (-b - sqrtf(disc)) / (2.0f * a);
+    return (-b + sqrtf(disc)) / (2.0f * a);
+}
+
+__device__ __forceinline__ void d_psf_append(gpet_psf_record *buf, unsigned long long *count, int64_t cap, float x,
+                                             float y, float z, float vx, float vy, float vz, float E, int parn,
+                                             int eid, double t)
+{
+    const unsigned long long i = atomicAdd(count, 1ull);
+    if ((int64_t)i < cap) {
+        gpet_psf_record r;
+        r.x = x; r.y = y; r.z = z; r.vx = vx; r.vy = vy; r.vz = vz; r.E = E;
+        r.parn = parn; r.eventid = eid; r.pad = 0; r.t = t;
+        buf[i] = r;
+    }
+}
+
 struct PhCounters {
     unsigned long long ph = 0, ent = 0, maj = 0, ps = 0, rs = 0;
 };
@@ -295,7 +339,8 @@ __device__ __forceinline__ int d_source_of(const RunParams &rp, int64_t h)
  * of both photons, panel entry and queue append. */
 template <int MODE>
 __device__ __forceinline__ void d_history(const RunParams &rp, const DevTables &tab, const DevPhantom &ph,
-                                          const DevDetector &det, const DevQueue &q, int64_t i, int64_t h, int si,
+                                          const DevDetector &det, const float4 *s_pf, const DevPanel *s_p,
+                                          const DevQueue &q, unsigned *s_qn, int64_t i, int64_t h, int si,
                                           PhCounters &C)
 {
     unsigned long long &c_ph = C.ph, &c_ent = C.ent, &c_maj = C.maj, &c_ps = C.ps, &c_rs = C.rs;
@@ -356,9 +401,21 @@ __device__ __forceinline__ void d_history(const RunParams &rp, const DevTables &
         gp_rng_init(&r, rp.seed, (uint64_t)h, (uint32_t)(GP_TAG_PHOTON + k), 0u);
         int ns = 0, nc = 0, nr = 0;
         bool alive = active;
+        const int parn = (MODE == RUN_PSF_PHOTON) ? (int)h : (int)(2 * h + k);
         if (active) {
             c_ph++;
+            if (rp.psf_src)
+                d_psf_append(rp.psf_src, rp.n_psf, rp.psf_cap, x, y, z, vx, vy, vz, E, parn, (int)h, t);
             if (ph.present) alive = d_phantom_walk(tab, ph, rp.eabs, x, y, z, vx, vy, vz, E, t, r, ns, nc, nr, c_maj);
+            if (alive && rp.sphere[3] > 0.0f) {
+                const float sd = d_sphere_dist(rp.sphere, x, y, z, vx, vy, vz);
+                x = x + sd * vx;
+                y = y + sd * vy;
+                z = z + sd * vz;
+                t = t + (double)sd * D_INV_C;
+            }
+            if (alive && rp.psf_ph)
+                d_psf_append(rp.psf_ph, rp.n_psf + 1, rp.psf_cap, x, y, z, vx, vy, vz, E, parn, (int)h, t);
         }
         c_ps += (unsigned long long)ns;
         if (tr) {
@@ -380,17 +437,18 @@ __device__ __forceinline__ void d_history(const RunParams &rp, const DevTables &
         if (alive && det.present) {
             const float vv = vx * vx + vy * vy + vz * vz;
             for (int pi = 0; pi < det.npanels; ++pi) {
-                const DevPanel &P = det.p[pi];
                 /* conservative prefilter: the line through the photon must
                  * pass within the face's bounding sphere (covers the
                  * reference's backward entries too); skipping a panel here
                  * never changes which panel the exact test picks */
                 {
-                    float wx = P.ox - x, wy = P.oy - y, wz = P.oz - z;
+                    const float4 f = s_pf[pi];
+                    float wx = f.x - x, wy = f.y - y, wz = f.z - z;
                     float a = vx * wx + vy * wy + vz * wz;
                     float w2 = wx * wx + wy * wy + wz * wz;
-                    if (w2 * vv - a * a > P.bound2 * vv) continue;
+                    if (w2 * vv - a * a > f.w * vv) continue;
                 }
+                const DevPanel &P = s_p[pi];
                 float rx = x - P.ox, ry = y - P.oy, rz = z - P.oz;
                 float tx = rx * P.ux[0] + ry * P.ux[1] + rz * P.ux[2];
                 float ty = rx * P.uy[0] + ry * P.uy[1] + rz * P.uy[2];
@@ -416,16 +474,26 @@ __device__ __forceinline__ void d_history(const RunParams &rp, const DevTables &
             tr->panel = pa;
             tr->status |= GPET_TR_PANEL;
         }
+        /* append to this block's queue region (LDS counter, one atomic per
+         * wave) */
         const bool want = pa >= 0;
-        int idx = wave_append(q.count, want);
+        const unsigned long long m = __ballot(want);
+        int64_t idx = q.region;
+        if (m) {
+            const int leader = __ffsll((unsigned long long)m) - 1;
+            unsigned b = 0;
+            if ((int)__lane_id() == leader) b = atomicAdd(s_qn, (unsigned)__popcll(m));
+            b = __shfl(b, leader);
+            idx = (int64_t)b + __popcll(m & lanemask_lt());
+        }
         if (want) {
             c_ent++;
-            if (idx < q.cap) {
+            if (idx < q.region) {
+                idx += (int64_t)blockIdx.x * q.region;
                 q.y[idx] = y2; q.z[idx] = z2;
                 q.vx[idx] = vx2; q.vy[idx] = vy2; q.vz[idx] = vz2;
                 q.E[idx] = E;
                 q.t[idx] = t;
-                int parn = (MODE == RUN_PSF_PHOTON) ? (int)h : (int)(2 * h + k);
                 q.info[idx] = make_int4(pa, parn, (int)h, (int)(r.n | ((unsigned)k << 31)));
                 if (q.trace_slot) q.trace_slot[idx] = tr ? (int)slot : -1;
             }
@@ -446,12 +514,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? W
 void k_phantom(RunParams rp, DevTables tab, DevPhantom ph, DevDetector det, DevQueue q)
 {
     PhCounters C;
+    __shared__ unsigned s_qn;
+    if (threadIdx.x == 0) s_qn = 0u;
+    /* panel table in LDS: the entry test scans every panel for every photon,
+     * and from LDS the scan is not a chain of memory round trips */
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    float4 *s_pf = reinterpret_cast<float4 *>(smem);
+    DevPanel *s_p = reinterpret_cast<DevPanel *>(smem + sizeof(float4) * (size_t)det.npanels);
+    if (det.present) {
+        for (int t = threadIdx.x; t < det.npanels; t += blockDim.x) {
+            const DevPanel P = det.p[t];
+            s_pf[t] = make_float4(P.ox, P.oy, P.oz, P.bound2);
+            s_p[t] = P;
+        }
+    }
+    __syncthreads();
     if (!BATCH) {
         const int64_t stride = (int64_t)gridDim.x * blockDim.x;
         for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < rp.nh; i += stride) {
             const int64_t h = rp.h0 + i;
             const int si = (MODE == RUN_DECAY) ? d_source_of(rp, h) : 0;
-            d_history<MODE>(rp, tab, ph, det, q, i, h, si, C);
+            d_history<MODE>(rp, tab, ph, det, s_pf, s_p, q, &s_qn, i, h, si, C);
         }
     } else {
         for (int64_t vb = blockIdx.x; vb < rp.n_vblocks; vb += gridDim.x) {
@@ -466,15 +549,18 @@ void k_phantom(RunParams rp, DevTables tab, DevPhantom ph, DevDetector det, DevQ
                 const int64_t h = S.h0 + off;
                 const int si = d_source_of(rp, h);
                 const DevPhantom P = rp.phs[S.phantom];
-                d_history<MODE>(rp, tab, P, det, q, h - rp.h0, h, si, C);
+                d_history<MODE>(rp, tab, P, det, s_pf, s_p, q, &s_qn, h - rp.h0, h, si, C);
             }
         }
     }
-    flush_counter(rp.cnt + CNT_PHOTONS, C.ph);
-    flush_counter(rp.cnt + CNT_PANEL_ENTRIES, C.ent);
-    flush_counter(rp.cnt + CNT_MAJORANT, C.maj);
-    flush_counter(rp.cnt + CNT_PHANTOM_STEPS, C.ps);
-    flush_counter(rp.cnt + CNT_RANGE_STEPS, C.rs);
+    __syncthreads();
+    if (threadIdx.x == 0) q.rcount[blockIdx.x] = min((unsigned)q.region, s_qn);
+    {
+        unsigned long long *const dst[5] = {rp.cnt + CNT_PHOTONS, rp.cnt + CNT_PANEL_ENTRIES, rp.cnt + CNT_MAJORANT,
+                                            rp.cnt + CNT_PHANTOM_STEPS, rp.cnt + CNT_RANGE_STEPS};
+        const unsigned long long v[5] = {C.ph, C.ent, C.maj, C.ps, C.rs};
+        flush_counters5(dst, v);
+    }
 }
 
 /* ---- panel -------------------------------------------------------------- */
@@ -559,9 +645,46 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? W
 void k_panel(PanelParams pp, DevTables tab, DevDetector det, DevQueue q)
 {
     unsigned long long c_hits = 0, c_drop = 0, c_droph = 0, c_maj = 0, c_steps = 0;
-    const int nq = (int)min((int64_t)*q.count, q.cap);
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    DevPanel *s_p = reinterpret_cast<DevPanel *>(smem);
+    unsigned *s_pre = reinterpret_cast<unsigned *>(smem + sizeof(DevPanel) * (size_t)det.npanels);
+    __shared__ unsigned s_part[256];
+    for (int t = threadIdx.x; t < det.npanels; t += blockDim.x) s_p[t] = det.p[t];
+    /* prefix over the phantom kernel's region counts: s_pre[r] = entries
+     * before region r, s_pre[nregions] = total */
+    const int G = q.nregions;
+    const int chunk = (G + (int)blockDim.x - 1) / (int)blockDim.x;
+    unsigned part = 0;
+    for (int r = threadIdx.x * chunk; r < min(G, (int)(threadIdx.x + 1) * chunk); ++r) part += q.rcount[r];
+    s_part[threadIdx.x] = part;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned acc = 0;
+        for (int t = 0; t < (int)blockDim.x; ++t) {
+            const unsigned v = s_part[t];
+            s_part[t] = acc;
+            acc += v;
+        }
+    }
+    __syncthreads();
+    {
+        unsigned acc = s_part[threadIdx.x];
+        for (int r = threadIdx.x * chunk; r < min(G, (int)(threadIdx.x + 1) * chunk); ++r) {
+            s_pre[r] = acc;
+            acc += q.rcount[r];
+        }
+        if ((int)threadIdx.x == (int)blockDim.x - 1) s_pre[G] = acc;
+    }
+    __syncthreads();
+    const int nq = (int)s_pre[G];
     const int stride = gridDim.x * blockDim.x;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += stride) {
+    for (int gi = blockIdx.x * blockDim.x + threadIdx.x; gi < nq; gi += stride) {
+        int lo = 0, hi = G; /* largest r with s_pre[r] <= gi */
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if ((int)s_pre[mid] <= gi) lo = mid; else hi = mid;
+        }
+        const int64_t i = (int64_t)lo * q.region + (gi - (int)s_pre[lo]);
         if (MODE == PANEL_SURROGATE) {
             const bool sv = surr_valid(q.E[i], q.vx[i]);
             const int slot = wave_append(pp.sq.count, sv);
@@ -588,7 +711,7 @@ void k_panel(PanelParams pp, DevTables tab, DevDetector det, DevQueue q)
         const unsigned k = (unsigned)info.w >> 31;
         gp_rng r;
         gp_rng_init(&r, pp.seed, (uint64_t)(int64_t)eid, GP_TAG_PHOTON + k, draws);
-        const DevPanel &P = det.p[pa];
+        const DevPanel &P = s_p[pa];
         const int pann = P.panel;
         const float hx = P.lenx, hy = P.leny * 0.5f, hz = P.lenz * 0.5f;
         EvList L;
@@ -767,11 +890,12 @@ void k_panel(PanelParams pp, DevTables tab, DevDetector det, DevQueue q)
             }
         }
     }
-    flush_counter(pp.cnt + CNT_HITS, c_hits);
-    flush_counter(pp.cnt + CNT_DROPPED_EVENTS, c_drop);
-    flush_counter(pp.cnt + CNT_DROPPED_HITS, c_droph);
-    flush_counter(pp.cnt + CNT_MAJORANT, c_maj);
-    flush_counter(pp.cnt + CNT_PANEL_STEPS, c_steps);
+    {
+        unsigned long long *const dst[5] = {pp.cnt + CNT_HITS, pp.cnt + CNT_DROPPED_EVENTS, pp.cnt + CNT_DROPPED_HITS,
+                                            pp.cnt + CNT_MAJORANT, pp.cnt + CNT_PANEL_STEPS};
+        const unsigned long long v[5] = {c_hits, c_drop, c_droph, c_maj, c_steps};
+        flush_counters5(dst, v);
+    }
 }
 
 } // namespace
@@ -791,12 +915,13 @@ template <int MODE, bool BATCH>
 void launch_phantom_wpe(const RunParams &rp, const DevTables &tab, const DevPhantom &ph, const DevDetector &det,
                         const DevQueue &q, int grid, int block, int wpe, hipStream_t s)
 {
+    const size_t lds = det.present ? (sizeof(float4) + sizeof(DevPanel)) * (size_t)det.npanels : 0;
     if (wpe == 8)
-        hipLaunchKernelGGL((k_phantom<MODE, BATCH, 8>), dim3(grid), dim3(block), 0, s, rp, tab, ph, det, q);
+        hipLaunchKernelGGL((k_phantom<MODE, BATCH, 8>), dim3(grid), dim3(block), lds, s, rp, tab, ph, det, q);
     else if (wpe == 6)
-        hipLaunchKernelGGL((k_phantom<MODE, BATCH, 6>), dim3(grid), dim3(block), 0, s, rp, tab, ph, det, q);
+        hipLaunchKernelGGL((k_phantom<MODE, BATCH, 6>), dim3(grid), dim3(block), lds, s, rp, tab, ph, det, q);
     else
-        hipLaunchKernelGGL((k_phantom<MODE, BATCH, 0>), dim3(grid), dim3(block), 0, s, rp, tab, ph, det, q);
+        hipLaunchKernelGGL((k_phantom<MODE, BATCH, 0>), dim3(grid), dim3(block), lds, s, rp, tab, ph, det, q);
 }
 
 __global__ void k_zero(unsigned *p, int n)
@@ -813,6 +938,8 @@ int gpet_launch_zero(unsigned *p, int n, hipStream_t s)
     hipLaunchKernelGGL(k_zero, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
     return (int)hipGetLastError();
 }
+
+int gpet_phantom_grid(int64_t items, const LaunchCfg &cfg, int n_cu) { return grid_of(items, cfg, n_cu); }
 
 int gpet_launch_phantom(const RunParams &rp, const DevTables &tab, const DevPhantom &ph, const DevDetector &det,
                         const DevQueue &q, bool batch, const LaunchCfg &cfg, int n_cu, hipStream_t s)
@@ -837,15 +964,16 @@ int gpet_launch_panel(const PanelParams &pp, const DevTables &tab, const DevDete
     if (max_items <= 0) return 0;
     if (cfg.block <= 0 || cfg.block > 256 || cfg.block % 64) return (int)hipErrorInvalidValue;
     const int grid = grid_of(max_items, cfg, n_cu);
+    const size_t lds = sizeof(DevPanel) * (size_t)det.npanels + sizeof(unsigned) * (size_t)(q.nregions + 1);
     if (mode == PANEL_HARVEST)
-        hipLaunchKernelGGL((k_panel<0, PANEL_HARVEST>), dim3(grid), dim3(cfg.block), 0, s, pp, tab, det, q);
+        hipLaunchKernelGGL((k_panel<0, PANEL_HARVEST>), dim3(grid), dim3(cfg.block), lds, s, pp, tab, det, q);
     else if (mode == PANEL_SURROGATE)
-        hipLaunchKernelGGL((k_panel<0, PANEL_SURROGATE>), dim3(grid), dim3(cfg.block), 0, s, pp, tab, det, q);
+        hipLaunchKernelGGL((k_panel<0, PANEL_SURROGATE>), dim3(grid), dim3(cfg.block), lds, s, pp, tab, det, q);
     else if (cfg.wpe == 8)
-        hipLaunchKernelGGL((k_panel<8, PANEL_PLAIN>), dim3(grid), dim3(cfg.block), 0, s, pp, tab, det, q);
+        hipLaunchKernelGGL((k_panel<8, PANEL_PLAIN>), dim3(grid), dim3(cfg.block), lds, s, pp, tab, det, q);
     else if (cfg.wpe == 6)
-        hipLaunchKernelGGL((k_panel<6, PANEL_PLAIN>), dim3(grid), dim3(cfg.block), 0, s, pp, tab, det, q);
+        hipLaunchKernelGGL((k_panel<6, PANEL_PLAIN>), dim3(grid), dim3(cfg.block), lds, s, pp, tab, det, q);
     else
-        hipLaunchKernelGGL((k_panel<0, PANEL_PLAIN>), dim3(grid), dim3(cfg.block), 0, s, pp, tab, det, q);
+        hipLaunchKernelGGL((k_panel<0, PANEL_PLAIN>), dim3(grid), dim3(cfg.block), lds, s, pp, tab, det, q);
     return (int)hipGetLastError();
 }

@@ -190,10 +190,12 @@ class RunResult:
     n_adder: int = 0
     n_hits: int = 0
     job_counts: np.ndarray = None   # batched runs: [n_jobs, 3] histories, singles, coincidences
+    psf_source: np.ndarray = None   # capi.PSF_DTYPE records (GPET_FLAG_PSF_SOURCE)
+    psf_phantom: np.ndarray = None  # capi.PSF_DTYPE records (GPET_FLAG_PSF_PHANTOM)
 
 
 def make_options(seed=1, eabs=1.0e3, noncol_sigma=0.0037056, positron_range=0, record_hits=0, batch=0,
-                 device=0, flags=0) -> capi.Options:
+                 device=0, flags=0, record_sphere=(0.0, 0.0, 0.0, 0.0)) -> capi.Options:
     o = capi.Options()
     o.device = device
     o.seed = seed
@@ -203,6 +205,8 @@ def make_options(seed=1, eabs=1.0e3, noncol_sigma=0.0037056, positron_range=0, r
     o.record_hits = record_hits
     o.batch = batch
     o.flags = flags
+    for k in range(4):
+        o.record_sphere[k] = record_sphere[k]
     return o
 
 
@@ -297,8 +301,10 @@ class Engine:
                  "get_majorant")
         return out, e0.value, idl.value
 
-    def _sink(self, singles_cap, coinc_cap, adder_cap, hits_cap, trace_cap, out_dir):
+    def _sink(self, singles_cap, coinc_cap, adder_cap, hits_cap, trace_cap, out_dir, psf_cap=0):
         bufs = {
+            "psf_source": np.zeros(psf_cap, capi.PSF_DTYPE),
+            "psf_phantom": np.zeros(psf_cap, capi.PSF_DTYPE),
             "singles": np.zeros(singles_cap, capi.EVENT_DTYPE),
             "coinc": np.zeros(2 * coinc_cap, capi.EVENT_DTYPE),
             "adder": np.zeros(adder_cap, capi.EVENT_DTYPE),
@@ -316,6 +322,8 @@ class Engine:
         s.adder, s.adder_cap = ptr(bufs["adder"]), adder_cap
         s.hit_id, s.hit_f, s.hits_cap = ptr(bufs["hit_id"]), ptr(bufs["hit_f"]), hits_cap
         s.trace, s.trace_cap = ptr(bufs["trace"]), trace_cap
+        s.psf_source, s.psf_source_cap = ptr(bufs["psf_source"]), psf_cap
+        s.psf_phantom, s.psf_phantom_cap = ptr(bufs["psf_phantom"]), psf_cap
         s.out_dir = out_dir.encode() if out_dir else None
         return s, bufs
 
@@ -327,15 +335,18 @@ class Engine:
         na = min(s.n_adder, s.adder_cap)
         nh = min(s.n_hits, s.hits_cap)
         nt = min(s.n_trace, s.trace_cap)
-        return RunResult(bufs["singles"][:ns], bufs["coinc"][: 2 * nc].reshape(-1, 2), bufs["adder"][:na],
-                         bufs["hit_id"][:nh], bufs["hit_f"][:nh], bufs["trace"][:nt], t,
-                         s.n_singles, s.n_coinc, s.n_adder, s.n_hits)
+        r = RunResult(bufs["singles"][:ns], bufs["coinc"][: 2 * nc].reshape(-1, 2), bufs["adder"][:na],
+                      bufs["hit_id"][:nh], bufs["hit_f"][:nh], bufs["trace"][:nt], t,
+                      s.n_singles, s.n_coinc, s.n_adder, s.n_hits)
+        r.psf_source = bufs["psf_source"][:min(s.n_psf_source, s.psf_source_cap)]
+        r.psf_phantom = bufs["psf_phantom"][:min(s.n_psf_phantom, s.psf_phantom_cap)]
+        return r
 
     def run_source(self, sources, isotopes, t0: float, t1: float, n_decays: int = 0, *, singles_cap=0,
-                   coinc_cap=0, adder_cap=0, hits_cap=0, trace_cap=0, out_dir=None) -> RunResult:
+                   coinc_cap=0, adder_cap=0, hits_cap=0, trace_cap=0, out_dir=None, psf_cap=0) -> RunResult:
         src = (capi.Source * len(sources))(*sources)
         iso = (capi.Isotope * len(isotopes))(*isotopes)
-        s, bufs = self._sink(singles_cap, coinc_cap, adder_cap, hits_cap, trace_cap, out_dir)
+        s, bufs = self._sink(singles_cap, coinc_cap, adder_cap, hits_cap, trace_cap, out_dir, psf_cap)
         self._ok(self.f["run_source"](self.ctx, src, len(sources), iso, len(isotopes), float(t0), float(t1),
                                       int(n_decays), C.byref(s)), "run_source")
         return self._result(s, bufs)
@@ -431,9 +442,9 @@ class Engine:
         return loss.value, g
 
     def run_psf(self, rec: np.ndarray, ptype: int, *, singles_cap=0, coinc_cap=0, adder_cap=0, hits_cap=0,
-                trace_cap=0, out_dir=None) -> RunResult:
+                trace_cap=0, out_dir=None, psf_cap=0) -> RunResult:
         rec = np.ascontiguousarray(rec, dtype=np.float64).reshape(-1, 8)
-        s, bufs = self._sink(singles_cap, coinc_cap, adder_cap, hits_cap, trace_cap, out_dir)
+        s, bufs = self._sink(singles_cap, coinc_cap, adder_cap, hits_cap, trace_cap, out_dir, psf_cap)
         self._ok(self.f["run_psf"](self.ctx, rec.ctypes.data_as(C.c_void_p), len(rec), int(ptype), C.byref(s)),
                  "run_psf")
         return self._result(s, bufs)

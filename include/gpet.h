@@ -113,6 +113,17 @@ typedef struct gpet_digitizer {
     double coinc_window_us;    /* coincidence window (0 disables the sorter)    */
     int32_t coinc_policy;      /* multiples: 0 kill all, 1 take all pairs       */
     int32_t record_rayleigh;   /* constants.h:7 RECORDRayleigh                  */
+    /* Background singles (addnoise, gPET_kernals.cu:699-735, never launched
+     * there): a Poisson stream of noise_rate singles per second over the
+     * whole scanner inside each decay run's [t0, t1], on a uniform random
+     * crystal, energy ~ N(noise_e_mean, noise_e_sigma), parn < 0.  They
+     * join the adder output before blur and go through the whole chain. */
+    double noise_rate;         /* singles per second (0: none)                  */
+    float noise_e_mean, noise_e_sigma; /* eV                                    */
+    /* Delayed-window randoms estimate: every prompt window opened at t also
+     * looks at the singles in [t + delay, t + delay + window]; the opener
+     * and those singles form pairs under the multiples policy. */
+    double delay_us;           /* 0: no delayed window                          */
 } gpet_digitizer;
 
 enum {
@@ -129,14 +140,35 @@ typedef struct gpet_options {
     int32_t record_hits;       /* constants.h:6 OUTPUTHIT                        */
     int64_t batch;             /* histories per transport launch; 0 = automatic */
     int32_t flags;             /* GPET_FLAG_* bits                              */
+    float record_sphere[4];    /* cx cy cz r (cm): with r > 0, a photon leaving
+                                  the phantom is moved along its direction onto
+                                  this sphere before the detector stage
+                                  (RECORDPSF -1: gPET_kernals.cu:148-171,288-294;
+                                  input_PET.in "record sphere" line)          */
 } gpet_options;
 
 enum {
     GPET_FLAG_NO_GRAPH = 1,    /* launch transport batches one by one instead of
                                   replaying one captured hipGraph per run       */
-    GPET_FLAG_AUTOTUNE = 2     /* time launch-shape variants on the first run of
+    GPET_FLAG_AUTOTUNE = 2,    /* time launch-shape variants on the first run of
                                   each kind and keep the fastest                */
+    GPET_FLAG_PSF_SOURCE = 4,  /* record every emitted photon (OUTPUTPSF 1,
+                                  gPET.cu:63-90)                                */
+    GPET_FLAG_PSF_PHANTOM = 8  /* record every photon leaving the phantom
+                                  (OUTPUTPSF 2, gPET.cu:91-114,296-346)         */
 };
+
+/* Phase-space record.  The reference writes three parallel files per
+ * surface: out*.dat (x y z vx vy vz E, 7 floats), id*.dat (eventid, int32),
+ * time*.dat (t, double); gpet_write_psf writes exactly those.  Records come
+ * out ordered by photon id. */
+typedef struct gpet_psf_record {
+    float x, y, z, vx, vy, vz, E;   /* cm, unit direction, eV                  */
+    int32_t parn;                   /* photon id (2h + k; h for photon PSF)     */
+    int32_t eventid;
+    int32_t pad;
+    double t;                       /* us                                      */
+} gpet_psf_record;
 
 /* Launch shape of the two transport kernels (index 0 phantom, 1 panel).
  * Results do not depend on it: every history draws from its own streams. */
@@ -216,6 +248,13 @@ typedef struct gpet_sink {
     gpet_trace *trace;   int64_t trace_cap;   int64_t n_trace;
     const char *out_dir; /* non-NULL: write singles.dat, adder.dat, coincidences.dat, Hits*.dat */
     int64_t *job_counts; /* gpet_run_batch: n_jobs x 3 {histories, singles, coincidences}, or NULL */
+    /* phase-space outputs (with GPET_FLAG_PSF_SOURCE / _PHANTOM); out_dir also
+     * gets outsource/idsource/timesource.dat and outphantom/... .dat */
+    gpet_psf_record *psf_source;  int64_t psf_source_cap;  int64_t n_psf_source;
+    gpet_psf_record *psf_phantom; int64_t psf_phantom_cap; int64_t n_psf_phantom;
+    int32_t out_append;  /* 1: append to the files in out_dir (the reference's mode, Q21) */
+    /* delayed-window pairs (2 events each; delayed.dat in out_dir) */
+    gpet_event *delayed; int64_t delayed_cap; int64_t n_delayed;
 } gpet_sink;
 
 /* Run statistics and tallies.  Arrays are caller-provided (may be NULL). */
@@ -252,6 +291,9 @@ typedef struct gpet_tallies {
     double seconds_tune;       /* autotuning time spent inside this run         */
     int64_t n_surrogate;       /* panel entries handed to the surrogate         */
     double seconds_surrogate;  /* network inference and sampling                */
+    int64_t n_noise;           /* background singles generated                  */
+    int64_t n_delayed;         /* delayed-window pairs (randoms estimate)       */
+    int64_t *sinogram_delayed; /* n_phi x n_s of the delayed pairs, or NULL     */
 } gpet_tallies;
 
 typedef struct gpet_ctx gpet_ctx;
@@ -286,7 +328,8 @@ gpet_status gpet_panel_counts(const gpet_panel *p0, int32_t *module_n, int32_t *
 /* input_PET.in: 20 label/value fields in fixed order (main.cu:50-184).
  * Optional trailing label/value pairs extend it (all absent = reference
  * behaviour): time blur FWHM in us; coincidence window in us and multiples
- * policy; Philox seed. */
+ * policy; Philox seed; background singles rate (1/s), energy mean and sigma
+ * (eV); delayed-window offset in us. */
 #define GPET_PATH_MAX 256
 typedef struct gpet_input {
     int32_t device;
@@ -314,6 +357,9 @@ typedef struct gpet_input {
     double time_blur_fwhm_us, coinc_window_us;
     int32_t coinc_policy;
     uint64_t seed;
+    double noise_rate;
+    float noise_e_mean, noise_e_sigma;
+    double delay_us;
 } gpet_input;
 
 gpet_status gpet_read_input(const char *path, gpet_input *out);
@@ -326,6 +372,10 @@ void gpet_input_digitizer(const gpet_input *in, gpet_digitizer *out);
 gpet_status gpet_write_events(const char *path, const gpet_event *ev, int64_t n, int32_t append);
 gpet_status gpet_write_hits(const char *id_path, const char *f_path, const int32_t *hit_id,
                             const float *hit_f, int64_t n, int32_t append);
+/* The reference's three PSF files (gPET.cu:69-88): <dir>/out<name>.dat,
+ * id<name>.dat, time<name>.dat, name = "source" or "phantom". */
+gpet_status gpet_write_psf(const char *dir, const char *name, const gpet_psf_record *rec, int64_t n,
+                           int32_t append);
 
 const char *gpet_status_string(gpet_status s);
 

@@ -137,6 +137,7 @@ struct HitRec {
 };
 
 struct Result {
+    std::vector<gpet_psf_record> psf[2];
     std::vector<int64_t> job_first;   /* batched runs: first history of each job */
     std::vector<int64_t> job_counts;  /* batched runs: per job {histories, singles, pairs} */
     std::vector<gpet_event> adder, singles, coinc;
@@ -450,6 +451,8 @@ struct PhotonOut {
     gpet_event ev[MAX_PH_EVENTS];
     int nev = 0;
     std::vector<HitRec> hits;
+    gpet_psf_record psf[2];    /* source, phantom exit                     */
+    bool has_psf[2] = {false, false};
 };
 
 /* photon (gPET_kernals.cu:256-345): Woodcock tracking in the phantom.
@@ -627,7 +630,36 @@ void photon_full(const gpet_ctx &C, const Phantom *ph, int64_t hist, int k, real
         tr->panel = -1;
     }
     bool alive = true;
+    auto record = [&](int which) {
+        gpet_psf_record &p = out.psf[which];
+        p.x = (float)x; p.y = (float)y; p.z = (float)z; p.vx = (float)vx; p.vy = (float)vy; p.vz = (float)vz;
+        p.E = (float)E; p.parn = parn; p.eventid = (int32_t)hist; p.pad = 0; p.t = t;
+        out.has_psf[which] = true;
+    };
+    if (C.opt.flags & GPET_FLAG_PSF_SOURCE) record(0);
     if (ph) alive = phantom_transport(C, *ph, x, y, z, vx, vy, vz, E, t, r, tr, cnt);
+    if (alive && C.opt.record_sphere[3] > 0.0f) {
+        /* getDistance (gPET_kernals.cu:148-171, 288-294): move onto the
+         * recording sphere; far root from inside, near root when
+         * approaching from outside, the receding root otherwise */
+        const float *sp = C.opt.record_sphere;
+        const real cx = x - sp[0], cy = y - sp[1], cz = z - sp[2];
+        const real a = vx * vx + vy * vy + vz * vz;
+        const real b = 2.0f * (vx * cx + vy * cy + vz * cz);
+        const real c = (cx * cx + cy * cy + cz * cz) - sp[3] * sp[3];
+        const real disc = b * b - 4.0f * a * c;
+        real sd = 0.0f;
+        if (!(disc < 0.0f)) {
+            if (c < 0.0f) sd = (-b + std::sqrt(disc)) / (2.0f * a);
+            else if (b < 0.0f) sd = (-b - std::sqrt(disc)) / (2.0f * a);
+            else sd = (-b + std::sqrt(disc)) / (2.0f * a);
+        }
+        x = x + sd * vx;
+        y = y + sd * vy;
+        z = z + sd * vz;
+        t = t + (double)sd * INV_C;
+    }
+    if (alive && (C.opt.flags & GPET_FLAG_PSF_PHANTOM)) record(1);
     if (tr) {
         tr->status |= alive ? GPET_TR_ESCAPED : GPET_TR_ABSORBED;
         tr->x = x; tr->y = y; tr->z = z; tr->vx = vx; tr->vy = vy; tr->vz = vz; tr->E = E; tr->t = t;
@@ -1174,6 +1206,8 @@ void run_histories(gpet_ctx &C, int64_t n, F hist_fn, gpet_sink *sink, Result &R
 #endif
     std::vector<std::vector<gpet_event>> tev(nthreads);
     std::vector<std::vector<HitRec>> thit(nthreads);
+    std::vector<std::vector<gpet_psf_record>> tpsf[2] = {std::vector<std::vector<gpet_psf_record>>(nthreads),
+                                                         std::vector<std::vector<gpet_psf_record>>(nthreads)};
     std::vector<Counters> tcnt(nthreads);
     std::vector<gpet_trace> trace;
     if (want_trace) trace.assign((size_t)std::min<int64_t>(2 * n, sink->trace_cap), gpet_trace());
@@ -1194,6 +1228,8 @@ void run_histories(gpet_ctx &C, int64_t n, F hist_fn, gpet_sink *sink, Result &R
         for (int k = 0; k < 2; ++k) {
             for (int i = 0; i < out[k].nev; ++i) tev[tid].push_back(out[k].ev[i]);
             for (auto &hr : out[k].hits) thit[tid].push_back(hr);
+            for (int w = 0; w < 2; ++w)
+                if (out[k].has_psf[w]) tpsf[w][tid].push_back(out[k].psf[w]);
         }
         if (tracing) {
             trace[2 * h] = tr2[0];
@@ -1203,9 +1239,13 @@ void run_histories(gpet_ctx &C, int64_t n, F hist_fn, gpet_sink *sink, Result &R
     for (int t = 0; t < nthreads; ++t) {
         R.adder.insert(R.adder.end(), tev[t].begin(), tev[t].end());
         R.hits.insert(R.hits.end(), thit[t].begin(), thit[t].end());
+        for (int w = 0; w < 2; ++w) R.psf[w].insert(R.psf[w].end(), tpsf[w][t].begin(), tpsf[w][t].end());
         R.c.add(tcnt[t]);
     }
     std::sort(R.adder.begin(), R.adder.end(), ev_less_id);
+    for (int w = 0; w < 2; ++w)
+        std::sort(R.psf[w].begin(), R.psf[w].end(),
+                  [](const gpet_psf_record &a, const gpet_psf_record &b) { return a.parn < b.parn; });
     /* per photon, hits stay in the order they happened */
     std::stable_sort(R.hits.begin(), R.hits.end(), [](const HitRec &a, const HitRec &b) { return a.id[0] < b.id[0]; });
     R.trace.swap(trace);
@@ -1241,6 +1281,8 @@ void fill_sink(const Result &R, gpet_sink *s)
         }
     }
     copy_out(R.trace, s->trace, s->trace_cap, s->n_trace);
+    copy_out(R.psf[0], s->psf_source, s->psf_source_cap, s->n_psf_source);
+    copy_out(R.psf[1], s->psf_phantom, s->psf_phantom_cap, s->n_psf_phantom);
     if (s->job_counts && !R.job_counts.empty())
         memcpy(s->job_counts, R.job_counts.data(), sizeof(int64_t) * R.job_counts.size());
 }

@@ -133,3 +133,23 @@ def test_psf_parity(tables, ptype):
                     lambda e: e.run_psf(rec, ptype, singles_cap=n, coinc_cap=n, adder_cap=4 * n, trace_cap=2 * n))
     assert a.tallies.n_adder > 10
     assert_parity(a, b)
+
+
+@pytest.mark.parametrize("sphere", [(0.0, 0.0, 0.0, 0.0), (0.05, -0.02, 0.0, 40.0)])
+def test_phase_space_parity(tables, sphere):
+    """Source and phantom-exit phase-space records (OUTPUTPSF 1/2) and the
+    recording sphere (RECORDPSF -1)."""
+    from gpet_amd import capi, engine
+    from gpet_amd import workloads as W
+
+    opt = engine.make_options(seed=21, positron_range=1, flags=capi.FLAG_PSF_SOURCE | capi.FLAG_PSF_PHANTOM,
+                              record_sphere=sphere)
+    src, iso, t0, t1 = W.workload_sources(2)
+    n = 20000
+    a, b = run_both(tables, opt, lambda e: W.setup_engine(e, tables, config=2, phantom_n=100),
+                    lambda e: e.run_source(src, iso, t0, t1, n_decays=n, singles_cap=n, coinc_cap=n,
+                                           adder_cap=4 * n, trace_cap=2 * n, psf_cap=2 * n))
+    assert len(a.psf_source) > 30000 and len(a.psf_phantom) > 20000
+    assert same_bytes(a.psf_source, b.psf_source)
+    assert same_bytes(a.psf_phantom, b.psf_phantom)
+    assert_parity(a, b)
