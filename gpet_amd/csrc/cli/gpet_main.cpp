@@ -140,12 +140,14 @@ int main(int argc, char **argv)
     gpet_get_tallies(c, &t);
     printf("{\"histories\": %lld, \"photons\": %lld, \"panel_entries\": %lld, \"hits\": %lld, \"adder\": %lld, "
            "\"threshold\": %lld, \"deadtime\": %lld, \"singles\": %lld, \"coincidences\": %lld, \"true_coincidences\": %lld, "
-           "\"dropped_events\": %lld, \"majorant_violations\": %lld, \"seconds_transport\": %.6f, "
+           "\"dropped_events\": %lld, \"majorant_violations\": %lld, \"background_singles\": %lld, "
+           "\"delayed_coincidences\": %lld, \"seconds_transport\": %.6f, "
            "\"seconds_digitizer\": %.6f, \"seconds_wall\": %.6f}\n",
            (long long)t.n_histories, (long long)t.n_photons, (long long)t.n_panel_entries, (long long)t.n_hits,
            (long long)t.n_adder, (long long)t.n_threshold, (long long)t.n_deadtime, (long long)t.n_singles,
            (long long)t.n_coinc, (long long)t.n_coinc_true, (long long)t.dropped_events,
-           (long long)t.majorant_violations, t.seconds_transport, t.seconds_digitizer,
+           (long long)t.majorant_violations, (long long)t.n_noise, (long long)t.n_delayed, t.seconds_transport,
+           t.seconds_digitizer,
            std::chrono::duration<double>(t1 - t0).count());
     gpet_destroy(c);
     return EXIT_SUCCESS;

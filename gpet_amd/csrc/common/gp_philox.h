@@ -56,7 +56,10 @@ GP_HD gp_u4 gp_philox4x32_10(gp_u4 c, uint32_t k0, uint32_t k1)
 enum {
     GP_TAG_DECAY = 0,     /* source, decay time, positron, pair direction */
     GP_TAG_PHOTON = 1,    /* aux = 1 + photon index (0/1) -> 1, 2          */
-    GP_TAG_DIGI = 4       /* digitizer: aux = 4 | (siten << 4), id = parn  */
+    GP_TAG_DIGI = 4,      /* digitizer: aux = 4 | (siten << 4), id = parn  */
+    GP_TAG_NOISE_T = 5,   /* background arrival gaps: aux = 5 | (job << 4), id = chunk */
+    GP_TAG_NOISE_A = 6    /* their crystal and energy: aux = 6 | (job << 4), id = chunk,
+                             draws 8 m .. 8 m + 7 for the chunk's m-th single  */
 };
 
 /* One stream: 32-bit draw counter plus the 4-word block it is reading. */

@@ -174,11 +174,14 @@ gpet_status check_ready(gpet_ctx *c)
 
 gpet_status write_outputs(gpet_ctx *c, const std::string &dir, const std::vector<gpet_event> &adder,
                           const std::vector<gpet_event> &singles, const std::vector<gpet_event> &coinc,
-                          const std::vector<int32_t> &hid, const std::vector<float> &hf, int32_t app)
+                          const std::vector<int32_t> &hid, const std::vector<float> &hf,
+                          const std::vector<gpet_event> &delayed, int32_t app)
 {
     gpet_status st = gpet_write_events((dir + "/adder.dat").c_str(), adder.data(), (int64_t)adder.size(), app);
     if (!st) st = gpet_write_events((dir + "/singles.dat").c_str(), singles.data(), (int64_t)singles.size(), app);
     if (!st) st = gpet_write_events((dir + "/coincidences.dat").c_str(), coinc.data(), (int64_t)coinc.size(), app);
+    if (!st && c->dig.delay_us > 0.0)
+        st = gpet_write_events((dir + "/delayed.dat").c_str(), delayed.data(), (int64_t)delayed.size(), app);
     if (!st && c->opt.record_hits)
         st = gpet_write_hits((dir + "/HitsID.dat").c_str(), (dir + "/Hits.dat").c_str(), hid.data(), hf.data(),
                              (int64_t)hid.size() / 5, app);
@@ -208,6 +211,13 @@ struct BatchInfo {
     std::vector<int64_t> first;      /* [nsrc + 1]  */
     std::vector<int> src_phantom;    /* [nsrc]      */
     std::vector<int64_t> job_first;  /* [njobs + 1] */
+};
+
+/* Acquisition windows (s) and first history of each job: where background
+ * singles are generated. */
+struct NoiseWindows {
+    std::vector<double> t0, t1;
+    std::vector<int64_t> first;
 };
 
 /* Segments of the launch covering histories [h0, h0 + nh): maximal runs of
@@ -515,10 +525,13 @@ gpet_status autotune(gpet_ctx *c, const RunParams &rp0, int64_t n_hist, const Pa
     return GPET_OK;
 }
 
-gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sink, const BatchInfo *bi)
+gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sink, const BatchInfo *bi,
+                       const NoiseWindows *nw)
 {
     gpet_status st;
     if (n_hist > GPET_MAX_HISTORIES) return fail(c, GPET_ERR_CAPACITY, "more than 2^30 histories in one run");
+    if (c->dig.noise_rate > 0.0 && !nw)
+        return fail(c, GPET_ERR_UNSUPPORTED, "background singles need a decay run's acquisition window");
     const bool batched = bi != nullptr;
     c->last_surr_items = 0;
     c->last_surr_seconds = 0;
@@ -712,6 +725,37 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
         c->surr.set_n += std::min<int64_t>((int64_t)nh, pp.harvest_cap);
     }
 
+    /* background singles join the adder output before blur */
+    int64_t n_noise = 0;
+    if (c->dig.noise_rate > 0.0 && nw && c->has_det) {
+        NoiseParams np;
+        memset(&np, 0, sizeof(np));
+        std::vector<NoiseJob> nj;
+        gpet_noise_layout(c->dig.noise_rate, nw->t0.data(), nw->t1.data(), nw->first.data(), (int)nw->t0.size(), nj,
+                          np.gap_us, np.chunk_us, np.nchunks);
+        np.seed = c->opt.seed;
+        np.e_mean = c->dig.noise_e_mean;
+        np.e_sigma = c->dig.noise_e_sigma;
+        np.jobs = nj.data();
+        np.njobs = (int)nj.size();
+        np.panels = c->ddet.p;
+        np.npanels = c->ddet.npanels;
+        np.moduleN = c->ddet.moduleN;
+        np.crystalN = c->ddet.crystalN;
+        np.moduleNy = c->ddet.moduleNy;
+        np.crystalNy = c->ddet.crystalNy;
+        np.depth = c->dig.readout_depth;
+        np.policy = c->dig.readout_policy;
+        char nerr[256] = {0};
+        if (gpet_digi_noise_count(c->dw, np, &n_noise, c->stream, nerr, sizeof(nerr)))
+            return fail(c, GPET_ERR_CAPACITY, std::string("background singles: ") + nerr);
+        if (n_noise + c->n_events > (int64_t)INT32_MAX)
+            return fail(c, GPET_ERR_CAPACITY, "more than 2^31 events with background singles");
+        if ((st = dgrow(c, c->d_events, c->n_events + n_noise, c->n_events))) return st;
+        if (gpet_digi_noise_write(c->dw, np, c->d_events.p + c->n_events, c->stream, nerr, sizeof(nerr)))
+            return fail(c, GPET_ERR_DEVICE, std::string("background singles: ") + nerr);
+    }
+
     DigiParams dp;
     memset(&dp, 0, sizeof(dp));
     dp.seed = c->opt.seed;
@@ -727,7 +771,7 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
         dp.job_bounds = c->d_job_cnt.p;
     }
     char err[256] = {0};
-    if (gpet_digitize(c->dw, dp, c->d_events.p, c->n_events, &c->dout, c->stream, err, sizeof(err)))
+    if (gpet_digitize(c->dw, dp, c->d_events.p, c->n_events + n_noise, &c->dout, c->stream, err, sizeof(err)))
         return fail(c, GPET_ERR_DEVICE, std::string("digitizer: ") + err);
     HIPOK(c, hipEventRecord(c->ev2, c->stream));
     HIPOK(c, hipEventSynchronize(c->ev2));
@@ -774,6 +818,8 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
     T.seconds_tune = tune_s;
     T.n_surrogate = c->last_surr_items;
     T.seconds_surrogate = c->last_surr_seconds;
+    T.n_noise = n_noise;
+    T.n_delayed = c->dout.n_delayed;
 
     if (!sink) return GPET_OK;
     if (batched && sink->job_counts) {
@@ -786,12 +832,13 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
         }
     }
     const bool to_files = sink->out_dir && sink->out_dir[0];
-    std::vector<gpet_event> adder, singles, coinc;
+    std::vector<gpet_event> adder, singles, coinc, delayed;
     std::vector<int32_t> hid;
     std::vector<float> hf;
     sink->n_adder = c->n_events;
     sink->n_singles = c->dout.n_singles;
     sink->n_coinc = c->dout.n_coinc;
+    sink->n_delayed = c->dout.n_delayed;
     sink->n_hits = std::min<int64_t>(small[2], hits_cap);
     if ((sink->adder && sink->adder_cap > 0) || to_files) {
         adder.resize((size_t)c->n_events);
@@ -814,6 +861,16 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
         if (m) HIPOK(c, hipMemcpy(coinc.data(), c->dout.coinc, sizeof(gpet_event) * (size_t)(2 * m), hipMemcpyDeviceToHost));
         if (sink->coinc)
             memcpy(sink->coinc, coinc.data(), sizeof(gpet_event) * (size_t)(2 * std::min<int64_t>(sink->coinc_cap, m)));
+    }
+    if ((sink->delayed && sink->delayed_cap > 0) || to_files) {
+        int64_t m = to_files ? c->dout.n_delayed : std::min<int64_t>(sink->delayed_cap, c->dout.n_delayed);
+        delayed.resize((size_t)(2 * m));
+        if (m)
+            HIPOK(c, hipMemcpy(delayed.data(), c->dout.delayed, sizeof(gpet_event) * (size_t)(2 * m),
+                               hipMemcpyDeviceToHost));
+        if (sink->delayed)
+            memcpy(sink->delayed, delayed.data(),
+                   sizeof(gpet_event) * (size_t)(2 * std::min<int64_t>(sink->delayed_cap, m)));
     }
     if (want_hits && sink->n_hits > 0 && ((sink->hit_id && sink->hit_f) || to_files)) {
         hid.resize((size_t)(5 * sink->n_hits));
@@ -855,7 +912,7 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
             }
         }
     }
-    if (to_files) return write_outputs(c, sink->out_dir, adder, singles, coinc, hid, hf, sink->out_append);
+    if (to_files) return write_outputs(c, sink->out_dir, adder, singles, coinc, hid, hf, delayed, sink->out_append);
     return GPET_OK;
 }
 
@@ -1117,7 +1174,11 @@ gpet_status gpet_run_source(gpet_ctx *c, const gpet_source *src, int32_t ns, con
     rp.src = c->d_src.p;
     rp.iso = c->d_iso.p;
     rp.psf = nullptr;
-    return run_common(c, rp, first[ns], sink, nullptr);
+    NoiseWindows nw;
+    nw.t0.assign(1, t0);
+    nw.t1.assign(1, t1);
+    nw.first.assign(1, 0);
+    return run_common(c, rp, first[ns], sink, nullptr, &nw);
 }
 
 gpet_status gpet_run_batch(gpet_ctx *c, const gpet_job *jobs, int32_t njobs, const gpet_source *src, int32_t nsrc,
@@ -1166,7 +1227,13 @@ gpet_status gpet_run_batch(gpet_ctx *c, const gpet_job *jobs, int32_t njobs, con
     rp.src = c->d_src.p;
     rp.iso = c->d_iso.p;
     rp.psf = nullptr;
-    return run_common(c, rp, bi.first[ns], sink, &bi);
+    NoiseWindows nw;
+    for (int j = 0; j < njobs; ++j) {
+        nw.t0.push_back(jobs[j].t0);
+        nw.t1.push_back(jobs[j].t1);
+        nw.first.push_back(bi.job_first[j]);
+    }
+    return run_common(c, rp, bi.first[ns], sink, &bi, &nw);
 }
 
 gpet_status gpet_set_launch(gpet_ctx *c, const gpet_launch *l)
@@ -1196,19 +1263,20 @@ gpet_status gpet_run_psf(gpet_ctx *c, const double *rec8, int64_t n, int32_t pty
     memset(&rp, 0, sizeof(rp));
     rp.mode = ptype == 0 ? RUN_PSF_POSITRON : RUN_PSF_PHOTON;
     rp.psf = c->d_psf.p;
-    return run_common(c, rp, n, sink, nullptr);
+    return run_common(c, rp, n, sink, nullptr, nullptr);
 }
 
 gpet_status gpet_get_tallies(gpet_ctx *c, gpet_tallies *t)
 {
     if (!c || !t) return GPET_ERR_ARG;
-    int64_t *cs = t->crystal_singles, *eh = t->energy_hist, *sg = t->sinogram;
+    int64_t *cs = t->crystal_singles, *eh = t->energy_hist, *sg = t->sinogram, *sgd = t->sinogram_delayed;
     int32_t nc = t->n_crystals, nb = t->n_ebins, nsb = t->n_s, nph = t->n_phi;
     float elo = t->e_lo, ehi = t->e_hi, smax = t->s_max;
     *t = c->last;
     t->crystal_singles = cs; t->n_crystals = nc;
     t->energy_hist = eh; t->n_ebins = nb; t->e_lo = elo; t->e_hi = ehi;
     t->sinogram = sg; t->n_s = nsb; t->n_phi = nph; t->s_max = smax;
+    t->sinogram_delayed = sgd;
     if ((cs && nc > 0) || (eh && nb > 0)) {
         HIPOK(c, hipSetDevice(c->device));
         DBuf<unsigned long long> dc, dh;
@@ -1230,16 +1298,19 @@ gpet_status gpet_get_tallies(gpet_ctx *c, gpet_tallies *t)
         dfree(dc);
         dfree(dh);
     }
-    if (sg && nsb > 0 && nph > 0 && smax > 0.0f && c->has_det) {
+    for (int k = 0; k < 2; ++k) {
+        int64_t *dst = k == 0 ? sg : sgd;
+        if (!(dst && nsb > 0 && nph > 0 && smax > 0.0f && c->has_det)) continue;
         DBuf<unsigned long long> ds;
         gpet_status st;
         const int64_t nb = (int64_t)nsb * nph;
         if ((st = dalloc(c, ds, nb))) return st;
         HIPOK(c, hipMemsetAsync(ds.p, 0, sizeof(unsigned long long) * nb, c->stream));
-        int e = gpet_digi_sinogram(c->dout.coinc, c->dout.n_coinc, c->ddet.p, c->ddet.npanels, c->ddet.moduleNy,
-                                   c->ddet.crystalNy, ds.p, nsb, nph, smax, c->stream);
+        int e = gpet_digi_sinogram(k == 0 ? c->dout.coinc : c->dout.delayed, k == 0 ? c->dout.n_coinc : c->dout.n_delayed,
+                                   c->ddet.p, c->ddet.npanels, c->ddet.moduleNy, c->ddet.crystalNy, ds.p, nsb, nph,
+                                   smax, c->stream);
         if (e) return fail(c, GPET_ERR_DEVICE, "sinogram kernel");
-        HIPOK(c, hipMemcpyAsync(sg, ds.p, sizeof(int64_t) * nb, hipMemcpyDeviceToHost, c->stream));
+        HIPOK(c, hipMemcpyAsync(dst, ds.p, sizeof(int64_t) * nb, hipMemcpyDeviceToHost, c->stream));
         HIPOK(c, hipStreamSynchronize(c->stream));
         dfree(ds);
     }

@@ -17,6 +17,7 @@
 #define GPET_INTERNAL_H
 
 #include <stdint.h>
+#include <vector>
 #include "../../../include/gpet.h"
 
 #define GPET_MAX_PH_EVENTS 4   /* per-photon adder list (reference: events[4]) */
@@ -247,12 +248,44 @@ struct DigiOut {
     gpet_event *coinc;         /* device, 2 per pair                          */
     int64_t n_coinc, n_coinc_true;
     int64_t n_threshold, n_deadtime;
+    gpet_event *delayed;       /* device, 2 per delayed-window pair           */
+    int64_t n_delayed;
 };
+
+/* Background singles.  Job j's window [t0_us, t1_us) is cut into chunks of
+ * chunk_us (the last one ends at t1_us); chunk c of job j draws exponential
+ * gaps of mean gap_us from its own Philox stream and keeps the arrivals
+ * before its end, at most NOISE_CHUNK_MAX.  Singles come out in (job, chunk,
+ * arrival) order; the k-th has parn = -1 - k and eventid = the job's first
+ * history (so job lookups by eventid place it in its job). */
+#define NOISE_PER_CHUNK 64.0
+#define NOISE_CHUNK_MAX 4096
+struct NoiseJob {
+    double t0_us, t1_us;
+    int64_t chunk0;            /* first global chunk of the job               */
+    int32_t eventid, pad;
+};
+struct NoiseParams {
+    uint64_t seed;
+    double gap_us, chunk_us;
+    float e_mean, e_sigma;
+    const NoiseJob *jobs;      /* host array, njobs entries                   */
+    int njobs;
+    int64_t nchunks;
+    const DevPanel *panels;    /* device                                      */
+    int npanels, moduleN, crystalN, moduleNy, crystalNy, depth, policy;
+};
+/* Job windows and chunk layout for a rate (singles/s) over windows in s. */
+void gpet_noise_layout(double rate, const double *t0_s, const double *t1_s, const int64_t *first_hist, int njobs,
+                       std::vector<NoiseJob> &jobs, double &gap_us, double &chunk_us, int64_t &nchunks);
 struct DigiWork;               /* scratch owned by the context                */
 DigiWork *gpet_digi_work_create();
 void gpet_digi_work_destroy(DigiWork *w);
 int gpet_digitize(DigiWork *w, const DigiParams &p, const gpet_event *events, int64_t n,
                   DigiOut *out, hipStream_t s, char *err, int errlen);
+/* Count (into *total, host) then write the background singles. */
+int gpet_digi_noise_count(DigiWork *w, const NoiseParams &p, int64_t *total, hipStream_t s, char *err, int errlen);
+int gpet_digi_noise_write(DigiWork *w, const NoiseParams &p, gpet_event *out, hipStream_t s, char *err, int errlen);
 int gpet_digi_sinogram(const gpet_event *pairs, int64_t npairs, const DevPanel *panels, int npanels, int moduleNy,
                        int crystalNy, unsigned long long *sino, int n_s, int n_phi, float s_max, hipStream_t s);
 int gpet_digi_tallies(DigiWork *w, const gpet_event *singles, int64_t n, int moduleN, int crystalN,

@@ -21,6 +21,7 @@
  */
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <vector>
 #include "../common/gp_math.h"
@@ -388,7 +389,7 @@ __global__ __launch_bounds__(256) void k_coinc(const gpet_event *s, int64_t n, d
                     if (WRITE) {
                         out[2 * (int64_t)(o + np)] = s[p];
                         out[2 * (int64_t)(o + np) + 1] = s[q];
-                        if (s[p].eventid == s[q].eventid) nt++;
+                        if (s[p].eventid == s[q].eventid && s[p].parn >= 0 && s[q].parn >= 0) nt++;
                     }
                     np++;
                 }
@@ -419,14 +420,21 @@ __global__ __launch_bounds__(256) void k_tally(const gpet_event *s, int64_t n, i
 
 /* Sinogram of coincidence pairs: same rule as the oracle (crystal centres,
  * transaxial line as (phi, s), uniform bins). */
-__device__ __forceinline__ void d_crystal_centre(const DevPanel &P, int modn, int cryn, int moduleNy, int crystalNy,
-                                                 float &gx, float &gy)
+__device__ __forceinline__ void d_crystal_local(const DevPanel &P, int modn, int cryn, int moduleNy, int crystalNy,
+                                                float &xl, float &yl, float &zl)
 {
     int my = modn % moduleNy, mz = modn / moduleNy;
     int ly = cryn % crystalNy, lz = cryn / crystalNy;
-    float xl = (P.dirx * P.lenx) * 0.5f;
-    float yl = (-P.leny * 0.5f + (float)my * (P.mody + P.mspy)) + ((float)ly * (P.cry + P.cspy) + P.cry * 0.5f);
-    float zl = (-P.lenz * 0.5f + (float)mz * (P.modz + P.mspz)) + ((float)lz * (P.crz + P.cspz) + P.crz * 0.5f);
+    xl = (P.dirx * P.lenx) * 0.5f;
+    yl = (-P.leny * 0.5f + (float)my * (P.mody + P.mspy)) + ((float)ly * (P.cry + P.cspy) + P.cry * 0.5f);
+    zl = (-P.lenz * 0.5f + (float)mz * (P.modz + P.mspz)) + ((float)lz * (P.crz + P.cspz) + P.crz * 0.5f);
+}
+
+__device__ __forceinline__ void d_crystal_centre(const DevPanel &P, int modn, int cryn, int moduleNy, int crystalNy,
+                                                 float &gx, float &gy)
+{
+    float xl, yl, zl;
+    d_crystal_local(P, modn, cryn, moduleNy, crystalNy, xl, yl, zl);
     gx = P.ox + xl * P.ux[0] + yl * P.uy[0] + zl * P.uz[0];
     gy = P.oy + xl * P.ux[1] + yl * P.uy[1] + zl * P.uz[1];
 }
@@ -458,6 +466,140 @@ __global__ __launch_bounds__(256) void k_sinogram(const gpet_event *pairs, int64
     atomicAdd(&sino[(int64_t)ip * n_s + (int)fs], 1ull);
 }
 
+/* ---- background singles (addnoise, gPET_kernals.cu:699-735) -------------
+ * The reference's kernel (never launched) gives each thread a time interval,
+ * draws exponential gaps of mean lambda with curand, an energy
+ * Emean + sigma N(0,1), a uniform panel, module and crystal, placeholder
+ * positions, parn -1, and appends in groups of six through an atomic.  Here
+ * a lane owns one chunk of a job's window and draws from Philox streams keyed
+ * by (job, chunk); a count pass and a write pass at scanned offsets give a
+ * fixed output order.  Positions are the crystal centre in panel-local
+ * coordinates; the site index is the one the readout gives that crystal. */
+struct NoiseDev {
+    uint64_t seed;
+    double gap_us, chunk_us;
+    float e_mean, e_sigma;
+    const NoiseJob *jobs;
+    int njobs;
+    int64_t nchunks;
+    const DevPanel *panels;
+    int npanels, moduleN, crystalN, moduleNy, crystalNy, depth, policy;
+};
+
+__device__ __forceinline__ int d_readout_site(int depth, int policy, int pann, int modn, int cryn, int moduleN,
+                                              int crystalN)
+{
+    if (depth == 3) return pann * moduleN * crystalN + modn * crystalN + cryn;
+    if (policy == 1) depth = 2;
+    if (depth == 0) return 0;
+    if (depth == 1) return pann;
+    return pann * moduleN + modn;
+}
+
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_noise(NoiseDev p, unsigned *cnt, const unsigned *pos, gpet_event *out)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= p.nchunks) return;
+    int lo = 0, hi = p.njobs;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (p.jobs[mid].chunk0 <= c) lo = mid; else hi = mid;
+    }
+    const NoiseJob J = p.jobs[lo];
+    const int64_t lc = c - J.chunk0;
+    const int64_t nch = (lo + 1 < p.njobs ? p.jobs[lo + 1].chunk0 : p.nchunks) - J.chunk0;
+    double t = J.t0_us + (double)lc * p.chunk_us;
+    const double end = (lc + 1 == nch) ? J.t1_us : J.t0_us + (double)(lc + 1) * p.chunk_us;
+    gp_rng rt;
+    gp_rng_init(&rt, p.seed, (uint64_t)lc, (uint32_t)GP_TAG_NOISE_T | ((uint32_t)lo << 4), 0u);
+    const unsigned base = WRITE ? pos[c] : 0u;
+    unsigned m = 0;
+    while (m < (unsigned)NOISE_CHUNK_MAX) {
+        const double u = gp_rng_uniform_d(&rt);
+        t = t + (-gp_log_d(u)) * p.gap_us;
+        if (!(t < end)) break;
+        if (WRITE) {
+            gp_rng ra;
+            gp_rng_init(&ra, p.seed, (uint64_t)lc, (uint32_t)GP_TAG_NOISE_A | ((uint32_t)lo << 4), 8u * m);
+            const gp_u4 b0 = gp_rng_block(&ra), b1 = gp_rng_block(&ra);
+            gpet_event e;
+            e.pann = (int)(((uint64_t)b0.z * (uint64_t)p.npanels) >> 32);
+            e.modn = (int)(((uint64_t)b0.w * (uint64_t)p.moduleN) >> 32);
+            e.cryn = (int)(((uint64_t)b1.x * (uint64_t)p.crystalN) >> 32);
+            e.siten = d_readout_site(p.depth, p.policy, e.pann, e.modn, e.cryn, p.moduleN, p.crystalN);
+            e.parn = -1 - (int)(base + m);
+            e.eventid = J.eventid;
+            e.t = t;
+            e.E = fmaxf(p.e_mean + p.e_sigma * gp_normal(gp_u01(b0.x), gp_u01(b0.y)), 1.0f);
+            d_crystal_local(p.panels[e.pann], e.modn, e.cryn, p.moduleNy, p.crystalNy, e.x, e.y, e.z);
+            out[base + m] = e;
+        }
+        ++m;
+    }
+    if (!WRITE) cnt[c] = m;
+}
+
+/* ---- delayed-window randoms estimate ------------------------------------
+ * The prompt windows of k_coinc are opened by the same singles; a window
+ * opened at t_a also collects the singles of its job with
+ * delay <= t - t_a <= delay + W.  With exactly one such single the opener and
+ * it are a delayed pair; with more, the multiples policy decides as for a
+ * prompt window holding the opener and those singles. */
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_delayed(const gpet_event *s, int64_t n, double W, double D, int policy,
+                                                 unsigned *cnt, const unsigned *pos, gpet_event *out,
+                                                 const int64_t *jf, int nj)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    auto same_job = [&](int64_t a, int64_t b) {
+        return nj <= 1 || d_job_of(jf, nj, s[a].eventid) == d_job_of(jf, nj, s[b].eventid);
+    };
+    bool start = (i == 0) || (s[i].t - s[i - 1].t > W) || !same_job(i, i - 1);
+    if (!start) {
+        if (!WRITE) cnt[i] = 0;
+        return;
+    }
+    const double DW = D + W;
+    unsigned np = 0;
+    const unsigned o = WRITE ? pos[i] : 0u;
+    int64_t a = i;
+    while (a < n) {
+        /* first single of a's job at or after t_a + D: galloping, then bisection */
+        auto before = [&](int64_t j) { return j < n && same_job(j, a) && s[j].t - s[a].t < D; };
+        int64_t lo = a, hi = a + 1, step = 1;
+        while (before(hi)) {
+            lo = hi;
+            step *= 2;
+            hi = a + step < n ? a + step : n;
+        }
+        while (hi - lo > 1) {
+            int64_t mid = lo + ((hi - lo) >> 1);
+            if (before(mid)) lo = mid; else hi = mid;
+        }
+        int64_t e = hi;
+        while (e < n && same_job(e, a) && s[e].t - s[a].t <= DW) ++e;
+        const int64_t m = e - hi;
+        if (m == 1 || (m > 1 && policy == GPET_COINC_TAKE_ALL_GOODS)) {
+            for (int64_t p = -1; p < m; ++p)
+                for (int64_t q = p + 1; q < m; ++q) {
+                    if (WRITE) {
+                        out[2 * (int64_t)(o + np)] = s[p < 0 ? a : hi + p];
+                        out[2 * (int64_t)(o + np) + 1] = s[hi + q];
+                    }
+                    np++;
+                }
+        }
+        /* next prompt window opener, as in k_coinc */
+        int64_t b = a + 1;
+        while (b < n && s[b].t - s[a].t <= W && same_job(b, a)) ++b;
+        a = b;
+        if (a >= n || s[a].t - s[a - 1].t > W || !same_job(a, a - 1)) break;
+    }
+    if (!WRITE) cnt[i] = np;
+}
+
 inline int grid_for(int64_t n, int block) { return (int)((n + block - 1) / block); }
 
 } // namespace
@@ -471,9 +613,13 @@ struct DigiWork {
     int64_t cap = 0;
     unsigned *tile_sums = nullptr, *hist = nullptr, *grand = nullptr;
     int64_t tile_cap = 0, hist_cap = 0;
-    gpet_event *singles = nullptr, *coinc = nullptr;
-    int64_t singles_cap = 0, coinc_cap = 0;
+    gpet_event *singles = nullptr, *coinc = nullptr, *delayed = nullptr;
+    int64_t singles_cap = 0, coinc_cap = 0, delayed_cap = 0;
     unsigned long long *ntrue = nullptr;
+    unsigned *noise_cnt = nullptr, *noise_pos = nullptr;
+    int64_t noise_cnt_cap = 0, noise_pos_cap = 0;
+    NoiseJob *noise_jobs = nullptr;
+    int64_t noise_jobs_cap = 0;
 };
 
 namespace {
@@ -539,8 +685,9 @@ DigiWork *gpet_digi_work_create() { return new DigiWork(); }
 void gpet_digi_work_destroy(DigiWork *w)
 {
     if (!w) return;
-    void *ptrs[] = {w->ev_a, w->ev_b, w->flag, w->pos, w->alive, w->key_a, w->key_b, w->val_a, w->val_b,
-                    w->tile_sums, w->hist, w->grand, w->singles, w->coinc, w->ntrue};
+    void *ptrs[] = {w->ev_a,  w->ev_b,  w->flag,      w->pos,  w->alive,   w->key_a,   w->key_b,
+                    w->val_a, w->val_b, w->tile_sums, w->hist, w->grand,   w->singles, w->coinc,
+                    w->ntrue, w->delayed, w->noise_cnt, w->noise_pos, w->noise_jobs};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete w;
@@ -669,6 +816,23 @@ int gpet_digitize(DigiWork *w, const DigiParams &p, const gpet_event *events, in
                            g.coinc_window_us, g.coinc_policy, cnt, cpos, w->coinc, w->ntrue, jf, nj);
         HIPCHK(hipGetLastError());
     }
+    /* delayed window: same openers, window shifted by delay_us */
+    unsigned nd = 0;
+    if (g.coinc_window_us > 0.0 && g.delay_us > 0.0 && ns > 1) {
+        unsigned *cnt = w->flag;
+        unsigned *cpos = w->pos;
+        hipLaunchKernelGGL(k_delayed<false>, dim3(grid_for(ns, 256)), dim3(256), 0, s, w->singles, (int64_t)ns,
+                           g.coinc_window_us, g.delay_us, g.coinc_policy, cnt, (const unsigned *)nullptr,
+                           (gpet_event *)nullptr, jf, nj);
+        HIPCHK(hipGetLastError());
+        if (scan(w, cnt, cpos, ns, &nd, s, err, errlen)) return -1;
+        if (ensure(w->delayed, w->delayed_cap, std::max<int64_t>(2 * (int64_t)nd, 2), err, errlen)) return -1;
+        hipLaunchKernelGGL(k_delayed<true>, dim3(grid_for(ns, 256)), dim3(256), 0, s, w->singles, (int64_t)ns,
+                           g.coinc_window_us, g.delay_us, g.coinc_policy, cnt, cpos, w->delayed, jf, nj);
+        HIPCHK(hipGetLastError());
+    }
+    out->delayed = w->delayed;
+    out->n_delayed = nd;
     if (nj > 1 && p.job_bounds) {
         hipLaunchKernelGGL(k_job_bounds, dim3(grid_for(nj + 1, 64)), dim3(64), 0, s, w->singles, (int64_t)ns, w->coinc,
                            (int64_t)npairs, jf, nj, p.job_bounds);
@@ -680,6 +844,84 @@ int gpet_digitize(DigiWork *w, const DigiParams &p, const gpet_event *events, in
     out->n_coinc = npairs;
     out->n_coinc_true = (int64_t)nt;
     out->coinc = w->coinc;
+    return 0;
+}
+
+void gpet_noise_layout(double rate, const double *t0_s, const double *t1_s, const int64_t *first_hist, int njobs,
+                       std::vector<NoiseJob> &jobs, double &gap_us, double &chunk_us, int64_t &nchunks)
+{
+    jobs.assign((size_t)std::max(njobs, 0), NoiseJob());
+    gap_us = 1.0e6 / rate;
+    chunk_us = NOISE_PER_CHUNK * gap_us;
+    nchunks = 0;
+    for (int j = 0; j < njobs; ++j) {
+        NoiseJob &J = jobs[j];
+        J.t0_us = t0_s[j] * 1.0e6;
+        J.t1_us = t1_s[j] * 1.0e6;
+        J.chunk0 = nchunks;
+        J.eventid = (int32_t)first_hist[j];
+        J.pad = 0;
+        const double T = J.t1_us - J.t0_us;
+        if (T > 0.0) nchunks += (int64_t)std::ceil(T / chunk_us);
+    }
+}
+
+namespace {
+
+NoiseDev noise_dev(DigiWork *w, const NoiseParams &p)
+{
+    NoiseDev d;
+    d.seed = p.seed;
+    d.gap_us = p.gap_us;
+    d.chunk_us = p.chunk_us;
+    d.e_mean = p.e_mean;
+    d.e_sigma = p.e_sigma;
+    d.jobs = w->noise_jobs;
+    d.njobs = p.njobs;
+    d.nchunks = p.nchunks;
+    d.panels = p.panels;
+    d.npanels = p.npanels;
+    d.moduleN = p.moduleN;
+    d.crystalN = p.crystalN;
+    d.moduleNy = p.moduleNy;
+    d.crystalNy = p.crystalNy;
+    d.depth = p.depth;
+    d.policy = p.policy;
+    return d;
+}
+
+} // namespace
+
+int gpet_digi_noise_count(DigiWork *w, const NoiseParams &p, int64_t *total, hipStream_t s, char *err, int errlen)
+{
+    *total = 0;
+    if (p.nchunks <= 0 || p.njobs <= 0) return 0;
+    if (p.nchunks > ((int64_t)1 << 25)) {
+        snprintf(err, errlen, "background singles: rate x window above 2^31 singles");
+        return -1;
+    }
+    if (ensure(w->noise_jobs, w->noise_jobs_cap, p.njobs, err, errlen) ||
+        ensure(w->noise_cnt, w->noise_cnt_cap, p.nchunks, err, errlen) ||
+        ensure(w->noise_pos, w->noise_pos_cap, p.nchunks, err, errlen))
+        return -1;
+    HIPCHK(hipMemcpy(w->noise_jobs, p.jobs, sizeof(NoiseJob) * (size_t)p.njobs, hipMemcpyHostToDevice));
+    const NoiseDev d = noise_dev(w, p);
+    hipLaunchKernelGGL(k_noise<false>, dim3(grid_for(p.nchunks, 256)), dim3(256), 0, s, d, w->noise_cnt,
+                       (const unsigned *)nullptr, (gpet_event *)nullptr);
+    HIPCHK(hipGetLastError());
+    unsigned n = 0;
+    if (scan(w, w->noise_cnt, w->noise_pos, p.nchunks, &n, s, err, errlen)) return -1;
+    *total = n;
+    return 0;
+}
+
+int gpet_digi_noise_write(DigiWork *w, const NoiseParams &p, gpet_event *out, hipStream_t s, char *err, int errlen)
+{
+    if (p.nchunks <= 0 || p.njobs <= 0) return 0;
+    const NoiseDev d = noise_dev(w, p);
+    hipLaunchKernelGGL(k_noise<true>, dim3(grid_for(p.nchunks, 256)), dim3(256), 0, s, d, w->noise_cnt,
+                       (const unsigned *)w->noise_pos, out);
+    HIPCHK(hipGetLastError());
     return 0;
 }
 

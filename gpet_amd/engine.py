@@ -192,6 +192,8 @@ class RunResult:
     job_counts: np.ndarray = None   # batched runs: [n_jobs, 3] histories, singles, coincidences
     psf_source: np.ndarray = None   # capi.PSF_DTYPE records (GPET_FLAG_PSF_SOURCE)
     psf_phantom: np.ndarray = None  # capi.PSF_DTYPE records (GPET_FLAG_PSF_PHANTOM)
+    delayed: np.ndarray = None      # [n, 2] delayed-window pairs (digitizer delay_us > 0)
+    n_delayed: int = 0
 
 
 def make_options(seed=1, eabs=1.0e3, noncol_sigma=0.0037056, positron_range=0, record_hits=0, batch=0,
@@ -301,8 +303,10 @@ class Engine:
                  "get_majorant")
         return out, e0.value, idl.value
 
-    def _sink(self, singles_cap, coinc_cap, adder_cap, hits_cap, trace_cap, out_dir, psf_cap=0):
+    def _sink(self, singles_cap, coinc_cap, adder_cap, hits_cap, trace_cap, out_dir, psf_cap=0, delayed_cap=0,
+              append=False):
         bufs = {
+            "delayed": np.zeros(2 * delayed_cap, capi.EVENT_DTYPE),
             "psf_source": np.zeros(psf_cap, capi.PSF_DTYPE),
             "psf_phantom": np.zeros(psf_cap, capi.PSF_DTYPE),
             "singles": np.zeros(singles_cap, capi.EVENT_DTYPE),
@@ -324,7 +328,9 @@ class Engine:
         s.trace, s.trace_cap = ptr(bufs["trace"]), trace_cap
         s.psf_source, s.psf_source_cap = ptr(bufs["psf_source"]), psf_cap
         s.psf_phantom, s.psf_phantom_cap = ptr(bufs["psf_phantom"]), psf_cap
+        s.delayed, s.delayed_cap = ptr(bufs["delayed"]), delayed_cap
         s.out_dir = out_dir.encode() if out_dir else None
+        s.out_append = 1 if append else 0
         return s, bufs
 
     def _result(self, s, bufs) -> RunResult:
@@ -340,19 +346,23 @@ class Engine:
                       s.n_singles, s.n_coinc, s.n_adder, s.n_hits)
         r.psf_source = bufs["psf_source"][:min(s.n_psf_source, s.psf_source_cap)]
         r.psf_phantom = bufs["psf_phantom"][:min(s.n_psf_phantom, s.psf_phantom_cap)]
+        r.delayed = bufs["delayed"][: 2 * min(s.n_delayed, s.delayed_cap)].reshape(-1, 2)
+        r.n_delayed = s.n_delayed
         return r
 
     def run_source(self, sources, isotopes, t0: float, t1: float, n_decays: int = 0, *, singles_cap=0,
-                   coinc_cap=0, adder_cap=0, hits_cap=0, trace_cap=0, out_dir=None, psf_cap=0) -> RunResult:
+                   coinc_cap=0, adder_cap=0, hits_cap=0, trace_cap=0, out_dir=None, psf_cap=0, delayed_cap=0,
+                   append=False) -> RunResult:
         src = (capi.Source * len(sources))(*sources)
         iso = (capi.Isotope * len(isotopes))(*isotopes)
-        s, bufs = self._sink(singles_cap, coinc_cap, adder_cap, hits_cap, trace_cap, out_dir, psf_cap)
+        s, bufs = self._sink(singles_cap, coinc_cap, adder_cap, hits_cap, trace_cap, out_dir, psf_cap, delayed_cap,
+                             append)
         self._ok(self.f["run_source"](self.ctx, src, len(sources), iso, len(isotopes), float(t0), float(t1),
                                       int(n_decays), C.byref(s)), "run_source")
         return self._result(s, bufs)
 
     def run_batch(self, jobs, sources, isotopes, *, singles_cap=0, coinc_cap=0, adder_cap=0, hits_cap=0,
-                  trace_cap=0, out_dir=None) -> RunResult:
+                  trace_cap=0, out_dir=None, delayed_cap=0, append=False) -> RunResult:
         """jobs: sequence of dicts {phantom, first_source, n_sources, t0, t1, n_decays}."""
         J = (capi.Job * len(jobs))()
         for k, j in enumerate(jobs):
@@ -364,7 +374,7 @@ class Engine:
             J[k].n_decays = j.get("n_decays", 0)
         src = (capi.Source * len(sources))(*sources)
         iso = (capi.Isotope * len(isotopes))(*isotopes)
-        s, bufs = self._sink(singles_cap, coinc_cap, adder_cap, hits_cap, trace_cap, out_dir)
+        s, bufs = self._sink(singles_cap, coinc_cap, adder_cap, hits_cap, trace_cap, out_dir, 0, delayed_cap, append)
         jc = np.zeros((len(jobs), 3), np.int64)
         s.job_counts = jc.ctypes.data_as(C.c_void_p)
         self._ok(self.f["run_batch"](self.ctx, J, len(jobs), src, len(sources), iso, len(isotopes), C.byref(s)),
@@ -442,22 +452,28 @@ class Engine:
         return loss.value, g
 
     def run_psf(self, rec: np.ndarray, ptype: int, *, singles_cap=0, coinc_cap=0, adder_cap=0, hits_cap=0,
-                trace_cap=0, out_dir=None, psf_cap=0) -> RunResult:
+                trace_cap=0, out_dir=None, psf_cap=0, delayed_cap=0, append=False) -> RunResult:
         rec = np.ascontiguousarray(rec, dtype=np.float64).reshape(-1, 8)
-        s, bufs = self._sink(singles_cap, coinc_cap, adder_cap, hits_cap, trace_cap, out_dir, psf_cap)
+        s, bufs = self._sink(singles_cap, coinc_cap, adder_cap, hits_cap, trace_cap, out_dir, psf_cap, delayed_cap,
+                             append)
         self._ok(self.f["run_psf"](self.ctx, rec.ctypes.data_as(C.c_void_p), len(rec), int(ptype), C.byref(s)),
                  "run_psf")
         return self._result(s, bufs)
 
-    def tallies(self, n_crystals=0, n_ebins=0, e_lo=0.0, e_hi=1.0, n_s=0, n_phi=0, s_max=0.0, with_sinogram=False):
+    def tallies(self, n_crystals=0, n_ebins=0, e_lo=0.0, e_hi=1.0, n_s=0, n_phi=0, s_max=0.0, with_sinogram=False,
+                with_delayed=False):
         """Run statistics, and on request per-crystal singles, the singles
-        energy histogram and the coincidence sinogram [n_phi, n_s]."""
+        energy histogram, the coincidence sinogram [n_phi, n_s] and (with
+        with_delayed) the delayed-window sinogram after it."""
         t = capi.Tallies()
         cs = np.zeros(n_crystals, np.int64)
         eh = np.zeros(n_ebins, np.int64)
         sg = np.zeros((n_phi, n_s), np.int64)
+        sgd = np.zeros((n_phi, n_s), np.int64)
         if n_s and n_phi and s_max > 0:
             t.sinogram = sg.ctypes.data_as(C.c_void_p)
+            if with_delayed:
+                t.sinogram_delayed = sgd.ctypes.data_as(C.c_void_p)
             t.n_s, t.n_phi, t.s_max = n_s, n_phi, s_max
         t.crystal_singles = cs.ctypes.data_as(C.c_void_p) if n_crystals else None
         t.n_crystals = n_crystals
@@ -466,6 +482,8 @@ class Engine:
         t.e_lo = e_lo
         t.e_hi = e_hi
         self._ok(self.f["get_tallies"](self.ctx, C.byref(t)), "get_tallies")
+        if with_delayed:
+            return t, cs, eh, sg, sgd
         if with_sinogram:
             return t, cs, eh, sg
         return t, cs, eh

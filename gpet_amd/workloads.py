@@ -298,9 +298,12 @@ def source(shape: int, centre=(0.0, 0.0, 0.0), c3=0.0, c4=0.0, c5=0.0, natom=1.0
 def digitizer(readout_depth=2, readout_policy=1, e_threshold=50e3, blur_policy=1, e_ref=511e3, r_ref=0.12,
               slope=0.0, space_blur=0.0, dead_level=3, dead_type=0, dead_time_us=0.0, win_min=350e3,
               win_max=650e3, time_blur_fwhm_us=300e-6, coinc_window_us=4e-3, coinc_policy=0,
-              record_rayleigh=0) -> capi.Digitizer:
+              record_rayleigh=0, noise_rate=0.0, noise_e_mean=0.0, noise_e_sigma=0.0,
+              delay_us=0.0) -> capi.Digitizer:
     """Digitizer of the benchmark scanner: 12% FWHM at 511 keV, 350-650 keV,
-    300 ps time blur, 4 ns coincidence window, no deadtime."""
+    300 ps time blur, 4 ns coincidence window, no deadtime.  noise_rate
+    (singles/s over the scanner) adds background singles; delay_us > 0 adds
+    the delayed-window randoms estimate."""
     d = capi.Digitizer()
     d.readout_depth = readout_depth
     d.readout_policy = readout_policy
@@ -320,6 +323,10 @@ def digitizer(readout_depth=2, readout_policy=1, e_threshold=50e3, blur_policy=1
     d.coinc_window_us = coinc_window_us
     d.coinc_policy = coinc_policy
     d.record_rayleigh = record_rayleigh
+    d.noise_rate = noise_rate
+    d.noise_e_mean = noise_e_mean
+    d.noise_e_sigma = noise_e_sigma
+    d.delay_us = delay_us
     return d
 
 

@@ -136,8 +136,18 @@ struct HitRec {
     float f[5];
 };
 
+/* Acquisition window (s) and first history of a job: where background
+ * singles are generated. */
+struct NoiseWin {
+    double t0, t1;
+    int64_t first;
+};
+
 struct Result {
     std::vector<gpet_psf_record> psf[2];
+    std::vector<NoiseWin> noise_win;
+    std::vector<gpet_event> delayed;  /* delayed-window pairs                     */
+    int64_t n_noise = 0;
     std::vector<int64_t> job_first;   /* batched runs: first history of each job */
     std::vector<int64_t> job_counts;  /* batched runs: per job {histories, singles, pairs} */
     std::vector<gpet_event> adder, singles, coinc;
@@ -1067,14 +1077,124 @@ inline int64_t job_of(const std::vector<int64_t> &jf, int64_t h)
     return (int64_t)(std::upper_bound(jf.begin(), jf.end(), h) - jf.begin()) - 1;
 }
 
+/* Readout site of a crystal (gPET_kernals.cu:756-813, as readout above). */
+inline int readout_site(int depth, int policy, int pann, int modn, int cryn, const Detector &d)
+{
+    if (depth == 3) return pann * d.moduleN * d.crystalN + modn * d.crystalN + cryn;
+    if (policy == 1) depth = 2;
+    if (depth == 0) return 0;
+    if (depth == 1) return pann;
+    return pann * d.moduleN + modn;
+}
+
+/* Crystal centre in panel-local coordinates. */
+inline void crystal_local(const Detector &d, int pann, int modn, int cryn, float &xl, float &yl, float &zl)
+{
+    const gpet_panel &P = d.p[pann];
+    int my = modn % d.moduleNy, mz = modn / d.moduleNy;
+    int ly = cryn % d.crystalNy, lz = cryn / d.crystalNy;
+    xl = (P.dirx * P.lenx) * 0.5f;
+    yl = (-P.leny * 0.5f + (float)my * (P.mody + P.mspy)) + ((float)ly * (P.cry + P.cspy) + P.cry * 0.5f);
+    zl = (-P.lenz * 0.5f + (float)mz * (P.modz + P.mspz)) + ((float)lz * (P.crz + P.cspz) + P.crz * 0.5f);
+}
+
+/* Background singles: the reference's addnoise (gPET_kernals.cu:699-735,
+ * never launched) draws exponential arrival gaps of mean lambda per thread
+ * interval, an energy Emean + sigma N(0,1), a uniform panel/module/crystal
+ * and placeholder positions, with parn -1.  Restated sequentially: the
+ * window [t0, t1] of job j (in us) is cut into chunks of 64 mean gaps (the
+ * last ends at t1); chunk c's arrivals come from the Philox stream
+ * (id c, aux 5 | j << 4), at most 4096 per chunk, and arrival m's crystal and
+ * energy from (id c, aux 6 | j << 4) draws 8m..8m+7.  The k-th single (in
+ * job, chunk, arrival order) has parn -1-k, eventid = the job's first
+ * history, the crystal centre as position and the readout site. */
+void gen_noise(const gpet_ctx &C, const std::vector<NoiseWin> &win, std::vector<gpet_event> &out)
+{
+    const gpet_digitizer &g = C.dig;
+    const Detector &d = C.det;
+    const double gap = 1.0e6 / g.noise_rate;
+    const double L = 64.0 * gap;
+    const int np = (int)d.p.size();
+    int64_t k = 0;
+    for (size_t j = 0; j < win.size(); ++j) {
+        const double t0 = win[j].t0 * 1.0e6, t1 = win[j].t1 * 1.0e6;
+        const double T = t1 - t0;
+        const int64_t nch = T > 0.0 ? (int64_t)std::ceil(T / L) : 0;
+        for (int64_t c = 0; c < nch; ++c) {
+            double t = t0 + (double)c * L;
+            const double end = (c + 1 == nch) ? t1 : t0 + (double)(c + 1) * L;
+            gp_rng rt;
+            gp_rng_init(&rt, C.opt.seed, (uint64_t)c, (uint32_t)GP_TAG_NOISE_T | ((uint32_t)j << 4), 0u);
+            for (unsigned m = 0; m < 4096u; ++m) {
+                t = t + (-gp_log_d(gp_rng_uniform_d(&rt))) * gap;
+                if (!(t < end)) break;
+                gp_rng ra;
+                gp_rng_init(&ra, C.opt.seed, (uint64_t)c, (uint32_t)GP_TAG_NOISE_A | ((uint32_t)j << 4), 8u * m);
+                const gp_u4 b0 = gp_rng_block(&ra), b1 = gp_rng_block(&ra);
+                gpet_event e;
+                e.pann = (int)(((uint64_t)b0.z * (uint64_t)np) >> 32);
+                e.modn = (int)(((uint64_t)b0.w * (uint64_t)d.moduleN) >> 32);
+                e.cryn = (int)(((uint64_t)b1.x * (uint64_t)d.crystalN) >> 32);
+                e.siten = readout_site(g.readout_depth, g.readout_policy, e.pann, e.modn, e.cryn, d);
+                e.parn = -1 - (int)k;
+                e.eventid = (int32_t)win[j].first;
+                e.t = t;
+                e.E = fmaxf(g.noise_e_mean + g.noise_e_sigma * gp_normal(gp_u01(b0.x), gp_u01(b0.y)), 1.0f);
+                crystal_local(d, e.pann, e.modn, e.cryn, e.x, e.y, e.z);
+                out.push_back(e);
+                ++k;
+            }
+        }
+    }
+}
+
+/* Delayed-window randoms estimate (new): the prompt windows' openers (the
+ * sequential rule below) each look at the singles of their job with
+ * delay <= t - t_open <= delay + W; one such single makes a delayed pair
+ * with the opener, more make the pairs of {opener, those singles} under the
+ * multiples policy. */
+void delayed_pairs(const gpet_digitizer &g, const std::vector<int64_t> &jf, const std::vector<gpet_event> &sg,
+                   std::vector<gpet_event> &out)
+{
+    out.clear();
+    const double W = g.coinc_window_us, D = g.delay_us;
+    const size_t n = sg.size();
+    size_t i = 0;
+    while (i < n) {
+        const int64_t ji = job_of(jf, sg[i].eventid);
+        size_t j0 = i + 1;
+        while (j0 < n && job_of(jf, sg[j0].eventid) == ji && sg[j0].t - sg[i].t < D) ++j0;
+        size_t e = j0;
+        while (e < n && job_of(jf, sg[e].eventid) == ji && sg[e].t - sg[i].t <= D + W) ++e;
+        const size_t m = e - j0;
+        if (m == 1 || (m > 1 && g.coinc_policy == GPET_COINC_TAKE_ALL_GOODS)) {
+            std::vector<size_t> set(1, i);
+            for (size_t q = j0; q < e; ++q) set.push_back(q);
+            for (size_t a = 0; a < set.size(); ++a)
+                for (size_t b = a + 1; b < set.size(); ++b) {
+                    out.push_back(sg[set[a]]);
+                    out.push_back(sg[set[b]]);
+                }
+        }
+        size_t j = i + 1;
+        while (j < n && sg[j].t - sg[i].t <= W && job_of(jf, sg[j].eventid) == ji) ++j;
+        i = j;
+    }
+}
+
 void run_digitizer(gpet_ctx &C, Result &R)
 {
     const gpet_digitizer &g = C.dig;
     const Detector &d = C.det;
     const std::vector<int64_t> &jf = R.job_first;
+    std::vector<gpet_event> noise;
+    if (g.noise_rate > 0.0) gen_noise(C, R.noise_win, noise);
+    R.n_noise = (int64_t)noise.size();
     std::vector<gpet_event> ev;
-    ev.reserve(R.adder.size());
-    for (gpet_event e : R.adder) {
+    ev.reserve(R.adder.size() + noise.size());
+    std::vector<gpet_event> all(R.adder);
+    all.insert(all.end(), noise.begin(), noise.end());
+    for (gpet_event e : all) {
         blur_event(C, e);
         if (e.E < g.e_threshold || e.E > g.e_upholder) continue; /* energywindow */
         ev.push_back(e);
@@ -1130,12 +1250,15 @@ void run_digitizer(gpet_ctx &C, Result &R)
                     for (size_t b = a + 1; b < j; ++b) {
                         R.coinc.push_back(sg[a]);
                         R.coinc.push_back(sg[b]);
-                        if (sg[a].eventid == sg[b].eventid) R.n_coinc_true++;
+                        if (sg[a].eventid == sg[b].eventid && sg[a].parn >= 0 && sg[b].parn >= 0)
+                            R.n_coinc_true++;
                     }
             }
             i = j;
         }
     }
+    R.delayed.clear();
+    if (g.coinc_window_us > 0.0 && g.delay_us > 0.0) delayed_pairs(g, jf, sg, R.delayed);
     if (jf.size() >= 2) {
         const size_t nj = jf.size() - 1;
         R.job_counts.assign(3 * nj, 0);
@@ -1192,10 +1315,11 @@ bool ev_less_id(const gpet_event &a, const gpet_event &b)
 /* Run the transport over n histories with OpenMP, collect, digitize. */
 template <class F>
 void run_histories(gpet_ctx &C, int64_t n, F hist_fn, gpet_sink *sink, Result &R,
-                   const std::vector<int64_t> *job_first = nullptr)
+                   const std::vector<int64_t> *job_first = nullptr, const std::vector<NoiseWin> *noise_win = nullptr)
 {
     R = Result();
     if (job_first) R.job_first = *job_first;
+    if (noise_win) R.noise_win = *noise_win;
     R.n_histories = n;
     const bool want_trace = sink && sink->trace && sink->trace_cap > 0;
     const bool want_hits = C.opt.record_hits != 0;
@@ -1271,6 +1395,7 @@ void fill_sink(const Result &R, gpet_sink *s)
     if (!s) return;
     copy_out(R.singles, s->singles, s->singles_cap, s->n_singles);
     copy_out(R.coinc, s->coinc, s->coinc_cap, s->n_coinc, 2);
+    copy_out(R.delayed, s->delayed, s->delayed_cap, s->n_delayed, 2);
     copy_out(R.adder, s->adder, s->adder_cap, s->n_adder);
     s->n_hits = (int64_t)R.hits.size();
     if (s->hit_id && s->hit_f && s->hits_cap > 0) {
@@ -1431,11 +1556,12 @@ gpet_status gpet_cpu_run_source(gpet_ctx *c, const gpet_source *src, int32_t ns,
     if (st) return fail(c, st, "bad source/isotope");
     if (pl.total > GPET_MAX_HISTORIES) return fail(c, GPET_ERR_CAPACITY, "more than 2^30 histories in one run");
     const gpet_ctx &C = *c;
+    const std::vector<NoiseWin> nw(1, NoiseWin{t0, t1, 0});
     run_histories(*c, pl.total,
                   [&](int64_t h, gpet_trace *tr, PhotonOut *out, Counters &cnt, bool wh) {
                       decay_history(C, pl, iso, h, tr, out, cnt, wh);
                   },
-                  sink, c->last);
+                  sink, c->last, nullptr, &nw);
     fill_sink(c->last, sink);
     return GPET_OK;
 }
@@ -1461,11 +1587,13 @@ gpet_status gpet_cpu_run_batch(gpet_ctx *c, const gpet_job *jobs, int32_t njobs,
     }
     if (pl.total > GPET_MAX_HISTORIES) return fail(c, GPET_ERR_CAPACITY, "more than 2^30 histories in one run");
     const gpet_ctx &C = *c;
+    std::vector<NoiseWin> nw;
+    for (int j = 0; j < njobs; ++j) nw.push_back(NoiseWin{jobs[j].t0, jobs[j].t1, pl.job_first[j]});
     run_histories(*c, pl.total,
                   [&](int64_t h, gpet_trace *tr, PhotonOut *out, Counters &cnt, bool wh) {
                       decay_history(C, pl, iso, h, tr, out, cnt, wh);
                   },
-                  sink, c->last, &pl.job_first);
+                  sink, c->last, &pl.job_first, &nw);
     fill_sink(c->last, sink);
     return GPET_OK;
 }
@@ -1490,6 +1618,8 @@ gpet_status gpet_cpu_run_psf(gpet_ctx *c, const double *rec8, int64_t n, int32_t
     gpet_status st = check_ready(c);
     if (st) return st;
     if (!rec8 || n < 0 || (ptype != 0 && ptype != 1)) return fail(c, GPET_ERR_ARG, "psf arguments");
+    if (c->dig.noise_rate > 0.0)
+        return fail(c, GPET_ERR_UNSUPPORTED, "background singles need a decay run's acquisition window");
     const gpet_ctx &C = *c;
     run_histories(*c, n,
                   [&](int64_t h, gpet_trace *tr, PhotonOut *out, Counters &cnt, bool wh) {
@@ -1522,6 +1652,8 @@ gpet_status gpet_cpu_get_tallies(gpet_ctx *c, gpet_tallies *t)
     t->panel_steps = R.c.panel_steps;
     t->seconds_transport = R.sec_transport;
     t->seconds_digitizer = R.sec_digi;
+    t->n_noise = R.n_noise;
+    t->n_delayed = (int64_t)R.delayed.size() / 2;
     const Detector &d = c->det;
     if (t->crystal_singles && t->n_crystals > 0) {
         memset(t->crystal_singles, 0, sizeof(int64_t) * t->n_crystals);
@@ -1530,11 +1662,14 @@ gpet_status gpet_cpu_get_tallies(gpet_ctx *c, gpet_tallies *t)
             if (g >= 0 && g < t->n_crystals) t->crystal_singles[g]++;
         }
     }
-    if (t->sinogram && t->n_s > 0 && t->n_phi > 0 && t->s_max > 0.0f && c->has_det) {
-        memset(t->sinogram, 0, sizeof(int64_t) * (size_t)t->n_s * t->n_phi);
-        for (size_t i = 0; i + 1 < R.coinc.size(); i += 2) {
-            int b = sino_bin(d, R.coinc[i], R.coinc[i + 1], t->n_s, t->n_phi, t->s_max);
-            if (b >= 0) t->sinogram[b]++;
+    for (int k = 0; k < 2; ++k) {
+        int64_t *dst = k == 0 ? t->sinogram : t->sinogram_delayed;
+        const std::vector<gpet_event> &pairs = k == 0 ? R.coinc : R.delayed;
+        if (!(dst && t->n_s > 0 && t->n_phi > 0 && t->s_max > 0.0f && c->has_det)) continue;
+        memset(dst, 0, sizeof(int64_t) * (size_t)t->n_s * t->n_phi);
+        for (size_t i = 0; i + 1 < pairs.size(); i += 2) {
+            int b = sino_bin(d, pairs[i], pairs[i + 1], t->n_s, t->n_phi, t->s_max);
+            if (b >= 0) dst[b]++;
         }
     }
     if (t->energy_hist && t->n_ebins > 0 && t->e_hi > t->e_lo) {

@@ -99,3 +99,31 @@ def test_autotune(tables, batch_case):
     assert first.tallies.seconds_tune > 0 and second.tallies.seconds_tune == 0
     check_same(first, ref)
     check_same(second, ref)
+
+
+def test_batch_noise_and_delayed_parity(tables):
+    """Background singles per job window and delayed-window pairs in a
+    batched run: the oracle's singles, pairs, delayed pairs and job counts."""
+    from gpet_amd import engine
+    from gpet_amd import workloads as W
+
+    phantoms, jobs, src, iso = small_batch()
+    dig = W.digitizer(dead_time_us=1.5, dead_level=1, coinc_policy=1, coinc_window_us=0.05, noise_rate=3.0e6,
+                      noise_e_mean=450e3, noise_e_sigma=40e3, delay_us=0.4)
+    n = sum(j["n_decays"] for j in jobs)
+    caps = dict(singles_cap=4 * n + 10**5, coinc_cap=4 * n + 10**5, adder_cap=4 * n, delayed_cap=4 * n + 10**5)
+    out = {}
+    for name, mk in (("gpu", hip_engine), ("oracle", oracle_engine)):
+        e = mk(engine.make_options(seed=5, positron_range=1, batch=6000))
+        try:
+            W.setup_batch(e, tables, phantoms, dig=dig)
+            out[name] = e.run_batch(jobs, src, iso, **caps)
+        finally:
+            e.close()
+    a, b = out["gpu"], out["oracle"]
+    assert a.tallies.n_noise == b.tallies.n_noise > 0
+    assert a.tallies.n_delayed == b.tallies.n_delayed > 0
+    for f in ("adder", "singles", "coinc", "delayed"):
+        x, y = getattr(a, f), getattr(b, f)
+        assert same_bytes(x, y), f + ": " + first_diff(x.ravel(), y.ravel())
+    assert np.array_equal(a.job_counts, b.job_counts)

@@ -153,3 +153,41 @@ def test_phase_space_parity(tables, sphere):
     assert same_bytes(a.psf_source, b.psf_source)
     assert same_bytes(a.psf_phantom, b.psf_phantom)
     assert_parity(a, b)
+
+
+@pytest.mark.parametrize("policy,dead", [(0, 0.0), (1, 0.005)])
+def test_noise_and_delayed_parity(tables, policy, dead):
+    """Background singles (addnoise) and the delayed-window pairs, on top of
+    decays: the same singles, prompt pairs, delayed pairs and delayed
+    sinogram as the oracle."""
+    from gpet_amd import engine
+    from gpet_amd import workloads as W
+
+    dig = W.digitizer(coinc_policy=policy, dead_time_us=dead, dead_level=3, noise_rate=8.0e7, noise_e_mean=430e3,
+                      noise_e_sigma=60e3, delay_us=0.25)
+    opt = engine.make_options(seed=21, positron_range=1, batch=9000)
+    src, iso, _, _ = W.workload_sources(2)
+    n, t1 = 40000, 4e-4
+    cap = 4 * n + int(8.0e7 * t1 * 2)
+
+    def tallies_delayed(eng):
+        t, cs, eh, sg, sgd = eng.tallies(n_crystals=48 * 400, n_ebins=100, e_lo=0.0, e_hi=1e6, n_s=64, n_phi=48,
+                                         s_max=8.0, with_delayed=True)
+        return cs, eh, sg, sgd
+
+    res = {}
+    for name, mk in (("gpu", hip_engine), ("oracle", oracle_engine)):
+        eng = mk(opt)
+        try:
+            W.setup_engine(eng, tables, config=2, phantom_n=100, dig=dig)
+            res[name] = eng.run_source(src, iso, 0.0, t1, n_decays=n, singles_cap=cap, coinc_cap=cap, adder_cap=cap,
+                                       trace_cap=2 * n, delayed_cap=cap)
+            res[name].arrays = tallies_delayed(eng)
+        finally:
+            eng.close()
+    a, b = res["gpu"], res["oracle"]
+    assert a.tallies.n_noise == b.tallies.n_noise > 20000
+    assert a.tallies.n_delayed == b.tallies.n_delayed > 100
+    assert same_bytes(a.delayed, b.delayed), "delayed: " + first_diff(a.delayed.ravel(), b.delayed.ravel())
+    assert a.arrays[3].sum() > 0
+    assert_parity(a, b)

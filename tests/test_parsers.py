@@ -41,6 +41,13 @@ def test_input_file_extension_fields(tmp_path):
     inp = engine.read_input(str(p))
     assert inp.time_blur_fwhm_us == 0.0003 and inp.coinc_window_us == 0.004 and inp.coinc_policy == 1
     assert inp.seed == 12345
+    assert inp.noise_rate == 0 and inp.delay_us == 0
+    txt += "background singles: rate (1/s), energy mean and sigma (eV)\n2e5 450000 25000\ndelay (us)\n0.1\n"
+    p.write_text(txt)
+    inp = engine.read_input(str(p))
+    assert (inp.noise_rate, inp.noise_e_mean, inp.noise_e_sigma, inp.delay_us) == (2e5, 450000, 25000, 0.1)
+    d = engine.input_digitizer(inp)
+    assert (d.noise_rate, d.noise_e_mean, d.noise_e_sigma, d.delay_us) == (2e5, 450000, 25000, 0.1)
 
 
 def test_config8_geometry():
