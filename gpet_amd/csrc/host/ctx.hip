@@ -492,7 +492,7 @@ gpet_status autotune(gpet_ctx *c, const RunParams &rp0, int64_t n_hist, const Pa
     auto search = [&](int k, auto timer) -> gpet_status {
         float best_ms = 0;
         if ((st = timer(L, best_ms))) return st;
-        const int wpes[] = {0, 6, 8};
+        const int wpes[] = {0, 5, 6};
         const int bpcs[] = {1, 2, 4, 8, 16};
         const int blocks[] = {128, 256};
         for (int stage = 0; stage < 3; ++stage) {
@@ -1044,6 +1044,11 @@ gpet_status gpet_set_phantom_slot(gpet_ctx *c, int32_t slot, const int32_t *mat,
         c->slots.resize((size_t)slot + 1);
         for (size_t k = n0; k < c->slots.size(); ++k) memset(&c->slots[k].dp, 0, sizeof(DevPhantom));
     }
+    {
+        /* the positron walk addresses the bricked densities in 32 bits */
+        const int64_t nb = (int64_t)((dims[0] + 3) / 4) * ((dims[1] + 3) / 4) * ((dims[2] + 3) / 4);
+        if (nb >= ((int64_t)1 << 24)) return fail(c, GPET_ERR_CAPACITY, "phantom above 2^24 bricks of 4^3 voxels");
+    }
     PhSlot &S = c->slots[slot];
     const size_t nv = (size_t)dims[0] * dims[1] * dims[2];
     std::vector<int2> vox(nv);
@@ -1072,6 +1077,7 @@ gpet_status gpet_set_phantom_slot(gpet_ctx *c, int32_t slot, const int32_t *mat,
     p.dens_b = S.dens_b.p;
     p.nbx = nbx;
     p.nby = nby;
+    p.dens_bytes = (unsigned)((size_t)nbx * nby * nbz * 64 * sizeof(float));
     build_majorant(c, rmax, S.h_wck, p.wck.e0, p.wck.idl);
     if ((st = upload(c, S.wck, S.h_wck.data(), GPET_NWCK))) return st;
     p.wck.lam = S.wck.p;

@@ -41,6 +41,7 @@ struct DevPhantom {
     const int2 *vox;           /* {float bits of density, material}           */
     const float *dens_b;       /* density in 4x4x4 bricks (positron walk)     */
     int nbx, nby;              /* bricks along x and y                        */
+    unsigned dens_bytes;       /* size of dens_b (fewer than 2^24 bricks)     */
     DevWck wck;
     int present;
 };

@@ -147,75 +147,140 @@ __device__ __forceinline__ void d_dda_axis(float x, float off, float d, int i, f
     }
 }
 
-__device__ void d_positron_range(const DevPhantom &p, float &x, float &y, float &z, float ek_ev, int usedir,
-                                 float vx, float vy, float vz, gp_rng &r, unsigned long long &nsteps)
-{
-    float ekin = ek_ev / 1.0e6f;
-    float rex = ((0.1f * 5.44040782f) * ekin * ekin) / (0.369516529f + ekin);
-    float sigma = rex / 2.0f;
-    float dx = sigma * gp_rng_normal(&r);
-    float dy = sigma * gp_rng_normal(&r);
-    float dz = sigma * gp_rng_normal(&r);
-    float rr = sqrtf(dx * dx + dy * dy + dz * dz);
-    if (usedir) {
-        float tmp = sqrtf(vx * vx + vy * vy + vz * vz);
-        dx = rr * vx / tmp;
-        dy = rr * vy / tmp;
-        dz = rr * vz / tmp;
-    }
-    if (!(rr > 0.0f)) return;
-    float inv_rr = 1.0f / rr;
-    float ux = dx * inv_rr, uy = dy * inv_rr, uz = dz * inv_rr;
-    int ix = 0, iy = 0, iz = 0;
-    float t;
-    if (!(p.present && d_vox(p, x, y, z, ix, iy, iz))) {
-        t = rr / D_RHO_OUTSIDE;
-    } else {
-        float tmx, tmy, tmz, tdx, tdy, tdz;
-        int sx, sy, sz;
+/* The walk as a state: begin() samples the displacement and sets up the DDA
+ * (or finishes at once outside the phantom), step4() advances four
+ * crossings, end() moves the point. */
+struct RangeWalk {
+    float ux, uy, uz, rr;
+    float tmx, tmy, tmz, tdx, tdy, tdz;
+    int ix, iy, iz, sx, sy, sz;
+    float tcur, s, t;
+    int it;
+    bool done;
+
+    __device__ __forceinline__ void begin(const DevPhantom &p, float x, float y, float z, float ek_ev, int usedir,
+                                          float vx, float vy, float vz, gp_rng &r)
+    {
+        float ekin = ek_ev / 1.0e6f;
+        float rex = ((0.1f * 5.44040782f) * ekin * ekin) / (0.369516529f + ekin);
+        float sigma = rex / 2.0f;
+        float dx = sigma * gp_rng_normal(&r);
+        float dy = sigma * gp_rng_normal(&r);
+        float dz = sigma * gp_rng_normal(&r);
+        rr = sqrtf(dx * dx + dy * dy + dz * dz);
+        if (usedir) {
+            float tmp = sqrtf(vx * vx + vy * vy + vz * vz);
+            dx = rr * vx / tmp;
+            dy = rr * vy / tmp;
+            dz = rr * vz / tmp;
+        }
+        t = 0.0f;
+        ux = uy = uz = 0.0f;
+        done = true;
+        if (!(rr > 0.0f)) return;
+        float inv_rr = 1.0f / rr;
+        ux = dx * inv_rr;
+        uy = dy * inv_rr;
+        uz = dz * inv_rr;
+        ix = iy = iz = 0;
+        if (!(p.present && d_vox(p, x, y, z, ix, iy, iz))) {
+            t = rr / D_RHO_OUTSIDE;
+            return;
+        }
         d_dda_axis(x, p.offx, p.dx, ix, ux, tmx, tdx, sx);
         d_dda_axis(y, p.offy, p.dy, iy, uy, tmy, tdy, sy);
         d_dda_axis(z, p.offz, p.dz, iz, uz, tmz, tdz, sz);
-        /* The crossing sequence does not depend on the densities: advance the
-         * DDA four crossings at a time and issue their gathers together, then
-         * accumulate in order (identical arithmetic, 4x memory parallelism). */
-        float tcur = 0.0f, s = 0.0f;
-        t = 0.0f;
-        bool done = false;
-        for (int it = 0; it < D_STEP_CAP && !done; it += 4) {
-            float tn[4];
-            bool in[4];
-            int g[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                if (tmx <= tmy && tmx <= tmz) { tn[k] = tmx; tmx = tmx + tdx; ix += sx; }
-                else if (tmy <= tmz) { tn[k] = tmy; tmy = tmy + tdy; iy += sy; }
-                else { tn[k] = tmz; tmz = tmz + tdz; iz += sz; }
-                in[k] = ix >= 0 && ix < p.nx && iy >= 0 && iy < p.ny && iz >= 0 && iz < p.nz;
-                g[k] = in[k] ? __float_as_int(p.dens_b[gpet_brick_offset(ix, iy, iz, p.nbx, p.nby)]) : 0;
-            }
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                if (!done) {
-                    float rho = in[k] ? __int_as_float(g[k]) : D_RHO_OUTSIDE;
-                    if (in[k]) nsteps++;
-                    s = s + (tn[k] - tcur) * rho;
-                    if (s >= rr) {
-                        t = (s > rr) ? tn[k] + (rr - s) / rho : tn[k];
-                        done = true;
-                    } else if (!in[k]) {
-                        t = tn[k] + (rr - s) / D_RHO_OUTSIDE;
-                        done = true;
-                    } else {
-                        tcur = tn[k];
-                    }
-                }
-            }
-        }
+        tcur = 0.0f;
+        s = 0.0f;
+        it = 0;
+        done = false;
     }
-    x = x + ux * t;
-    y = y + uy * t;
-    z = z + uz * t;
+
+    /* The crossing sequence does not depend on the densities: advance the DDA
+     * four crossings, issue their gathers together, then accumulate in order
+     * (identical arithmetic, 4x memory parallelism).  Written with selects,
+     * not branches: lanes of a wave take different axes at every crossing.
+     * The walk ends at the crossing where the accumulated density-weighted
+     * length reaches rr (or the walk leaves the grid); there
+     * t = tn + (rr - s)/rho, or tn when s == rr exactly (outside the grid
+     * rho is D_RHO_OUTSIDE, so one rule covers both ends). */
+    __device__ __forceinline__ void step4(const DevPhantom &p, unsigned &nsteps)
+    {
+        float tn[4];
+        bool in[4];
+        float g[4];
+        /* The phantom is wave-uniform: the bricked densities are read through
+         * a buffer descriptor with the array's size as its range, so the four
+         * gathers are issued unconditionally (a read past the range returns
+         * 0; out-of-grid results are replaced below) and the address
+         * arithmetic stays in 32 bits. */
+        const uint64_t base = (uint64_t)(uintptr_t)p.dens_b;
+        const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)base);
+        const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(base >> 32));
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            (void *)(uintptr_t)(((uint64_t)hi << 32) | lo), (short)0,
+            (int)__builtin_amdgcn_readfirstlane(p.dens_bytes), 0x00020000);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const bool bx = (tmx <= tmy) & (tmx <= tmz);
+            const bool by = !bx & (tmy <= tmz);
+            const bool bz = !bx & !by;
+            tn[k] = bx ? tmx : (by ? tmy : tmz);
+            const float nx_ = tmx + tdx, ny_ = tmy + tdy, nz_ = tmz + tdz;
+            tmx = bx ? nx_ : tmx;
+            tmy = by ? ny_ : tmy;
+            tmz = bz ? nz_ : tmz;
+            ix += bx ? sx : 0;
+            iy += by ? sy : 0;
+            iz += bz ? sz : 0;
+            in[k] = ((unsigned)ix < (unsigned)p.nx) & ((unsigned)iy < (unsigned)p.ny) & ((unsigned)iz < (unsigned)p.nz);
+            /* brick index with 24-bit multiplies (fewer than 2^24 bricks) */
+            const unsigned b = __umul24(__umul24((unsigned)iz >> 2, (unsigned)p.nby) + ((unsigned)iy >> 2),
+                                        (unsigned)p.nbx) + ((unsigned)ix >> 2);
+            const unsigned o = (b << 6) | (((unsigned)iz & 3u) << 4) | (((unsigned)iy & 3u) << 2) | ((unsigned)ix & 3u);
+            const float v = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, o << 2, 0, 0));
+            g[k] = in[k] ? v : D_RHO_OUTSIDE;
+        }
+        float ftn = 0.0f, fs = 0.0f, frho = 1.0f;
+        bool fin = false;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float rho = g[k];
+            const float sn = s + (tn[k] - tcur) * rho;
+            const bool live = !done;
+            const bool end = live & ((sn >= rr) | !in[k]);
+            nsteps += (live & in[k]) ? 1u : 0u;
+            s = live ? sn : s;
+            tcur = (live & !end) ? tn[k] : tcur;
+            ftn = end ? tn[k] : ftn;
+            fs = end ? sn : fs;
+            frho = end ? rho : frho;
+            fin = fin | end;
+            done = done | end;
+        }
+        if (fin) t = (fs == rr) ? ftn : ftn + (rr - fs) / frho;
+        it += 4;
+        if (it >= D_STEP_CAP) done = true;
+    }
+
+    __device__ __forceinline__ void end(float &x, float &y, float &z) const
+    {
+        if (!(rr > 0.0f)) return;
+        x = x + ux * t;
+        y = y + uy * t;
+        z = z + uz * t;
+    }
+};
+
+__device__ void d_positron_range(const DevPhantom &p, float &x, float &y, float &z, float ek_ev, int usedir,
+                                 float vx, float vy, float vz, gp_rng &r, unsigned long long &nsteps)
+{
+    RangeWalk w;
+    w.begin(p, x, y, z, ek_ev, usedir, vx, vy, vz, r);
+    unsigned n = 0;
+    while (!w.done) w.step4(p, n);
+    nsteps += n;
+    w.end(x, y, z);
 }
 
 __device__ void d_emit_pair(float sigma, gp_rng &r, float v[2][3], float e[2])
@@ -335,6 +400,31 @@ __device__ __forceinline__ int d_source_of(const RunParams &rp, int64_t h)
     return lo;
 }
 
+/* Decay of history h of source si up to the positron range: decay time
+ * (us), whether it emits a pair, the emission point and the positron's
+ * kinetic energy (eV; sampled only with positron range).  The stream is left
+ * where the range walk starts. */
+__device__ __forceinline__ bool d_decay_begin(const RunParams &rp, int64_t h, int si, gp_rng &r, double &tstart,
+                                              float &px, float &py, float &pz, float &ek)
+{
+    const DevSource S = rp.src[si];
+    const DevIsotope I = rp.iso[S.iso];
+    gp_rng_init(&r, rp.seed, (uint64_t)h, GP_TAG_DECAY, 0u);
+    double u = gp_rng_uniform_d(&r);
+    double ts = S.t0 + S.tau * (-gp_log1p_d(-u * S.pdec));
+    tstart = ts * 1.0e6;
+    float rb = gp_rng_uniform(&r);
+    if (!(rb < I.ratio)) return false;
+    gp_rng_align(&r);
+    d_sample_shape(S, r, px, py, pz);
+    if (rp.prange) {
+        gp_rng_align(&r);
+        ek = d_sample_ek(I, r);
+        gp_rng_align(&r);
+    }
+    return true;
+}
+
 /* One history: decay (or PSF record i), positron range, pair, phantom walk
  * of both photons, panel entry and queue append. */
 template <int MODE>
@@ -350,23 +440,10 @@ __device__ __forceinline__ void d_history(const RunParams &rp, const DevTables &
     double tstart = 0.0;
     int nph = 0;
     if (MODE == RUN_DECAY) {
-        const DevSource S = rp.src[si];
-        const DevIsotope I = rp.iso[S.iso];
         gp_rng r;
-        gp_rng_init(&r, rp.seed, (uint64_t)h, GP_TAG_DECAY, 0u);
-        double u = gp_rng_uniform_d(&r);
-        double ts = S.t0 + S.tau * (-gp_log1p_d(-u * S.pdec));
-        tstart = ts * 1.0e6;
-        float rb = gp_rng_uniform(&r);
-        if (rb < I.ratio) {
-            gp_rng_align(&r);
-            d_sample_shape(S, r, px, py, pz);
-            if (rp.prange) {
-                gp_rng_align(&r);
-                float ek = d_sample_ek(I, r);
-                gp_rng_align(&r);
-                d_positron_range(ph, px, py, pz, ek, 0, 0.f, 0.f, 0.f, r, c_rs);
-            }
+        float ek = 0.0f;
+        if (d_decay_begin(rp, h, si, r, tstart, px, py, pz, ek)) {
+            if (rp.prange) d_positron_range(ph, px, py, pz, ek, 0, 0.f, 0.f, 0.f, r, c_rs);
             gp_rng_align(&r);
             d_emit_pair(rp.sigma, r, v, e);
             nph = 2;
@@ -900,7 +977,7 @@ void k_panel(PanelParams pp, DevTables tab, DevDetector det, DevQueue q)
 
 } // namespace
 
-const int gpet_wpe_variants[GPET_N_WPE] = {0, 6, 8};
+const int gpet_wpe_variants[GPET_N_WPE] = {0, 5, 6};
 
 namespace {
 
@@ -916,8 +993,8 @@ void launch_phantom_wpe(const RunParams &rp, const DevTables &tab, const DevPhan
                         const DevQueue &q, int grid, int block, int wpe, hipStream_t s)
 {
     const size_t lds = det.present ? (sizeof(float4) + sizeof(DevPanel)) * (size_t)det.npanels : 0;
-    if (wpe == 8)
-        hipLaunchKernelGGL((k_phantom<MODE, BATCH, 8>), dim3(grid), dim3(block), lds, s, rp, tab, ph, det, q);
+    if (wpe == 5)
+        hipLaunchKernelGGL((k_phantom<MODE, BATCH, 5>), dim3(grid), dim3(block), lds, s, rp, tab, ph, det, q);
     else if (wpe == 6)
         hipLaunchKernelGGL((k_phantom<MODE, BATCH, 6>), dim3(grid), dim3(block), lds, s, rp, tab, ph, det, q);
     else
@@ -969,8 +1046,8 @@ int gpet_launch_panel(const PanelParams &pp, const DevTables &tab, const DevDete
         hipLaunchKernelGGL((k_panel<0, PANEL_HARVEST>), dim3(grid), dim3(cfg.block), lds, s, pp, tab, det, q);
     else if (mode == PANEL_SURROGATE)
         hipLaunchKernelGGL((k_panel<0, PANEL_SURROGATE>), dim3(grid), dim3(cfg.block), lds, s, pp, tab, det, q);
-    else if (cfg.wpe == 8)
-        hipLaunchKernelGGL((k_panel<8, PANEL_PLAIN>), dim3(grid), dim3(cfg.block), lds, s, pp, tab, det, q);
+    else if (cfg.wpe == 5)
+        hipLaunchKernelGGL((k_panel<5, PANEL_PLAIN>), dim3(grid), dim3(cfg.block), lds, s, pp, tab, det, q);
     else if (cfg.wpe == 6)
         hipLaunchKernelGGL((k_panel<6, PANEL_PLAIN>), dim3(grid), dim3(cfg.block), lds, s, pp, tab, det, q);
     else

@@ -175,7 +175,7 @@ typedef struct gpet_psf_record {
 typedef struct gpet_launch {
     int32_t block[2];          /* threads per block: 64, 128, 192 or 256         */
     int32_t wpe[2];            /* register budget as minimum waves per SIMD:
-                                  0 (compiler default), 6 or 8                  */
+                                  0 (compiler default), 5 or 6                  */
     int32_t blocks_per_cu[2];  /* grid = CUs x this, grid-stride; 0 = one lane
                                   per item                                      */
     int64_t batch;             /* histories per phantom-kernel launch           */

@@ -82,7 +82,7 @@ def test_graph_replays(tables, batch_case):
 @pytest.mark.parametrize("launch", [
     dict(block=(128, 128), wpe=(0, 0), blocks_per_cu=(1, 8)),
     dict(block=(256, 256), wpe=(6, 6), blocks_per_cu=(4, 4)),
-    dict(block=(64, 192), wpe=(8, 8), blocks_per_cu=(2, 16)),
+    dict(block=(64, 192), wpe=(5, 5), blocks_per_cu=(2, 16)),
 ])
 def test_launch_shapes(tables, batch_case, launch):
     ref = batch_case[-1]
