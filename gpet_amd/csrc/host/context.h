@@ -127,6 +127,8 @@ struct gpet_ctx {
     DBuf<gpet_event> d_events;
     DBuf<gpet_trace> d_trace;
     DBuf<unsigned> d_rcount;     /* panel-queue region counts                */
+    DBuf<LongWalkItem> d_lw;     /* deferred positron walks (queue / 8)      */
+    DBuf<unsigned> d_lw_count;
     DBuf<gpet_psf_record> d_psf_src, d_psf_ph;
     DBuf<unsigned long long> d_npsf;
     DBuf<int32_t> d_hit_id;

@@ -265,7 +265,7 @@ gpet_launch default_launch()
     memset(&L, 0, sizeof(L));
     L.block[0] = 256; L.block[1] = 256;
     L.wpe[0] = 0; L.wpe[1] = 0;
-    L.blocks_per_cu[0] = 4; L.blocks_per_cu[1] = 4;   /* the tuner's pick on the cylinder workload */
+    L.blocks_per_cu[0] = 16; L.blocks_per_cu[1] = 2;  /* the tuner's pick on the cylinder workload */
     L.batch = DEFAULT_BATCH;
     return L;
 }
@@ -290,6 +290,8 @@ gpet_status alloc_queue(gpet_ctx *c, int64_t qcap)
 {
     gpet_status st;
     if ((st = dalloc(c, c->d_rcount, 64 * (int64_t)c->n_cu + 1))) return st;
+    if ((st = dalloc(c, c->d_lw_count, 64 * (int64_t)c->n_cu + 1))) return st;
+    if ((st = dalloc(c, c->d_lw, qcap / 8 + 1))) return st;
     if (qcap > c->q_cap) {
         for (int k = 0; k < 6; ++k)
             if ((st = dalloc(c, c->q_f[k], qcap))) return st;
@@ -310,6 +312,8 @@ DevQueue make_queue(gpet_ctx *c, int64_t qcap, bool want_trace)
     q.trace_slot = want_trace ? c->q_slot.p : nullptr;
     q.count = (int *)c->d_small.p;
     q.cap = qcap;
+    q.lw = c->d_lw.p;
+    q.lw_count = c->d_lw_count.p;
     return q;
 }
 
@@ -968,6 +972,7 @@ gpet_status gpet_destroy(gpet_ctx *c)
     dfree(c->d_panels); dfree(c->d_wck_det);
     for (auto &b : c->q_f) dfree(b);
     dfree(c->q_t); dfree(c->q_info); dfree(c->q_slot); dfree(c->d_small); dfree(c->d_cnt);
+    dfree(c->d_rcount); dfree(c->d_lw); dfree(c->d_lw_count);
     dfree(c->d_events); dfree(c->d_trace); dfree(c->d_hit_id); dfree(c->d_hit_f);
     dfree(c->d_psf_src); dfree(c->d_psf_ph); dfree(c->d_npsf);
     dfree(c->d_first); dfree(c->d_src); dfree(c->d_iso); dfree(c->d_psf);

@@ -100,6 +100,18 @@ struct DevSegment {
  * is split into one region per phantom-kernel block (no global atomics):
  * region r holds slots [r * region, r * region + rcount[r]).  The run-wide
  * surrogate queue uses the single counter `count` instead (region = 0). */
+/* A positron-range walk handed from the phantom kernel to k_longwalk: the
+ * walk state after LW_STEPS crossings and what the history needs after it. */
+struct LongWalkItem {
+    float ux, uy, uz, rr, tmx, tmy, tmz, tdx, tdy, tdz, tcur, s;
+    int ix, iy, iz, sx, sy, sz, it;
+    int i;                     /* launch index of the history                 */
+    float px, py, pz;
+    unsigned n;                /* aligned decay-stream position of the pair   */
+    double tstart;
+    int slot, pad;             /* phantom slot (batched runs)                 */
+};
+
 struct DevQueue {
     float *y, *z, *vx, *vy, *vz, *E;
     double *t;
@@ -110,6 +122,10 @@ struct DevQueue {
     unsigned *rcount;          /* [nregions] entries per region               */
     int64_t region;            /* slots per region                            */
     int nregions;
+    /* long positron walks: region r holds items [r * region / 8, ...) and
+     * lw_count[r] of them are used; null: walks are not deferred */
+    LongWalkItem *lw;
+    unsigned *lw_count;
 };
 
 /* Device-side counters, 64-bit, accumulated with atomics. */

@@ -161,6 +161,13 @@ struct RangeWalk {
     __device__ __forceinline__ void begin(const DevPhantom &p, float x, float y, float z, float ek_ev, int usedir,
                                           float vx, float vy, float vz, gp_rng &r)
     {
+        if (sample(ek_ev, usedir, vx, vy, vz, r)) setup(p, x, y, z);
+    }
+
+    /* displacement: range rr (mass thickness) and unit direction; false
+     * (nothing to walk) unless rr > 0 */
+    __device__ __forceinline__ bool sample(float ek_ev, int usedir, float vx, float vy, float vz, gp_rng &r)
+    {
         float ekin = ek_ev / 1.0e6f;
         float rex = ((0.1f * 5.44040782f) * ekin * ekin) / (0.369516529f + ekin);
         float sigma = rex / 2.0f;
@@ -177,11 +184,20 @@ struct RangeWalk {
         t = 0.0f;
         ux = uy = uz = 0.0f;
         done = true;
-        if (!(rr > 0.0f)) return;
+        if (!(rr > 0.0f)) return false;
         float inv_rr = 1.0f / rr;
         ux = dx * inv_rr;
         uy = dy * inv_rr;
         uz = dz * inv_rr;
+        return true;
+    }
+
+    /* the DDA from (x, y, z) along (ux, uy, uz); outside the phantom the
+     * walk ends at once in air */
+    __device__ __forceinline__ void setup(const DevPhantom &p, float x, float y, float z)
+    {
+        t = 0.0f;
+        done = true;
         ix = iy = iz = 0;
         if (!(p.present && d_vox(p, x, y, z, ix, iy, iz))) {
             t = rr / D_RHO_OUTSIDE;
@@ -204,22 +220,25 @@ struct RangeWalk {
      * length reaches rr (or the walk leaves the grid); there
      * t = tn + (rr - s)/rho, or tn when s == rr exactly (outside the grid
      * rho is D_RHO_OUTSIDE, so one rule covers both ends). */
+    template <bool UNIFORM = true>
     __device__ __forceinline__ void step4(const DevPhantom &p, unsigned &nsteps)
     {
         float tn[4];
         bool in[4];
         float g[4];
-        /* The phantom is wave-uniform: the bricked densities are read through
-         * a buffer descriptor with the array's size as its range, so the four
-         * gathers are issued unconditionally (a read past the range returns
-         * 0; out-of-grid results are replaced below) and the address
-         * arithmetic stays in 32 bits. */
-        const uint64_t base = (uint64_t)(uintptr_t)p.dens_b;
-        const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)base);
-        const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(base >> 32));
-        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-            (void *)(uintptr_t)(((uint64_t)hi << 32) | lo), (short)0,
-            (int)__builtin_amdgcn_readfirstlane(p.dens_bytes), 0x00020000);
+        /* UNIFORM (one phantom per wave): the bricked densities are read
+         * through a buffer descriptor with the array's size as its range, so
+         * the four gathers are issued unconditionally (a read past the range
+         * returns 0; out-of-grid results are replaced below) and the address
+         * arithmetic stays in 32 bits.  Otherwise plain guarded loads. */
+        __amdgpu_buffer_rsrc_t rs;
+        if (UNIFORM) {
+            const uint64_t base = (uint64_t)(uintptr_t)p.dens_b;
+            const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)base);
+            const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(base >> 32));
+            rs = __builtin_amdgcn_make_buffer_rsrc((void *)(uintptr_t)(((uint64_t)hi << 32) | lo), (short)0,
+                                                   (int)__builtin_amdgcn_readfirstlane(p.dens_bytes), 0x00020000);
+        }
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const bool bx = (tmx <= tmy) & (tmx <= tmz);
@@ -238,8 +257,12 @@ struct RangeWalk {
             const unsigned b = __umul24(__umul24((unsigned)iz >> 2, (unsigned)p.nby) + ((unsigned)iy >> 2),
                                         (unsigned)p.nbx) + ((unsigned)ix >> 2);
             const unsigned o = (b << 6) | (((unsigned)iz & 3u) << 4) | (((unsigned)iy & 3u) << 2) | ((unsigned)ix & 3u);
-            const float v = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, o << 2, 0, 0));
-            g[k] = in[k] ? v : D_RHO_OUTSIDE;
+            if (UNIFORM) {
+                const float v = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, o << 2, 0, 0));
+                g[k] = in[k] ? v : D_RHO_OUTSIDE;
+            } else {
+                g[k] = in[k] ? p.dens_b[o] : D_RHO_OUTSIDE;
+            }
         }
         float ftn = 0.0f, fs = 0.0f, frho = 1.0f;
         bool fin = false;
@@ -425,15 +448,32 @@ __device__ __forceinline__ bool d_decay_begin(const RunParams &rp, int64_t h, in
     return true;
 }
 
+template <int MODE>
+__device__ __forceinline__ void d_photons(const RunParams &rp, const DevTables &tab, const DevPhantom &ph,
+                                          const DevDetector &det, const float4 *s_pf, const DevPanel *s_p,
+                                          const DevQueue &q, unsigned *s_qn, int64_t h, float px, float py,
+                                          float pz, const float (&v)[2][3], const float (&e)[2], double tstart,
+                                          int nph, PhCounters &C);
+
 /* One history: decay (or PSF record i), positron range, pair, phantom walk
  * of both photons, panel entry and queue append. */
+/* Crossings a positron walk may take in the phantom kernel before it is
+ * handed to k_longwalk (multiple of 4).  Walk lengths have a long tail (a
+ * positron that leaves the water crosses the air voxels to the grid's
+ * edge), and a wave runs until its longest walk ends; deferring the tail
+ * lets the short walks' waves move on, and k_longwalk runs the long ones
+ * together. */
+#ifndef LW_STEPS
+#define LW_STEPS 32
+#endif
+
 template <int MODE>
 __device__ __forceinline__ void d_history(const RunParams &rp, const DevTables &tab, const DevPhantom &ph,
                                           const DevDetector &det, const float4 *s_pf, const DevPanel *s_p,
-                                          const DevQueue &q, unsigned *s_qn, int64_t i, int64_t h, int si,
-                                          PhCounters &C)
+                                          const DevQueue &q, unsigned *s_qn, unsigned *s_ln, int64_t i, int64_t h,
+                                          int si, int slot, PhCounters &C)
 {
-    unsigned long long &c_ph = C.ph, &c_ent = C.ent, &c_maj = C.maj, &c_ps = C.ps, &c_rs = C.rs;
+    unsigned long long &c_rs = C.rs;
     float px = 0.f, py = 0.f, pz = 0.f;
     float v[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
     float e[2] = {0.f, 0.f};
@@ -443,7 +483,40 @@ __device__ __forceinline__ void d_history(const RunParams &rp, const DevTables &
         gp_rng r;
         float ek = 0.0f;
         if (d_decay_begin(rp, h, si, r, tstart, px, py, pz, ek)) {
-            if (rp.prange) d_positron_range(ph, px, py, pz, ek, 0, 0.f, 0.f, 0.f, r, c_rs);
+            if (rp.prange) {
+                RangeWalk w;
+                w.begin(ph, px, py, pz, ek, 0, 0.f, 0.f, 0.f, r);
+                unsigned ns = 0;
+                if (q.lw) {
+                    for (int k = 0; k < LW_STEPS / 4 && !w.done; ++k) w.step4(ph, ns);
+                    if (!w.done) {
+                        const unsigned lcap = (unsigned)(q.region >> 3);
+                        const unsigned li = atomicAdd(s_ln, 1u);
+                        if (li < lcap) {
+                            LongWalkItem L;
+                            L.ux = w.ux; L.uy = w.uy; L.uz = w.uz; L.rr = w.rr;
+                            L.tmx = w.tmx; L.tmy = w.tmy; L.tmz = w.tmz;
+                            L.tdx = w.tdx; L.tdy = w.tdy; L.tdz = w.tdz;
+                            L.tcur = w.tcur; L.s = w.s;
+                            L.ix = w.ix; L.iy = w.iy; L.iz = w.iz;
+                            L.sx = w.sx; L.sy = w.sy; L.sz = w.sz;
+                            L.it = w.it;
+                            L.i = (int)i;
+                            L.px = px; L.py = py; L.pz = pz;
+                            L.n = (r.n + 3u) & ~3u;
+                            L.tstart = tstart;
+                            L.slot = slot;
+                            L.pad = 0;
+                            q.lw[(int64_t)blockIdx.x * lcap + li] = L;
+                            c_rs += ns;
+                            return;   /* k_longwalk finishes this history */
+                        }
+                    }
+                }
+                while (!w.done) w.step4(ph, ns);
+                c_rs += ns;
+                w.end(px, py, pz);
+            }
             gp_rng_align(&r);
             d_emit_pair(rp.sigma, r, v, e);
             nph = 2;
@@ -467,6 +540,19 @@ __device__ __forceinline__ void d_history(const RunParams &rp, const DevTables &
             nph = 2;
         }
     }
+    d_photons<MODE>(rp, tab, ph, det, s_pf, s_p, q, s_qn, h, px, py, pz, v, e, tstart, nph, C);
+}
+
+/* Both photons of history h (nph of them active) from (px, py, pz) at
+ * tstart: phantom walk, phase-space records, panel entry, queue append. */
+template <int MODE>
+__device__ __forceinline__ void d_photons(const RunParams &rp, const DevTables &tab, const DevPhantom &ph,
+                                          const DevDetector &det, const float4 *s_pf, const DevPanel *s_p,
+                                          const DevQueue &q, unsigned *s_qn, int64_t h, float px, float py,
+                                          float pz, const float (&v)[2][3], const float (&e)[2], double tstart,
+                                          int nph, PhCounters &C)
+{
+    unsigned long long &c_ph = C.ph, &c_ent = C.ent, &c_maj = C.maj, &c_ps = C.ps;
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const bool active = k < nph;
@@ -591,8 +677,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? W
 void k_phantom(RunParams rp, DevTables tab, DevPhantom ph, DevDetector det, DevQueue q)
 {
     PhCounters C;
-    __shared__ unsigned s_qn;
-    if (threadIdx.x == 0) s_qn = 0u;
+    __shared__ unsigned s_qn, s_ln;
+    if (threadIdx.x == 0) {
+        s_qn = 0u;
+        s_ln = 0u;
+    }
     /* panel table in LDS: the entry test scans every panel for every photon,
      * and from LDS the scan is not a chain of memory round trips */
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -611,7 +700,7 @@ void k_phantom(RunParams rp, DevTables tab, DevPhantom ph, DevDetector det, DevQ
         for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < rp.nh; i += stride) {
             const int64_t h = rp.h0 + i;
             const int si = (MODE == RUN_DECAY) ? d_source_of(rp, h) : 0;
-            d_history<MODE>(rp, tab, ph, det, s_pf, s_p, q, &s_qn, i, h, si, C);
+            d_history<MODE>(rp, tab, ph, det, s_pf, s_p, q, &s_qn, &s_ln, i, h, si, 0, C);
         }
     } else {
         for (int64_t vb = blockIdx.x; vb < rp.n_vblocks; vb += gridDim.x) {
@@ -622,13 +711,74 @@ void k_phantom(RunParams rp, DevTables tab, DevPhantom ph, DevDetector det, DevQ
             }
             const DevSegment S = rp.seg[lo];
             const int64_t off = (vb - S.vb0) * blockDim.x + threadIdx.x;
+            const DevPhantom P = rp.phs[S.phantom];
             if (off < S.nh) {
                 const int64_t h = S.h0 + off;
                 const int si = d_source_of(rp, h);
-                const DevPhantom P = rp.phs[S.phantom];
-                d_history<MODE>(rp, tab, P, det, s_pf, s_p, q, &s_qn, h - rp.h0, h, si, C);
+                d_history<MODE>(rp, tab, P, det, s_pf, s_p, q, &s_qn, &s_ln, h - rp.h0, h, si, S.phantom, C);
             }
         }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        q.rcount[blockIdx.x] = min((unsigned)q.region, s_qn);
+        if (q.lw) q.lw_count[blockIdx.x] = min((unsigned)(q.region >> 3), s_ln);
+    }
+    {
+        unsigned long long *const dst[5] = {rp.cnt + CNT_PHOTONS, rp.cnt + CNT_PANEL_ENTRIES, rp.cnt + CNT_MAJORANT,
+                                            rp.cnt + CNT_PHANTOM_STEPS, rp.cnt + CNT_RANGE_STEPS};
+        const unsigned long long v[5] = {C.ph, C.ent, C.maj, C.ps, C.rs};
+        flush_counters5(dst, v);
+    }
+}
+
+/* The long positron walks of one phantom-kernel block (same grid): finish
+ * each walk, emit the pair and transport both photons, appending to that
+ * block's queue region after the entries the phantom kernel left there. */
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_longwalk(RunParams rp, DevTables tab, DevPhantom ph, DevDetector det,
+                                                  DevQueue q)
+{
+    PhCounters C;
+    __shared__ unsigned s_qn;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    float4 *s_pf = reinterpret_cast<float4 *>(smem);
+    DevPanel *s_p = reinterpret_cast<DevPanel *>(smem + sizeof(float4) * (size_t)det.npanels);
+    if (threadIdx.x == 0) s_qn = q.rcount[blockIdx.x];
+    if (det.present) {
+        for (int t = threadIdx.x; t < det.npanels; t += blockDim.x) {
+            const DevPanel P = det.p[t];
+            s_pf[t] = make_float4(P.ox, P.oy, P.oz, P.bound2);
+            s_p[t] = P;
+        }
+    }
+    __syncthreads();
+    const unsigned lcap = (unsigned)(q.region >> 3);
+    const unsigned n = min(q.lw_count[blockIdx.x], lcap);
+    for (unsigned j = threadIdx.x; j < n; j += blockDim.x) {
+        const LongWalkItem L = q.lw[(int64_t)blockIdx.x * lcap + j];
+        const DevPhantom P = BATCH ? rp.phs[L.slot] : ph;
+        RangeWalk w;
+        w.ux = L.ux; w.uy = L.uy; w.uz = L.uz; w.rr = L.rr;
+        w.tmx = L.tmx; w.tmy = L.tmy; w.tmz = L.tmz;
+        w.tdx = L.tdx; w.tdy = L.tdy; w.tdz = L.tdz;
+        w.tcur = L.tcur; w.s = L.s;
+        w.ix = L.ix; w.iy = L.iy; w.iz = L.iz;
+        w.sx = L.sx; w.sy = L.sy; w.sz = L.sz;
+        w.it = L.it;
+        w.t = 0.0f;
+        w.done = false;
+        unsigned ns = 0;
+        while (!w.done) w.step4<!BATCH>(P, ns);
+        C.rs += ns;
+        float px = L.px, py = L.py, pz = L.pz;
+        w.end(px, py, pz);
+        const int64_t h = rp.h0 + L.i;
+        gp_rng r;
+        gp_rng_init(&r, rp.seed, (uint64_t)h, GP_TAG_DECAY, L.n);
+        float v[2][3], e[2];
+        d_emit_pair(rp.sigma, r, v, e);
+        d_photons<RUN_DECAY>(rp, tab, P, det, s_pf, s_p, q, &s_qn, h, px, py, pz, v, e, L.tstart, 2, C);
     }
     __syncthreads();
     if (threadIdx.x == 0) q.rcount[blockIdx.x] = min((unsigned)q.region, s_qn);
@@ -1027,6 +1177,15 @@ int gpet_launch_phantom(const RunParams &rp, const DevTables &tab, const DevPhan
     if (rp.mode == RUN_DECAY) {
         if (batch) launch_phantom_wpe<RUN_DECAY, true>(rp, tab, ph, det, q, grid, cfg.block, cfg.wpe, s);
         else launch_phantom_wpe<RUN_DECAY, false>(rp, tab, ph, det, q, grid, cfg.block, cfg.wpe, s);
+        if (rp.prange && q.lw) {
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return (int)e;
+            const size_t lds = det.present ? (sizeof(float4) + sizeof(DevPanel)) * (size_t)det.npanels : 0;
+            if (batch)
+                hipLaunchKernelGGL((k_longwalk<true>), dim3(grid), dim3(cfg.block), lds, s, rp, tab, ph, det, q);
+            else
+                hipLaunchKernelGGL((k_longwalk<false>), dim3(grid), dim3(cfg.block), lds, s, rp, tab, ph, det, q);
+        }
     } else if (rp.mode == RUN_PSF_POSITRON) {
         launch_phantom_wpe<RUN_PSF_POSITRON, false>(rp, tab, ph, det, q, grid, cfg.block, cfg.wpe, s);
     } else {

@@ -268,7 +268,7 @@ class Engine:
             self._ok(self.f["set_phantom_slot"](self.ctx, int(slot), mat.ctypes.data_as(C.c_void_p),
                                                 dens.ctypes.data_as(C.c_void_p), d, o, s), "set_phantom_slot")
 
-    def set_launch(self, block=(256, 256), wpe=(0, 0), blocks_per_cu=(4, 4), batch=1 << 22):
+    def set_launch(self, block=(256, 256), wpe=(0, 0), blocks_per_cu=(16, 2), batch=1 << 22):
         L = capi.Launch()
         L.block[0], L.block[1] = block
         L.wpe[0], L.wpe[1] = wpe
