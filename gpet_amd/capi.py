@@ -102,6 +102,12 @@ PSF_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("vx", "<f4"), (
                       ("E", "<f4"), ("parn", "<i4"), ("eventid", "<i4"), ("pad", "<i4"), ("t", "<f8")])
 assert PSF_DTYPE.itemsize == 48
 
+# list-mode coincidence record (gpet_lm_record, 32 B)
+LM_DTYPE = np.dtype([("crystal_a", "<u4"), ("crystal_b", "<u4"), ("t_us", "<f8"), ("dt_ps", "<f4"),
+                     ("e_a_kev", "<f4"), ("e_b_kev", "<f4"), ("flags", "<u4")])
+assert LM_DTYPE.itemsize == 32
+LM_TRUE, LM_DELAYED, LM_BACKGROUND = 1, 2, 4
+
 
 class Launch(C.Structure):
     _fields_ = [("block", C.c_int32 * 2), ("wpe", C.c_int32 * 2), ("blocks_per_cu", C.c_int32 * 2),
@@ -151,6 +157,7 @@ FLAG_NO_GRAPH = 1
 FLAG_AUTOTUNE = 2
 FLAG_PSF_SOURCE = 4
 FLAG_PSF_PHANTOM = 8
+FLAG_LISTMODE = 16
 
 
 class Input(C.Structure):
@@ -218,6 +225,10 @@ HOST_FUNCS = {
     "gpet_write_events": (C.c_int32, [C.c_char_p, _VP, C.c_int64, C.c_int32]),
     "gpet_write_hits": (C.c_int32, [C.c_char_p, C.c_char_p, _VP, _VP, C.c_int64, C.c_int32]),
     "gpet_write_psf": (C.c_int32, [C.c_char_p, C.c_char_p, _VP, C.c_int64, C.c_int32]),
+    "gpet_listmode_records": (C.c_int32, [_VP, C.c_int64, C.c_int32, C.c_int32, C.c_uint32, _VP]),
+    "gpet_write_listmode": (C.c_int32, [C.c_char_p, _VP, C.c_int64, _VP, C.c_int64, C.c_int32, C.c_int32,
+                                        C.c_int32, C.c_int32]),
+    "gpet_write_sinogram": (C.c_int32, [C.c_char_p, _VP, C.c_int32, C.c_int32, C.c_float]),
     "gpet_build_majorant": (C.c_int32, [P(Tables), _VP, _VP, P(C.c_float), P(C.c_float)]),
 }
 

@@ -11,7 +11,9 @@
  * --psf-source / --psf-phantom (outsource/idsource/timesource.dat and
  * outphantom/idphantom/timephantom.dat: OUTPUTPSF 1 / 2, gPET.cu:63-114),
  * --record-sphere (move photons leaving the phantom onto the input file's
- * record sphere: RECORDPSF -1), --autotune (pick launch shapes by timing).
+ * record sphere: RECORDPSF -1), --autotune (pick launch shapes by timing),
+ * --listmode (coincidences.lm + .lmh), --sinogram NS NPHI SMAX_CM
+ * (sinogram.s + .hs, and sinogram_delayed.* with a delayed window).
  */
 #include "../../../include/gpet.h"
 
@@ -46,6 +48,8 @@ int main(int argc, char **argv)
     bool hits = false, append = false, sphere = false;
     long long batch = 0;
     int32_t flags = 0;
+    int sino_s = 0, sino_phi = 0;
+    float sino_smax = 0.0f;
     for (int i = 1; i < argc; ++i) {
         if (!strcmp(argv[i], "--data") && i + 1 < argc) data_dir = argv[++i];
         else if (!strcmp(argv[i], "--out") && i + 1 < argc) out_dir = argv[++i];
@@ -56,11 +60,18 @@ int main(int argc, char **argv)
         else if (!strcmp(argv[i], "--psf-phantom")) flags |= GPET_FLAG_PSF_PHANTOM;
         else if (!strcmp(argv[i], "--autotune")) flags |= GPET_FLAG_AUTOTUNE;
         else if (!strcmp(argv[i], "--record-sphere")) sphere = true;
+        else if (!strcmp(argv[i], "--listmode")) flags |= GPET_FLAG_LISTMODE;
+        else if (!strcmp(argv[i], "--sinogram") && i + 3 < argc) {
+            sino_s = atoi(argv[++i]);
+            sino_phi = atoi(argv[++i]);
+            sino_smax = (float)atof(argv[++i]);
+        }
         else if (!input) input = argv[i];
     }
     if (!input) {
         printf("Please execute ./gpet input_file [--data DIR] [--out DIR] [--hits] [--append] [--batch N] "
-               "[--psf-source] [--psf-phantom] [--record-sphere] [--autotune]\n");
+               "[--psf-source] [--psf-phantom] [--record-sphere] [--autotune] [--listmode] "
+               "[--sinogram NS NPHI SMAX_CM]\n");
         return EXIT_FAILURE;
     }
     gpet_input in;
@@ -137,7 +148,25 @@ int main(int argc, char **argv)
     auto t1 = std::chrono::steady_clock::now();
     gpet_tallies t;
     memset(&t, 0, sizeof(t));
-    gpet_get_tallies(c, &t);
+    std::vector<int64_t> sino, sino_d;
+    if (sino_s > 0 && sino_phi > 0 && sino_smax > 0.0f) {
+        sino.assign((size_t)sino_s * sino_phi, 0);
+        sino_d.assign(sino.size(), 0);
+        t.sinogram = sino.data();
+        t.sinogram_delayed = sino_d.data();
+        t.n_s = sino_s;
+        t.n_phi = sino_phi;
+        t.s_max = sino_smax;
+    }
+    if ((st = gpet_get_tallies(c, &t))) return die("tallies", st, c);
+    if (!sino.empty()) {
+        if ((st = gpet_write_sinogram((out_dir + "/sinogram").c_str(), sino.data(), sino_s, sino_phi, sino_smax)))
+            return die("writing the sinogram", st, c);
+        if (dg.delay_us > 0.0 &&
+            (st = gpet_write_sinogram((out_dir + "/sinogram_delayed").c_str(), sino_d.data(), sino_s, sino_phi,
+                                      sino_smax)))
+            return die("writing the delayed sinogram", st, c);
+    }
     printf("{\"histories\": %lld, \"photons\": %lld, \"panel_entries\": %lld, \"hits\": %lld, \"adder\": %lld, "
            "\"threshold\": %lld, \"deadtime\": %lld, \"singles\": %lld, \"coincidences\": %lld, \"true_coincidences\": %lld, "
            "\"dropped_events\": %lld, \"majorant_violations\": %lld, \"background_singles\": %lld, "

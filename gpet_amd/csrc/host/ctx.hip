@@ -182,6 +182,10 @@ gpet_status write_outputs(gpet_ctx *c, const std::string &dir, const std::vector
     if (!st) st = gpet_write_events((dir + "/coincidences.dat").c_str(), coinc.data(), (int64_t)coinc.size(), app);
     if (!st && c->dig.delay_us > 0.0)
         st = gpet_write_events((dir + "/delayed.dat").c_str(), delayed.data(), (int64_t)delayed.size(), app);
+    if (!st && (c->opt.flags & GPET_FLAG_LISTMODE))
+        st = gpet_write_listmode((dir + "/coincidences.lm").c_str(), coinc.data(), (int64_t)coinc.size() / 2,
+                                 delayed.data(), (int64_t)delayed.size() / 2, c->ddet.moduleN, c->ddet.crystalN,
+                                 c->ddet.npanels, app);
     if (!st && c->opt.record_hits)
         st = gpet_write_hits((dir + "/HitsID.dat").c_str(), (dir + "/Hits.dat").c_str(), hid.data(), hf.data(),
                              (int64_t)hid.size() / 5, app);

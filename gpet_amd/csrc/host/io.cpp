@@ -7,6 +7,7 @@
  */
 #include "../../../include/gpet.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -614,6 +615,129 @@ gpet_status gpet_write_psf(const char *dir, const char *name, const gpet_psf_rec
     for (FILE *x : f)
         if (x && fclose(x) != 0) ok = false;
     return ok ? GPET_OK : GPET_ERR_IO;
+}
+
+gpet_status gpet_listmode_records(const gpet_event *pairs, int64_t n_pairs, int32_t module_n, int32_t crystal_n,
+                                  uint32_t flags, gpet_lm_record *out)
+{
+    if (n_pairs < 0 || (n_pairs > 0 && (!pairs || !out)) || module_n <= 0 || crystal_n <= 0) return GPET_ERR_ARG;
+    for (int64_t k = 0; k < n_pairs; ++k) {
+        const gpet_event &a = pairs[2 * k], &b = pairs[2 * k + 1];
+        gpet_lm_record &r = out[k];
+        r.crystal_a = (uint32_t)(((int64_t)a.pann * module_n + a.modn) * crystal_n + a.cryn);
+        r.crystal_b = (uint32_t)(((int64_t)b.pann * module_n + b.modn) * crystal_n + b.cryn);
+        r.t_us = a.t;
+        r.dt_ps = (float)((b.t - a.t) * 1.0e6);
+        r.e_a_kev = a.E * 1.0e-3f;
+        r.e_b_kev = b.E * 1.0e-3f;
+        r.flags = flags;
+        if (a.parn < 0 || b.parn < 0) r.flags |= GPET_LM_BACKGROUND;
+        else if (a.eventid == b.eventid && !(flags & GPET_LM_DELAYED)) r.flags |= GPET_LM_TRUE;
+    }
+    return GPET_OK;
+}
+
+gpet_status gpet_write_listmode(const char *path, const gpet_event *prompts, int64_t n_prompts,
+                                const gpet_event *delayed, int64_t n_delayed, int32_t module_n, int32_t crystal_n,
+                                int32_t n_panels, int32_t append)
+{
+    if (!path || n_prompts < 0 || n_delayed < 0 || (n_prompts > 0 && !prompts) || (n_delayed > 0 && !delayed) ||
+        module_n <= 0 || crystal_n <= 0 || n_panels <= 0)
+        return GPET_ERR_ARG;
+    /* counts of what the file already holds (append mode) */
+    int64_t old_p = 0, old_d = 0;
+    const std::string hpath = std::string(path) + "h";
+    if (append) {
+        FILE *h = fopen(hpath.c_str(), "r");
+        if (h) {
+            char line[512];
+            while (fgets(line, sizeof(line), h)) {
+                long long v = 0;
+                if (sscanf(line, "number of prompts := %lld", &v) == 1) old_p = v;
+                if (sscanf(line, "number of delayed := %lld", &v) == 1) old_d = v;
+            }
+            fclose(h);
+        }
+    }
+    FILE *f = fopen(path, append ? "ab" : "wb");
+    if (!f) return GPET_ERR_IO;
+    bool ok = true;
+    std::vector<gpet_lm_record> buf;
+    for (int pass = 0; pass < 2 && ok; ++pass) {
+        const gpet_event *ev = pass == 0 ? prompts : delayed;
+        const int64_t n = pass == 0 ? n_prompts : n_delayed;
+        const int64_t chunk = 1 << 20;
+        for (int64_t k0 = 0; k0 < n && ok; k0 += chunk) {
+            const int64_t m = std::min(chunk, n - k0);
+            buf.resize((size_t)m);
+            gpet_listmode_records(ev + 2 * k0, m, module_n, crystal_n, pass == 0 ? 0u : (uint32_t)GPET_LM_DELAYED,
+                                  buf.data());
+            ok = fwrite(buf.data(), sizeof(gpet_lm_record), (size_t)m, f) == (size_t)m;
+        }
+    }
+    if (fclose(f) != 0) ok = false;
+    if (!ok) return GPET_ERR_IO;
+    FILE *h = fopen(hpath.c_str(), "w");
+    if (!h) return GPET_ERR_IO;
+    const char *base = strrchr(path, '/');
+    base = base ? base + 1 : path;
+    fprintf(h, "!GPET LIST MODE :=\n");
+    fprintf(h, "name of data file := %s\n", base);
+    fprintf(h, "record size (bytes) := %d\n", (int)sizeof(gpet_lm_record));
+    fprintf(h, "record fields := uint32 crystal_a, uint32 crystal_b, float64 t_us, float32 dt_ps, "
+               "float32 e_a_kev, float32 e_b_kev, uint32 flags\n");
+    fprintf(h, "flags := 1 true (both singles from one decay), 2 delayed window, 4 background single\n");
+    fprintf(h, "crystal index := (panel * modules per panel + module) * crystals per module + crystal\n");
+    fprintf(h, "number of panels := %d\n", n_panels);
+    fprintf(h, "modules per panel := %d\n", module_n);
+    fprintf(h, "crystals per module := %d\n", crystal_n);
+    fprintf(h, "number of prompts := %lld\n", (long long)(old_p + n_prompts));
+    fprintf(h, "number of delayed := %lld\n", (long long)(old_d + n_delayed));
+    fprintf(h, "record order := %s\n", append ? "runs appended, each run prompts then delayed"
+                                               : "prompts (time order), then delayed");
+    fprintf(h, "!END OF GPET LIST MODE :=\n");
+    return fclose(h) == 0 ? GPET_OK : GPET_ERR_IO;
+}
+
+gpet_status gpet_write_sinogram(const char *prefix, const int64_t *sino, int32_t n_s, int32_t n_phi, float s_max)
+{
+    if (!prefix || !sino || n_s <= 0 || n_phi <= 0 || !(s_max > 0.0f)) return GPET_ERR_ARG;
+    const std::string p(prefix);
+    std::vector<float> v((size_t)n_s * n_phi);
+    for (size_t k = 0; k < v.size(); ++k) v[k] = (float)sino[k];
+    FILE *f = fopen((p + ".s").c_str(), "wb");
+    if (!f) return GPET_ERR_IO;
+    bool ok = fwrite(v.data(), sizeof(float), v.size(), f) == v.size();
+    if (fclose(f) != 0) ok = false;
+    if (!ok) return GPET_ERR_IO;
+    FILE *h = fopen((p + ".hs").c_str(), "w");
+    if (!h) return GPET_ERR_IO;
+    const char *base = strrchr(prefix, '/');
+    base = base ? base + 1 : prefix;
+    fprintf(h, "!INTERFILE :=\n");
+    fprintf(h, "!imaging modality := PT\n");
+    fprintf(h, "name of data file := %s.s\n", base);
+    fprintf(h, "!GENERAL DATA :=\n");
+    fprintf(h, "!GENERAL IMAGE DATA :=\n");
+    fprintf(h, "!type of data := PET\n");
+    fprintf(h, "imagedata byte order := LITTLEENDIAN\n");
+    fprintf(h, "!PET STUDY (General) :=\n");
+    fprintf(h, "!PET data type := Emission\n");
+    fprintf(h, "applied corrections := {None}\n");
+    fprintf(h, "!number format := float\n");
+    fprintf(h, "!number of bytes per pixel := 4\n");
+    fprintf(h, "number of dimensions := 2\n");
+    fprintf(h, "matrix axis label [2] := view\n");
+    fprintf(h, "!matrix size [2] := %d\n", n_phi);
+    fprintf(h, "matrix axis label [1] := tangential coordinate\n");
+    fprintf(h, "!matrix size [1] := %d\n", n_s);
+    fprintf(h, "scale factor (mm/pixel) [1] := %.9g\n", 20.0 * (double)s_max / n_s);
+    fprintf(h, "scale factor (rad/pixel) [2] := %.9g\n", 3.14159265358979323846 / n_phi);
+    fprintf(h, "tangential range (cm) := %.9g %.9g\n", -(double)s_max, (double)s_max);
+    fprintf(h, "view range (rad) := 0 3.14159265358979\n");
+    fprintf(h, "number of time frames := 1\n");
+    fprintf(h, "!END OF INTERFILE :=\n");
+    return fclose(h) == 0 ? GPET_OK : GPET_ERR_IO;
 }
 
 } /* extern "C" */

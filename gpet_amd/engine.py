@@ -174,6 +174,34 @@ def write_events(path: str, ev: np.ndarray, append: bool = False) -> None:
            "writing " + path)
 
 
+def listmode_records(pairs: np.ndarray, module_n: int, crystal_n: int, flags: int = 0) -> np.ndarray:
+    """List-mode records (capi.LM_DTYPE) of coincidence pairs ([n, 2] events)."""
+    pairs = np.ascontiguousarray(pairs, dtype=capi.EVENT_DTYPE).reshape(-1, 2)
+    out = np.zeros(len(pairs), capi.LM_DTYPE)
+    _check(host_lib().gpet_listmode_records(pairs.ctypes.data_as(C.c_void_p), len(pairs), module_n, crystal_n,
+                                            flags, out.ctypes.data_as(C.c_void_p)), "list-mode records")
+    return out
+
+
+def write_listmode(path: str, prompts: np.ndarray, delayed=None, module_n: int = 1, crystal_n: int = 1,
+                   n_panels: int = 1, append: bool = False) -> None:
+    """<path> (prompt pairs, then delayed pairs) and its header <path>h."""
+    p = np.ascontiguousarray(prompts, dtype=capi.EVENT_DTYPE).reshape(-1, 2)
+    d = np.ascontiguousarray(delayed if delayed is not None else np.zeros((0, 2), capi.EVENT_DTYPE),
+                             dtype=capi.EVENT_DTYPE).reshape(-1, 2)
+    _check(host_lib().gpet_write_listmode(path.encode(), p.ctypes.data_as(C.c_void_p), len(p),
+                                          d.ctypes.data_as(C.c_void_p), len(d), module_n, crystal_n, n_panels,
+                                          int(append)), "writing " + path)
+
+
+def write_sinogram(prefix: str, sino: np.ndarray, s_max: float) -> None:
+    """<prefix>.s (float32 [n_phi, n_s]) and the Interfile header <prefix>.hs."""
+    s = np.ascontiguousarray(sino, dtype=np.int64)
+    n_phi, n_s = s.shape
+    _check(host_lib().gpet_write_sinogram(prefix.encode(), s.ctypes.data_as(C.c_void_p), n_s, n_phi, float(s_max)),
+           "writing " + prefix)
+
+
 # ------------------------------------------------------------------ runs --
 
 @dataclass

@@ -154,8 +154,9 @@ enum {
                                   each kind and keep the fastest                */
     GPET_FLAG_PSF_SOURCE = 4,  /* record every emitted photon (OUTPUTPSF 1,
                                   gPET.cu:63-90)                                */
-    GPET_FLAG_PSF_PHANTOM = 8  /* record every photon leaving the phantom
+    GPET_FLAG_PSF_PHANTOM = 8, /* record every photon leaving the phantom
                                   (OUTPUTPSF 2, gPET.cu:91-114,296-346)         */
+    GPET_FLAG_LISTMODE = 16    /* out_dir also gets coincidences.lm/.lmh       */
 };
 
 /* Phase-space record.  The reference writes three parallel files per
@@ -376,6 +377,31 @@ gpet_status gpet_write_hits(const char *id_path, const char *f_path, const int32
  * id<name>.dat, time<name>.dat, name = "source" or "phantom". */
 gpet_status gpet_write_psf(const char *dir, const char *name, const gpet_psf_record *rec, int64_t n,
                            int32_t append);
+
+/* Exporters (new; the reference leaves its binary outputs to the MATLAB
+ * reader output/readOutput.m:1-54).  A list-mode record is one coincidence,
+ * 32 bytes, little endian: the two singles' global crystals
+ * ((panel * module_n + module) * crystal_n + crystal), the first single's
+ * time, the time difference (the TOF information), both energies and flags.
+ * gpet_write_listmode writes <path> (prompt pairs, then delayed-window pairs)
+ * and the text header <path>h (key := value lines) describing it. */
+typedef struct gpet_lm_record {
+    uint32_t crystal_a, crystal_b;
+    double t_us;                    /* time of single a                         */
+    float dt_ps;                    /* t_b - t_a                                */
+    float e_a_kev, e_b_kev;
+    uint32_t flags;                 /* GPET_LM_*                                */
+} gpet_lm_record;
+enum { GPET_LM_TRUE = 1, GPET_LM_DELAYED = 2, GPET_LM_BACKGROUND = 4 };
+gpet_status gpet_listmode_records(const gpet_event *pairs, int64_t n_pairs, int32_t module_n, int32_t crystal_n,
+                                  uint32_t flags, gpet_lm_record *out);
+gpet_status gpet_write_listmode(const char *path, const gpet_event *prompts, int64_t n_prompts,
+                                const gpet_event *delayed, int64_t n_delayed, int32_t module_n, int32_t crystal_n,
+                                int32_t n_panels, int32_t append);
+/* A sinogram tally (n_phi x n_s counts, views over [0, pi), tangential bins
+ * over [-s_max, s_max] cm) as float32 <prefix>.s with an Interfile-style
+ * header <prefix>.hs. */
+gpet_status gpet_write_sinogram(const char *prefix, const int64_t *sino, int32_t n_s, int32_t n_phi, float s_max);
 
 const char *gpet_status_string(gpet_status s);
 
