@@ -53,9 +53,10 @@ WORKLOADS = {
 
 
 def measured_traffic(wl):
-    """HBM bytes per history of k_phantom from the committed rocprofv3
-    counter summary of this workload (FETCH_SIZE x 2.0 + WRITE_SIZE, scaled
-    by a plain copy's counters), or None."""
+    """HBM bytes per history of the phantom stage (k_phantom + k_longwalk)
+    from the committed rocprofv3 counter summary of this workload
+    (FETCH_SIZE x 2.0 + WRITE_SIZE, scaled by a plain copy's counters), or
+    None."""
     import re
 
     name = wl.get("counters")
@@ -248,10 +249,12 @@ def main():
 
     M = S if S is not None else plain
     last = M["last"]
-    # dominant kernel: the decay + positron range + phantom kernel.
-    # Algorithmic bytes per history (4 B per positron-walk voxel, 8 B per
-    # Woodcock step in the phantom, one queue record per panel entry) times
-    # the histories of an average timed launch, over its average duration.
+    # dominant stage: the decay + positron range + phantom kernel and the
+    # kernel that finishes its long positron walks (launched back to back,
+    # timed together by the library's HIP events).  Algorithmic bytes per
+    # history (4 B per positron-walk voxel, 8 B per Woodcock step in the
+    # phantom, one queue record per panel entry) times the histories of an
+    # average timed launch, over its average duration.
     avg_launch = M["t_k"][0] / max(M["l_k"][0], 1)
     bytes_per_launch = M["b_alg"] / max(M["hist"], 1) * M["h_timed"] / max(M["l_k"][0], 1)
     achieved = bytes_per_launch / avg_launch / 1e9 if avg_launch > 0 else 0.0
@@ -283,7 +286,8 @@ def main():
                           "launches_per_step": last.launches},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_from": tsrc,
-                     "kernel": "k_phantom", "bytes_per_launch": bytes_per_launch, "avg_launch_ms": 1e3 * avg_launch},
+                     "kernel": "k_phantom + k_longwalk", "bytes_per_launch": bytes_per_launch,
+                     "avg_launch_ms": 1e3 * avg_launch},
     }
     if S is not None:
         flops = SURR_FLOPS_PER_ROW * last.n_surrogate

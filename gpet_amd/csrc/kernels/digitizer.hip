@@ -365,20 +365,15 @@ __global__ __launch_bounds__(256) void k_coinc(const gpet_event *s, int64_t n, d
                                                const unsigned *pos, gpet_event *out, unsigned long long *ntrue,
                                                const int64_t *jf, int nj)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     /* singles are in (job, t) order; a job change closes every window */
     auto same_job = [&](int64_t a, int64_t b) {
         return nj <= 1 || d_job_of(jf, nj, s[a].eventid) == d_job_of(jf, nj, s[b].eventid);
     };
-    bool start = (i == 0) || (s[i].t - s[i - 1].t > W) || !same_job(i, i - 1);
-    if (!start) {
-        if (!WRITE) cnt[i] = 0;
-        return;
-    }
+    const bool start = i < n && ((i == 0) || (s[i].t - s[i - 1].t > W) || !same_job(i, i - 1));
     unsigned np = 0, nt = 0;
-    unsigned o = WRITE ? pos[i] : 0u;
-    int64_t a = i;
+    const unsigned o = (WRITE && start) ? pos[i] : 0u;
+    int64_t a = start ? i : n;
     while (a < n) {
         int64_t b = a + 1;
         while (b < n && s[b].t - s[a].t <= W && same_job(b, a)) ++b;
@@ -397,8 +392,15 @@ __global__ __launch_bounds__(256) void k_coinc(const gpet_event *s, int64_t n, d
         a = b;
         if (a >= n || s[a].t - s[a - 1].t > W || !same_job(a, a - 1)) break;
     }
-    if (!WRITE) cnt[i] = np;
-    else if (nt) atomicAdd(ntrue, (unsigned long long)nt);
+    if (!WRITE) {
+        if (i < n) cnt[i] = np;
+    } else {
+        /* true pairs: one atomic per wave (a per-cluster atomic on one
+         * address serializes when there are many clusters) */
+        unsigned long long v = nt;
+        for (int k = 32; k > 0; k >>= 1) v += __shfl_xor(v, k);
+        if (__lane_id() == 0 && v) atomicAdd(ntrue, v);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_tally(const gpet_event *s, int64_t n, int moduleN, int crystalN,

@@ -1,19 +1,22 @@
 """Kernel summary of a rocprofv3 database (rocpd SQLite), as text.
 
-usage: python -m gpet_amd.rocpd_summary <results.db> [top_n]
-Prints, per kernel: dispatches, total and average duration, share of GPU
-time, and VGPR/SGPR/scratch/LDS of the dispatch.
+usage: python -m gpet_amd.rocpd_summary <results.db> [top_n] [--by-grid]
+Prints, per kernel (per kernel and grid size with --by-grid, which keeps the
+launch-shape tuner's trial dispatches apart from the chosen shape's): the
+dispatches, total and average duration, share of GPU time, and
+VGPR/SGPR/scratch/LDS of the dispatch.
 """
 import sqlite3
 import sys
 
 
-def summary(path, top=15):
+def summary(path, top=15, by_grid=False):
     db = sqlite3.connect(path)
+    group = "name, grid_x" if by_grid else "name"
     rows = db.execute(
         "select name, count(*), sum(duration), avg(duration), max(vgpr_count), max(sgpr_count), "
-        "max(scratch_size), max(lds_size), max(workgroup_x), max(grid_x) from kernels group by name "
-        "order by sum(duration) desc").fetchall()
+        "max(scratch_size), max(lds_size), max(workgroup_x), max(grid_x) from kernels group by %s "
+        "order by sum(duration) desc" % group).fetchall()
     total = sum(r[2] for r in rows) or 1
     out = ["| kernel | calls | total us | avg us | % GPU | VGPR | SGPR | scratch B | LDS B | block | grid |",
            "|---|---|---|---|---|---|---|---|---|---|---|"]
@@ -25,4 +28,5 @@ def summary(path, top=15):
 
 
 if __name__ == "__main__":
-    print(summary(sys.argv[1], int(sys.argv[2]) if len(sys.argv) > 2 else 15))
+    args = [a for a in sys.argv[1:] if not a.startswith("--")]
+    print(summary(args[0], int(args[1]) if len(args) > 1 else 15, "--by-grid" in sys.argv))
