@@ -543,7 +543,7 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
     const bool batched = bi != nullptr;
     c->last_surr_items = 0;
     c->last_surr_seconds = 0;
-    if ((st = dalloc(c, c->d_small, 4)) || (st = dalloc(c, c->d_cnt, CNT_NUM))) return st;
+    if ((st = dalloc(c, c->d_small, 4)) || (st = dalloc(c, c->d_cnt, CNT_NUM * CNT_STRIPES))) return st;
 
     rp.seed = c->opt.seed;
     rp.eabs = c->opt.eabs;
@@ -665,10 +665,11 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
 
     unsigned small[4] = {0, 0, 0, 0};
     unsigned long long cnt[CNT_NUM];
+    std::vector<unsigned long long> cnt_s((size_t)CNT_NUM * CNT_STRIPES);
     bool used_graph = false;
     for (int attempt = 0;; ++attempt) {
         HIPOK(c, hipMemsetAsync(c->d_small.p, 0, sizeof(unsigned) * 4, c->stream));
-        HIPOK(c, hipMemsetAsync(c->d_cnt.p, 0, sizeof(unsigned long long) * CNT_NUM, c->stream));
+        HIPOK(c, hipMemsetAsync(c->d_cnt.p, 0, sizeof(unsigned long long) * CNT_NUM * CNT_STRIPES, c->stream));
         if (pmode == PANEL_SURROGATE) HIPOK(c, hipMemsetAsync(pp.sq.count, 0, sizeof(int), c->stream));
         if (pmode == PANEL_HARVEST) HIPOK(c, hipMemsetAsync(pp.harvest_count, 0, sizeof(unsigned long long), c->stream));
         if (rp.n_psf) HIPOK(c, hipMemsetAsync(rp.n_psf, 0, 2 * sizeof(unsigned long long), c->stream));
@@ -697,11 +698,16 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
         }
         HIPOK(c, hipEventRecord(c->ev1, c->stream));
         HIPOK(c, hipMemcpyAsync(small, c->d_small.p, sizeof(small), hipMemcpyDeviceToHost, c->stream));
-        HIPOK(c, hipMemcpyAsync(cnt, c->d_cnt.p, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+        HIPOK(c, hipMemcpyAsync(cnt_s.data(), c->d_cnt.p, sizeof(unsigned long long) * cnt_s.size(),
+                                hipMemcpyDeviceToHost, c->stream));
         int n_surr = 0;
         if (pmode == PANEL_SURROGATE)
             HIPOK(c, hipMemcpyAsync(&n_surr, pp.sq.count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPOK(c, hipStreamSynchronize(c->stream));
+        for (int k = 0; k < CNT_NUM; ++k) {
+            cnt[k] = 0;
+            for (int j = 0; j < CNT_STRIPES; ++j) cnt[k] += cnt_s[(size_t)k * CNT_STRIPES + j];
+        }
         const bool ev_ok = (int64_t)small[1] <= c->d_events.cap;
         const bool sq_ok = (int64_t)n_surr <= pp.sq.cap;
         if (ev_ok && sq_ok) {
@@ -730,7 +736,14 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
     if (pmode == PANEL_HARVEST) {
         unsigned long long nh = 0;
         HIPOK(c, hipMemcpy(&nh, c->surr.set_count.p, sizeof(nh), hipMemcpyDeviceToHost));
-        c->surr.set_n += std::min<int64_t>((int64_t)nh, pp.harvest_cap);
+        const int64_t added = std::min<int64_t>((int64_t)nh, pp.harvest_cap);
+        /* the panel kernel appends records in scheduling order; put them in
+         * photon order so that training on them is reproducible */
+        char serr[256] = {0};
+        if (gpet_sort_records(c->dw, pp.harvest, SURR_REC, 11, added, c->stream, serr, sizeof(serr)))
+            return fail(c, GPET_ERR_DEVICE, std::string("sorting harvested records: ") + serr);
+        HIPOK(c, hipStreamSynchronize(c->stream));
+        c->surr.set_n += added;
     }
 
     /* background singles join the adder output before blur */

@@ -133,6 +133,11 @@ enum {
     CNT_PHOTONS = 0, CNT_PANEL_ENTRIES, CNT_HITS, CNT_DROPPED_EVENTS, CNT_DROPPED_HITS,
     CNT_MAJORANT, CNT_PHANTOM_STEPS, CNT_RANGE_STEPS, CNT_PANEL_STEPS, CNT_NUM
 };
+/* Each counter is striped over CNT_STRIPES words (counter k of block b goes
+ * to cnt[k * CNT_STRIPES + b % CNT_STRIPES]; the host sums the stripes):
+ * atomics on one address serialize, and thousands of blocks finishing
+ * together would queue on it. */
+#define CNT_STRIPES 64
 
 enum { RUN_DECAY = 0, RUN_PSF_POSITRON = 1, RUN_PSF_PHOTON = 2 };
 
@@ -153,7 +158,7 @@ struct RunParams {
     const double *psf;         /* mode 1/2: records of 8 doubles for [h0,...)  */
     gpet_trace *trace;         /* device trace buffer or null                  */
     int64_t trace_cap;         /* slots: 2*(h - 0) < trace_cap are written     */
-    unsigned long long *cnt;   /* [CNT_NUM]                                    */
+    unsigned long long *cnt;   /* [CNT_NUM * CNT_STRIPES]                      */
     float sphere[4];           /* recording sphere (r = sphere[3] > 0: on)     */
     gpet_psf_record *psf_src, *psf_ph; /* phase-space outputs or null         */
     unsigned long long *n_psf;  /* [2] appended records                         */
@@ -300,6 +305,10 @@ DigiWork *gpet_digi_work_create();
 void gpet_digi_work_destroy(DigiWork *w);
 int gpet_digitize(DigiWork *w, const DigiParams &p, const gpet_event *events, int64_t n,
                   DigiOut *out, hipStream_t s, char *err, int errlen);
+/* Stable sort of n records of `stride` floats by the non-negative int in
+ * field key_field (ascending). */
+int gpet_sort_records(DigiWork *w, float *rec, int stride, int key_field, int64_t n, hipStream_t s, char *err,
+                      int errlen);
 /* Count (into *total, host) then write the background singles. */
 int gpet_digi_noise_count(DigiWork *w, const NoiseParams &p, int64_t *total, hipStream_t s, char *err, int errlen);
 int gpet_digi_noise_write(DigiWork *w, const NoiseParams &p, gpet_event *out, hipStream_t s, char *err, int errlen);

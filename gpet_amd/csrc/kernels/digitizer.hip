@@ -361,9 +361,10 @@ __global__ __launch_bounds__(256) void k_gather_singles(const gpet_event *ev, co
  * opens at the first single not inside the previous window and holds the
  * singles with t - t_open <= W; exactly two -> a pair, more -> policy. */
 template <bool WRITE>
-__global__ __launch_bounds__(256) void k_coinc(const gpet_event *s, int64_t n, double W, int policy, unsigned *cnt,
-                                               const unsigned *pos, gpet_event *out, unsigned long long *ntrue,
-                                               const int64_t *jf, int nj)
+__global__ __launch_bounds__(256) void k_coinc(const gpet_event *__restrict__ s, int64_t n, double W, int policy,
+                                               unsigned *__restrict__ cnt, const unsigned *__restrict__ pos,
+                                               unsigned *__restrict__ out, unsigned long long *ntrue,
+                                               const int64_t *__restrict__ jf, int nj)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     /* singles are in (job, t) order; a job change closes every window */
@@ -382,9 +383,10 @@ __global__ __launch_bounds__(256) void k_coinc(const gpet_event *s, int64_t n, d
             for (int64_t p = a; p < b; ++p)
                 for (int64_t q = p + 1; q < b; ++q) {
                     if (WRITE) {
-                        out[2 * (int64_t)(o + np)] = s[p];
-                        out[2 * (int64_t)(o + np) + 1] = s[q];
+                        /* the pair as two indices; k_gather_pairs copies the records */
                         if (s[p].eventid == s[q].eventid && s[p].parn >= 0 && s[q].parn >= 0) nt++;
+                        out[2 * (int64_t)(o + np)] = (unsigned)p;
+                        out[2 * (int64_t)(o + np) + 1] = (unsigned)q;
                     }
                     np++;
                 }
@@ -395,11 +397,17 @@ __global__ __launch_bounds__(256) void k_coinc(const gpet_event *s, int64_t n, d
     if (!WRITE) {
         if (i < n) cnt[i] = np;
     } else {
-        /* true pairs: one atomic per wave (a per-cluster atomic on one
-         * address serializes when there are many clusters) */
+        /* true pairs: one atomic per block into one of CNT_STRIPES words
+         * (atomics on one address serialize when many blocks end together) */
+        __shared__ unsigned long long s_nt[4];
         unsigned long long v = nt;
         for (int k = 32; k > 0; k >>= 1) v += __shfl_xor(v, k);
-        if (__lane_id() == 0 && v) atomicAdd(ntrue, v);
+        if (__lane_id() == 0) s_nt[threadIdx.x >> 6] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned long long t = s_nt[0] + s_nt[1] + s_nt[2] + s_nt[3];
+            if (t) atomicAdd(ntrue + (blockIdx.x % CNT_STRIPES), t);
+        }
     }
 }
 
@@ -549,9 +557,10 @@ __global__ __launch_bounds__(256) void k_noise(NoiseDev p, unsigned *cnt, const 
  * it are a delayed pair; with more, the multiples policy decides as for a
  * prompt window holding the opener and those singles. */
 template <bool WRITE>
-__global__ __launch_bounds__(256) void k_delayed(const gpet_event *s, int64_t n, double W, double D, int policy,
-                                                 unsigned *cnt, const unsigned *pos, gpet_event *out,
-                                                 const int64_t *jf, int nj)
+__global__ __launch_bounds__(256) void k_delayed(const gpet_event *__restrict__ s, int64_t n, double W, double D,
+                                                 int policy, unsigned *__restrict__ cnt,
+                                                 const unsigned *__restrict__ pos, unsigned *__restrict__ out,
+                                                 const int64_t *__restrict__ jf, int nj)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -587,8 +596,8 @@ __global__ __launch_bounds__(256) void k_delayed(const gpet_event *s, int64_t n,
             for (int64_t p = -1; p < m; ++p)
                 for (int64_t q = p + 1; q < m; ++q) {
                     if (WRITE) {
-                        out[2 * (int64_t)(o + np)] = s[p < 0 ? a : hi + p];
-                        out[2 * (int64_t)(o + np) + 1] = s[hi + q];
+                        out[2 * (int64_t)(o + np)] = (unsigned)(p < 0 ? a : hi + p);
+                        out[2 * (int64_t)(o + np) + 1] = (unsigned)(hi + q);
                     }
                     np++;
                 }
@@ -600,6 +609,35 @@ __global__ __launch_bounds__(256) void k_delayed(const gpet_event *s, int64_t n,
         if (a >= n || s[a].t - s[a - 1].t > W || !same_job(a, a - 1)) break;
     }
     if (!WRITE) cnt[i] = np;
+}
+
+/* Records of `stride` floats keyed by the int in field `kf`: sort keys. */
+__global__ __launch_bounds__(256) void k_rec_keys(const float *rec, int stride, int kf, int64_t n,
+                                                  unsigned long long *key, unsigned *val)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        key[i] = (unsigned long long)__float_as_uint(rec[i * stride + kf]);
+        val[i] = (unsigned)i;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_rec_gather(const float *rec, int stride, const unsigned *perm, float *out,
+                                                    int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n * stride) {
+        const int64_t r = i / stride, f = i - r * stride;
+        out[i] = rec[(int64_t)perm[r] * stride + f];
+    }
+}
+
+/* Pair records from the sorters' index pairs: one thread per output record. */
+__global__ __launch_bounds__(256) void k_gather_pairs(const gpet_event *__restrict__ s, const unsigned *__restrict__ idx,
+                                                      gpet_event *__restrict__ out, int64_t m)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < m) out[k] = s[idx[k]];
 }
 
 inline int grid_for(int64_t n, int block) { return (int)((n + block - 1) / block); }
@@ -618,6 +656,10 @@ struct DigiWork {
     gpet_event *singles = nullptr, *coinc = nullptr, *delayed = nullptr;
     int64_t singles_cap = 0, coinc_cap = 0, delayed_cap = 0;
     unsigned long long *ntrue = nullptr;
+    unsigned *pidx = nullptr;
+    int64_t pidx_cap = 0;
+    float *rec_tmp = nullptr;
+    int64_t rec_tmp_cap = 0;
     unsigned *noise_cnt = nullptr, *noise_pos = nullptr;
     int64_t noise_cnt_cap = 0, noise_pos_cap = 0;
     NoiseJob *noise_jobs = nullptr;
@@ -689,7 +731,7 @@ void gpet_digi_work_destroy(DigiWork *w)
     if (!w) return;
     void *ptrs[] = {w->ev_a,  w->ev_b,  w->flag,      w->pos,  w->alive,   w->key_a,   w->key_b,
                     w->val_a, w->val_b, w->tile_sums, w->hist, w->grand,   w->singles, w->coinc,
-                    w->ntrue, w->delayed, w->noise_cnt, w->noise_pos, w->noise_jobs};
+                    w->ntrue, w->delayed, w->noise_cnt, w->noise_pos, w->noise_jobs, w->pidx, w->rec_tmp};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete w;
@@ -717,8 +759,8 @@ int gpet_digitize(DigiWork *w, const DigiParams &p, const gpet_event *events, in
     int64_t ce = w->ev_cap, ce2 = w->ev_cap;
     if (ensure(w->ev_a, ce, need, err, errlen) || ensure(w->ev_b, ce2, need, err, errlen)) return -1;
     w->ev_cap = std::min(ce, ce2);
-    if (!w->ntrue) HIPCHK(hipMalloc((void **)&w->ntrue, sizeof(unsigned long long)));
-    HIPCHK(hipMemsetAsync(w->ntrue, 0, sizeof(unsigned long long), s));
+    if (!w->ntrue) HIPCHK(hipMalloc((void **)&w->ntrue, sizeof(unsigned long long) * CNT_STRIPES));
+    HIPCHK(hipMemsetAsync(w->ntrue, 0, sizeof(unsigned long long) * CNT_STRIPES, s));
 
     /* blur + thresholder, compaction (input order kept) */
     unsigned nT = 0;
@@ -809,14 +851,21 @@ int gpet_digitize(DigiWork *w, const DigiParams &p, const gpet_event *events, in
         unsigned *cnt = w->flag;
         unsigned *cpos = w->pos;
         hipLaunchKernelGGL(k_coinc<false>, dim3(grid_for(ns, 256)), dim3(256), 0, s, w->singles, (int64_t)ns,
-                           g.coinc_window_us, g.coinc_policy, cnt, (const unsigned *)nullptr, (gpet_event *)nullptr,
+                           g.coinc_window_us, g.coinc_policy, cnt, (const unsigned *)nullptr, (unsigned *)nullptr,
                            (unsigned long long *)nullptr, jf, nj);
         HIPCHK(hipGetLastError());
         if (scan(w, cnt, cpos, ns, &npairs, s, err, errlen)) return -1;
-        if (ensure(w->coinc, w->coinc_cap, std::max<int64_t>(2 * (int64_t)npairs, 2), err, errlen)) return -1;
+        const int64_t m = 2 * (int64_t)npairs;
+        if (ensure(w->coinc, w->coinc_cap, std::max<int64_t>(m, 2), err, errlen) ||
+            ensure(w->pidx, w->pidx_cap, std::max<int64_t>(m, 2), err, errlen))
+            return -1;
         hipLaunchKernelGGL(k_coinc<true>, dim3(grid_for(ns, 256)), dim3(256), 0, s, w->singles, (int64_t)ns,
-                           g.coinc_window_us, g.coinc_policy, cnt, cpos, w->coinc, w->ntrue, jf, nj);
+                           g.coinc_window_us, g.coinc_policy, cnt, cpos, w->pidx, w->ntrue, jf, nj);
         HIPCHK(hipGetLastError());
+        if (m > 0) {
+            hipLaunchKernelGGL(k_gather_pairs, dim3(grid_for(m, 256)), dim3(256), 0, s, w->singles, w->pidx, w->coinc, m);
+            HIPCHK(hipGetLastError());
+        }
     }
     /* delayed window: same openers, window shifted by delay_us */
     unsigned nd = 0;
@@ -825,13 +874,21 @@ int gpet_digitize(DigiWork *w, const DigiParams &p, const gpet_event *events, in
         unsigned *cpos = w->pos;
         hipLaunchKernelGGL(k_delayed<false>, dim3(grid_for(ns, 256)), dim3(256), 0, s, w->singles, (int64_t)ns,
                            g.coinc_window_us, g.delay_us, g.coinc_policy, cnt, (const unsigned *)nullptr,
-                           (gpet_event *)nullptr, jf, nj);
+                           (unsigned *)nullptr, jf, nj);
         HIPCHK(hipGetLastError());
         if (scan(w, cnt, cpos, ns, &nd, s, err, errlen)) return -1;
-        if (ensure(w->delayed, w->delayed_cap, std::max<int64_t>(2 * (int64_t)nd, 2), err, errlen)) return -1;
+        const int64_t m = 2 * (int64_t)nd;
+        if (ensure(w->delayed, w->delayed_cap, std::max<int64_t>(m, 2), err, errlen) ||
+            ensure(w->pidx, w->pidx_cap, std::max<int64_t>(m, 2), err, errlen))
+            return -1;
         hipLaunchKernelGGL(k_delayed<true>, dim3(grid_for(ns, 256)), dim3(256), 0, s, w->singles, (int64_t)ns,
-                           g.coinc_window_us, g.delay_us, g.coinc_policy, cnt, cpos, w->delayed, jf, nj);
+                           g.coinc_window_us, g.delay_us, g.coinc_policy, cnt, cpos, w->pidx, jf, nj);
         HIPCHK(hipGetLastError());
+        if (m > 0) {
+            hipLaunchKernelGGL(k_gather_pairs, dim3(grid_for(m, 256)), dim3(256), 0, s, w->singles, w->pidx,
+                               w->delayed, m);
+            HIPCHK(hipGetLastError());
+        }
     }
     out->delayed = w->delayed;
     out->n_delayed = nd;
@@ -841,8 +898,10 @@ int gpet_digitize(DigiWork *w, const DigiParams &p, const gpet_event *events, in
         HIPCHK(hipGetLastError());
     }
     unsigned long long nt = 0;
-    HIPCHK(hipMemcpyAsync(&nt, w->ntrue, sizeof(nt), hipMemcpyDeviceToHost, s));
+    unsigned long long ntv[CNT_STRIPES];
+    HIPCHK(hipMemcpyAsync(ntv, w->ntrue, sizeof(ntv), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    for (int k = 0; k < CNT_STRIPES; ++k) nt += ntv[k];
     out->n_coinc = npairs;
     out->n_coinc_true = (int64_t)nt;
     out->coinc = w->coinc;
@@ -924,6 +983,31 @@ int gpet_digi_noise_write(DigiWork *w, const NoiseParams &p, gpet_event *out, hi
     hipLaunchKernelGGL(k_noise<true>, dim3(grid_for(p.nchunks, 256)), dim3(256), 0, s, d, w->noise_cnt,
                        (const unsigned *)w->noise_pos, out);
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int gpet_sort_records(DigiWork *w, float *rec, int stride, int key_field, int64_t n, hipStream_t s, char *err,
+                      int errlen)
+{
+    if (n <= 1) return 0;
+    if (n > w->cap) {
+        int64_t c4 = w->cap, c5 = w->cap, c6 = w->cap, c7 = w->cap, c1 = w->cap, c2 = w->cap, c3 = w->cap;
+        if (ensure(w->flag, c1, n, err, errlen) || ensure(w->pos, c2, n, err, errlen) ||
+            ensure(w->alive, c3, n, err, errlen) || ensure(w->key_a, c4, n, err, errlen) ||
+            ensure(w->key_b, c5, n, err, errlen) || ensure(w->val_a, c6, n, err, errlen) ||
+            ensure(w->val_b, c7, n, err, errlen))
+            return -1;
+        w->cap = std::min({c1, c2, c3, c4, c5, c6, c7});
+    }
+    if (ensure(w->rec_tmp, w->rec_tmp_cap, n * stride, err, errlen)) return -1;
+    hipLaunchKernelGGL(k_rec_keys, dim3(grid_for(n, 256)), dim3(256), 0, s, rec, stride, key_field, n, w->key_a,
+                       w->val_a);
+    HIPCHK(hipGetLastError());
+    if (radix_sort(w, n, 32, s, err, errlen)) return -1;
+    hipLaunchKernelGGL(k_rec_gather, dim3(grid_for(n * stride, 256)), dim3(256), 0, s, rec, stride, w->val_a,
+                       w->rec_tmp, n);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(rec, w->rec_tmp, sizeof(float) * (size_t)(n * stride), hipMemcpyDeviceToDevice, s));
     return 0;
 }
 

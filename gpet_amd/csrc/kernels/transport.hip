@@ -50,6 +50,19 @@ __device__ __forceinline__ int wave_append(int *counter, bool want)
     return base + (int)__popcll(m & lanemask_lt());
 }
 
+/* The same for a 64-bit or unsigned counter, every calling lane appending
+ * one item (one atomic per wave instead of one per lane). */
+template <class T>
+__device__ __forceinline__ T wave_append1(T *counter)
+{
+    const unsigned long long m = __ballot(1);
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    T base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(counter, (T)__popcll(m));
+    base = __shfl(base, leader);
+    return base + (T)__popcll(m & lanemask_lt());
+}
+
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
 {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -71,7 +84,7 @@ __device__ __forceinline__ void flush_counters5(unsigned long long *const dst[5]
     if (threadIdx.x < 5) {
         unsigned long long s = 0;
         for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += s_red[i][threadIdx.x];
-        if (s) atomicAdd(dst[threadIdx.x], s);
+        if (s) atomicAdd(dst[threadIdx.x] + (blockIdx.x % CNT_STRIPES), s);
     }
 }
 
@@ -353,26 +366,22 @@ __device__ bool d_phantom_walk(const DevTables &tab, const DevPhantom &p, float 
         if (prob < 0.0f) { prob = 0.0f; nmaj++; }
         float rn = gp_u01(b.y);
         if (rn < prob) continue;
-        prob = prob + lamden * mu.y;
-        if (rn < prob) {
-            float cth = d_surf(tab, GPET_SURF_COMPTON, mat, E, gp_u01(b.z));
-            float efrac = 1.0f / (1.0f + (E * D_IMC2F) * (1.0f - cth));
-            float uphi = gp_u01(b.w);
+        /* Compton and Rayleigh share one path (surface lookup, rotation), so
+         * a wave that holds both pays for it once */
+        const float pc = prob + lamden * mu.y;
+        const bool comp = rn < pc;
+        if (!comp && !(rn < pc + lamden * mu.z)) return false;   /* photoelectric */
+        const float cth = d_surf(tab, comp ? GPET_SURF_COMPTON : GPET_SURF_RAYLEIGH, mat, E, gp_u01(b.z));
+        const float uphi = gp_u01(b.w);
+        if (comp) {
+            const float efrac = 1.0f / (1.0f + (E * D_IMC2F) * (1.0f - cth));
             ncomp++;
             E = E * efrac;
             if (E < eabs) return false;
-            d_rotate(vx, vy, vz, cth, uphi);
-            continue;
-        }
-        prob = prob + lamden * mu.z;
-        if (rn < prob) {
-            float cth = d_surf(tab, GPET_SURF_RAYLEIGH, mat, E, gp_u01(b.z));
-            float uphi = gp_u01(b.w);
+        } else {
             nrayl++;
-            d_rotate(vx, vy, vz, cth, uphi);
-            continue;
         }
-        return false;
+        d_rotate(vx, vy, vz, cth, uphi);
     }
     return false;
 }
@@ -399,7 +408,7 @@ __device__ __forceinline__ void d_psf_append(gpet_psf_record *buf, unsigned long
                                              float y, float z, float vx, float vy, float vz, float E, int parn,
                                              int eid, double t)
 {
-    const unsigned long long i = atomicAdd(count, 1ull);
+    const unsigned long long i = wave_append1(count);
     if ((int64_t)i < cap) {
         gpet_psf_record r;
         r.x = x; r.y = y; r.z = z; r.vx = vx; r.vy = vy; r.vz = vz; r.E = E;
@@ -725,8 +734,10 @@ void k_phantom(RunParams rp, DevTables tab, DevPhantom ph, DevDetector det, DevQ
         if (q.lw) q.lw_count[blockIdx.x] = min((unsigned)(q.region >> 3), s_ln);
     }
     {
-        unsigned long long *const dst[5] = {rp.cnt + CNT_PHOTONS, rp.cnt + CNT_PANEL_ENTRIES, rp.cnt + CNT_MAJORANT,
-                                            rp.cnt + CNT_PHANTOM_STEPS, rp.cnt + CNT_RANGE_STEPS};
+        unsigned long long *const dst[5] = {
+            rp.cnt + CNT_PHOTONS * CNT_STRIPES, rp.cnt + CNT_PANEL_ENTRIES * CNT_STRIPES,
+            rp.cnt + CNT_MAJORANT * CNT_STRIPES, rp.cnt + CNT_PHANTOM_STEPS * CNT_STRIPES,
+            rp.cnt + CNT_RANGE_STEPS * CNT_STRIPES};
         const unsigned long long v[5] = {C.ph, C.ent, C.maj, C.ps, C.rs};
         flush_counters5(dst, v);
     }
@@ -783,8 +794,10 @@ __global__ __launch_bounds__(256) void k_longwalk(RunParams rp, DevTables tab, D
     __syncthreads();
     if (threadIdx.x == 0) q.rcount[blockIdx.x] = min((unsigned)q.region, s_qn);
     {
-        unsigned long long *const dst[5] = {rp.cnt + CNT_PHOTONS, rp.cnt + CNT_PANEL_ENTRIES, rp.cnt + CNT_MAJORANT,
-                                            rp.cnt + CNT_PHANTOM_STEPS, rp.cnt + CNT_RANGE_STEPS};
+        unsigned long long *const dst[5] = {
+            rp.cnt + CNT_PHOTONS * CNT_STRIPES, rp.cnt + CNT_PANEL_ENTRIES * CNT_STRIPES,
+            rp.cnt + CNT_MAJORANT * CNT_STRIPES, rp.cnt + CNT_PHANTOM_STEPS * CNT_STRIPES,
+            rp.cnt + CNT_RANGE_STEPS * CNT_STRIPES};
         const unsigned long long v[5] = {C.ph, C.ent, C.maj, C.ps, C.rs};
         flush_counters5(dst, v);
     }
@@ -1011,7 +1024,7 @@ void k_panel(PanelParams pp, DevTables tab, DevDetector det, DevQueue q)
                     const float ed = rec ? de2 : (type == 3 ? 0.0f : edep);
                     c_hits++;
                     if (pp.hit_id) {
-                        unsigned hidx = atomicAdd(pp.n_hits, 1u);
+                        const unsigned hidx = wave_append1(pp.n_hits);
                         if ((int64_t)hidx < pp.hits_cap) {
                             int32_t *hi = pp.hit_id + 5 * (size_t)hidx;
                             float *hf = pp.hit_f + 5 * (size_t)hidx;
@@ -1101,7 +1114,7 @@ void k_panel(PanelParams pp, DevTables tab, DevDetector det, DevQueue q)
                     rec[8] = dep;
                     rec[9] = tdy;
                     rec[10] = tdz;
-                    rec[11] = 0.0f;
+                    rec[11] = __int_as_float(parn);   /* sort key: records come out in photon order */
                 }
             }
         }
@@ -1118,8 +1131,10 @@ void k_panel(PanelParams pp, DevTables tab, DevDetector det, DevQueue q)
         }
     }
     {
-        unsigned long long *const dst[5] = {pp.cnt + CNT_HITS, pp.cnt + CNT_DROPPED_EVENTS, pp.cnt + CNT_DROPPED_HITS,
-                                            pp.cnt + CNT_MAJORANT, pp.cnt + CNT_PANEL_STEPS};
+        unsigned long long *const dst[5] = {
+            pp.cnt + CNT_HITS * CNT_STRIPES, pp.cnt + CNT_DROPPED_EVENTS * CNT_STRIPES,
+            pp.cnt + CNT_DROPPED_HITS * CNT_STRIPES, pp.cnt + CNT_MAJORANT * CNT_STRIPES,
+            pp.cnt + CNT_PANEL_STEPS * CNT_STRIPES};
         const unsigned long long v[5] = {c_hits, c_drop, c_droph, c_maj, c_steps};
         flush_counters5(dst, v);
     }
