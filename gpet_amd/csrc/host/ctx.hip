@@ -1068,7 +1068,8 @@ gpet_status gpet_set_phantom_slot(gpet_ctx *c, int32_t slot, const int32_t *mat,
     }
     {
         /* the positron walk addresses the bricked densities in 32 bits */
-        const int64_t nb = (int64_t)((dims[0] + 3) / 4) * ((dims[1] + 3) / 4) * ((dims[2] + 3) / 4);
+        const int64_t nb = (int64_t)gpet_padded_bricks(dims[0]) * gpet_padded_bricks(dims[1]) *
+                           gpet_padded_bricks(dims[2]);
         if (nb >= ((int64_t)1 << 24)) return fail(c, GPET_ERR_CAPACITY, "phantom above 2^24 bricks of 4^3 voxels");
     }
     PhSlot &S = c->slots[slot];
@@ -1078,6 +1079,7 @@ gpet_status gpet_set_phantom_slot(gpet_ctx *c, int32_t slot, const int32_t *mat,
     for (size_t i = 0; i < nv; ++i) {
         int m = mat[i];
         if (m < 0 || m >= c->nmat) return fail(c, GPET_ERR_ARG, "phantom material out of range");
+        if (dens[i] < 0.0f) return fail(c, GPET_ERR_ARG, "negative phantom density");
         if (dens[i] > rmax[m]) rmax[m] = dens[i];
         int b;
         memcpy(&b, &dens[i], 4);
@@ -1085,13 +1087,16 @@ gpet_status gpet_set_phantom_slot(gpet_ctx *c, int32_t slot, const int32_t *mat,
     }
     gpet_status st = upload(c, S.vox, vox.data(), (int64_t)nv);
     if (st) return st;
-    const int nbx = (dims[0] + 3) / 4, nby = (dims[1] + 3) / 4, nbz = (dims[2] + 3) / 4;
+    const int nbx = gpet_padded_bricks(dims[0]), nby = gpet_padded_bricks(dims[1]), nbz = gpet_padded_bricks(dims[2]);
     {
-        std::vector<float> db((size_t)nbx * nby * nbz * 64, 0.0f);
+        /* padded bricked copy for the positron walk: the border (and the
+         * bricks' slack beyond it) holds the negative outside density */
+        std::vector<float> db((size_t)nbx * nby * nbz * 64, -D_RHO_OUTSIDE);
         for (int iz = 0; iz < dims[2]; ++iz)
             for (int iy = 0; iy < dims[1]; ++iy)
                 for (int ix = 0; ix < dims[0]; ++ix)
-                    db[gpet_brick_offset(ix, iy, iz, nbx, nby)] = dens[((size_t)iz * dims[1] + iy) * dims[0] + ix];
+                    db[gpet_brick_offset(ix + 1, iy + 1, iz + 1, nbx, nby)] =
+                        dens[((size_t)iz * dims[1] + iy) * dims[0] + ix];
         if ((st = upload(c, S.dens_b, db.data(), (int64_t)db.size()))) return st;
     }
     DevPhantom &p = S.dp;

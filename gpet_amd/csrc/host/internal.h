@@ -35,25 +35,36 @@ struct DevTables {
     const float *surf[2];      /* [nmat][ncp][ne]                             */
 };
 
+/* density outside the phantom for the positron walk (gPET_kernals.cu:403) */
+#define D_RHO_OUTSIDE 0.0012905f
+
 struct DevPhantom {
     int nx, ny, nz;
     float offx, offy, offz, dx, dy, dz, idx, idy, idz;
     const int2 *vox;           /* {float bits of density, material}           */
-    const float *dens_b;       /* density in 4x4x4 bricks (positron walk)     */
-    int nbx, nby;              /* bricks along x and y                        */
+    const float *dens_b;       /* padded density in 4x4x4 bricks (positron
+                                  walk); the border layer holds -D_RHO_OUTSIDE */
+    int nbx, nby;              /* bricks along x and y (padded grid)          */
     unsigned dens_bytes;       /* size of dens_b (fewer than 2^24 bricks)     */
     DevWck wck;
     int present;
 };
 
-/* Offset of voxel (ix, iy, iz) in the bricked density array: 64 voxels per
- * brick, x fastest inside a brick, bricks x fastest.  A 128-byte line holds
- * a 4 x 4 x 2 block, so the short random walks of positrons reuse lines. */
+/* Offset (in floats) of voxel (ix, iy, iz) of the padded grid in the bricked
+ * density array: 64 voxels per brick, x fastest inside a brick, bricks x
+ * fastest.  A 128-byte line holds a 4 x 4 x 2 block, so the short random
+ * walks of positrons reuse lines.  The padded grid has one extra layer on
+ * every side (phantom voxel (i, j, k) is padded voxel (i+1, j+1, k+1)); the
+ * layer holds -D_RHO_OUTSIDE, so a walk learns that it left the phantom from
+ * the sign of the density it reads. */
 __host__ __device__ __forceinline__ size_t gpet_brick_offset(int ix, int iy, int iz, int nbx, int nby)
 {
     size_t b = ((size_t)(iz >> 2) * nby + (size_t)(iy >> 2)) * nbx + (size_t)(ix >> 2);
     return (b << 6) | (size_t)(((iz & 3) << 4) | ((iy & 3) << 2) | (ix & 3));
 }
+
+/* Bricks of the padded grid along an axis of n voxels. */
+__host__ __device__ __forceinline__ int gpet_padded_bricks(int n) { return (n + 2 + 3) / 4; }
 
 struct DevPanel {
     int panel;
@@ -104,7 +115,8 @@ struct DevSegment {
  * walk state after LW_STEPS crossings and what the history needs after it. */
 struct LongWalkItem {
     float ux, uy, uz, rr, tmx, tmy, tmz, tdx, tdy, tdz, tcur, s;
-    int ix, iy, iz, sx, sy, sz, it;
+    unsigned o;                /* byte offset of the current voxel in dens_b  */
+    int it;
     int i;                     /* launch index of the history                 */
     float px, py, pz;
     unsigned n;                /* aligned decay-stream position of the pair   */

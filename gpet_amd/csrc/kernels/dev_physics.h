@@ -19,7 +19,6 @@
 #define D_IMC2F 1.95695061e-06f    /* constants.h:35 */
 #define D_SZERO 1.0e-4f
 #define D_ZEROF 1.0e-20f
-#define D_RHO_OUTSIDE 0.0012905f   /* gPET_kernals.cu:403 */
 #define D_STEP_CAP 1000000
 
 /* Linear interpolation on a uniform grid (replaces 1-D texture filtering,

@@ -166,7 +166,13 @@ __device__ __forceinline__ void d_dda_axis(float x, float off, float d, int i, f
 struct RangeWalk {
     float ux, uy, uz, rr;
     float tmx, tmy, tmz, tdx, tdy, tdz;
-    int ix, iy, iz, sx, sy, sz;
+    /* o: byte offset of the current voxel in the padded bricked densities;
+     * d?i / d?c: change of o for a crossing along an axis that stays inside
+     * the brick / enters the next brick; flip: the local-coordinate bits of
+     * the axes walked in the + direction (a crossing enters the next brick
+     * when the axis' local coordinate is 3 going up or 0 going down). */
+    unsigned o, flip;
+    int dxi, dyi, dzi, dxc, dyc, dzc;
     float tcur, s, t;
     int it;
     bool done;
@@ -205,13 +211,29 @@ struct RangeWalk {
         return true;
     }
 
+    /* offset steps of the walk direction (signs of ux, uy, uz, as the DDA's
+     * steps) in a padded grid of nbx x nby bricks per z layer */
+    __device__ __forceinline__ void deltas(int nbx, int nby)
+    {
+        const int sx = ux > 0.0f ? 1 : (ux < 0.0f ? -1 : 0);
+        const int sy = uy > 0.0f ? 1 : (uy < 0.0f ? -1 : 0);
+        const int sz = uz > 0.0f ? 1 : (uz < 0.0f ? -1 : 0);
+        dxi = sx * 4;
+        dyi = sy * 16;
+        dzi = sz * 64;
+        dxc = sx * (256 - 12);
+        dyc = sy * (256 * nbx - 48);
+        dzc = sz * (256 * nbx * nby - 192);
+        flip = (sx > 0 ? 12u : 0u) | (sy > 0 ? 48u : 0u) | (sz > 0 ? 192u : 0u);
+    }
+
     /* the DDA from (x, y, z) along (ux, uy, uz); outside the phantom the
      * walk ends at once in air */
     __device__ __forceinline__ void setup(const DevPhantom &p, float x, float y, float z)
     {
         t = 0.0f;
         done = true;
-        ix = iy = iz = 0;
+        int ix = 0, iy = 0, iz = 0, sx, sy, sz;
         if (!(p.present && d_vox(p, x, y, z, ix, iy, iz))) {
             t = rr / D_RHO_OUTSIDE;
             return;
@@ -219,6 +241,8 @@ struct RangeWalk {
         d_dda_axis(x, p.offx, p.dx, ix, ux, tmx, tdx, sx);
         d_dda_axis(y, p.offy, p.dy, iy, uy, tmy, tdy, sy);
         d_dda_axis(z, p.offz, p.dz, iz, uz, tmz, tdz, sz);
+        o = (unsigned)gpet_brick_offset(ix + 1, iy + 1, iz + 1, p.nbx, p.nby) << 2;
+        deltas(p.nbx, p.nby);
         tcur = 0.0f;
         s = 0.0f;
         it = 0;
@@ -230,20 +254,21 @@ struct RangeWalk {
      * (identical arithmetic, 4x memory parallelism).  Written with selects,
      * not branches: lanes of a wave take different axes at every crossing.
      * The walk ends at the crossing where the accumulated density-weighted
-     * length reaches rr (or the walk leaves the grid); there
-     * t = tn + (rr - s)/rho, or tn when s == rr exactly (outside the grid
-     * rho is D_RHO_OUTSIDE, so one rule covers both ends). */
+     * length reaches rr, or at the first crossing out of the phantom, which
+     * reads the border's -D_RHO_OUTSIDE; there t = tn + (rr - s)/rho, or tn
+     * when s == rr exactly (rho = |density|, so one rule covers both ends).
+     * Crossings after the end in the same call read whatever lies there and
+     * are ignored. */
     template <bool UNIFORM = true>
     __device__ __forceinline__ void step4(const DevPhantom &p, unsigned &nsteps)
     {
         float tn[4];
-        bool in[4];
         float g[4];
-        /* UNIFORM (one phantom per wave): the bricked densities are read
-         * through a buffer descriptor with the array's size as its range, so
-         * the four gathers are issued unconditionally (a read past the range
-         * returns 0; out-of-grid results are replaced below) and the address
-         * arithmetic stays in 32 bits.  Otherwise plain guarded loads. */
+        /* UNIFORM (one phantom per wave): the densities are read through a
+         * buffer descriptor with the array's size as its range (a read past
+         * the range, possible only after the end, returns 0) and 32-bit
+         * offsets.  Otherwise plain loads at offsets clamped into the
+         * array. */
         __amdgpu_buffer_rsrc_t rs;
         if (UNIFORM) {
             const uint64_t base = (uint64_t)(uintptr_t)p.dens_b;
@@ -252,6 +277,12 @@ struct RangeWalk {
             rs = __builtin_amdgcn_make_buffer_rsrc((void *)(uintptr_t)(((uint64_t)hi << 32) | lo), (short)0,
                                                    (int)__builtin_amdgcn_readfirstlane(p.dens_bytes), 0x00020000);
         }
+        /* values, not members, in the selects below: a select between
+         * members becomes a load through a selected address, and the walk
+         * state then lives in scratch memory */
+        const int Dxi = dxi, Dyi = dyi, Dzi = dzi, Dxc = dxc, Dyc = dyc, Dzc = dzc;
+        const unsigned F = flip;
+        unsigned O = o;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const bool bx = (tmx <= tmy) & (tmx <= tmz);
@@ -262,39 +293,37 @@ struct RangeWalk {
             tmx = bx ? nx_ : tmx;
             tmy = by ? ny_ : tmy;
             tmz = bz ? nz_ : tmz;
-            ix += bx ? sx : 0;
-            iy += by ? sy : 0;
-            iz += bz ? sz : 0;
-            in[k] = ((unsigned)ix < (unsigned)p.nx) & ((unsigned)iy < (unsigned)p.ny) & ((unsigned)iz < (unsigned)p.nz);
-            /* brick index with 24-bit multiplies (fewer than 2^24 bricks) */
-            const unsigned b = __umul24(__umul24((unsigned)iz >> 2, (unsigned)p.nby) + ((unsigned)iy >> 2),
-                                        (unsigned)p.nbx) + ((unsigned)ix >> 2);
-            const unsigned o = (b << 6) | (((unsigned)iz & 3u) << 4) | (((unsigned)iy & 3u) << 2) | ((unsigned)ix & 3u);
+            const unsigned msk = bx ? 12u : (by ? 48u : 192u);
+            const bool wrap = ((O ^ F) & msk) == 0u;
+            const int di = bx ? Dxi : (by ? Dyi : Dzi);
+            const int dc = bx ? Dxc : (by ? Dyc : Dzc);
+            O += (unsigned)(wrap ? dc : di);
             if (UNIFORM) {
-                const float v = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, o << 2, 0, 0));
-                g[k] = in[k] ? v : D_RHO_OUTSIDE;
+                g[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, O, 0, 0));
             } else {
-                g[k] = in[k] ? p.dens_b[o] : D_RHO_OUTSIDE;
+                const unsigned oc = min(O, p.dens_bytes - 4u);
+                g[k] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(p.dens_b) + oc);
             }
         }
+        o = O;
+        const bool was = done;
         float ftn = 0.0f, fs = 0.0f, frho = 1.0f;
-        bool fin = false;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const float rho = g[k];
+            const float rho = fabsf(g[k]);
+            const bool inb = !(g[k] < 0.0f);
             const float sn = s + (tn[k] - tcur) * rho;
-            const bool live = !done;
-            const bool end = live & ((sn >= rr) | !in[k]);
-            nsteps += (live & in[k]) ? 1u : 0u;
-            s = live ? sn : s;
-            tcur = (live & !end) ? tn[k] : tcur;
-            ftn = end ? tn[k] : ftn;
-            fs = end ? sn : fs;
-            frho = end ? rho : frho;
-            fin = fin | end;
+            const bool end = (sn >= rr) | !inb;
+            const bool first = end & !done;
+            nsteps += (!done & inb) ? 1u : 0u;
+            ftn = first ? tn[k] : ftn;
+            fs = first ? sn : fs;
+            frho = first ? rho : frho;
             done = done | end;
+            s = sn;
+            tcur = tn[k];
         }
-        if (fin) t = (fs == rr) ? ftn : ftn + (rr - fs) / frho;
+        if (done & !was) t = (fs == rr) ? ftn : ftn + (rr - fs) / frho;
         it += 4;
         if (it >= D_STEP_CAP) done = true;
     }
@@ -507,8 +536,7 @@ __device__ __forceinline__ void d_history(const RunParams &rp, const DevTables &
                             L.tmx = w.tmx; L.tmy = w.tmy; L.tmz = w.tmz;
                             L.tdx = w.tdx; L.tdy = w.tdy; L.tdz = w.tdz;
                             L.tcur = w.tcur; L.s = w.s;
-                            L.ix = w.ix; L.iy = w.iy; L.iz = w.iz;
-                            L.sx = w.sx; L.sy = w.sy; L.sz = w.sz;
+                            L.o = w.o;
                             L.it = w.it;
                             L.i = (int)i;
                             L.px = px; L.py = py; L.pz = pz;
@@ -612,13 +640,20 @@ __device__ __forceinline__ void d_photons(const RunParams &rp, const DevTables &
                 /* conservative prefilter: the line through the photon must
                  * pass within the face's bounding sphere (covers the
                  * reference's backward entries too); skipping a panel here
-                 * never changes which panel the exact test picks */
+                 * never changes which panel the exact test picks.  The
+                 * squared distance loses about 2^-22 w2 to rounding, and the
+                 * exact test's own rounding moves its crossing point by a
+                 * similar relative amount, so the bound grows with w2 (1e-5
+                 * w2: nothing at scanner distances, and photons many metres
+                 * away -- after a long flight through an empty phantom --
+                 * are not skipped where the exact test would still accept a
+                 * face).  An overflow to inf or NaN never skips. */
                 {
                     const float4 f = s_pf[pi];
                     float wx = f.x - x, wy = f.y - y, wz = f.z - z;
                     float a = vx * wx + vy * wy + vz * wz;
                     float w2 = wx * wx + wy * wy + wz * wz;
-                    if (w2 * vv - a * a > f.w * vv) continue;
+                    if (w2 * vv - a * a > (f.w + 1.0e-5f * w2) * vv) continue;
                 }
                 const DevPanel &P = s_p[pi];
                 float rx = x - P.ox, ry = y - P.oy, rz = z - P.oz;
@@ -774,8 +809,8 @@ __global__ __launch_bounds__(256) void k_longwalk(RunParams rp, DevTables tab, D
         w.tmx = L.tmx; w.tmy = L.tmy; w.tmz = L.tmz;
         w.tdx = L.tdx; w.tdy = L.tdy; w.tdz = L.tdz;
         w.tcur = L.tcur; w.s = L.s;
-        w.ix = L.ix; w.iy = L.iy; w.iz = L.iz;
-        w.sx = L.sx; w.sy = L.sy; w.sz = L.sz;
+        w.o = L.o;
+        w.deltas(P.nbx, P.nby);
         w.it = L.it;
         w.t = 0.0f;
         w.done = false;

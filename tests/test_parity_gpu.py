@@ -191,3 +191,54 @@ def test_noise_and_delayed_parity(tables, policy, dead):
     assert same_bytes(a.delayed, b.delayed), "delayed: " + first_diff(a.delayed.ravel(), b.delayed.ravel())
     assert a.arrays[3].sum() > 0
     assert_parity(a, b)
+
+
+@pytest.mark.parametrize("dims", [(7, 5, 9), (1, 1, 1), (13, 4, 6)])
+def test_positron_walk_small_grids(tables, dims):
+    """Positron walks on grids whose sizes are not multiples of the 4-voxel
+    brick, with empty (zero-density) voxels, from a box source filling the
+    phantom: walks that cross empty voxels, end inside or leave through
+    every face (on the 1-voxel grid every walk leaves at its first crossing
+    and goes on in air) agree with the oracle history by history."""
+    from gpet_amd import engine
+    from gpet_amd import workloads as W
+
+    rng = np.random.default_rng(sum(dims))
+    nx, ny, nz = dims
+    mat = rng.choice([W.AIR, W.WATER, W.BONE], size=(nz, ny, nx)).astype(np.int32)
+    den = rng.choice(np.array([0.0, 0.3, 1.0, 1.85], np.float32), size=(nz, ny, nx))
+    vox = 0.01
+    size = (nx * vox, ny * vox, nz * vox)
+    off = tuple(-s / 2 for s in size)
+    src = [W.source(0, c3=size[0] * 0.98, c4=size[1] * 0.98, c5=size[2] * 0.98, natom=1.0e11)]
+    iso = [W.isotope_f18()]
+    n = 20000
+    opt = engine.make_options(seed=31, positron_range=1, batch=6000)
+
+    def setup(e):
+        W.setup_engine(e, tables, config=2, phantom_n=8)
+        e.set_phantom(mat, den, dims, off, size)
+
+    a, b = run_both(tables, opt, setup,
+                    lambda e: e.run_source(src, iso, 0.0, 10.0, n_decays=n, singles_cap=n, coinc_cap=n,
+                                           adder_cap=4 * n, trace_cap=n))
+    assert a.tallies.n_singles > 100
+    if min(dims) > 1:
+        assert a.tallies.range_steps > n
+    assert_parity(a, b)
+
+
+def test_negative_density_rejected(tables):
+    from gpet_amd import engine
+    from gpet_amd import workloads as W
+
+    e = hip_engine(engine.make_options(seed=1))
+    try:
+        e.set_tables(tables)
+        mat, den, dims, off, size = W.cylinder_phantom(8)
+        den = den.copy()
+        den[3, 4, 5] = -1.0
+        with pytest.raises(engine.GpetError, match="negative phantom density"):
+            e.set_phantom(mat, den, dims, off, size)
+    finally:
+        e.close()
