@@ -23,6 +23,14 @@
 #define GP_HD static inline
 #endif
 
+/* Three-way xor: one v_bitop3_b32 (truth table 0x96) in gfx950 code, where
+ * the compiler would otherwise issue two v_xor_b32; plain C elsewhere. */
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GP_XOR3(a, b, c) __builtin_amdgcn_bitop3_b32((a), (b), (c), 0x96)
+#else
+#define GP_XOR3(a, b, c) ((a) ^ (b) ^ (c))
+#endif
+
 #define GP_PHILOX_M0 0xD2511F53u
 #define GP_PHILOX_M1 0xCD9E8D57u
 #define GP_PHILOX_W0 0x9E3779B9u
@@ -41,9 +49,9 @@ GP_HD gp_u4 gp_philox4x32_10(gp_u4 c, uint32_t k0, uint32_t k1)
         uint64_t p0 = (uint64_t)GP_PHILOX_M0 * (uint64_t)c.x;
         uint64_t p1 = (uint64_t)GP_PHILOX_M1 * (uint64_t)c.z;
         gp_u4 n;
-        n.x = (uint32_t)(p1 >> 32) ^ c.y ^ k0;
+        n.x = GP_XOR3((uint32_t)(p1 >> 32), c.y, k0);
         n.y = (uint32_t)p1;
-        n.z = (uint32_t)(p0 >> 32) ^ c.w ^ k1;
+        n.z = GP_XOR3((uint32_t)(p0 >> 32), c.w, k1);
         n.w = (uint32_t)p0;
         c = n;
         k0 += GP_PHILOX_W0;

@@ -163,6 +163,11 @@ __device__ __forceinline__ void d_dda_axis(float x, float off, float d, int i, f
 /* The walk as a state: begin() samples the displacement and sets up the DDA
  * (or finishes at once outside the phantom), step4() advances four
  * crossings, end() moves the point. */
+/* crossings per walk step (gathers in flight per lane) */
+#ifndef WALK_K
+#define WALK_K 4
+#endif
+
 struct RangeWalk {
     float ux, uy, uz, rr;
     float tmx, tmy, tmz, tdx, tdy, tdz;
@@ -262,8 +267,8 @@ struct RangeWalk {
     template <bool UNIFORM = true>
     __device__ __forceinline__ void step4(const DevPhantom &p, unsigned &nsteps)
     {
-        float tn[4];
-        float g[4];
+        float tn[WALK_K];
+        float g[WALK_K];
         /* UNIFORM (one phantom per wave): the densities are read through a
          * buffer descriptor with the array's size as its range (a read past
          * the range, possible only after the end, returns 0) and 32-bit
@@ -284,7 +289,7 @@ struct RangeWalk {
         const unsigned F = flip;
         unsigned O = o;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < WALK_K; ++k) {
             const bool bx = (tmx <= tmy) & (tmx <= tmz);
             const bool by = !bx & (tmy <= tmz);
             const bool bz = !bx & !by;
@@ -309,7 +314,7 @@ struct RangeWalk {
         const bool was = done;
         float ftn = 0.0f, fs = 0.0f, frho = 1.0f;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < WALK_K; ++k) {
             const float rho = fabsf(g[k]);
             const bool inb = !(g[k] < 0.0f);
             const float sn = s + (tn[k] - tcur) * rho;
@@ -324,7 +329,7 @@ struct RangeWalk {
             tcur = tn[k];
         }
         if (done & !was) t = (fs == rr) ? ftn : ftn + (rr - fs) / frho;
-        it += 4;
+        it += WALK_K;
         if (it >= D_STEP_CAP) done = true;
     }
 
@@ -526,7 +531,7 @@ __device__ __forceinline__ void d_history(const RunParams &rp, const DevTables &
                 w.begin(ph, px, py, pz, ek, 0, 0.f, 0.f, 0.f, r);
                 unsigned ns = 0;
                 if (q.lw) {
-                    for (int k = 0; k < LW_STEPS / 4 && !w.done; ++k) w.step4(ph, ns);
+                    for (int k = 0; k < LW_STEPS / WALK_K && !w.done; ++k) w.step4(ph, ns);
                     if (!w.done) {
                         const unsigned lcap = (unsigned)(q.region >> 3);
                         const unsigned li = atomicAdd(s_ln, 1u);
