@@ -916,6 +916,9 @@ __device__ __forceinline__ void d_readout(EvList &L, int n, int depth, int polic
     }
 }
 
+/* events staged in LDS per panel-kernel block (48 B each) */
+#define EV_STAGE 512u
+
 /* MODE: PANEL_PLAIN transports every queued photon; PANEL_SURROGATE lists
  * the photons inside the network's validity range for the surrogate and
  * transports the rest; PANEL_HARVEST transports every photon and also writes
@@ -925,6 +928,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? W
 void k_panel(PanelParams pp, DevTables tab, DevDetector det, DevQueue q)
 {
     unsigned long long c_hits = 0, c_drop = 0, c_droph = 0, c_maj = 0, c_steps = 0;
+    /* the block's events are staged in LDS and written out together at the
+     * end with one reservation in the global event array (many waves adding
+     * to one global counter serialize); a wave whose events do not fit
+     * writes the rest straight to the array */
+    __shared__ gpet_event s_ev[EV_STAGE];
+    __shared__ unsigned s_evn, s_evbase;
+    if (threadIdx.x == 0) s_evn = 0u;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     DevPanel *s_p = reinterpret_cast<DevPanel *>(smem);
     unsigned *s_pre = reinterpret_cast<unsigned *>(smem + sizeof(DevPanel) * (size_t)det.npanels);
@@ -1101,24 +1111,34 @@ void k_panel(PanelParams pp, DevTables tab, DevDetector det, DevQueue q)
             tot += (int)__popcll(b);
         }
         /* leader = first active lane (lanes that handed their photon to
-         * the surrogate have left this iteration) */
-        unsigned base = 0;
+         * the surrogate have left this iteration); the wave's events take
+         * LDS slots [lb, lb + nl) and, past the stage, global slots from gb */
+        unsigned lb = 0, nl = 0, gb = 0;
         if (tot) {
             const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
-            if ((int)__lane_id() == leader) base = atomicAdd(pp.n_events, (unsigned)tot);
-            base = __shfl(base, leader);
+            if ((int)__lane_id() == leader) {
+                lb = atomicAdd(&s_evn, (unsigned)tot);
+                nl = lb < EV_STAGE ? min((unsigned)tot, EV_STAGE - lb) : 0u;
+                if (nl < (unsigned)tot) gb = atomicAdd(pp.n_events, (unsigned)tot - nl);
+            }
+            lb = __shfl(lb, leader);
+            nl = __shfl(nl, leader);
+            gb = __shfl(gb, leader);
         }
         int o = 0;
 #pragma unroll
         for (int j = 0; j < GPET_MAX_PH_EVENTS; ++j) {
             if (alive[j]) {
-                int64_t dst = (int64_t)base + off + o;
-                if (dst < pp.events_cap) {
-                    gpet_event ev;
-                    ev.parn = parn; ev.pann = pann; ev.modn = L.mod[j]; ev.cryn = L.cry[j];
-                    ev.siten = L.site[j]; ev.eventid = eid; ev.t = L.t[j];
-                    ev.E = L.E[j]; ev.x = L.x[j]; ev.y = L.y[j]; ev.z = L.z[j];
-                    pp.events[dst] = ev;
+                gpet_event ev;
+                ev.parn = parn; ev.pann = pann; ev.modn = L.mod[j]; ev.cryn = L.cry[j];
+                ev.siten = L.site[j]; ev.eventid = eid; ev.t = L.t[j];
+                ev.E = L.E[j]; ev.x = L.x[j]; ev.y = L.y[j]; ev.z = L.z[j];
+                const unsigned e = (unsigned)(off + o);
+                if (e < nl) {
+                    s_ev[lb + e] = ev;
+                } else {
+                    const int64_t dst = (int64_t)gb + (e - nl);
+                    if (dst < pp.events_cap) pp.events[dst] = ev;
                 }
                 o++;
             }
@@ -1169,6 +1189,20 @@ void k_panel(PanelParams pp, DevTables tab, DevDetector det, DevQueue q)
                 tr->draws = r.n;
             }
         }
+    }
+    /* write out the staged events: one reservation for the block, then a
+     * coalesced copy (12 dwords per event) */
+    __syncthreads();
+    const unsigned ns = min(s_evn, EV_STAGE);
+    if (threadIdx.x == 0) s_evbase = ns ? atomicAdd(pp.n_events, ns) : 0u;
+    __syncthreads();
+    {
+        const int64_t base = (int64_t)s_evbase;
+        const int64_t nfit = max<int64_t>(0, min<int64_t>((int64_t)ns, pp.events_cap - base));
+        const unsigned *src = reinterpret_cast<const unsigned *>(s_ev);
+        unsigned *dstw = reinterpret_cast<unsigned *>(pp.events + base);
+        const int nw = (int)nfit * (int)(sizeof(gpet_event) / 4);
+        for (int k = threadIdx.x; k < nw; k += blockDim.x) dstw[k] = src[k];
     }
     {
         unsigned long long *const dst[5] = {
