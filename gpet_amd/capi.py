@@ -111,11 +111,11 @@ LM_TRUE, LM_DELAYED, LM_BACKGROUND = 1, 2, 4
 
 class Launch(C.Structure):
     _fields_ = [("block", C.c_int32 * 2), ("wpe", C.c_int32 * 2), ("blocks_per_cu", C.c_int32 * 2),
-                ("batch", C.c_int64)]
+                ("batch", C.c_int64), ("transit", C.c_int32 * 4)]
 
     def as_dict(self):
         return {"block": list(self.block), "wpe": list(self.wpe), "blocks_per_cu": list(self.blocks_per_cu),
-                "batch": int(self.batch)}
+                "batch": int(self.batch), "transit": list(self.transit)}
 
 
 class SurrogateConfig(C.Structure):

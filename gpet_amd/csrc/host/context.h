@@ -129,6 +129,8 @@ struct gpet_ctx {
     DBuf<unsigned> d_rcount;     /* panel-queue region counts                */
     DBuf<LongWalkItem> d_lw;     /* deferred positron walks (queue / 8)      */
     DBuf<unsigned> d_lw_count;
+    DBuf<TransitItem> d_tq[2];   /* photons in transit (two per history slot) */
+    DBuf<unsigned> d_tq_count[2];
     DBuf<gpet_psf_record> d_psf_src, d_psf_ph;
     DBuf<unsigned long long> d_npsf;
     DBuf<int32_t> d_hit_id;

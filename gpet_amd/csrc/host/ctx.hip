@@ -260,6 +260,8 @@ LaunchCfg cfg_of(const gpet_launch &L, int k)
     c.block = L.block[k];
     c.wpe = L.wpe[k];
     c.blocks_per_cu = L.blocks_per_cu[k];
+    if (k == 0)
+        for (int i = 0; i < 3; ++i) c.rounds[i] = L.transit[i + 1];
     return c;
 }
 
@@ -280,13 +282,17 @@ gpet_status check_launch(gpet_ctx *c, const gpet_launch &L)
         if (L.block[k] < 64 || L.block[k] > 256 || L.block[k] % 64) return fail(c, GPET_ERR_ARG, "launch block size");
         bool ok = false;
         for (int v : gpet_wpe_variants) ok = ok || v == L.wpe[k];
-        if (!ok) return fail(c, GPET_ERR_ARG, "launch register variant (wpe) must be 0, 6 or 8");
+        if (!ok) return fail(c, GPET_ERR_ARG, "launch register variant (wpe) must be 0, 5 or 6");
         if (L.blocks_per_cu[k] < 0 || L.blocks_per_cu[k] > 64) return fail(c, GPET_ERR_ARG, "launch blocks per CU");
         /* the phantom kernel's grid sets the panel queue's regions: 1..16 per CU */
         if (k == 0 && (L.blocks_per_cu[0] < 1 || L.blocks_per_cu[0] > 16))
             return fail(c, GPET_ERR_ARG, "phantom-kernel blocks per CU must be 1..16");
     }
     if (L.batch < 256 || L.batch > ((int64_t)1 << 26)) return fail(c, GPET_ERR_ARG, "launch batch size");
+    if (L.transit[0] < 0 || L.transit[0] > 1024) return fail(c, GPET_ERR_ARG, "launch transit steps");
+    for (int k = 1; k < 4; ++k)
+        if (L.transit[k] < 0 || L.transit[k] > 1 << 20 || (L.transit[k] > 0 && k > 1 && L.transit[k - 1] == 0))
+            return fail(c, GPET_ERR_ARG, "launch transit rounds (0 ends the list)");
     return GPET_OK;
 }
 
@@ -296,6 +302,13 @@ gpet_status alloc_queue(gpet_ctx *c, int64_t qcap)
     if ((st = dalloc(c, c->d_rcount, 64 * (int64_t)c->n_cu + 1))) return st;
     if ((st = dalloc(c, c->d_lw_count, 64 * (int64_t)c->n_cu + 1))) return st;
     if ((st = dalloc(c, c->d_lw, qcap / 8 + 1))) return st;
+    const gpet_launch &L = c->launch;
+    for (int k = 0; k < 2; ++k) {
+        if (L.transit[0] > 0 && (k == 0 || L.transit[1] > 0)) {
+            if ((st = dalloc(c, c->d_tq[k], qcap))) return st;
+            if ((st = dalloc(c, c->d_tq_count[k], 64 * (int64_t)c->n_cu + 1))) return st;
+        }
+    }
     if (qcap > c->q_cap) {
         for (int k = 0; k < 6; ++k)
             if ((st = dalloc(c, c->q_f[k], qcap))) return st;
@@ -318,6 +331,16 @@ DevQueue make_queue(gpet_ctx *c, int64_t qcap, bool want_trace)
     q.cap = qcap;
     q.lw = c->d_lw.p;
     q.lw_count = c->d_lw_count.p;
+    const gpet_launch &L = c->launch;
+    if (L.transit[0] > 0) {
+        q.tq = c->d_tq[0].p;
+        q.tq_count = c->d_tq_count[0].p;
+        q.tq_steps = L.transit[0];
+        if (L.transit[1] > 0) {
+            q.tq2 = c->d_tq[1].p;
+            q.tq2_count = c->d_tq_count[1].p;
+        }
+    }
     return q;
 }
 
@@ -990,6 +1013,7 @@ gpet_status gpet_destroy(gpet_ctx *c)
     for (auto &b : c->q_f) dfree(b);
     dfree(c->q_t); dfree(c->q_info); dfree(c->q_slot); dfree(c->d_small); dfree(c->d_cnt);
     dfree(c->d_rcount); dfree(c->d_lw); dfree(c->d_lw_count);
+    for (int k = 0; k < 2; ++k) { dfree(c->d_tq[k]); dfree(c->d_tq_count[k]); }
     dfree(c->d_events); dfree(c->d_trace); dfree(c->d_hit_id); dfree(c->d_hit_f);
     dfree(c->d_psf_src); dfree(c->d_psf_ph); dfree(c->d_npsf);
     dfree(c->d_first); dfree(c->d_src); dfree(c->d_iso); dfree(c->d_psf);

@@ -124,6 +124,18 @@ struct LongWalkItem {
     int slot, pad;             /* phantom slot (batched runs)                 */
 };
 
+/* A photon still in the phantom after the phantom kernel's first Woodcock
+ * steps, handed to the transit kernel (96 B). */
+struct TransitItem {
+    double t, tstart;          /* time now / at emission (us)                 */
+    int64_t h;                 /* history                                     */
+    float x, y, z, vx, vy, vz, E;
+    float px, py, pz, E0;      /* emission point and energy (trace)           */
+    int ns, nc, nr, it;        /* steps inside, Compton, Rayleigh; all steps  */
+    unsigned n;                /* stream position, photon index in bit 31     */
+    int parn, slot;            /* photon id; phantom slot (batched runs)      */
+};
+
 struct DevQueue {
     float *y, *z, *vx, *vy, *vz, *E;
     double *t;
@@ -138,6 +150,14 @@ struct DevQueue {
      * lw_count[r] of them are used; null: walks are not deferred */
     LongWalkItem *lw;
     unsigned *lw_count;
+    /* photons in transit: region r holds items [r * region, ...) and
+     * tq_count[r] of them are used; tq_steps Woodcock steps are taken in the
+     * phantom kernel before a photon is put here (0: never) */
+    TransitItem *tq;
+    unsigned *tq_count;
+    int tq_steps;
+    TransitItem *tq2;          /* second buffer: rounds alternate between them */
+    unsigned *tq2_count;
 };
 
 /* Device-side counters, 64-bit, accumulated with atomics. */
@@ -209,6 +229,9 @@ struct LaunchCfg {
     int block = 256;
     int wpe = 0;
     int blocks_per_cu = 0;     /* 0: one lane per item (no grid-stride reuse) */
+    int rounds[3] = {0, 0, 0}; /* phantom stage: Woodcock steps per transit
+                                  round (0: not launched; the last launched
+                                  round finishes every photon)              */
 };
 #define GPET_N_WPE 3
 extern const int gpet_wpe_variants[GPET_N_WPE];

@@ -375,11 +375,18 @@ __device__ void d_emit_pair(float sigma, gp_rng &r, float v[2][3], float e[2])
 }
 
 /* Woodcock tracking in the phantom; returns false when absorbed. */
-__device__ bool d_phantom_walk(const DevTables &tab, const DevPhantom &p, float eabs, float &x, float &y, float &z,
-                               float &vx, float &vy, float &vz, float &E, double &tof, gp_rng &r, int &nsteps,
-                               int &ncomp, int &nrayl, unsigned long long &nmaj)
+enum { WALK_ABSORBED = 0, WALK_ESCAPED = 1, WALK_IN_FLIGHT = 2 };
+
+/* it counts the photon's Woodcock steps so far (across calls); the call
+ * takes at most maxit more and reports WALK_IN_FLIGHT if the photon is
+ * still in the phantom then.  A photon reaching D_STEP_CAP steps is
+ * absorbed, as in one uninterrupted walk. */
+__device__ int d_phantom_walk(const DevTables &tab, const DevPhantom &p, float eabs, float &x, float &y, float &z,
+                              float &vx, float &vy, float &vz, float &E, double &tof, gp_rng &r, int &nsteps,
+                              int &ncomp, int &nrayl, unsigned long long &nmaj, int &it, int maxit)
 {
-    for (int it = 0; it < D_STEP_CAP; ++it) {
+    const int stop = (maxit >= D_STEP_CAP - it) ? D_STEP_CAP : it + maxit;
+    for (; it < stop; ++it) {
         /* one 4-word block per step: length, selector, scattering CDF, azimuth */
         gp_u4 b = gp_rng_block(&r);
         float lammin = d_lam(p.wck, E);
@@ -389,7 +396,7 @@ __device__ bool d_phantom_walk(const DevTables &tab, const DevPhantom &p, float 
         z = z + s * vz;
         tof = tof + (double)s * D_INV_C;
         int ix, iy, iz;
-        if (!d_vox(p, x, y, z, ix, iy, iz)) return true;
+        if (!d_vox(p, x, y, z, ix, iy, iz)) { ++it; return WALK_ESCAPED; }
         nsteps++;
         int2 vv = p.vox[((size_t)iz * p.ny + iy) * p.nx + ix];
         float den = __int_as_float(vv.x);
@@ -404,20 +411,20 @@ __device__ bool d_phantom_walk(const DevTables &tab, const DevPhantom &p, float 
          * a wave that holds both pays for it once */
         const float pc = prob + lamden * mu.y;
         const bool comp = rn < pc;
-        if (!comp && !(rn < pc + lamden * mu.z)) return false;   /* photoelectric */
+        if (!comp && !(rn < pc + lamden * mu.z)) { ++it; return WALK_ABSORBED; }   /* photoelectric */
         const float cth = d_surf(tab, comp ? GPET_SURF_COMPTON : GPET_SURF_RAYLEIGH, mat, E, gp_u01(b.z));
         const float uphi = gp_u01(b.w);
         if (comp) {
             const float efrac = 1.0f / (1.0f + (E * D_IMC2F) * (1.0f - cth));
             ncomp++;
             E = E * efrac;
-            if (E < eabs) return false;
+            if (E < eabs) { ++it; return WALK_ABSORBED; }
         } else {
             nrayl++;
         }
         d_rotate(vx, vy, vz, cth, uphi);
     }
-    return false;
+    return it >= D_STEP_CAP ? WALK_ABSORBED : WALK_IN_FLIGHT;
 }
 
 /* Distance along the direction to the recording sphere (getDistance,
@@ -491,12 +498,12 @@ __device__ __forceinline__ bool d_decay_begin(const RunParams &rp, int64_t h, in
     return true;
 }
 
-template <int MODE>
+template <int MODE, bool TQ>
 __device__ __forceinline__ void d_photons(const RunParams &rp, const DevTables &tab, const DevPhantom &ph,
                                           const DevDetector &det, const float4 *s_pf, const DevPanel *s_p,
-                                          const DevQueue &q, unsigned *s_qn, int64_t h, float px, float py,
-                                          float pz, const float (&v)[2][3], const float (&e)[2], double tstart,
-                                          int nph, PhCounters &C);
+                                          const DevQueue &q, unsigned *s_qn, unsigned *s_tn, int64_t h, float px,
+                                          float py, float pz, const float (&v)[2][3], const float (&e)[2],
+                                          double tstart, int nph, int slot, PhCounters &C);
 
 /* One history: decay (or PSF record i), positron range, pair, phantom walk
  * of both photons, panel entry and queue append. */
@@ -510,11 +517,11 @@ __device__ __forceinline__ void d_photons(const RunParams &rp, const DevTables &
 #define LW_STEPS 32
 #endif
 
-template <int MODE>
+template <int MODE, bool TQ>
 __device__ __forceinline__ void d_history(const RunParams &rp, const DevTables &tab, const DevPhantom &ph,
                                           const DevDetector &det, const float4 *s_pf, const DevPanel *s_p,
-                                          const DevQueue &q, unsigned *s_qn, unsigned *s_ln, int64_t i, int64_t h,
-                                          int si, int slot, PhCounters &C)
+                                          const DevQueue &q, unsigned *s_qn, unsigned *s_ln, unsigned *s_tn,
+                                          int64_t i, int64_t h, int si, int slot, PhCounters &C)
 {
     unsigned long long &c_rs = C.rs;
     float px = 0.f, py = 0.f, pz = 0.f;
@@ -582,134 +589,188 @@ __device__ __forceinline__ void d_history(const RunParams &rp, const DevTables &
             nph = 2;
         }
     }
-    d_photons<MODE>(rp, tab, ph, det, s_pf, s_p, q, s_qn, h, px, py, pz, v, e, tstart, nph, C);
+    d_photons<MODE, TQ>(rp, tab, ph, det, s_pf, s_p, q, s_qn, s_tn, h, px, py, pz, v, e, tstart, nph, slot, C);
+}
+
+/* What follows a photon's phantom walk: recording sphere, phase-space
+ * record, trace record, panel entry and the append to this block's queue
+ * region (LDS counter, one atomic per wave among the lanes that call it).
+ * draws: the photon's stream position, where the panel kernel goes on. */
+__device__ __forceinline__ void d_photon_tail(const RunParams &rp, const DevDetector &det, const float4 *s_pf,
+                                              const DevPanel *s_p, const DevQueue &q, unsigned *s_qn, int64_t h,
+                                              int k, int parn, bool active, bool alive, float x, float y, float z,
+                                              float vx, float vy, float vz, float E, double t, int ns, int nc,
+                                              int nr, unsigned draws, float px, float py, float pz, float E0,
+                                              double tstart, PhCounters &C)
+{
+    const int64_t slot = 2 * h + k;
+    gpet_trace *tr = (rp.trace && slot < rp.trace_cap) ? rp.trace + slot : nullptr;
+    if (active) {
+        if (alive && rp.sphere[3] > 0.0f) {
+            const float sd = d_sphere_dist(rp.sphere, x, y, z, vx, vy, vz);
+            x = x + sd * vx;
+            y = y + sd * vy;
+            z = z + sd * vz;
+            t = t + (double)sd * D_INV_C;
+        }
+        if (alive && rp.psf_ph)
+            d_psf_append(rp.psf_ph, rp.n_psf + 1, rp.psf_cap, x, y, z, vx, vy, vz, E, parn, (int)h, t);
+    }
+    C.ps += (unsigned long long)ns;
+    if (tr) {
+        tr->history = h;
+        tr->photon = k;
+        tr->panel = -1;
+        tr->t0 = tstart;
+        if (active) {
+            tr->status = GPET_TR_ALIVE | (alive ? GPET_TR_ESCAPED : GPET_TR_ABSORBED);
+            tr->n_steps = ns; tr->n_compton = nc; tr->n_rayleigh = nr;
+            tr->x0 = px; tr->y0 = py; tr->z0 = pz; tr->E0 = E0;
+            tr->x = x; tr->y = y; tr->z = z; tr->vx = vx; tr->vy = vy; tr->vz = vz; tr->E = E;
+            tr->t = t;
+            tr->draws = draws;
+        }
+    }
+    int pa = -1;
+    float y2 = 0.f, z2 = 0.f, vx2 = 0.f, vy2 = 0.f, vz2 = 0.f;
+    if (alive && det.present) {
+        const float vv = vx * vx + vy * vy + vz * vz;
+        for (int pi = 0; pi < det.npanels; ++pi) {
+            /* conservative prefilter: the line through the photon must
+             * pass within the face's bounding sphere (covers the
+             * reference's backward entries too); skipping a panel here
+             * never changes which panel the exact test picks.  The
+             * squared distance loses about 2^-22 w2 to rounding, and the
+             * exact test's own rounding moves its crossing point by a
+             * similar relative amount, so the bound grows with w2 (1e-5
+             * w2: nothing at scanner distances, and photons many metres
+             * away -- after a long flight through an empty phantom --
+             * are not skipped where the exact test would still accept a
+             * face).  An overflow to inf or NaN never skips. */
+            {
+                const float4 f = s_pf[pi];
+                float wx = f.x - x, wy = f.y - y, wz = f.z - z;
+                float a = vx * wx + vy * wy + vz * wz;
+                float w2 = wx * wx + wy * wy + wz * wz;
+                if (w2 * vv - a * a > (f.w + 1.0e-5f * w2) * vv) continue;
+            }
+            const DevPanel &P = s_p[pi];
+            float rx = x - P.ox, ry = y - P.oy, rz = z - P.oz;
+            float tx = rx * P.ux[0] + ry * P.ux[1] + rz * P.ux[2];
+            float ty = rx * P.uy[0] + ry * P.uy[1] + rz * P.uy[2];
+            float tz = rx * P.uz[0] + ry * P.uz[1] + rz * P.uz[2];
+            float tvx = vx * P.ux[0] + vy * P.ux[1] + vz * P.ux[2];
+            float tvy = vx * P.uy[0] + vy * P.uy[1] + vz * P.uy[2];
+            float tvz = vx * P.uz[0] + vy * P.uz[1] + vz * P.uz[2];
+            if (tvx * P.dirx >= 0.0f) {
+                float ty2 = ty - (tx / tvx) * tvy;
+                float tz2 = tz - (tx / tvx) * tvz;
+                if (fabsf(ty2) < P.leny / 2.0f && fabsf(tz2) < P.lenz / 2.0f) {
+                    y2 = ty2;
+                    z2 = tz2;
+                    t = t + (double)(-tx) / (D_C_LIGHT * (double)tvx);
+                    vx2 = tvx; vy2 = tvy; vz2 = tvz;
+                    pa = pi;
+                    break;
+                }
+            }
+        }
+    }
+    if (tr && pa >= 0) {
+        tr->panel = pa;
+        tr->status |= GPET_TR_PANEL;
+    }
+    const bool want = pa >= 0;
+    const unsigned long long m = __ballot(want);
+    int64_t idx = q.region;
+    if (m) {
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        unsigned b = 0;
+        if ((int)__lane_id() == leader) b = atomicAdd(s_qn, (unsigned)__popcll(m));
+        b = __shfl(b, leader);
+        idx = (int64_t)b + __popcll(m & lanemask_lt());
+    }
+    if (want) {
+        C.ent++;
+        if (idx < q.region) {
+            idx += (int64_t)blockIdx.x * q.region;
+            q.y[idx] = y2; q.z[idx] = z2;
+            q.vx[idx] = vx2; q.vy[idx] = vy2; q.vz[idx] = vz2;
+            q.E[idx] = E;
+            q.t[idx] = t;
+            q.info[idx] = make_int4(pa, parn, (int)h, (int)(draws | ((unsigned)k << 31)));
+            if (q.trace_slot) q.trace_slot[idx] = tr ? (int)slot : -1;
+        }
+    }
+}
+
+/* Slot of a photon that goes to this block's transit region (LDS counter,
+ * one atomic per wave among the lanes that call it), or null.  A region
+ * holds two photons per history of its block, so it is never full. */
+__device__ __forceinline__ TransitItem *d_transit_slot(TransitItem *tq, int64_t region, unsigned *s_tn, bool push)
+{
+    const unsigned long long m = __ballot(push);
+    if (!m) return nullptr;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    unsigned b = 0;
+    if ((int)__lane_id() == leader) b = atomicAdd(s_tn, (unsigned)__popcll(m));
+    b = __shfl(b, leader);
+    const int64_t idx = (int64_t)b + __popcll(m & lanemask_lt());
+    return (push && idx < region) ? tq + (int64_t)blockIdx.x * region + idx : nullptr;
+}
+
+__device__ __forceinline__ void d_transit_write(TransitItem *T, double t, double tstart, int64_t h, float x, float y,
+                                                float z, float vx, float vy, float vz, float E, float px, float py,
+                                                float pz, float E0, int ns, int nc, int nr, int it, unsigned n,
+                                                int parn, int slot)
+{
+    T->t = t; T->tstart = tstart; T->h = h;
+    T->x = x; T->y = y; T->z = z; T->vx = vx; T->vy = vy; T->vz = vz; T->E = E;
+    T->px = px; T->py = py; T->pz = pz; T->E0 = E0;
+    T->ns = ns; T->nc = nc; T->nr = nr; T->it = it;
+    T->n = n; T->parn = parn; T->slot = slot;
 }
 
 /* Both photons of history h (nph of them active) from (px, py, pz) at
- * tstart: phantom walk, phase-space records, panel entry, queue append. */
-template <int MODE>
+ * tstart: phantom walk (TQ: a photon still in the phantom after q.tq_steps
+ * steps is handed to the transit kernel), then d_photon_tail. */
+template <int MODE, bool TQ>
 __device__ __forceinline__ void d_photons(const RunParams &rp, const DevTables &tab, const DevPhantom &ph,
                                           const DevDetector &det, const float4 *s_pf, const DevPanel *s_p,
-                                          const DevQueue &q, unsigned *s_qn, int64_t h, float px, float py,
-                                          float pz, const float (&v)[2][3], const float (&e)[2], double tstart,
-                                          int nph, PhCounters &C)
+                                          const DevQueue &q, unsigned *s_qn, unsigned *s_tn, int64_t h, float px,
+                                          float py, float pz, const float (&v)[2][3], const float (&e)[2],
+                                          double tstart, int nph, int slot, PhCounters &C)
 {
-    unsigned long long &c_ph = C.ph, &c_ent = C.ent, &c_maj = C.maj, &c_ps = C.ps;
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const bool active = k < nph;
-        const int64_t slot = 2 * h + k;
-        gpet_trace *tr = (rp.trace && slot < rp.trace_cap) ? rp.trace + slot : nullptr;
         float x = px, y = py, z = pz, vx = v[k][0], vy = v[k][1], vz = v[k][2], E = e[k];
         double t = tstart;
         gp_rng r;
         gp_rng_init(&r, rp.seed, (uint64_t)h, (uint32_t)(GP_TAG_PHOTON + k), 0u);
-        int ns = 0, nc = 0, nr = 0;
-        bool alive = active;
+        int ns = 0, nc = 0, nr = 0, it = 0;
+        bool alive = active, defer = false;
         const int parn = (MODE == RUN_PSF_PHOTON) ? (int)h : (int)(2 * h + k);
         if (active) {
-            c_ph++;
+            C.ph++;
             if (rp.psf_src)
                 d_psf_append(rp.psf_src, rp.n_psf, rp.psf_cap, x, y, z, vx, vy, vz, E, parn, (int)h, t);
-            if (ph.present) alive = d_phantom_walk(tab, ph, rp.eabs, x, y, z, vx, vy, vz, E, t, r, ns, nc, nr, c_maj);
-            if (alive && rp.sphere[3] > 0.0f) {
-                const float sd = d_sphere_dist(rp.sphere, x, y, z, vx, vy, vz);
-                x = x + sd * vx;
-                y = y + sd * vy;
-                z = z + sd * vz;
-                t = t + (double)sd * D_INV_C;
-            }
-            if (alive && rp.psf_ph)
-                d_psf_append(rp.psf_ph, rp.n_psf + 1, rp.psf_cap, x, y, z, vx, vy, vz, E, parn, (int)h, t);
-        }
-        c_ps += (unsigned long long)ns;
-        if (tr) {
-            tr->history = h;
-            tr->photon = k;
-            tr->panel = -1;
-            tr->t0 = tstart;
-            if (active) {
-                tr->status = GPET_TR_ALIVE | (alive ? GPET_TR_ESCAPED : GPET_TR_ABSORBED);
-                tr->n_steps = ns; tr->n_compton = nc; tr->n_rayleigh = nr;
-                tr->x0 = px; tr->y0 = py; tr->z0 = pz; tr->E0 = e[k];
-                tr->x = x; tr->y = y; tr->z = z; tr->vx = vx; tr->vy = vy; tr->vz = vz; tr->E = E;
-                tr->t = t;
-                tr->draws = r.n;
+            if (ph.present) {
+                const int st = d_phantom_walk(tab, ph, rp.eabs, x, y, z, vx, vy, vz, E, t, r, ns, nc, nr, C.maj, it,
+                                              TQ ? q.tq_steps : D_STEP_CAP);
+                alive = st != WALK_ABSORBED;
+                defer = st == WALK_IN_FLIGHT;
             }
         }
-        int pa = -1;
-        float y2 = 0.f, z2 = 0.f, vx2 = 0.f, vy2 = 0.f, vz2 = 0.f;
-        if (alive && det.present) {
-            const float vv = vx * vx + vy * vy + vz * vz;
-            for (int pi = 0; pi < det.npanels; ++pi) {
-                /* conservative prefilter: the line through the photon must
-                 * pass within the face's bounding sphere (covers the
-                 * reference's backward entries too); skipping a panel here
-                 * never changes which panel the exact test picks.  The
-                 * squared distance loses about 2^-22 w2 to rounding, and the
-                 * exact test's own rounding moves its crossing point by a
-                 * similar relative amount, so the bound grows with w2 (1e-5
-                 * w2: nothing at scanner distances, and photons many metres
-                 * away -- after a long flight through an empty phantom --
-                 * are not skipped where the exact test would still accept a
-                 * face).  An overflow to inf or NaN never skips. */
-                {
-                    const float4 f = s_pf[pi];
-                    float wx = f.x - x, wy = f.y - y, wz = f.z - z;
-                    float a = vx * wx + vy * wy + vz * wz;
-                    float w2 = wx * wx + wy * wy + wz * wz;
-                    if (w2 * vv - a * a > (f.w + 1.0e-5f * w2) * vv) continue;
-                }
-                const DevPanel &P = s_p[pi];
-                float rx = x - P.ox, ry = y - P.oy, rz = z - P.oz;
-                float tx = rx * P.ux[0] + ry * P.ux[1] + rz * P.ux[2];
-                float ty = rx * P.uy[0] + ry * P.uy[1] + rz * P.uy[2];
-                float tz = rx * P.uz[0] + ry * P.uz[1] + rz * P.uz[2];
-                float tvx = vx * P.ux[0] + vy * P.ux[1] + vz * P.ux[2];
-                float tvy = vx * P.uy[0] + vy * P.uy[1] + vz * P.uy[2];
-                float tvz = vx * P.uz[0] + vy * P.uz[1] + vz * P.uz[2];
-                if (tvx * P.dirx >= 0.0f) {
-                    float ty2 = ty - (tx / tvx) * tvy;
-                    float tz2 = tz - (tx / tvx) * tvz;
-                    if (fabsf(ty2) < P.leny / 2.0f && fabsf(tz2) < P.lenz / 2.0f) {
-                        y2 = ty2;
-                        z2 = tz2;
-                        t = t + (double)(-tx) / (D_C_LIGHT * (double)tvx);
-                        vx2 = tvx; vy2 = tvy; vz2 = tvz;
-                        pa = pi;
-                        break;
-                    }
-                }
-            }
+        if (TQ) {
+            TransitItem *T = d_transit_slot(q.tq, q.region, s_tn, defer);
+            if (T)
+                d_transit_write(T, t, tstart, h, x, y, z, vx, vy, vz, E, px, py, pz, e[k], ns, nc, nr, it,
+                                r.n | ((unsigned)k << 31), parn, slot);
         }
-        if (tr && pa >= 0) {
-            tr->panel = pa;
-            tr->status |= GPET_TR_PANEL;
-        }
-        /* append to this block's queue region (LDS counter, one atomic per
-         * wave) */
-        const bool want = pa >= 0;
-        const unsigned long long m = __ballot(want);
-        int64_t idx = q.region;
-        if (m) {
-            const int leader = __ffsll((unsigned long long)m) - 1;
-            unsigned b = 0;
-            if ((int)__lane_id() == leader) b = atomicAdd(s_qn, (unsigned)__popcll(m));
-            b = __shfl(b, leader);
-            idx = (int64_t)b + __popcll(m & lanemask_lt());
-        }
-        if (want) {
-            c_ent++;
-            if (idx < q.region) {
-                idx += (int64_t)blockIdx.x * q.region;
-                q.y[idx] = y2; q.z[idx] = z2;
-                q.vx[idx] = vx2; q.vy[idx] = vy2; q.vz[idx] = vz2;
-                q.E[idx] = E;
-                q.t[idx] = t;
-                q.info[idx] = make_int4(pa, parn, (int)h, (int)(r.n | ((unsigned)k << 31)));
-                if (q.trace_slot) q.trace_slot[idx] = tr ? (int)slot : -1;
-            }
-        }
+        if (!defer)
+            d_photon_tail(rp, det, s_pf, s_p, q, s_qn, h, k, parn, active, alive, x, y, z, vx, vy, vz, E, t, ns, nc,
+                          nr, r.n, px, py, pz, e[k], tstart, C);
     }
 }
 
@@ -721,15 +782,16 @@ __device__ __forceinline__ void d_photons(const RunParams &rp, const DevTables &
  * read with scalar loads and held in SGPRs, and no wave mixes phantoms.
  * WPE > 0 asks the compiler for at least WPE waves per SIMD (a register
  * budget), one of the launch variants the tuner picks from. */
-template <int MODE, bool BATCH, int WPE>
+template <int MODE, bool BATCH, int WPE, bool TQ>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
 void k_phantom(RunParams rp, DevTables tab, DevPhantom ph, DevDetector det, DevQueue q)
 {
     PhCounters C;
-    __shared__ unsigned s_qn, s_ln;
+    __shared__ unsigned s_qn, s_ln, s_tn;
     if (threadIdx.x == 0) {
         s_qn = 0u;
         s_ln = 0u;
+        s_tn = 0u;
     }
     /* panel table in LDS: the entry test scans every panel for every photon,
      * and from LDS the scan is not a chain of memory round trips */
@@ -749,7 +811,7 @@ void k_phantom(RunParams rp, DevTables tab, DevPhantom ph, DevDetector det, DevQ
         for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < rp.nh; i += stride) {
             const int64_t h = rp.h0 + i;
             const int si = (MODE == RUN_DECAY) ? d_source_of(rp, h) : 0;
-            d_history<MODE>(rp, tab, ph, det, s_pf, s_p, q, &s_qn, &s_ln, i, h, si, 0, C);
+            d_history<MODE, TQ>(rp, tab, ph, det, s_pf, s_p, q, &s_qn, &s_ln, &s_tn, i, h, si, 0, C);
         }
     } else {
         for (int64_t vb = blockIdx.x; vb < rp.n_vblocks; vb += gridDim.x) {
@@ -764,7 +826,8 @@ void k_phantom(RunParams rp, DevTables tab, DevPhantom ph, DevDetector det, DevQ
             if (off < S.nh) {
                 const int64_t h = S.h0 + off;
                 const int si = d_source_of(rp, h);
-                d_history<MODE>(rp, tab, P, det, s_pf, s_p, q, &s_qn, &s_ln, h - rp.h0, h, si, S.phantom, C);
+                d_history<MODE, TQ>(rp, tab, P, det, s_pf, s_p, q, &s_qn, &s_ln, &s_tn, h - rp.h0, h, si, S.phantom,
+                                    C);
             }
         }
     }
@@ -772,6 +835,7 @@ void k_phantom(RunParams rp, DevTables tab, DevPhantom ph, DevDetector det, DevQ
     if (threadIdx.x == 0) {
         q.rcount[blockIdx.x] = min((unsigned)q.region, s_qn);
         if (q.lw) q.lw_count[blockIdx.x] = min((unsigned)(q.region >> 3), s_ln);
+        if (TQ) q.tq_count[blockIdx.x] = min((unsigned)q.region, s_tn);
     }
     {
         unsigned long long *const dst[5] = {
@@ -786,16 +850,19 @@ void k_phantom(RunParams rp, DevTables tab, DevPhantom ph, DevDetector det, DevQ
 /* The long positron walks of one phantom-kernel block (same grid): finish
  * each walk, emit the pair and transport both photons, appending to that
  * block's queue region after the entries the phantom kernel left there. */
-template <bool BATCH>
+template <bool BATCH, bool TQ>
 __global__ __launch_bounds__(256) void k_longwalk(RunParams rp, DevTables tab, DevPhantom ph, DevDetector det,
                                                   DevQueue q)
 {
     PhCounters C;
-    __shared__ unsigned s_qn;
+    __shared__ unsigned s_qn, s_tn;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     float4 *s_pf = reinterpret_cast<float4 *>(smem);
     DevPanel *s_p = reinterpret_cast<DevPanel *>(smem + sizeof(float4) * (size_t)det.npanels);
-    if (threadIdx.x == 0) s_qn = q.rcount[blockIdx.x];
+    if (threadIdx.x == 0) {
+        s_qn = q.rcount[blockIdx.x];
+        s_tn = TQ ? q.tq_count[blockIdx.x] : 0u;
+    }
     if (det.present) {
         for (int t = threadIdx.x; t < det.npanels; t += blockDim.x) {
             const DevPanel P = det.p[t];
@@ -829,10 +896,81 @@ __global__ __launch_bounds__(256) void k_longwalk(RunParams rp, DevTables tab, D
         gp_rng_init(&r, rp.seed, (uint64_t)h, GP_TAG_DECAY, L.n);
         float v[2][3], e[2];
         d_emit_pair(rp.sigma, r, v, e);
-        d_photons<RUN_DECAY>(rp, tab, P, det, s_pf, s_p, q, &s_qn, h, px, py, pz, v, e, L.tstart, 2, C);
+        d_photons<RUN_DECAY, TQ>(rp, tab, P, det, s_pf, s_p, q, &s_qn, &s_tn, h, px, py, pz, v, e, L.tstart, 2,
+                                 L.slot, C);
     }
     __syncthreads();
-    if (threadIdx.x == 0) q.rcount[blockIdx.x] = min((unsigned)q.region, s_qn);
+    if (threadIdx.x == 0) {
+        q.rcount[blockIdx.x] = min((unsigned)q.region, s_qn);
+        if (TQ) q.tq_count[blockIdx.x] = min((unsigned)q.region, s_tn);
+    }
+    {
+        unsigned long long *const dst[5] = {
+            rp.cnt + CNT_PHOTONS * CNT_STRIPES, rp.cnt + CNT_PANEL_ENTRIES * CNT_STRIPES,
+            rp.cnt + CNT_MAJORANT * CNT_STRIPES, rp.cnt + CNT_PHANTOM_STEPS * CNT_STRIPES,
+            rp.cnt + CNT_RANGE_STEPS * CNT_STRIPES};
+        const unsigned long long v[5] = {C.ph, C.ent, C.maj, C.ps, C.rs};
+        flush_counters5(dst, v);
+    }
+}
+
+/* Photons in transit, one round: block b continues the photons of its
+ * region of `in` (same grid as the phantom kernel) by at most maxit
+ * Woodcock steps; those still in the phantom go to its region of `out`
+ * (null: the round finishes every photon), the rest take d_photon_tail and
+ * append to the block's panel-queue region after the entries already there.
+ * Every photon is continued from its own stream position, so the rounds
+ * change nothing in the results. */
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_transit(RunParams rp, DevTables tab, DevPhantom ph, DevDetector det,
+                                                 DevQueue q, const TransitItem *in, const unsigned *in_count,
+                                                 TransitItem *out, unsigned *out_count, int maxit)
+{
+    PhCounters C;
+    __shared__ unsigned s_qn, s_tn;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    float4 *s_pf = reinterpret_cast<float4 *>(smem);
+    DevPanel *s_p = reinterpret_cast<DevPanel *>(smem + sizeof(float4) * (size_t)det.npanels);
+    if (threadIdx.x == 0) {
+        s_qn = q.rcount[blockIdx.x];
+        s_tn = 0u;
+    }
+    if (det.present) {
+        for (int t = threadIdx.x; t < det.npanels; t += blockDim.x) {
+            const DevPanel P = det.p[t];
+            s_pf[t] = make_float4(P.ox, P.oy, P.oz, P.bound2);
+            s_p[t] = P;
+        }
+    }
+    __syncthreads();
+    const unsigned n = min(in_count[blockIdx.x], (unsigned)q.region);
+    for (unsigned j = threadIdx.x; j < n; j += blockDim.x) {
+        const TransitItem T = in[(int64_t)blockIdx.x * q.region + j];
+        const DevPhantom P = BATCH ? rp.phs[T.slot] : ph;
+        const int k = (int)(T.n >> 31);
+        gp_rng r;
+        gp_rng_init(&r, rp.seed, (uint64_t)T.h, (uint32_t)(GP_TAG_PHOTON + k), T.n & 0x7fffffffu);
+        float x = T.x, y = T.y, z = T.z, vx = T.vx, vy = T.vy, vz = T.vz, E = T.E;
+        double t = T.t;
+        int ns = T.ns, nc = T.nc, nr = T.nr, it = T.it;
+        const int st = d_phantom_walk(tab, P, rp.eabs, x, y, z, vx, vy, vz, E, t, r, ns, nc, nr, C.maj, it,
+                                      out ? maxit : D_STEP_CAP);
+        const bool defer = st == WALK_IN_FLIGHT;
+        if (out) {
+            TransitItem *U = d_transit_slot(out, q.region, &s_tn, defer);
+            if (U)
+                d_transit_write(U, t, T.tstart, T.h, x, y, z, vx, vy, vz, E, T.px, T.py, T.pz, T.E0, ns, nc, nr, it,
+                                r.n | ((unsigned)k << 31), T.parn, T.slot);
+        }
+        if (!defer)
+            d_photon_tail(rp, det, s_pf, s_p, q, &s_qn, T.h, k, T.parn, true, st != WALK_ABSORBED, x, y, z, vx, vy,
+                          vz, E, t, ns, nc, nr, r.n, T.px, T.py, T.pz, T.E0, T.tstart, C);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        q.rcount[blockIdx.x] = min((unsigned)q.region, s_qn);
+        if (out) out_count[blockIdx.x] = min((unsigned)q.region, s_tn);
+    }
     {
         unsigned long long *const dst[5] = {
             rp.cnt + CNT_PHOTONS * CNT_STRIPES, rp.cnt + CNT_PANEL_ENTRIES * CNT_STRIPES,
@@ -1227,17 +1365,17 @@ int grid_of(int64_t n, const LaunchCfg &cfg, int n_cu)
     return (int)std::max<int64_t>(1, std::min(nb, cap));
 }
 
-template <int MODE, bool BATCH>
+template <int MODE, bool BATCH, bool TQ>
 void launch_phantom_wpe(const RunParams &rp, const DevTables &tab, const DevPhantom &ph, const DevDetector &det,
                         const DevQueue &q, int grid, int block, int wpe, hipStream_t s)
 {
     const size_t lds = det.present ? (sizeof(float4) + sizeof(DevPanel)) * (size_t)det.npanels : 0;
     if (wpe == 5)
-        hipLaunchKernelGGL((k_phantom<MODE, BATCH, 5>), dim3(grid), dim3(block), lds, s, rp, tab, ph, det, q);
+        hipLaunchKernelGGL((k_phantom<MODE, BATCH, 5, TQ>), dim3(grid), dim3(block), lds, s, rp, tab, ph, det, q);
     else if (wpe == 6)
-        hipLaunchKernelGGL((k_phantom<MODE, BATCH, 6>), dim3(grid), dim3(block), lds, s, rp, tab, ph, det, q);
+        hipLaunchKernelGGL((k_phantom<MODE, BATCH, 6, TQ>), dim3(grid), dim3(block), lds, s, rp, tab, ph, det, q);
     else
-        hipLaunchKernelGGL((k_phantom<MODE, BATCH, 0>), dim3(grid), dim3(block), lds, s, rp, tab, ph, det, q);
+        hipLaunchKernelGGL((k_phantom<MODE, BATCH, 0, TQ>), dim3(grid), dim3(block), lds, s, rp, tab, ph, det, q);
 }
 
 __global__ void k_zero(unsigned *p, int n)
@@ -1263,22 +1401,63 @@ int gpet_launch_phantom(const RunParams &rp, const DevTables &tab, const DevPhan
     if (rp.nh <= 0) return 0;
     if (cfg.block <= 0 || cfg.block > 256 || cfg.block % 64) return (int)hipErrorInvalidValue;
     const int grid = grid_of(batch ? rp.n_vblocks * cfg.block : rp.nh, cfg, n_cu);
+    /* photon regrouping: source runs only (the phase-space modes finish
+     * their photons in the phantom kernel) */
+    const bool tq = q.tq && rp.mode == RUN_DECAY && (batch || ph.present);
     if (rp.mode == RUN_DECAY) {
-        if (batch) launch_phantom_wpe<RUN_DECAY, true>(rp, tab, ph, det, q, grid, cfg.block, cfg.wpe, s);
-        else launch_phantom_wpe<RUN_DECAY, false>(rp, tab, ph, det, q, grid, cfg.block, cfg.wpe, s);
+        const size_t lds = det.present ? (sizeof(float4) + sizeof(DevPanel)) * (size_t)det.npanels : 0;
+        if (tq) {
+            if (batch) launch_phantom_wpe<RUN_DECAY, true, true>(rp, tab, ph, det, q, grid, cfg.block, cfg.wpe, s);
+            else launch_phantom_wpe<RUN_DECAY, false, true>(rp, tab, ph, det, q, grid, cfg.block, cfg.wpe, s);
+        } else {
+            if (batch) launch_phantom_wpe<RUN_DECAY, true, false>(rp, tab, ph, det, q, grid, cfg.block, cfg.wpe, s);
+            else launch_phantom_wpe<RUN_DECAY, false, false>(rp, tab, ph, det, q, grid, cfg.block, cfg.wpe, s);
+        }
         if (rp.prange && q.lw) {
             const hipError_t e = hipGetLastError();
             if (e != hipSuccess) return (int)e;
-            const size_t lds = det.present ? (sizeof(float4) + sizeof(DevPanel)) * (size_t)det.npanels : 0;
-            if (batch)
-                hipLaunchKernelGGL((k_longwalk<true>), dim3(grid), dim3(cfg.block), lds, s, rp, tab, ph, det, q);
+            if (batch && tq)
+                hipLaunchKernelGGL((k_longwalk<true, true>), dim3(grid), dim3(cfg.block), lds, s, rp, tab, ph, det, q);
+            else if (batch)
+                hipLaunchKernelGGL((k_longwalk<true, false>), dim3(grid), dim3(cfg.block), lds, s, rp, tab, ph, det,
+                                   q);
+            else if (tq)
+                hipLaunchKernelGGL((k_longwalk<false, true>), dim3(grid), dim3(cfg.block), lds, s, rp, tab, ph, det,
+                                   q);
             else
-                hipLaunchKernelGGL((k_longwalk<false>), dim3(grid), dim3(cfg.block), lds, s, rp, tab, ph, det, q);
+                hipLaunchKernelGGL((k_longwalk<false, false>), dim3(grid), dim3(cfg.block), lds, s, rp, tab, ph, det,
+                                   q);
         }
     } else if (rp.mode == RUN_PSF_POSITRON) {
-        launch_phantom_wpe<RUN_PSF_POSITRON, false>(rp, tab, ph, det, q, grid, cfg.block, cfg.wpe, s);
+        launch_phantom_wpe<RUN_PSF_POSITRON, false, false>(rp, tab, ph, det, q, grid, cfg.block, cfg.wpe, s);
     } else {
-        launch_phantom_wpe<RUN_PSF_PHOTON, false>(rp, tab, ph, det, q, grid, cfg.block, cfg.wpe, s);
+        launch_phantom_wpe<RUN_PSF_PHOTON, false, false>(rp, tab, ph, det, q, grid, cfg.block, cfg.wpe, s);
+    }
+    if (tq) {
+        /* transit rounds: alternate between the two buffers; the last
+         * launched round (or the only one) finishes every photon */
+        int nr = 0;
+        while (nr < 3 && cfg.rounds[nr] > 0) ++nr;
+        if (nr == 0) nr = 1;
+        const size_t lds = det.present ? (sizeof(float4) + sizeof(DevPanel)) * (size_t)det.npanels : 0;
+        const bool b = batch;
+        for (int i = 0; i < nr; ++i) {
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return (int)e;
+            const bool last = i == nr - 1;
+            if (!last && !q.tq2) return (int)hipErrorInvalidValue;
+            const TransitItem *in = (i & 1) ? q.tq2 : q.tq;
+            const unsigned *inc = (i & 1) ? q.tq2_count : q.tq_count;
+            TransitItem *out = last ? nullptr : ((i & 1) ? q.tq : q.tq2);
+            unsigned *outc = last ? nullptr : ((i & 1) ? q.tq_count : q.tq2_count);
+            const int steps = cfg.rounds[i] > 0 ? cfg.rounds[i] : D_STEP_CAP;
+            if (b)
+                hipLaunchKernelGGL((k_transit<true>), dim3(grid), dim3(cfg.block), lds, s, rp, tab, ph, det, q, in,
+                                   inc, out, outc, steps);
+            else
+                hipLaunchKernelGGL((k_transit<false>), dim3(grid), dim3(cfg.block), lds, s, rp, tab, ph, det, q, in,
+                                   inc, out, outc, steps);
+        }
     }
     return (int)hipGetLastError();
 }

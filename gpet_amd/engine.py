@@ -296,8 +296,16 @@ class Engine:
             self._ok(self.f["set_phantom_slot"](self.ctx, int(slot), mat.ctypes.data_as(C.c_void_p),
                                                 dens.ctypes.data_as(C.c_void_p), d, o, s), "set_phantom_slot")
 
-    def set_launch(self, block=(256, 256), wpe=(0, 0), blocks_per_cu=(16, 2), batch=1 << 22):
+    def set_launch(self, block=(256, 256), wpe=(0, 0), blocks_per_cu=(16, 2), batch=1 << 22, transit=None):
+        """transit: (steps in the phantom kernel, steps of transit rounds
+        1..3); None keeps the library's current setting."""
         L = capi.Launch()
+        if transit is None:
+            self._ok(self.f["get_launch"](self.ctx, C.byref(L)), "get_launch")
+        else:
+            t = list(transit) + [0] * (4 - len(transit))
+            for k in range(4):
+                L.transit[k] = t[k]
         L.block[0], L.block[1] = block
         L.wpe[0], L.wpe[1] = wpe
         L.blocks_per_cu[0], L.blocks_per_cu[1] = blocks_per_cu

@@ -180,6 +180,16 @@ typedef struct gpet_launch {
     int32_t blocks_per_cu[2];  /* grid = CUs x this, grid-stride; 0 = one lane
                                   per item                                      */
     int64_t batch;             /* histories per phantom-kernel launch           */
+    int32_t transit[4];        /* photon regrouping in the phantom (new): [0]
+                                  Woodcock steps a photon takes in the phantom
+                                  kernel before one still in the phantom is
+                                  handed on (0 = never: the phantom kernel
+                                  finishes every photon); [1..3] steps per
+                                  transit round, each round continuing the
+                                  handed-on photons compacted per block
+                                  (0 = round not launched; the last launched
+                                  round, or a single implied one, finishes
+                                  every photon)                                 */
 } gpet_launch;
 
 /* One acquisition of a batched run (new; the reference runs one phantom per

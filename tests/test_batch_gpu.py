@@ -77,12 +77,20 @@ def test_graph_replays(tables, batch_case):
     (eager,) = gpu_run(tables, batch_case, flags=capi.FLAG_NO_GRAPH)
     assert eager.tallies.graph == 0
     check_same(eager, ref)
+    # with transit rounds in the captured batches
+    runs = gpu_run(tables, batch_case, repeat=2,
+                   launch=dict(block=(256, 256), wpe=(0, 0), blocks_per_cu=(4, 4), batch=6000, transit=(1, 2)))
+    assert all(r.tallies.graph == 1 for r in runs)
+    for r in runs:
+        check_same(r, ref)
 
 
 @pytest.mark.parametrize("launch", [
     dict(block=(128, 128), wpe=(0, 0), blocks_per_cu=(1, 8)),
     dict(block=(256, 256), wpe=(6, 6), blocks_per_cu=(4, 4)),
     dict(block=(64, 192), wpe=(5, 5), blocks_per_cu=(2, 16)),
+    dict(block=(256, 256), wpe=(0, 0), blocks_per_cu=(4, 4), transit=(1,)),
+    dict(block=(128, 256), wpe=(5, 0), blocks_per_cu=(2, 4), transit=(1, 1, 2, 3)),
 ])
 def test_launch_shapes(tables, batch_case, launch):
     ref = batch_case[-1]

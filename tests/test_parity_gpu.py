@@ -53,32 +53,54 @@ def assert_parity(a, b, hits=False):
         assert np.array_equal(a.hit_f[ka].view(np.uint32), b.hit_f[kb].view(np.uint32))
 
 
-@pytest.mark.parametrize("config,n,prange", [(2, 30000, 1), (1, 30000, 0)])
-def test_source_parity(tables, config, n, prange):
+def with_transit(setup, transit):
+    """setup, then (if given) the photon-regrouping schedule of the launch."""
+    def f(e):
+        setup(e)
+        if transit is not None:
+            L = e.get_launch()
+            e.set_launch(block=tuple(L["block"]), wpe=tuple(L["wpe"]), blocks_per_cu=tuple(L["blocks_per_cu"]),
+                         batch=L["batch"], transit=transit)
+    return f
+
+
+# transit: photons still in the phantom after the phantom kernel's first
+# steps are continued by later rounds, compacted per block (results must not
+# change)
+@pytest.mark.parametrize("config,n,prange,transit", [(2, 30000, 1, None), (1, 30000, 0, None),
+                                                     (2, 30000, 1, (1,)), (1, 30000, 0, (1, 1, 2))])
+def test_source_parity(tables, config, n, prange, transit):
     from gpet_amd import engine
     from gpet_amd import workloads as W
 
     opt = engine.make_options(seed=11 + config, positron_range=prange, record_hits=1, batch=7000)
     src, iso, t0, t1 = W.workload_sources(config)
-    a, b = run_both(tables, opt, lambda e: W.setup_engine(e, tables, config=config, phantom_n=100 if config == 2 else 64),
+    setup = with_transit(lambda e: W.setup_engine(e, tables, config=config, phantom_n=100 if config == 2 else 64),
+                         transit)
+    a, b = run_both(tables, opt, setup,
                     lambda e: e.run_source(src, iso, t0, t1, n_decays=n, singles_cap=n, coinc_cap=n, adder_cap=4 * n,
                                            hits_cap=20 * n, trace_cap=2 * n))
     assert a.tallies.n_singles > 100 and a.tallies.n_coinc > 10
     assert_parity(a, b, hits=True)
 
 
-def test_body_parity(tables):
+@pytest.mark.parametrize("transit", [None, (2, 3, 5, 8)])
+def test_body_parity(tables, transit):
     """Body workload (ellipsoid background + hot spheres, lungs, spine) in
-    the 4-ring 48-panel scanner, on a 32 x 32 x 64 grid of the same extent."""
+    the 4-ring 48-panel scanner, on a 32 x 32 x 64 grid of the same extent
+    (photons take many steps in it: with transit rounds most of them are
+    handed on several times)."""
     from gpet_amd import engine
     from gpet_amd import workloads as W
 
     opt = engine.make_options(seed=5, positron_range=1, record_hits=1, batch=5000)
     src, iso, t0, t1 = W.workload_sources(5)
     n = 12000
-    a, b = run_both(tables, opt, lambda e: W.setup_engine(e, tables, config=5, phantom_n=32),
+    a, b = run_both(tables, opt, with_transit(lambda e: W.setup_engine(e, tables, config=5, phantom_n=32), transit),
                     lambda e: e.run_source(src, iso, t0, t1, n_decays=n, singles_cap=n, coinc_cap=n, adder_cap=4 * n,
                                            hits_cap=20 * n, trace_cap=2 * n))
+    if transit is not None:
+        assert a.tallies.launch.as_dict()["transit"] == list(transit)
     assert a.tallies.n_singles > 100 and a.tallies.n_coinc > 10
     assert a.arrays[0][8 * 400:].sum() > 0      # panels beyond the first ring see singles
     assert_parity(a, b, hits=True)
