@@ -10,7 +10,8 @@
  *   compaction              block scan + scatter (stable)
  *   time order              LSD radix sort of (sortable t bits, index):
  *                           8 passes of 8 bits, ranks inside a wavefront
- *                           from 8 ballots per digit
+ *                           from 8 ballots per digit, tiles staged in LDS
+ *                           in digit order before the coalesced write
  *   deadtime                stable radix sort by site over the time order,
  *                           then one lane per chain walks it
  *                           (gPET_kernals.cu:657-698)
@@ -143,15 +144,39 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const unsigned long long *
     hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
 }
 
+/* One tile per block: items are ranked stably by digit (8 ballots per
+ * round of 256), staged in LDS in digit order, then written out so that
+ * consecutive threads write consecutive addresses of each digit's run. */
 __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const unsigned long long *kin, const unsigned *vin,
                                                          unsigned long long *kout, unsigned *vout, int64_t n, int shift,
                                                          const unsigned *offs, int nblocks)
 {
-    __shared__ unsigned run[256];
+    __shared__ unsigned long long s_key[RS_TILE];
+    __shared__ unsigned s_val[RS_TILE];
+    __shared__ unsigned run[256], gstart[256], lstart[256], wsum[RS_BLOCK / 64];
     __shared__ unsigned wcnt[RS_BLOCK / 64][256];
     const int lane = __lane_id(), wid = threadIdx.x >> 6;
-    run[threadIdx.x] = offs[(int64_t)threadIdx.x * nblocks + blockIdx.x];
+    /* this tile's count of digit threadIdx.x from the scanned histogram
+     * (entry d * nblocks + tile is the digit's global start) */
+    const int64_t hn = (int64_t)256 * nblocks;
+    const int64_t e = (int64_t)threadIdx.x * nblocks + blockIdx.x;
+    const unsigned g0 = offs[e];
+    const unsigned c = ((e + 1 < hn) ? offs[e + 1] : (unsigned)n) - g0;
+    gstart[threadIdx.x] = g0;
+    unsigned v = c;
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned t = __shfl_up(v, o);
+        if (lane >= o) v += t;
+    }
+    if (lane == 63) wsum[wid] = v;
     for (int w = 0; w < RS_BLOCK / 64; ++w) wcnt[w][threadIdx.x] = 0;
+    __syncthreads();
+    {
+        unsigned add = 0;
+        for (int w = 0; w < wid; ++w) add += wsum[w];
+        lstart[threadIdx.x] = v - c + add;
+        run[threadIdx.x] = v - c + add;
+    }
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
     const unsigned long long lt = lanemask_lt();
@@ -186,11 +211,18 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const unsigned long lon
             run[threadIdx.x] += add;
         }
         if (valid) {
-            kout[pos] = key;
-            vout[pos] = val;
+            s_key[pos] = key;
+            s_val[pos] = val;
         }
         __syncthreads();
-        (void)lane;
+    }
+    const int tn = (int)min<int64_t>((int64_t)RS_TILE, n - base);
+    for (int j = threadIdx.x; j < tn; j += RS_BLOCK) {
+        const unsigned long long key = s_key[j];
+        const unsigned d = (unsigned)(key >> shift) & 255u;
+        const unsigned g = gstart[d] + ((unsigned)j - lstart[d]);
+        kout[g] = key;
+        vout[g] = s_val[j];
     }
 }
 
