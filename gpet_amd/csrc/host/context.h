@@ -107,6 +107,8 @@ struct gpet_ctx {
     /* detector */
     bool has_det = false;
     DBuf<DevPanel> d_panels;
+    DBuf<float4> d_pgroup_sph;   /* panel groups of the entry test           */
+    DBuf<unsigned long long> d_pgroup_mask;
     DBuf<float> d_wck_det;
     std::vector<float> h_wck_det;
     std::vector<gpet_panel> panels;

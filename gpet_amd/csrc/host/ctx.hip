@@ -1009,7 +1009,7 @@ gpet_status gpet_destroy(gpet_ctx *c)
     dfree(c->d_phs); dfree(c->d_seg); dfree(c->d_job_first); dfree(c->d_job_cnt);
     drop_graph(c);
     surr_free(c);
-    dfree(c->d_panels); dfree(c->d_wck_det);
+    dfree(c->d_panels); dfree(c->d_wck_det); dfree(c->d_pgroup_sph); dfree(c->d_pgroup_mask);
     for (auto &b : c->q_f) dfree(b);
     dfree(c->q_t); dfree(c->q_info); dfree(c->q_slot); dfree(c->d_small); dfree(c->d_cnt);
     dfree(c->d_rcount); dfree(c->d_lw); dfree(c->d_lw_count);
@@ -1153,6 +1153,75 @@ gpet_status gpet_set_phantom(gpet_ctx *c, const int32_t *mat, const float *dens,
     return gpet_set_phantom_slot(c, 0, mat, dens, dims, off, size);
 }
 
+/* Groups of panels for the entry test of large scanners (16 < n <= 64):
+ * panels sorted by the azimuth of their face centre (then z, then index)
+ * are cut into runs of s, and each group's sphere encloses its members'
+ * entry spheres (|c_g - c_i| + r_i, in double, rounded up).  A line that
+ * misses a group sphere misses every member sphere, so skipping the group
+ * skips only panels the per-panel prefilter would skip.  s is chosen to
+ * minimise groups + sum over groups of members x (r_g / |c_g|)^2, the
+ * expected tests for directions spread evenly over the scanner. */
+static void panel_groups(const std::vector<DevPanel> &dp, std::vector<float4> &gs,
+                         std::vector<unsigned long long> &gm)
+{
+    const int n = (int)dp.size();
+    gs.clear();
+    gm.clear();
+    if (n <= 16 || n > 64) return;
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    std::vector<double> az(n);
+    for (int i = 0; i < n; ++i) az[i] = std::atan2((double)dp[i].oy, (double)dp[i].ox);
+    std::sort(order.begin(), order.end(), [&](int a, int b) {
+        if (std::fabs(az[a] - az[b]) > 1e-6) return az[a] < az[b];
+        if (dp[a].oz != dp[b].oz) return dp[a].oz < dp[b].oz;
+        return a < b;
+    });
+    auto build = [&](int s, std::vector<float4> &S, std::vector<unsigned long long> &M) {
+        S.clear();
+        M.clear();
+        double cost = 0;
+        for (int g0 = 0; g0 < n; g0 += s) {
+            const int g1 = std::min(n, g0 + s);
+            double cx = 0, cy = 0, cz = 0;
+            for (int k = g0; k < g1; ++k) {
+                cx += dp[order[k]].ox; cy += dp[order[k]].oy; cz += dp[order[k]].oz;
+            }
+            cx /= (g1 - g0); cy /= (g1 - g0); cz /= (g1 - g0);
+            double r = 0;
+            unsigned long long m = 0;
+            for (int k = g0; k < g1; ++k) {
+                const DevPanel &P = dp[order[k]];
+                const double dx = P.ox - cx, dy = P.oy - cy, dz = P.oz - cz;
+                r = std::max(r, std::sqrt(dx * dx + dy * dy + dz * dz) + std::sqrt((double)P.bound2));
+                m |= 1ull << order[k];
+            }
+            r = r * (1.0 + 1e-6) + 1e-4;
+            S.push_back(make_float4((float)cx, (float)cy, (float)cz, (float)(r * r * (1.0 + 1e-6))));
+            M.push_back(m);
+            const double c2 = std::max(cx * cx + cy * cy + cz * cz, r * r);
+            cost += (double)(g1 - g0) * (r * r) / c2;
+        }
+        return (double)S.size() + cost;
+    };
+    double best = 1e300;
+    std::vector<float4> S;
+    std::vector<unsigned long long> M;
+    for (int s = 2; s <= 8; ++s) {
+        const double cost = build(s, S, M);
+        if (cost < best) {
+            best = cost;
+            gs = S;
+            gm = M;
+        }
+    }
+    /* no grouping unless it beats testing every panel */
+    if (!(best < (double)n)) {
+        gs.clear();
+        gm.clear();
+    }
+}
+
 gpet_status gpet_set_detector(gpet_ctx *c, const gpet_panel *panels, int32_t n, const gpet_panel_material mats[2],
                               const float *quad, int32_t nq)
 {
@@ -1180,6 +1249,21 @@ gpet_status gpet_set_detector(gpet_ctx *c, const gpet_panel *panels, int32_t n, 
     DevDetector &d = c->ddet;
     d.npanels = n;
     d.p = c->d_panels.p;
+    d.ngroups = 0;
+    d.gsph = nullptr;
+    d.gmask = nullptr;
+    {
+        std::vector<float4> gs;
+        std::vector<unsigned long long> gm;
+        panel_groups(dp, gs, gm);
+        if (!gs.empty()) {
+            if ((st = upload(c, c->d_pgroup_sph, gs.data(), (int64_t)gs.size()))) return st;
+            if ((st = upload(c, c->d_pgroup_mask, gm.data(), (int64_t)gm.size()))) return st;
+            d.ngroups = (int)gs.size();
+            d.gsph = c->d_pgroup_sph.p;
+            d.gmask = c->d_pgroup_mask.p;
+        }
+    }
     float rmax[GPET_MAXMAT] = {0};
     for (int i = 0; i < 2; ++i) {
         int m = mats[i].material;

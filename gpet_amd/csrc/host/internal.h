@@ -85,7 +85,21 @@ struct DevDetector {
     int moduleN, crystalN, moduleNy, crystalNy;
     DevWck wck;
     int present;
+    /* panel groups for the entry test (16 < npanels <= 64, else 0): a
+     * sphere enclosing the members' entry spheres, and the members as bits */
+    int ngroups;
+    const float4 *gsph;
+    const unsigned long long *gmask;
 };
+
+/* LDS bytes of the panel tables the transport kernels stage: entry spheres,
+ * group spheres and masks, panels. */
+__host__ __device__ __forceinline__ size_t gpet_panel_lds_bytes(const DevDetector &d)
+{
+    if (!d.present) return 0;
+    return (sizeof(float4) + sizeof(DevPanel)) * (size_t)d.npanels +
+           (sizeof(float4) + sizeof(unsigned long long)) * (size_t)d.ngroups;
+}
 
 struct DevSource {
     double tau, pdec;          /* T1/2 / ln 2 (s); 1 - 2^(-(t1-t0)/T1/2)      */

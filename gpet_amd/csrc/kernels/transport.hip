@@ -592,6 +592,46 @@ __device__ __forceinline__ void d_history(const RunParams &rp, const DevTables &
     d_photons<MODE, TQ>(rp, tab, ph, det, s_pf, s_p, q, s_qn, s_tn, h, px, py, pz, v, e, tstart, nph, slot, C);
 }
 
+/* Panel tables in LDS, laid out as [entry spheres][group spheres][group
+ * masks][panels] (gpet_panel_lds_bytes); filled by the whole block, read
+ * after a barrier.  Returns the panel array. */
+__device__ __forceinline__ DevPanel *d_panel_lds(unsigned char *smem, const DevDetector &det)
+{
+    float4 *pf = reinterpret_cast<float4 *>(smem);
+    float4 *gf = pf + det.npanels;
+    unsigned long long *gm = reinterpret_cast<unsigned long long *>(gf + det.ngroups);
+    DevPanel *sp = reinterpret_cast<DevPanel *>(gm + det.ngroups);
+    if (det.present) {
+        for (int t = threadIdx.x; t < det.npanels; t += blockDim.x) {
+            const DevPanel P = det.p[t];
+            pf[t] = make_float4(P.ox, P.oy, P.oz, P.bound2);
+            sp[t] = P;
+        }
+        for (int t = threadIdx.x; t < det.ngroups; t += blockDim.x) {
+            gf[t] = det.gsph[t];
+            gm[t] = det.gmask[t];
+        }
+    }
+    return sp;
+}
+
+/* The line through (x, y, z) along v (|v|^2 = vv) passes within the sphere
+ * (f.xyz, f.w = squared radius), with a bound that grows with the squared
+ * distance w2 to the centre: the squared distance loses about 2^-22 w2 to
+ * rounding, and the exact entry test's own rounding moves its crossing
+ * point by a similar relative amount (1e-5 w2: nothing at scanner
+ * distances; photons many metres away -- after a long flight through an
+ * empty phantom -- are not skipped where the exact test would still accept
+ * a face).  An overflow to inf or NaN passes. */
+__device__ __forceinline__ bool d_line_near(const float4 f, float x, float y, float z, float vx, float vy, float vz,
+                                            float vv)
+{
+    float wx = f.x - x, wy = f.y - y, wz = f.z - z;
+    float a = vx * wx + vy * wy + vz * wz;
+    float w2 = wx * wx + wy * wy + wz * wz;
+    return !(w2 * vv - a * a > (f.w + 1.0e-5f * w2) * vv);
+}
+
 /* What follows a photon's phantom walk: recording sphere, phase-space
  * record, trace record, panel entry and the append to this block's queue
  * region (LDS counter, one atomic per wave among the lanes that call it).
@@ -635,25 +675,27 @@ __device__ __forceinline__ void d_photon_tail(const RunParams &rp, const DevDete
     float y2 = 0.f, z2 = 0.f, vx2 = 0.f, vy2 = 0.f, vz2 = 0.f;
     if (alive && det.present) {
         const float vv = vx * vx + vy * vy + vz * vz;
-        for (int pi = 0; pi < det.npanels; ++pi) {
+        /* candidates: with groups, the members of the groups whose sphere
+         * the line passes near, else every panel; tried in index order, so
+         * the first panel the exact test accepts is the reference's */
+        unsigned long long cand = 0;
+        const int G = det.ngroups;
+        if (G > 0) {
+            const float4 *s_gf = s_pf + det.npanels;
+            const unsigned long long *s_gm = reinterpret_cast<const unsigned long long *>(s_gf + G);
+            for (int g = 0; g < G; ++g)
+                if (d_line_near(s_gf[g], x, y, z, vx, vy, vz, vv)) cand |= s_gm[g];
+        }
+        for (int pi = 0; G > 0 ? (cand != 0) : (pi < det.npanels); ++pi) {
+            if (G > 0) {
+                pi = __ffsll(cand) - 1;
+                cand &= cand - 1;
+            }
             /* conservative prefilter: the line through the photon must
              * pass within the face's bounding sphere (covers the
              * reference's backward entries too); skipping a panel here
-             * never changes which panel the exact test picks.  The
-             * squared distance loses about 2^-22 w2 to rounding, and the
-             * exact test's own rounding moves its crossing point by a
-             * similar relative amount, so the bound grows with w2 (1e-5
-             * w2: nothing at scanner distances, and photons many metres
-             * away -- after a long flight through an empty phantom --
-             * are not skipped where the exact test would still accept a
-             * face).  An overflow to inf or NaN never skips. */
-            {
-                const float4 f = s_pf[pi];
-                float wx = f.x - x, wy = f.y - y, wz = f.z - z;
-                float a = vx * wx + vy * wy + vz * wz;
-                float w2 = wx * wx + wy * wy + wz * wz;
-                if (w2 * vv - a * a > (f.w + 1.0e-5f * w2) * vv) continue;
-            }
+             * never changes which panel the exact test picks */
+            if (!d_line_near(s_pf[pi], x, y, z, vx, vy, vz, vv)) continue;
             const DevPanel &P = s_p[pi];
             float rx = x - P.ox, ry = y - P.oy, rz = z - P.oz;
             float tx = rx * P.ux[0] + ry * P.ux[1] + rz * P.ux[2];
@@ -796,15 +838,8 @@ void k_phantom(RunParams rp, DevTables tab, DevPhantom ph, DevDetector det, DevQ
     /* panel table in LDS: the entry test scans every panel for every photon,
      * and from LDS the scan is not a chain of memory round trips */
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    float4 *s_pf = reinterpret_cast<float4 *>(smem);
-    DevPanel *s_p = reinterpret_cast<DevPanel *>(smem + sizeof(float4) * (size_t)det.npanels);
-    if (det.present) {
-        for (int t = threadIdx.x; t < det.npanels; t += blockDim.x) {
-            const DevPanel P = det.p[t];
-            s_pf[t] = make_float4(P.ox, P.oy, P.oz, P.bound2);
-            s_p[t] = P;
-        }
-    }
+    const float4 *s_pf = reinterpret_cast<const float4 *>(smem);
+    const DevPanel *s_p = d_panel_lds(smem, det);
     __syncthreads();
     if (!BATCH) {
         const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -857,19 +892,12 @@ __global__ __launch_bounds__(256) void k_longwalk(RunParams rp, DevTables tab, D
     PhCounters C;
     __shared__ unsigned s_qn, s_tn;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    float4 *s_pf = reinterpret_cast<float4 *>(smem);
-    DevPanel *s_p = reinterpret_cast<DevPanel *>(smem + sizeof(float4) * (size_t)det.npanels);
+    const float4 *s_pf = reinterpret_cast<const float4 *>(smem);
     if (threadIdx.x == 0) {
         s_qn = q.rcount[blockIdx.x];
         s_tn = TQ ? q.tq_count[blockIdx.x] : 0u;
     }
-    if (det.present) {
-        for (int t = threadIdx.x; t < det.npanels; t += blockDim.x) {
-            const DevPanel P = det.p[t];
-            s_pf[t] = make_float4(P.ox, P.oy, P.oz, P.bound2);
-            s_p[t] = P;
-        }
-    }
+    const DevPanel *s_p = d_panel_lds(smem, det);
     __syncthreads();
     const unsigned lcap = (unsigned)(q.region >> 3);
     const unsigned n = min(q.lw_count[blockIdx.x], lcap);
@@ -929,19 +957,12 @@ __global__ __launch_bounds__(256) void k_transit(RunParams rp, DevTables tab, De
     PhCounters C;
     __shared__ unsigned s_qn, s_tn;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    float4 *s_pf = reinterpret_cast<float4 *>(smem);
-    DevPanel *s_p = reinterpret_cast<DevPanel *>(smem + sizeof(float4) * (size_t)det.npanels);
+    const float4 *s_pf = reinterpret_cast<const float4 *>(smem);
     if (threadIdx.x == 0) {
         s_qn = q.rcount[blockIdx.x];
         s_tn = 0u;
     }
-    if (det.present) {
-        for (int t = threadIdx.x; t < det.npanels; t += blockDim.x) {
-            const DevPanel P = det.p[t];
-            s_pf[t] = make_float4(P.ox, P.oy, P.oz, P.bound2);
-            s_p[t] = P;
-        }
-    }
+    const DevPanel *s_p = d_panel_lds(smem, det);
     __syncthreads();
     const unsigned n = min(in_count[blockIdx.x], (unsigned)q.region);
     for (unsigned j = threadIdx.x; j < n; j += blockDim.x) {
@@ -1369,7 +1390,7 @@ template <int MODE, bool BATCH, bool TQ>
 void launch_phantom_wpe(const RunParams &rp, const DevTables &tab, const DevPhantom &ph, const DevDetector &det,
                         const DevQueue &q, int grid, int block, int wpe, hipStream_t s)
 {
-    const size_t lds = det.present ? (sizeof(float4) + sizeof(DevPanel)) * (size_t)det.npanels : 0;
+    const size_t lds = gpet_panel_lds_bytes(det);
     if (wpe == 5)
         hipLaunchKernelGGL((k_phantom<MODE, BATCH, 5, TQ>), dim3(grid), dim3(block), lds, s, rp, tab, ph, det, q);
     else if (wpe == 6)
@@ -1405,7 +1426,7 @@ int gpet_launch_phantom(const RunParams &rp, const DevTables &tab, const DevPhan
      * their photons in the phantom kernel) */
     const bool tq = q.tq && rp.mode == RUN_DECAY && (batch || ph.present);
     if (rp.mode == RUN_DECAY) {
-        const size_t lds = det.present ? (sizeof(float4) + sizeof(DevPanel)) * (size_t)det.npanels : 0;
+        const size_t lds = gpet_panel_lds_bytes(det);
         if (tq) {
             if (batch) launch_phantom_wpe<RUN_DECAY, true, true>(rp, tab, ph, det, q, grid, cfg.block, cfg.wpe, s);
             else launch_phantom_wpe<RUN_DECAY, false, true>(rp, tab, ph, det, q, grid, cfg.block, cfg.wpe, s);
@@ -1439,7 +1460,7 @@ int gpet_launch_phantom(const RunParams &rp, const DevTables &tab, const DevPhan
         int nr = 0;
         while (nr < 3 && cfg.rounds[nr] > 0) ++nr;
         if (nr == 0) nr = 1;
-        const size_t lds = det.present ? (sizeof(float4) + sizeof(DevPanel)) * (size_t)det.npanels : 0;
+        const size_t lds = gpet_panel_lds_bytes(det);
         const bool b = batch;
         for (int i = 0; i < nr; ++i) {
             const hipError_t e = hipGetLastError();
