@@ -1,19 +1,22 @@
 /*
  * transport.hip - photon transport kernels for gfx950 (MI355X).
  *
- * Two kernels per batch of histories, regrouped through a queue:
+ * Two stages per batch of histories, regrouped through a queue:
  *
  *   k_phantom  one lane per history: decay sampling (setPosition,
  *              gPET_kernals.cu:483-561) or PSF record (setPositionForPhoton,
  *              :563-604), positron range (:347-418), the annihilation pair,
  *              Woodcock tracking in the voxel phantom (photon, :256-345) and
  *              the panel entry test (photonde, :958-1009).  Only photons that
- *              enter a panel (about a tenth) are appended to the queue, one
- *              atomic per wavefront.
+ *              enter a panel (about a tenth) are appended to the block's
+ *              queue region (LDS counter, one atomic per wavefront).
+ *              Positron walks longer than LW_STEPS crossings go on in
+ *              k_longwalk; optionally, photons still in the phantom after a
+ *              step budget go on in k_transit rounds.
  *   k_panel    one lane per queued photon: Woodcock tracking in the crystal
  *              panel with analytic Klein-Nishina, Rayleigh and photoelectric
  *              (:1011-1193), hit records, adder and readout (:737-813), and
- *              the event append (one atomic per wavefront).
+ *              the event append (staged in LDS, one reservation per block).
  *
  * Splitting at the panel face keeps the panel kernel's waves full: in the
  * reference one kernel per stage runs every photon slot, and ~90% of the
