@@ -267,11 +267,11 @@ struct RangeWalk {
      * when s == rr exactly (rho = |density|, so one rule covers both ends).
      * Crossings after the end in the same call read whatever lies there and
      * are ignored. */
-    template <bool UNIFORM = true>
+    template <bool UNIFORM = true, int K = WALK_K>
     __device__ __forceinline__ void step4(const DevPhantom &p, unsigned &nsteps)
     {
-        float tn[WALK_K];
-        float g[WALK_K];
+        float tn[K];
+        float g[K];
         /* UNIFORM (one phantom per wave): the densities are read through a
          * buffer descriptor with the array's size as its range (a read past
          * the range, possible only after the end, returns 0) and 32-bit
@@ -292,7 +292,7 @@ struct RangeWalk {
         const unsigned F = flip;
         unsigned O = o;
 #pragma unroll
-        for (int k = 0; k < WALK_K; ++k) {
+        for (int k = 0; k < K; ++k) {
             const bool bx = (tmx <= tmy) & (tmx <= tmz);
             const bool by = !bx & (tmy <= tmz);
             const bool bz = !bx & !by;
@@ -317,7 +317,7 @@ struct RangeWalk {
         const bool was = done;
         float ftn = 0.0f, fs = 0.0f, frho = 1.0f;
 #pragma unroll
-        for (int k = 0; k < WALK_K; ++k) {
+        for (int k = 0; k < K; ++k) {
             const float rho = fabsf(g[k]);
             const bool inb = !(g[k] < 0.0f);
             const float sn = s + (tn[k] - tcur) * rho;
@@ -332,7 +332,7 @@ struct RangeWalk {
             tcur = tn[k];
         }
         if (done & !was) t = (fs == rr) ? ftn : ftn + (rr - fs) / frho;
-        it += WALK_K;
+        it += K;
         if (it >= D_STEP_CAP) done = true;
     }
 
@@ -519,6 +519,10 @@ __device__ __forceinline__ void d_photons(const RunParams &rp, const DevTables &
 #ifndef LW_STEPS
 #define LW_STEPS 32
 #endif
+/* crossings per walk step in k_longwalk */
+#ifndef LW_WALK_K
+#define LW_WALK_K 8
+#endif
 
 template <int MODE, bool TQ>
 __device__ __forceinline__ void d_history(const RunParams &rp, const DevTables &tab, const DevPhantom &ph,
@@ -678,27 +682,13 @@ __device__ __forceinline__ void d_photon_tail(const RunParams &rp, const DevDete
     float y2 = 0.f, z2 = 0.f, vx2 = 0.f, vy2 = 0.f, vz2 = 0.f;
     if (alive && det.present) {
         const float vv = vx * vx + vy * vy + vz * vz;
-        /* candidates: with groups, the members of the groups whose sphere
-         * the line passes near, else every panel; tried in index order, so
-         * the first panel the exact test accepts is the reference's */
-        unsigned long long cand = 0;
-        const int G = det.ngroups;
-        if (G > 0) {
-            const float4 *s_gf = s_pf + det.npanels;
-            const unsigned long long *s_gm = reinterpret_cast<const unsigned long long *>(s_gf + G);
-            for (int g = 0; g < G; ++g)
-                if (d_line_near(s_gf[g], x, y, z, vx, vy, vz, vv)) cand |= s_gm[g];
-        }
-        for (int pi = 0; G > 0 ? (cand != 0) : (pi < det.npanels); ++pi) {
-            if (G > 0) {
-                pi = __ffsll(cand) - 1;
-                cand &= cand - 1;
-            }
-            /* conservative prefilter: the line through the photon must
-             * pass within the face's bounding sphere (covers the
-             * reference's backward entries too); skipping a panel here
-             * never changes which panel the exact test picks */
-            if (!d_line_near(s_pf[pi], x, y, z, vx, vy, vz, vv)) continue;
+        /* the exact entry test of panel pi (conservative prefilter first:
+         * the line through the photon must pass within the face's bounding
+         * sphere, which covers the reference's backward entries too;
+         * skipping a panel there never changes which panel the exact test
+         * picks) */
+        auto try_panel = [&](int pi) -> bool {
+            if (!d_line_near(s_pf[pi], x, y, z, vx, vy, vz, vv)) return false;
             const DevPanel &P = s_p[pi];
             float rx = x - P.ox, ry = y - P.oy, rz = z - P.oz;
             float tx = rx * P.ux[0] + ry * P.ux[1] + rz * P.ux[2];
@@ -707,18 +697,35 @@ __device__ __forceinline__ void d_photon_tail(const RunParams &rp, const DevDete
             float tvx = vx * P.ux[0] + vy * P.ux[1] + vz * P.ux[2];
             float tvy = vx * P.uy[0] + vy * P.uy[1] + vz * P.uy[2];
             float tvz = vx * P.uz[0] + vy * P.uz[1] + vz * P.uz[2];
-            if (tvx * P.dirx >= 0.0f) {
-                float ty2 = ty - (tx / tvx) * tvy;
-                float tz2 = tz - (tx / tvx) * tvz;
-                if (fabsf(ty2) < P.leny / 2.0f && fabsf(tz2) < P.lenz / 2.0f) {
-                    y2 = ty2;
-                    z2 = tz2;
-                    t = t + (double)(-tx) / (D_C_LIGHT * (double)tvx);
-                    vx2 = tvx; vy2 = tvy; vz2 = tvz;
-                    pa = pi;
-                    break;
-                }
+            if (!(tvx * P.dirx >= 0.0f)) return false;
+            float ty2 = ty - (tx / tvx) * tvy;
+            float tz2 = tz - (tx / tvx) * tvz;
+            if (!(fabsf(ty2) < P.leny / 2.0f && fabsf(tz2) < P.lenz / 2.0f)) return false;
+            y2 = ty2;
+            z2 = tz2;
+            t = t + (double)(-tx) / (D_C_LIGHT * (double)tvx);
+            vx2 = tvx; vy2 = tvy; vz2 = tvz;
+            pa = pi;
+            return true;
+        };
+        const int G = det.ngroups;
+        if (G > 0) {
+            /* candidates: the members of the groups whose sphere the line
+             * passes near, tried in index order, so the first panel the
+             * exact test accepts is the reference's */
+            const float4 *s_gf = s_pf + det.npanels;
+            const unsigned long long *s_gm = reinterpret_cast<const unsigned long long *>(s_gf + G);
+            unsigned long long cand = 0;
+            for (int g = 0; g < G; ++g)
+                if (d_line_near(s_gf[g], x, y, z, vx, vy, vz, vv)) cand |= s_gm[g];
+            while (cand) {
+                const int pi = __ffsll(cand) - 1;
+                cand &= cand - 1;
+                if (try_panel(pi)) break;
             }
+        } else {
+            for (int pi = 0; pi < det.npanels; ++pi)
+                if (try_panel(pi)) break;
         }
     }
     if (tr && pa >= 0) {
@@ -918,7 +925,7 @@ __global__ __launch_bounds__(256) void k_longwalk(RunParams rp, DevTables tab, D
         w.t = 0.0f;
         w.done = false;
         unsigned ns = 0;
-        while (!w.done) w.step4<!BATCH>(P, ns);
+        while (!w.done) w.step4<!BATCH, LW_WALK_K>(P, ns);
         C.rs += ns;
         float px = L.px, py = L.py, pz = L.pz;
         w.end(px, py, pz);
