@@ -93,15 +93,30 @@ __device__ __forceinline__ void flush_counters5(unsigned long long *const dst[5]
 
 /* ---- source ------------------------------------------------------------ */
 
+/* Positron kinetic energy by rejection from the isotope's spectrum
+ * polynomial: candidate i is (E, u) from draws 2i, 2i+1, the first accepted
+ * one (at most 100000 candidates) is kept.  Called at a block boundary and
+ * followed by a re-alignment, so one generator block yields two candidates
+ * in draw order and a wave's lanes, which accept after different numbers of
+ * tries, run half as many loop trips; the result and the stream position
+ * after the alignment are those of one candidate per trip. */
 __device__ float d_sample_ek(const DevIsotope &iso, gp_rng &r)
 {
     float E = 0.511f;
-    for (int it = 0; it < 100000; ++it) {
-        E = gp_rng_uniform(&r) * (iso.coef[0] - 0.511f) + 0.511f;
-        float u = iso.coef[1] * gp_rng_uniform(&r);
-        float p = iso.coef[2];
-        for (int i = 3; i < 8; ++i) p = p * E + iso.coef[i];
-        if (u <= p) break;
+    for (int it = 0; it < 100000; it += 2) {
+        const gp_u4 b = gp_rng_block(&r);
+        float E1 = gp_u01(b.x) * (iso.coef[0] - 0.511f) + 0.511f;
+        float u1 = iso.coef[1] * gp_u01(b.y);
+        float p1 = iso.coef[2];
+        for (int i = 3; i < 8; ++i) p1 = p1 * E1 + iso.coef[i];
+        E = E1;
+        if (u1 <= p1) break;
+        float E2 = gp_u01(b.z) * (iso.coef[0] - 0.511f) + 0.511f;
+        float u2 = iso.coef[1] * gp_u01(b.w);
+        float p2 = iso.coef[2];
+        for (int i = 3; i < 8; ++i) p2 = p2 * E2 + iso.coef[i];
+        E = E2;
+        if (u2 <= p2) break;
     }
     return (E - 0.511f) * 1.0e6f;
 }
