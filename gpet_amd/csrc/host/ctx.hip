@@ -560,20 +560,26 @@ gpet_status autotune(gpet_ctx *c, const RunParams &rp0, int64_t n_hist, const Pa
         for (int k = 0; k < 2; ++k)
             if ((st = dalloc(c, c->d_tq[k], qcap)) || (st = dalloc(c, c->d_tq_count[k], 64 * (int64_t)c->n_cu + 1)))
                 return st;
-        const int32_t presets[2][4] = {{0, 0, 0, 0}, {3, 6, 12, 0}};
+        const int32_t presets[3][4] = {{0, 0, 0, 0}, {3, 6, 12, 0}, {4, 8, 16, 0}};
         float best_ms = 0;
         set_tq(L);
         if ((st = time_phantom(L, best_ms))) return st;
+        /* each schedule with each register budget (the regrouped kernel
+         * holds more state, so its best budget differs) */
+        const gpet_launch base = L;
         for (const auto &pr : presets) {
-            gpet_launch T = L;
-            memcpy(T.transit, pr, sizeof(T.transit));
-            if (memcmp(T.transit, L.transit, sizeof(T.transit)) == 0) continue;
-            set_tq(T);
-            float ms = 0;
-            if ((st = time_phantom(T, ms))) return st;
-            if (ms < best_ms * 0.98f) {
-                best_ms = ms;
-                L = T;
+            for (int w : gpet_wpe_variants) {
+                gpet_launch T = base;
+                memcpy(T.transit, pr, sizeof(T.transit));
+                T.wpe[0] = w;
+                if (memcmp(&T, &base, sizeof(T)) == 0) continue;
+                set_tq(T);
+                float ms = 0;
+                if ((st = time_phantom(T, ms))) return st;
+                if (ms < best_ms * 0.98f) {
+                    best_ms = ms;
+                    L = T;
+                }
             }
         }
         set_tq(L);

@@ -139,12 +139,12 @@ struct LongWalkItem {
 };
 
 /* A photon still in the phantom after the phantom kernel's first Woodcock
- * steps, handed to the transit kernel (96 B). */
+ * steps, handed to the transit kernel (72 B; a traced photon's emission
+ * fields go to its trace record at the hand-off). */
 struct TransitItem {
-    double t, tstart;          /* time now / at emission (us)                 */
+    double t;                  /* time now (us)                               */
     int64_t h;                 /* history                                     */
     float x, y, z, vx, vy, vz, E;
-    float px, py, pz, E0;      /* emission point and energy (trace)           */
     int ns, nc, nr, it;        /* steps inside, Compton, Rayleigh; all steps  */
     unsigned n;                /* stream position, photon index in bit 31     */
     int parn, slot;            /* photon id; phantom slot (batched runs)      */

@@ -663,7 +663,7 @@ __device__ __forceinline__ void d_photon_tail(const RunParams &rp, const DevDete
                                               int k, int parn, bool active, bool alive, float x, float y, float z,
                                               float vx, float vy, float vz, float E, double t, int ns, int nc,
                                               int nr, unsigned draws, float px, float py, float pz, float E0,
-                                              double tstart, PhCounters &C)
+                                              double tstart, PhCounters &C, bool head = true)
 {
     const int64_t slot = 2 * h + k;
     gpet_trace *tr = (rp.trace && slot < rp.trace_cap) ? rp.trace + slot : nullptr;
@@ -683,11 +683,11 @@ __device__ __forceinline__ void d_photon_tail(const RunParams &rp, const DevDete
         tr->history = h;
         tr->photon = k;
         tr->panel = -1;
-        tr->t0 = tstart;
+        if (head) tr->t0 = tstart;
         if (active) {
             tr->status = GPET_TR_ALIVE | (alive ? GPET_TR_ESCAPED : GPET_TR_ABSORBED);
             tr->n_steps = ns; tr->n_compton = nc; tr->n_rayleigh = nr;
-            tr->x0 = px; tr->y0 = py; tr->z0 = pz; tr->E0 = E0;
+            if (head) { tr->x0 = px; tr->y0 = py; tr->z0 = pz; tr->E0 = E0; }
             tr->x = x; tr->y = y; tr->z = z; tr->vx = vx; tr->vy = vy; tr->vz = vz; tr->E = E;
             tr->t = t;
             tr->draws = draws;
@@ -786,14 +786,12 @@ __device__ __forceinline__ TransitItem *d_transit_slot(TransitItem *tq, int64_t 
     return (push && idx < region) ? tq + (int64_t)blockIdx.x * region + idx : nullptr;
 }
 
-__device__ __forceinline__ void d_transit_write(TransitItem *T, double t, double tstart, int64_t h, float x, float y,
-                                                float z, float vx, float vy, float vz, float E, float px, float py,
-                                                float pz, float E0, int ns, int nc, int nr, int it, unsigned n,
-                                                int parn, int slot)
+__device__ __forceinline__ void d_transit_write(TransitItem *T, double t, int64_t h, float x, float y, float z,
+                                                float vx, float vy, float vz, float E, int ns, int nc, int nr, int it,
+                                                unsigned n, int parn, int slot)
 {
-    T->t = t; T->tstart = tstart; T->h = h;
+    T->t = t; T->h = h;
     T->x = x; T->y = y; T->z = z; T->vx = vx; T->vy = vy; T->vz = vz; T->E = E;
-    T->px = px; T->py = py; T->pz = pz; T->E0 = E0;
     T->ns = ns; T->nc = nc; T->nr = nr; T->it = it;
     T->n = n; T->parn = parn; T->slot = slot;
 }
@@ -831,9 +829,17 @@ __device__ __forceinline__ void d_photons(const RunParams &rp, const DevTables &
         }
         if (TQ) {
             TransitItem *T = d_transit_slot(q.tq, q.region, s_tn, defer);
-            if (T)
-                d_transit_write(T, t, tstart, h, x, y, z, vx, vy, vz, E, px, py, pz, e[k], ns, nc, nr, it,
-                                r.n | ((unsigned)k << 31), parn, slot);
+            if (T) {
+                d_transit_write(T, t, h, x, y, z, vx, vy, vz, E, ns, nc, nr, it, r.n | ((unsigned)k << 31), parn,
+                                slot);
+                /* emission fields of a traced photon: written now */
+                const int64_t ts = 2 * h + k;
+                if (rp.trace && ts < rp.trace_cap) {
+                    gpet_trace *tr = rp.trace + ts;
+                    tr->t0 = tstart;
+                    tr->x0 = px; tr->y0 = py; tr->z0 = pz; tr->E0 = e[k];
+                }
+            }
         }
         if (!defer)
             d_photon_tail(rp, det, s_pf, s_p, q, s_qn, h, k, parn, active, alive, x, y, z, vx, vy, vz, E, t, ns, nc,
@@ -1005,12 +1011,12 @@ __global__ __launch_bounds__(256) void k_transit(RunParams rp, DevTables tab, De
         if (out) {
             TransitItem *U = d_transit_slot(out, q.region, &s_tn, defer);
             if (U)
-                d_transit_write(U, t, T.tstart, T.h, x, y, z, vx, vy, vz, E, T.px, T.py, T.pz, T.E0, ns, nc, nr, it,
-                                r.n | ((unsigned)k << 31), T.parn, T.slot);
+                d_transit_write(U, t, T.h, x, y, z, vx, vy, vz, E, ns, nc, nr, it, r.n | ((unsigned)k << 31), T.parn,
+                                T.slot);
         }
         if (!defer)
             d_photon_tail(rp, det, s_pf, s_p, q, &s_qn, T.h, k, T.parn, true, st != WALK_ABSORBED, x, y, z, vx, vy,
-                          vz, E, t, ns, nc, nr, r.n, T.px, T.py, T.pz, T.E0, T.tstart, C);
+                          vz, E, t, ns, nc, nr, r.n, 0.0f, 0.0f, 0.0f, 0.0f, 0.0, C, false);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
