@@ -137,8 +137,10 @@ def test_deadtime_and_policies(tables, case):
     assert_parity(a, b)
 
 
-@pytest.mark.parametrize("ptype", [0, 1])
-def test_psf_parity(tables, ptype):
+# phase-space runs finish their photons in the phantom kernel: a launch
+# with regrouping on must give the same results
+@pytest.mark.parametrize("ptype,transit", [(0, None), (1, None), (0, (1, 2)), (1, (1,))])
+def test_psf_parity(tables, ptype, transit):
     from gpet_amd import engine
     from gpet_amd import workloads as W
 
@@ -151,7 +153,7 @@ def test_psf_parity(tables, ptype):
     rec[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
     rec[:, 7] = 511e3 if ptype == 1 else rng.uniform(1e4, 6e5, n)
     opt = engine.make_options(seed=9, positron_range=1)
-    a, b = run_both(tables, opt, lambda e: W.setup_engine(e, tables, config=2, phantom_n=80),
+    a, b = run_both(tables, opt, with_transit(lambda e: W.setup_engine(e, tables, config=2, phantom_n=80), transit),
                     lambda e: e.run_psf(rec, ptype, singles_cap=n, coinc_cap=n, adder_cap=4 * n, trace_cap=2 * n))
     assert a.tallies.n_adder > 10
     assert_parity(a, b)
