@@ -806,8 +806,7 @@ __device__ __forceinline__ void d_photons(const RunParams &rp, const DevTables &
                                           float py, float pz, const float (&v)[2][3], const float (&e)[2],
                                           double tstart, int nph, int slot, PhCounters &C)
 {
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
+    auto photon = [&](const int k) {
         const bool active = k < nph;
         float x = px, y = py, z = pz, vx = v[k][0], vy = v[k][1], vz = v[k][2], E = e[k];
         double t = tstart;
@@ -844,6 +843,15 @@ __device__ __forceinline__ void d_photons(const RunParams &rp, const DevTables &
         if (!defer)
             d_photon_tail(rp, det, s_pf, s_p, q, s_qn, h, k, parn, active, alive, x, y, z, vx, vy, vz, E, t, ns, nc,
                           nr, r.n, px, py, pz, e[k], tstart, C);
+    };
+    /* the regrouping variant keeps one photon's state at a time (unrolled,
+     * its hand-off would keep both and cost a wave per SIMD) */
+    if (TQ) {
+#pragma unroll 1
+        for (int k = 0; k < 2; ++k) photon(k);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) photon(k);
     }
 }
 
