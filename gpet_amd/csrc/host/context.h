@@ -18,7 +18,7 @@
 #include "../kernels/surr_common.h"
 #include "internal.h"
 
-constexpr int64_t DEFAULT_BATCH = 1 << 22;
+constexpr int64_t DEFAULT_BATCH = 1 << 23;
 
 template <class T>
 struct DBuf {
