@@ -271,7 +271,7 @@ gpet_launch default_launch()
     memset(&L, 0, sizeof(L));
     L.block[0] = 256; L.block[1] = 256;
     L.wpe[0] = 0; L.wpe[1] = 0;
-    L.blocks_per_cu[0] = 16; L.blocks_per_cu[1] = 2;  /* the tuner's pick on the cylinder workload */
+    L.blocks_per_cu[0] = 16; L.blocks_per_cu[1] = 8;  /* the tuner's pick on the cylinder workload */
     L.batch = DEFAULT_BATCH;
     return L;
 }
@@ -452,7 +452,9 @@ gpet_status autotune(gpet_ctx *c, const RunParams &rp0, int64_t n_hist, const Pa
 {
     auto t0 = std::chrono::steady_clock::now();
     gpet_status st;
-    const int64_t ns = std::min<int64_t>(n_hist, (int64_t)1 << 21);
+    /* trials on one launch's worth of the run's histories (at least 2M):
+     * the best grid depends on how many histories a launch covers */
+    const int64_t ns = std::min<int64_t>(n_hist, std::max<int64_t>((int64_t)1 << 21, c->launch.batch));
     /* queue large enough for every grid the search tries (>= 1 block per CU,
      * <= 256 threads per block) */
     const int64_t qcap = 2 * ns + 2 * (int64_t)64 * c->n_cu * 256;

@@ -296,7 +296,7 @@ class Engine:
             self._ok(self.f["set_phantom_slot"](self.ctx, int(slot), mat.ctypes.data_as(C.c_void_p),
                                                 dens.ctypes.data_as(C.c_void_p), d, o, s), "set_phantom_slot")
 
-    def set_launch(self, block=(256, 256), wpe=(0, 0), blocks_per_cu=(16, 2), batch=1 << 23, transit=None):
+    def set_launch(self, block=(256, 256), wpe=(0, 0), blocks_per_cu=(16, 8), batch=1 << 23, transit=None):
         """transit: (steps in the phantom kernel, steps of transit rounds
         1..3); None keeps the library's current setting."""
         L = capi.Launch()
