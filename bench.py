@@ -38,12 +38,12 @@ QUEUE_BYTES = 6 * 4 + 8 + 16   # one queued photon: 6 floats, a double, an int4
 
 WORKLOADS = {
     # batch: histories per phantom-kernel launch (0: the library's 2^23).  The
-    # cylinder workload's 1e8 decays run in six launches of 2^24: per-launch
+    # cylinder workload's 1e8 decays run in three launches of 2^25: per-launch
     # fixed costs (queue-region prefixes, counter flushes, the last waves of each
-    # grid) then weigh less.  The body workload keeps the default: its event
-    # buffer (1.2 events per decay) plus a 2^24 batch's headroom does not fit
-    # with the digitizer's work buffers.
-    "cylinder": dict(config=2, decays=1e8, seed=2, batch=1 << 24, counters="k_phantom_cylinder200_1e8_counters.md",
+    # grid) then weigh less (2^25 measured 3.6% above 2^24).  The body workload
+    # keeps the default: its event buffer (1.2 events per decay) plus a 2^24
+    # batch's headroom does not fit with the digitizer's work buffers.
+    "cylinder": dict(config=2, decays=1e8, seed=2, batch=1 << 25, counters="k_phantom_cylinder200_1e8_counters.md",
                      text="cylinder phantom 200^3 over 1 cm (water r=0.4, bone rod r=0.1, air), uniform F-18 "
                           "with positron range, %.0e decays over 10 s, 8-panel LSO octagon at 8 cm, "
                           "plain MC (no surrogate)"),
