@@ -23,6 +23,7 @@
 
 #include "../../../include/gpet.h"
 #include "../common/gp_math.h"
+#include "../common/gp_binomial.h"
 #include "internal.h"
 
 #include "context.h"
@@ -82,34 +83,6 @@ void build_majorant(const gpet_ctx *c, const float *rho_max, std::vector<float> 
 
 /* ---- source plan (restates the sampling rule documented in gpet.h) ----- */
 
-int64_t binomial_draw(double n, double p, uint64_t seed, int src)
-{
-    if (n <= 0.0 || p <= 0.0) return 0;
-    if (p >= 1.0) return (int64_t)std::floor(n);
-    gp_rng r;
-    gp_rng_init(&r, seed, (uint64_t)src, 15u, 0u);
-    double mean = n * p;
-    double u = gp_rng_uniform_d(&r);
-    if (mean < 200.0) {
-        double pk = std::exp(n * std::log1p(-p));
-        double cdf = pk;
-        int64_t k = 0;
-        while (u > cdf && (double)k < n) {
-            pk = pk * (n - (double)k) / (double)(k + 1) * p / (1.0 - p);
-            k++;
-            cdf += pk;
-            if (pk < 1e-300 && (double)k > mean) break;
-        }
-        return k;
-    }
-    double u2 = gp_rng_uniform_d(&r);
-    double z = std::sqrt(-2.0 * std::log(u)) * std::cos(6.283185307179586 * u2);
-    double k = std::floor(mean + std::sqrt(mean * (1.0 - p)) * z + 0.5);
-    if (k < 0.0) k = 0.0;
-    if (k > n) k = std::floor(n);
-    return (int64_t)k;
-}
-
 /* Decay counts of one acquisition's sources (appended to cnt and ds).
  * src_base is the index of src[0] in the run's source array: it keys the
  * binomial stream, so a source draws the same count alone or in a batch. */
@@ -151,7 +124,7 @@ gpet_status plan_job(gpet_ctx *c, const gpet_source *src, int ns, const gpet_iso
                          [](const std::pair<double, int> &a, const std::pair<double, int> &b) { return a.first > b.first; });
         for (int k = 0; assigned < n_decays; ++k, ++assigned) cnt[rem[k % ns].second]++;
     } else {
-        for (int i = 0; i < ns; ++i) cnt[i] = binomial_draw(navail[i], ds[i].pdec, c->opt.seed, src_base + i);
+        for (int i = 0; i < ns; ++i) cnt[i] = gp_binomial_draw(navail[i], ds[i].pdec, c->opt.seed, src_base + i);
     }
     cnt_out.insert(cnt_out.end(), cnt.begin(), cnt.end());
     ds_out.insert(ds_out.end(), ds.begin(), ds.end());

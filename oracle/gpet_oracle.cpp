@@ -19,14 +19,26 @@
  * Parity pins (what checks this file besides the GPU path):
  *   philox / math     - KATs and error bounds (tests/test_rng_math.py)
  *   table lerp        - node values of input4gPET.* (tests/test_tables.py)
+ *   surface read      - node / mid-cell values and a chi-square against the
+ *                       angular distribution the surface encodes
+ *                       (tests/test_oracle_pins.py)
  *   majorant          - invariant mu*rho_max*lambda <= 1 on a dense grid
  *   samplers          - chi2/KS against closed forms (tests/test_oracle_physics.py)
+ *   decay counts/times- chi-square against Binomial(n, p); per-source counts
+ *                       in a run; KS of emission times against the truncated
+ *                       exponential (tests/test_oracle_pins.py)
+ *   positron walk     - Maxwell KS in homogeneous media; two-slab closed
+ *                       form and brute-force crossing lists in heterogeneous
+ *                       media, exits into air (tests/test_oracle_pins.py)
  *   geometry          - brute-force crystal index, closed-form panel entry
  *   transport         - narrow-beam exp(-mu L) in water and LSO
+ *   adder / readout   - rebuilt from the hit records at every readout depth
+ *                       and policy (tests/test_oracle_pins.py)
  *   digitizer         - brute-force O(n^2) restatements on small lists
  */
 #include "../include/gpet.h"
 #include "../gpet_amd/csrc/common/gp_math.h"
+#include "../gpet_amd/csrc/common/gp_binomial.h"
 
 #include <algorithm>
 #include <chrono>
@@ -763,25 +775,11 @@ inline void dda_axis(real x, real off, real d, int i, real u, real &tmax, real &
     }
 }
 
-void positron_range(const Phantom &p, real &x, real &y, real &z, real ek_ev, int usedir,
-                    real vx, real vy, real vz, gp_rng &r, Counters &cnt)
+/* The walk of a positron with mass range rr (g/cm^2 over the reference's
+ * unit density: rr cm of water) from (x, y, z) along the unit direction
+ * (ux, uy, uz): the loop of setPositronRange, gPET_kernals.cu:380-415. */
+void range_walk(const Phantom &p, real &x, real &y, real &z, real ux, real uy, real uz, real rr, Counters &cnt)
 {
-    real ekin = ek_ev / 1.0e6f;
-    real rex = ((0.1f * 5.44040782f) * ekin * ekin) / (0.369516529f + ekin);
-    real sigma = rex / 2.0f;
-    real dx = sigma * R_NORMAL(&r);
-    real dy = sigma * R_NORMAL(&r);
-    real dz = sigma * R_NORMAL(&r);
-    real rr = std::sqrt(dx * dx + dy * dy + dz * dz);
-    if (usedir) {
-        real tmp = std::sqrt(vx * vx + vy * vy + vz * vz);
-        dx = rr * vx / tmp;
-        dy = rr * vy / tmp;
-        dz = rr * vz / tmp;
-    }
-    if (!(rr > 0.0f)) return;
-    real inv_rr = 1.0f / rr;
-    real ux = dx * inv_rr, uy = dy * inv_rr, uz = dz * inv_rr;
     int ix = 0, iy = 0, iz = 0;
     real t;
     if (!(p.nx > 0 && vox_index(p, x, y, z, ix, iy, iz))) {
@@ -822,6 +820,31 @@ void positron_range(const Phantom &p, real &x, real &y, real &z, real ek_ev, int
     z = z + uz * t;
 }
 
+/* setPositronRange (gPET_kernals.cu:347-418): a Gaussian 3-D displacement
+ * of sigma = Rex/2 per axis (Rex from the positron's kinetic energy) gives
+ * the range rr = |displacement| and, unless usedir, the direction; then the
+ * walk above. */
+void positron_range(const Phantom &p, real &x, real &y, real &z, real ek_ev, int usedir,
+                    real vx, real vy, real vz, gp_rng &r, Counters &cnt)
+{
+    real ekin = ek_ev / 1.0e6f;
+    real rex = ((0.1f * 5.44040782f) * ekin * ekin) / (0.369516529f + ekin);
+    real sigma = rex / 2.0f;
+    real dx = sigma * R_NORMAL(&r);
+    real dy = sigma * R_NORMAL(&r);
+    real dz = sigma * R_NORMAL(&r);
+    real rr = std::sqrt(dx * dx + dy * dy + dz * dz);
+    if (usedir) {
+        real tmp = std::sqrt(vx * vx + vy * vy + vz * vz);
+        dx = rr * vx / tmp;
+        dy = rr * vy / tmp;
+        dz = rr * vz / tmp;
+    }
+    if (!(rr > 0.0f)) return;
+    real inv_rr = 1.0f / rr;
+    range_walk(p, x, y, z, dx * inv_rr, dy * inv_rr, dz * inv_rr, rr, cnt);
+}
+
 /* Pair emission (gPET_kernals.cu:536-555): isotropic direction, Gaussian
  * noncollinearity delta; photon 1 along d with E = mc2 + delta mc2/2,
  * photon 2 rotated by (cos = -cos delta, phi) with E = mc2 - delta mc2/2. */
@@ -853,39 +876,11 @@ struct SourcePlan {
     int64_t total = 0;
 };
 
-/* Binomial(n, p) from one uniform pair: inversion for small means, a
- * rounded normal approximation otherwise.  Host-only, FP64 libm. */
-int64_t binomial_draw(double n, double p, uint64_t seed, int src)
-{
-    if (n <= 0.0 || p <= 0.0) return 0;
-    if (p >= 1.0) return (int64_t)std::floor(n);
-    gp_rng r;
-    gp_rng_init(&r, seed, (uint64_t)src, 15u, 0u);
-    double mean = n * p;
-    double u = gp_rng_uniform_d(&r);
-    if (mean < 200.0) {
-        double pk = std::exp(n * std::log1p(-p));
-        double cdf = pk;
-        int64_t k = 0;
-        while (u > cdf && (double)k < n) {
-            pk = pk * (n - (double)k) / (double)(k + 1) * p / (1.0 - p);
-            k++;
-            cdf += pk;
-            if (pk < 1e-300 && (double)k > mean) break;
-        }
-        return k;
-    }
-    double u2 = gp_rng_uniform_d(&r);
-    double z = std::sqrt(-2.0 * std::log(u)) * std::cos(6.283185307179586 * u2);
-    double k = std::floor(mean + std::sqrt(mean * (1.0 - p)) * z + 0.5);
-    if (k < 0.0) k = 0.0;
-    if (k > n) k = std::floor(n);
-    return (int64_t)k;
-}
-
 /* Decay counts of one acquisition's sources, appended to the plan.
  * src_base is the index of src[0] in the run's source array (it keys the
- * binomial stream). */
+ * binomial stream).  Counts: the exact Binomial(navail, pdec) of
+ * gp_binomial.h (shared with the library's run planning; pinned on its own
+ * by a chi-square against the binomial distribution). */
 gpet_status plan_job(const gpet_ctx &C, const gpet_source *src, int ns, const gpet_isotope *iso, int niso, double t0,
                      double t1, int64_t n_decays, int src_base, int phantom, SourcePlan &pl)
 {
@@ -918,7 +913,7 @@ gpet_status plan_job(const gpet_ctx &C, const gpet_source *src, int ns, const gp
         });
         for (int k = 0; assigned < n_decays; ++k, ++assigned) cnt[rem[k % ns].second]++;
     } else {
-        for (int i = 0; i < ns; ++i) cnt[i] = binomial_draw(navail[i], pdec[i], C.opt.seed, src_base + i);
+        for (int i = 0; i < ns; ++i) cnt[i] = gp_binomial_draw(navail[i], pdec[i], C.opt.seed, src_base + i);
     }
     for (int i = 0; i < ns; ++i) {
         pl.first.push_back(pl.first.back() + cnt[i]);
@@ -1709,6 +1704,12 @@ float gpet_cpu_surface(gpet_ctx *c, int32_t which, int32_t mat, float E, float u
     return surf_read(c->tab, which, mat, E, u);
 }
 
+/* The same read for n cumulative probabilities u[i] at one energy. */
+void gpet_cpu_surface_n(gpet_ctx *c, int32_t which, int32_t mat, float E, const float *u, int64_t n, float *out)
+{
+    for (int64_t i = 0; i < n; ++i) out[i] = surf_read(c->tab, which, mat, E, u[i]);
+}
+
 void gpet_cpu_sample_kn(float E, uint64_t seed, int64_t n, float *efrac, float *costh)
 {
     gp_rng r;
@@ -1763,6 +1764,31 @@ gpet_status gpet_cpu_positron_range(gpet_ctx *c, const float *xyz0, float ek, ui
         xyz[3 * i] = x; xyz[3 * i + 1] = y; xyz[3 * i + 2] = z;
     }
     return GPET_OK;
+}
+
+/* The positron walk alone for n given ranges rr[i] (g/cm^2) and unit
+ * directions dir[3i..3i+2] from xyz0[3i..3i+2] (needs a phantom); returns
+ * the end points and, if steps is non-NULL, the voxel crossings inside the
+ * phantom per walk. */
+gpet_status gpet_cpu_range_walk(gpet_ctx *c, const float *xyz0, const float *dir, const float *rr, int64_t n,
+                                float *xyz, int64_t *steps)
+{
+    if (!c || !c->has_ph) return GPET_ERR_STATE;
+    for (int64_t i = 0; i < n; ++i) {
+        Counters cnt;
+        float x = xyz0[3 * i], y = xyz0[3 * i + 1], z = xyz0[3 * i + 2];
+        if (rr[i] > 0.0f)
+            range_walk(c->slots[0], x, y, z, dir[3 * i], dir[3 * i + 1], dir[3 * i + 2], rr[i], cnt);
+        xyz[3 * i] = x; xyz[3 * i + 1] = y; xyz[3 * i + 2] = z;
+        if (steps) steps[i] = cnt.range_steps;
+    }
+    return GPET_OK;
+}
+
+/* Binomial count probe: the draw of source `src` of a run with this seed. */
+int64_t gpet_cpu_binomial(double n, double p, uint64_t seed, int32_t src)
+{
+    return gp_binomial_draw(n, p, seed, src);
 }
 
 /* Crystal index probe: returns m_id and writes module/crystal ids. */
