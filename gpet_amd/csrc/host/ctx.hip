@@ -427,10 +427,24 @@ gpet_status autotune(gpet_ctx *c, const RunParams &rp0, int64_t n_hist, const Pa
     gpet_status st;
     /* trials on one launch's worth of the run's histories (at least 2M):
      * the best grid depends on how many histories a launch covers */
-    const int64_t ns = std::min<int64_t>(n_hist, std::max<int64_t>((int64_t)1 << 21, c->launch.batch));
+    int64_t ns = std::min<int64_t>(n_hist, std::max<int64_t>((int64_t)1 << 21, c->launch.batch));
     /* queue large enough for every grid the search tries (>= 1 block per CU,
      * <= 256 threads per block) */
-    const int64_t qcap = 2 * ns + 2 * (int64_t)64 * c->n_cu * 256;
+    auto qcap_of = [&](int64_t n) { return 2 * n + 2 * (int64_t)64 * c->n_cu * 256; };
+    /* the trials hold the panel queue, the long-walk buffer, two transit
+     * buffers and an event buffer of that size: halve the trial (down to
+     * 2M histories) until they take at most half of the free memory */
+    {
+        size_t free_b = 0, total_b = 0;
+        HIPOK(c, hipMemGetInfo(&free_b, &total_b));
+        auto bytes = [&](int64_t n) {
+            const double q = (double)qcap_of(n);
+            return q * (48.0 + sizeof(LongWalkItem) / 8.0 + 2.0 * sizeof(TransitItem)
+                        + GPET_MAX_PH_EVENTS * sizeof(gpet_event));
+        };
+        while (ns > ((int64_t)1 << 21) && bytes(ns) > 0.5 * (double)free_b) ns /= 2;
+    }
+    const int64_t qcap = qcap_of(ns);
     if ((st = alloc_queue(c, qcap))) return st;
     if ((st = dgrow(c, c->d_events, GPET_MAX_PH_EVENTS * qcap, 0))) return st;
     DevQueue q = make_queue(c, qcap, false);
@@ -564,6 +578,9 @@ gpet_status autotune(gpet_ctx *c, const RunParams &rp0, int64_t n_hist, const Pa
     if ((st = time_phantom(L, dummy))) return st;
     if ((st = search(1, time_panel))) return st;
     c->launch = L;
+    /* transit buffers the chosen schedule does not use */
+    for (int k = 0; k < 2; ++k)
+        if (L.transit[0] == 0 || (k == 1 && L.transit[1] == 0)) { dfree(c->d_tq[k]); dfree(c->d_tq_count[k]); }
     *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return GPET_OK;
 }

@@ -1379,7 +1379,7 @@ template <class T>
 void copy_out(const std::vector<T> &v, T *dst, int64_t cap, int64_t &n, int64_t per = 1)
 {
     n = (int64_t)v.size() / per;
-    if (dst && cap > 0) {
+    if (dst && cap > 0 && n > 0) {
         int64_t m = std::min<int64_t>(n, cap);
         memcpy(dst, v.data(), (size_t)(m * per) * sizeof(T));
     }

@@ -8,7 +8,7 @@ of events) are compared value by value: deposited energy, interaction
 position and timestamps of the events after adder/readout, and the phantom
 exit state of every photon.
 
-usage: python -m gpet_amd.precision_report [config] [decays]
+usage: python tests/precision_report.py [config] [decays]
 """
 from __future__ import annotations
 
@@ -23,6 +23,7 @@ ROOT = os.path.dirname(HERE)
 
 
 def run(lib, config, n, seed=17):
+    sys.path.insert(0, ROOT)
     from gpet_amd import engine, workloads as W
     eng = engine.Engine(engine.make_options(seed=seed, positron_range=1 if config == 2 else 0),
                         lib_path=lib, prefix="gpet_cpu_")

@@ -4,7 +4,7 @@ discrete path, and the continuous results agree to FP32 rounding."""
 
 
 def test_fp32_error_against_fp64():
-    from gpet_amd.precision_report import report
+    from precision_report import report
     r = report(config=2, n=50000)
     assert r["same_path_fraction"] > 0.9999
     assert r["events_matched"] > 3000 and r["events_only_one_side"] <= 2
