@@ -151,7 +151,7 @@ class Tallies(C.Structure):
         ("launches", C.c_int64),
         ("launch", Launch), ("graph", C.c_int32), ("n_jobs", C.c_int32), ("seconds_tune", C.c_double),
         ("n_surrogate", C.c_int64), ("seconds_surrogate", C.c_double), ("n_noise", C.c_int64),
-        ("n_delayed", C.c_int64), ("sinogram_delayed", C.c_void_p)]
+        ("n_delayed", C.c_int64), ("sinogram_delayed", C.c_void_p), ("range_jumps", C.c_int64)]
 
 FLAG_NO_GRAPH = 1
 FLAG_AUTOTUNE = 2

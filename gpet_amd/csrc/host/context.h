@@ -30,6 +30,7 @@ struct DBuf {
 struct PhSlot {
     DBuf<int2> vox;
     DBuf<float> dens_b;
+    DBuf<uint4> wmap;
     DBuf<float> wck;
     std::vector<float> h_wck;
     DevPhantom dp;

@@ -24,6 +24,7 @@
 #include "../../../include/gpet.h"
 #include "../common/gp_math.h"
 #include "../common/gp_binomial.h"
+#include "../common/gp_walkmap.h"
 #include "internal.h"
 
 #include "context.h"
@@ -869,6 +870,7 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
     T.majorant_violations = (int64_t)cnt[CNT_MAJORANT];
     T.phantom_steps = (int64_t)cnt[CNT_PHANTOM_STEPS];
     T.range_steps = (int64_t)cnt[CNT_RANGE_STEPS];
+    T.range_jumps = (int64_t)cnt[CNT_RANGE_JUMPS];
     T.panel_steps = (int64_t)cnt[CNT_PANEL_STEPS];
     T.seconds_transport = ms_t * 1e-3;
     T.seconds_digitizer = ms_d * 1e-3;
@@ -1033,7 +1035,7 @@ gpet_status gpet_destroy(gpet_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     dfree(c->d_mu4); dfree(c->d_surf[0]); dfree(c->d_surf[1]);
     for (PhSlot &sl : c->slots) {
-        dfree(sl.vox); dfree(sl.dens_b); dfree(sl.wck);
+        dfree(sl.vox); dfree(sl.dens_b); dfree(sl.wck); dfree(sl.wmap);
     }
     dfree(c->d_phs); dfree(c->d_seg); dfree(c->d_job_first); dfree(c->d_job_cnt);
     drop_graph(c);
@@ -1152,8 +1154,18 @@ gpet_status gpet_set_phantom_slot(gpet_ctx *c, int32_t slot, const int32_t *mat,
                         dens[((size_t)iz * dims[1] + iy) * dims[0] + ix];
         if ((st = upload(c, S.dens_b, db.data(), (int64_t)db.size()))) return st;
     }
+    /* uniform boxes of the positron walk (none: no map) */
+    std::vector<uint32_t> wm;
+    const bool has_map = gp_build_walkmap(dims[0], dims[1], dims[2], dens, wm);
+    if (has_map) {
+        if ((st = upload(c, S.wmap, reinterpret_cast<const uint4 *>(wm.data()), (int64_t)(wm.size() / 4)))) return st;
+    } else {
+        dfree(S.wmap);
+    }
     DevPhantom &p = S.dp;
     memset(&p, 0, sizeof(p));
+    p.wmap = has_map ? S.wmap.p : nullptr;
+    p.wmap_bytes = has_map ? (unsigned)(wm.size() * sizeof(uint32_t)) : 0u;
     p.dens_b = S.dens_b.p;
     p.nbx = nbx;
     p.nby = nby;

@@ -46,6 +46,9 @@ struct DevPhantom {
                                   walk); the border layer holds -D_RHO_OUTSIDE */
     int nbx, nby;              /* bricks along x and y (padded grid)          */
     unsigned dens_bytes;       /* size of dens_b (fewer than 2^24 bricks)     */
+    const uint4 *wmap;         /* uniform box per padded brick (gp_walkmap.h),
+                                  or null when the phantom has no uniform brick */
+    unsigned wmap_bytes;
     DevWck wck;
     int present;
 };
@@ -177,7 +180,7 @@ struct DevQueue {
 /* Device-side counters, 64-bit, accumulated with atomics. */
 enum {
     CNT_PHOTONS = 0, CNT_PANEL_ENTRIES, CNT_HITS, CNT_DROPPED_EVENTS, CNT_DROPPED_HITS,
-    CNT_MAJORANT, CNT_PHANTOM_STEPS, CNT_RANGE_STEPS, CNT_PANEL_STEPS, CNT_NUM
+    CNT_MAJORANT, CNT_PHANTOM_STEPS, CNT_RANGE_STEPS, CNT_PANEL_STEPS, CNT_RANGE_JUMPS, CNT_NUM
 };
 /* Each counter is striped over CNT_STRIPES words (counter k of block b goes
  * to cnt[k * CNT_STRIPES + b % CNT_STRIPES]; the host sums the stripes):

@@ -72,19 +72,20 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
     return v;
 }
 
-/* Block-wide sums of five per-lane counters, one atomic per counter and
+/* Block-wide sums of N per-lane counters, one atomic per counter and
  * block (called by every thread of the block, after its work loop). */
-__device__ __forceinline__ void flush_counters5(unsigned long long *const dst[5], const unsigned long long v[5])
+template <int N>
+__device__ __forceinline__ void flush_counters(unsigned long long *const dst[N], const unsigned long long v[N])
 {
-    __shared__ unsigned long long s_red[4][5];
+    __shared__ unsigned long long s_red[4][N];
     const int w = threadIdx.x >> 6;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
+    for (int k = 0; k < N; ++k) {
         const unsigned long long s = wave_sum(v[k]);
         if (__lane_id() == 0) s_red[w][k] = s;
     }
     __syncthreads();
-    if (threadIdx.x < 5) {
+    if (threadIdx.x < N) {
         unsigned long long s = 0;
         for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += s_red[i][threadIdx.x];
         if (s) atomicAdd(dst[threadIdx.x] + (blockIdx.x % CNT_STRIPES), s);
@@ -196,6 +197,7 @@ struct RangeWalk {
      * when the axis' local coordinate is 3 going up or 0 going down). */
     unsigned o, flip;
     int dxi, dyi, dzi, dxc, dyc, dzc;
+    float ivx, ivy, ivz;       /* 1/u per axis (0 for a zero component)  */
     float tcur, s, t;
     int it;
     bool done;
@@ -250,6 +252,80 @@ struct RangeWalk {
         flip = (sx > 0 ? 12u : 0u) | (sy > 0 ? 48u : 0u) | (sz > 0 ? 192u : 0u);
     }
 
+    __device__ __forceinline__ void inverses()
+    {
+        ivx = ux != 0.0f ? 1.0f / ux : 0.0f;
+        ivy = uy != 0.0f ? 1.0f / uy : 0.0f;
+        ivz = uz != 0.0f ? 1.0f / uz : 0.0f;
+    }
+
+    /* One jump across a uniform box of the phantom's walk map
+     * (gp_walkmap.h; the oracle's range_jump, same arithmetic): every
+     * crossing before the box's exit enters a voxel of the box's density, so
+     * the walk moves in one step to the crossing into the box's last voxel
+     * layer on its exit side (ties between axes: x, then y, then z); the
+     * crossings after it are left to the voxel steps.  The new voxel along
+     * the other axes is the one holding the point at the new parameter,
+     * kept inside the box.  (x0, y0, z0): the walk's start.  UNIFORM: one
+     * phantom per wave (buffer load bounded by the map's size); otherwise a
+     * plain load (the brick index is always inside the map). */
+    template <bool UNIFORM>
+    __device__ __forceinline__ void jump(const DevPhantom &p, float x0, float y0, float z0, unsigned &njump)
+    {
+        if (done || !p.wmap) return;
+        ++njump;
+        uint4 m;
+        if (UNIFORM) {
+            const uint64_t base = (uint64_t)(uintptr_t)p.wmap;
+            const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)base);
+            const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(base >> 32));
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                (void *)(uintptr_t)(((uint64_t)hi << 32) | lo), (short)0,
+                (int)__builtin_amdgcn_readfirstlane(p.wmap_bytes), 0x00020000);
+            const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, (o >> 8) << 4, 0, 0);
+            m = make_uint4(v[0], v[1], v[2], v[3]);
+        } else {
+            m = p.wmap[o >> 8];
+        }
+        const float rho = __uint_as_float(m.w);
+        if (!(rho >= 0.0f)) return;
+        const int lx = (int)(m.x & 0xffffu), hx = (int)(m.x >> 16);
+        const int ly = (int)(m.y & 0xffffu), hy = (int)(m.y >> 16);
+        const int lz = (int)(m.z & 0xffffu), hz = (int)(m.z >> 16);
+        /* padded bounds: the crossing into the exit layer is boundary hi - 1
+         * going up, lo going down (phantom voxel i is padded i + 1) */
+        const float Tx = ux > 0.0f ? (p.offx + (float)(hx - 1) * p.dx - x0) * ivx
+                                   : (ux < 0.0f ? (p.offx + (float)lx * p.dx - x0) * ivx : INFINITY);
+        const float Ty = uy > 0.0f ? (p.offy + (float)(hy - 1) * p.dy - y0) * ivy
+                                   : (uy < 0.0f ? (p.offy + (float)ly * p.dy - y0) * ivy : INFINITY);
+        const float Tz = uz > 0.0f ? (p.offz + (float)(hz - 1) * p.dz - z0) * ivz
+                                   : (uz < 0.0f ? (p.offz + (float)lz * p.dz - z0) * ivz : INFINITY);
+        const bool ax = (Tx <= Ty) & (Tx <= Tz);
+        const bool ay = !ax & (Ty <= Tz);
+        const bool az = !ax & !ay;
+        const float T = ax ? Tx : (ay ? Ty : Tz);
+        if (!(T > tcur)) return;
+        const float sn = s + (T - tcur) * rho;
+        if (sn >= rr) {
+            t = tcur + (rr - s) / rho;
+            done = true;
+            return;
+        }
+        s = sn;
+        tcur = T;
+        int px = (int)floorf(((x0 + T * ux) - p.offx) * p.idx) + 1;
+        int py = (int)floorf(((y0 + T * uy) - p.offy) * p.idy) + 1;
+        int pz = (int)floorf(((z0 + T * uz) - p.offz) * p.idz) + 1;
+        px = ax ? (ux > 0.0f ? hx : lx) : min(max(px, lx), hx);
+        py = ay ? (uy > 0.0f ? hy : ly) : min(max(py, ly), hy);
+        pz = az ? (uz > 0.0f ? hz : lz) : min(max(pz, lz), hz);
+        if (ux != 0.0f) tmx = (p.offx + (float)(ux > 0.0f ? px : px - 1) * p.dx - x0) * ivx;
+        if (uy != 0.0f) tmy = (p.offy + (float)(uy > 0.0f ? py : py - 1) * p.dy - y0) * ivy;
+        if (uz != 0.0f) tmz = (p.offz + (float)(uz > 0.0f ? pz : pz - 1) * p.dz - z0) * ivz;
+        o = ((((unsigned)(pz >> 2) * (unsigned)p.nby + (unsigned)(py >> 2)) * (unsigned)p.nbx + (unsigned)(px >> 2))
+             << 8) | ((unsigned)(((pz & 3) << 4) | ((py & 3) << 2) | (px & 3)) << 2);
+    }
+
     /* the DDA from (x, y, z) along (ux, uy, uz); outside the phantom the
      * walk ends at once in air */
     __device__ __forceinline__ void setup(const DevPhantom &p, float x, float y, float z)
@@ -266,6 +342,7 @@ struct RangeWalk {
         d_dda_axis(z, p.offz, p.dz, iz, uz, tmz, tdz, sz);
         o = (unsigned)gpet_brick_offset(ix + 1, iy + 1, iz + 1, p.nbx, p.nby) << 2;
         deltas(p.nbx, p.nby);
+        inverses();
         tcur = 0.0f;
         s = 0.0f;
         it = 0;
@@ -283,8 +360,11 @@ struct RangeWalk {
      * Crossings after the end in the same call read whatever lies there and
      * are ignored. */
     template <bool UNIFORM = true, int K = WALK_K>
-    __device__ __forceinline__ void step4(const DevPhantom &p, unsigned &nsteps)
+    __device__ __forceinline__ void step4(const DevPhantom &p, unsigned &nsteps, unsigned &njump, float x0, float y0,
+                                          float z0)
     {
+        jump<UNIFORM>(p, x0, y0, z0, njump);
+        if (done) return;
         float tn[K];
         float g[K];
         /* UNIFORM (one phantom per wave): the densities are read through a
@@ -361,13 +441,15 @@ struct RangeWalk {
 };
 
 __device__ void d_positron_range(const DevPhantom &p, float &x, float &y, float &z, float ek_ev, int usedir,
-                                 float vx, float vy, float vz, gp_rng &r, unsigned long long &nsteps)
+                                 float vx, float vy, float vz, gp_rng &r, unsigned long long &nsteps,
+                                 unsigned long long &njumps)
 {
     RangeWalk w;
     w.begin(p, x, y, z, ek_ev, usedir, vx, vy, vz, r);
-    unsigned n = 0;
-    while (!w.done) w.step4(p, n);
+    unsigned n = 0, nj = 0;
+    while (!w.done) w.step4(p, n, nj, x, y, z);
     nsteps += n;
+    njumps += nj;
     w.end(x, y, z);
 }
 
@@ -477,7 +559,7 @@ __device__ __forceinline__ void d_psf_append(gpet_psf_record *buf, unsigned long
 }
 
 struct PhCounters {
-    unsigned long long ph = 0, ent = 0, maj = 0, ps = 0, rs = 0;
+    unsigned long long ph = 0, ent = 0, maj = 0, ps = 0, rs = 0, rj = 0;
 };
 
 /* Source of history h: the largest si with first[si] <= h. */
@@ -558,9 +640,9 @@ __device__ __forceinline__ void d_history(const RunParams &rp, const DevTables &
             if (rp.prange) {
                 RangeWalk w;
                 w.begin(ph, px, py, pz, ek, 0, 0.f, 0.f, 0.f, r);
-                unsigned ns = 0;
+                unsigned ns = 0, nj = 0;
                 if (q.lw) {
-                    for (int k = 0; k < LW_STEPS / WALK_K && !w.done; ++k) w.step4(ph, ns);
+                    for (int k = 0; k < LW_STEPS / WALK_K && !w.done; ++k) w.step4(ph, ns, nj, px, py, pz);
                     if (!w.done) {
                         const unsigned lcap = (unsigned)(q.region >> 3);
                         const unsigned li = atomicAdd(s_ln, 1u);
@@ -580,12 +662,14 @@ __device__ __forceinline__ void d_history(const RunParams &rp, const DevTables &
                             L.pad = 0;
                             q.lw[(int64_t)blockIdx.x * lcap + li] = L;
                             c_rs += ns;
+                            C.rj += nj;
                             return;   /* k_longwalk finishes this history */
                         }
                     }
                 }
-                while (!w.done) w.step4(ph, ns);
+                while (!w.done) w.step4(ph, ns, nj, px, py, pz);
                 c_rs += ns;
+                C.rj += nj;
                 w.end(px, py, pz);
             }
             gp_rng_align(&r);
@@ -605,7 +689,7 @@ __device__ __forceinline__ void d_history(const RunParams &rp, const DevTables &
         } else {
             gp_rng r;
             gp_rng_init(&r, rp.seed, (uint64_t)h, GP_TAG_DECAY, 0u);
-            if (rp.prange) d_positron_range(ph, px, py, pz, rE, 1, rvx, rvy, rvz, r, c_rs);
+            if (rp.prange) d_positron_range(ph, px, py, pz, rE, 1, rvx, rvy, rvz, r, c_rs, C.rj);
             gp_rng_align(&r);
             d_emit_pair(rp.sigma, r, v, e);
             nph = 2;
@@ -912,12 +996,12 @@ void k_phantom(RunParams rp, DevTables tab, DevPhantom ph, DevDetector det, DevQ
         if (TQ) q.tq_count[blockIdx.x] = min((unsigned)q.region, s_tn);
     }
     {
-        unsigned long long *const dst[5] = {
+        unsigned long long *const dst[6] = {
             rp.cnt + CNT_PHOTONS * CNT_STRIPES, rp.cnt + CNT_PANEL_ENTRIES * CNT_STRIPES,
             rp.cnt + CNT_MAJORANT * CNT_STRIPES, rp.cnt + CNT_PHANTOM_STEPS * CNT_STRIPES,
-            rp.cnt + CNT_RANGE_STEPS * CNT_STRIPES};
-        const unsigned long long v[5] = {C.ph, C.ent, C.maj, C.ps, C.rs};
-        flush_counters5(dst, v);
+            rp.cnt + CNT_RANGE_STEPS * CNT_STRIPES, rp.cnt + CNT_RANGE_JUMPS * CNT_STRIPES};
+        const unsigned long long v[6] = {C.ph, C.ent, C.maj, C.ps, C.rs, C.rj};
+        flush_counters<6>(dst, v);
     }
 }
 
@@ -950,13 +1034,15 @@ __global__ __launch_bounds__(256) void k_longwalk(RunParams rp, DevTables tab, D
         w.tcur = L.tcur; w.s = L.s;
         w.o = L.o;
         w.deltas(P.nbx, P.nby);
+        w.inverses();
         w.it = L.it;
         w.t = 0.0f;
         w.done = false;
-        unsigned ns = 0;
-        while (!w.done) w.step4<!BATCH, LW_WALK_K>(P, ns);
-        C.rs += ns;
+        unsigned ns = 0, nj = 0;
         float px = L.px, py = L.py, pz = L.pz;
+        while (!w.done) w.step4<!BATCH, WALK_K>(P, ns, nj, px, py, pz);
+        C.rs += ns;
+        C.rj += nj;
         w.end(px, py, pz);
         const int64_t h = rp.h0 + L.i;
         gp_rng r;
@@ -972,12 +1058,12 @@ __global__ __launch_bounds__(256) void k_longwalk(RunParams rp, DevTables tab, D
         if (TQ) q.tq_count[blockIdx.x] = min((unsigned)q.region, s_tn);
     }
     {
-        unsigned long long *const dst[5] = {
+        unsigned long long *const dst[6] = {
             rp.cnt + CNT_PHOTONS * CNT_STRIPES, rp.cnt + CNT_PANEL_ENTRIES * CNT_STRIPES,
             rp.cnt + CNT_MAJORANT * CNT_STRIPES, rp.cnt + CNT_PHANTOM_STEPS * CNT_STRIPES,
-            rp.cnt + CNT_RANGE_STEPS * CNT_STRIPES};
-        const unsigned long long v[5] = {C.ph, C.ent, C.maj, C.ps, C.rs};
-        flush_counters5(dst, v);
+            rp.cnt + CNT_RANGE_STEPS * CNT_STRIPES, rp.cnt + CNT_RANGE_JUMPS * CNT_STRIPES};
+        const unsigned long long v[6] = {C.ph, C.ent, C.maj, C.ps, C.rs, C.rj};
+        flush_counters<6>(dst, v);
     }
 }
 
@@ -1032,12 +1118,12 @@ __global__ __launch_bounds__(256) void k_transit(RunParams rp, DevTables tab, De
         if (out) out_count[blockIdx.x] = min((unsigned)q.region, s_tn);
     }
     {
-        unsigned long long *const dst[5] = {
+        unsigned long long *const dst[6] = {
             rp.cnt + CNT_PHOTONS * CNT_STRIPES, rp.cnt + CNT_PANEL_ENTRIES * CNT_STRIPES,
             rp.cnt + CNT_MAJORANT * CNT_STRIPES, rp.cnt + CNT_PHANTOM_STEPS * CNT_STRIPES,
-            rp.cnt + CNT_RANGE_STEPS * CNT_STRIPES};
-        const unsigned long long v[5] = {C.ph, C.ent, C.maj, C.ps, C.rs};
-        flush_counters5(dst, v);
+            rp.cnt + CNT_RANGE_STEPS * CNT_STRIPES, rp.cnt + CNT_RANGE_JUMPS * CNT_STRIPES};
+        const unsigned long long v[6] = {C.ph, C.ent, C.maj, C.ps, C.rs, C.rj};
+        flush_counters<6>(dst, v);
     }
 }
 
@@ -1408,7 +1494,7 @@ void k_panel(PanelParams pp, DevTables tab, DevDetector det, DevQueue q)
             pp.cnt + CNT_DROPPED_HITS * CNT_STRIPES, pp.cnt + CNT_MAJORANT * CNT_STRIPES,
             pp.cnt + CNT_PANEL_STEPS * CNT_STRIPES};
         const unsigned long long v[5] = {c_hits, c_drop, c_droph, c_maj, c_steps};
-        flush_counters5(dst, v);
+        flush_counters<5>(dst, v);
     }
 }
 

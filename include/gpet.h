@@ -305,6 +305,9 @@ typedef struct gpet_tallies {
     int64_t n_noise;           /* background singles generated                  */
     int64_t n_delayed;         /* delayed-window pairs (randoms estimate)       */
     int64_t *sinogram_delayed; /* n_phi x n_s of the delayed pairs, or NULL     */
+    int64_t range_jumps;       /* walk-map reads of the positron range walk (one
+                                  per group of voxel crossings; a read inside a
+                                  uniform box moves the walk across it)         */
 } gpet_tallies;
 
 typedef struct gpet_ctx gpet_ctx;

@@ -39,6 +39,7 @@
 #include "../include/gpet.h"
 #include "../gpet_amd/csrc/common/gp_math.h"
 #include "../gpet_amd/csrc/common/gp_binomial.h"
+#include "../gpet_amd/csrc/common/gp_walkmap.h"
 
 #include <algorithm>
 #include <chrono>
@@ -120,6 +121,8 @@ struct Phantom {
     float dx = 0, dy = 0, dz = 0, idx = 0, idy = 0, idz = 0;
     std::vector<int32_t> mat;
     std::vector<float> dens;
+    std::vector<uint32_t> wmap;      /* uniform boxes per padded brick (gp_walkmap.h) */
+    int wbx = 0, wby = 0;
     Wck wck;
 };
 
@@ -134,12 +137,13 @@ struct Detector {
 
 struct Counters {
     int64_t photons = 0, panel_entries = 0, hits = 0, dropped_events = 0, majorant = 0;
-    int64_t phantom_steps = 0, range_steps = 0, panel_steps = 0;
+    int64_t phantom_steps = 0, range_steps = 0, panel_steps = 0, range_jumps = 0;
     void add(const Counters &o)
     {
         photons += o.photons; panel_entries += o.panel_entries; hits += o.hits;
         dropped_events += o.dropped_events; majorant += o.majorant;
         phantom_steps += o.phantom_steps; range_steps += o.range_steps; panel_steps += o.panel_steps;
+        range_jumps += o.range_jumps;
     }
 };
 
@@ -775,9 +779,88 @@ inline void dda_axis(real x, real off, real d, int i, real u, real &tmax, real &
     }
 }
 
+/* Next-crossing parameter along one axis from voxel i (the DDA's first
+ * boundary, as dda_axis computes it). */
+inline real dda_next(real x, real off, real d, int i, real u, int step)
+{
+    if (step > 0) return (off + (real)(i + 1) * d - x) * (1.0f / u);
+    if (step < 0) return (off + (real)i * d - x) * (1.0f / u);
+    return INFINITY;
+}
+
+/* One jump of the walk across a uniform box of the phantom's walk map
+ * (gp_walkmap.h).  The walk sits in voxel (ix, iy, iz) at parameter tcur
+ * with accumulated charge s; every crossing before the box's exit enters a
+ * voxel of the box's density rho, so the walk moves in one step to the
+ * crossing into the box's last voxel layer on its exit side (the exit of
+ * the box shrunk by one layer there: the crossing after it, possibly out of
+ * the box, is left to the voxel steps, which charge it at the density of
+ * the voxel it enters).  Ties between axes go to x, then y, then z, as in
+ * the voxel steps.  The new voxel along the other axes is the one holding
+ * the point at the new parameter, kept inside the box.  Returns true when
+ * the range runs out inside the box (t set). */
+bool range_jump(const Phantom &p, real x, real y, real z, real ux, real uy, real uz, real rr, int &ix, int &iy,
+                int &iz, real &tmx, real &tmy, real &tmz, int sx, int sy, int sz, real &tcur, real &s, real &t,
+                int64_t &cnt_jumps)
+{
+    const size_t b = ((size_t)((iz + 1) >> 2) * p.wby + (size_t)((iy + 1) >> 2)) * p.wbx + (size_t)((ix + 1) >> 2);
+    const uint32_t *w = &p.wmap[4 * b];
+    cnt_jumps++;
+    float rho_f;
+    memcpy(&rho_f, &w[3], 4);
+    if (!(rho_f >= 0.0f)) return false;
+    const real rho = rho_f;
+    const real pos[3] = {x, y, z}, u[3] = {ux, uy, uz}, off[3] = {p.offx, p.offy, p.offz}, d[3] = {p.dx, p.dy, p.dz};
+    const real id[3] = {p.idx, p.idy, p.idz};
+    const int st[3] = {sx, sy, sz};
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = (int)(w[a] & 0xffffu) - 1;      /* unpadded voxel bounds */
+        hi[a] = (int)(w[a] >> 16) - 1;
+    }
+    real T = INFINITY;
+    int ax = -1;
+    for (int a = 0; a < 3; ++a) {
+        if (st[a] == 0) continue;
+        /* s > 0: the crossing into voxel hi (boundary hi); s < 0: into voxel
+         * lo (boundary lo + 1) */
+        const int k = st[a] > 0 ? hi[a] : lo[a] + 1;
+        const real ta = (off[a] + (real)k * d[a] - pos[a]) * (1.0f / u[a]);
+        if (ta < T) { T = ta; ax = a; }
+    }
+    if (!(T > tcur)) return false;
+    const real sn = s + (T - tcur) * rho;
+    if (sn >= rr) {
+        t = tcur + (rr - s) / rho;
+        return true;
+    }
+    s = sn;
+    tcur = T;
+    int idx3[3];
+    for (int a = 0; a < 3; ++a) {
+        int i;
+        if (a == ax) {
+            i = st[a] > 0 ? hi[a] : lo[a];
+        } else {
+            /* along an axis the walk does not move this is its voxel */
+            i = (int)std::floor(((pos[a] + T * u[a]) - off[a]) * id[a]);
+            i = std::min(std::max(i, lo[a]), hi[a]);
+        }
+        idx3[a] = i;
+    }
+    ix = idx3[0]; iy = idx3[1]; iz = idx3[2];
+    tmx = dda_next(x, p.offx, p.dx, ix, ux, sx);
+    tmy = dda_next(y, p.offy, p.dy, iy, uy, sy);
+    tmz = dda_next(z, p.offz, p.dz, iz, uz, sz);
+    return false;
+}
+
 /* The walk of a positron with mass range rr (g/cm^2 over the reference's
  * unit density: rr cm of water) from (x, y, z) along the unit direction
- * (ux, uy, uz): the loop of setPositronRange, gPET_kernals.cu:380-415. */
+ * (ux, uy, uz): the loop of setPositronRange, gPET_kernals.cu:380-415, in
+ * groups of WALK_GROUP voxel crossings, each group preceded by a jump
+ * across a uniform box when the walk sits in one (range_jump). */
+constexpr int WALK_GROUP = 4;
 void range_walk(const Phantom &p, real &x, real &y, real &z, real ux, real uy, real uz, real rr, Counters &cnt)
 {
     int ix = 0, iy = 0, iz = 0;
@@ -792,27 +875,35 @@ void range_walk(const Phantom &p, real &x, real &y, real &z, real ux, real uy, r
         dda_axis(z, p.offz, p.dz, iz, uz, tmz, tdz, sz);
         real tcur = 0.0f, s = 0.0f;
         t = 0.0f;
-        for (int it = 0; it < STEP_CAP; ++it) {
-            real tnext;
-            if (tmx <= tmy && tmx <= tmz) { tnext = tmx; tmx = tmx + tdx; ix += sx; }
-            else if (tmy <= tmz) { tnext = tmy; tmy = tmy + tdy; iy += sy; }
-            else { tnext = tmz; tmz = tmz + tdz; iz += sz; }
-            bool in = ix >= 0 && ix < p.nx && iy >= 0 && iy < p.ny && iz >= 0 && iz < p.nz;
-            real rho = RHO_OUTSIDE;
-            if (in) {
-                rho = p.dens[((size_t)iz * p.ny + iy) * p.nx + ix];
-                cnt.range_steps++;
-            }
-            s = s + (tnext - tcur) * rho;
-            if (s >= rr) {
-                t = (s > rr) ? tnext + (rr - s) / rho : tnext;
+        bool done = false;
+        for (int it = 0; it < STEP_CAP && !done; it += WALK_GROUP) {
+            if (!p.wmap.empty() && range_jump(p, x, y, z, ux, uy, uz, rr, ix, iy, iz, tmx, tmy, tmz, sx, sy, sz, tcur,
+                                              s, t, cnt.range_jumps))
                 break;
+            for (int k = 0; k < WALK_GROUP; ++k) {
+                real tnext;
+                if (tmx <= tmy && tmx <= tmz) { tnext = tmx; tmx = tmx + tdx; ix += sx; }
+                else if (tmy <= tmz) { tnext = tmy; tmy = tmy + tdy; iy += sy; }
+                else { tnext = tmz; tmz = tmz + tdz; iz += sz; }
+                bool in = ix >= 0 && ix < p.nx && iy >= 0 && iy < p.ny && iz >= 0 && iz < p.nz;
+                real rho = RHO_OUTSIDE;
+                if (in) {
+                    rho = p.dens[((size_t)iz * p.ny + iy) * p.nx + ix];
+                    cnt.range_steps++;
+                }
+                s = s + (tnext - tcur) * rho;
+                if (s >= rr) {
+                    t = (s > rr) ? tnext + (rr - s) / rho : tnext;
+                    done = true;
+                    break;
+                }
+                if (!in) {
+                    t = tnext + (rr - s) / RHO_OUTSIDE;
+                    done = true;
+                    break;
+                }
+                tcur = tnext;
             }
-            if (!in) {
-                t = tnext + (rr - s) / RHO_OUTSIDE;
-                break;
-            }
-            tcur = tnext;
         }
     }
     x = x + ux * t;
@@ -1491,6 +1582,9 @@ gpet_status gpet_cpu_set_phantom_slot(gpet_ctx *c, int32_t slot, const int32_t *
         if (p.dens[i] > rmax[m]) rmax[m] = p.dens[i];
     }
     build_wck(c->tab, rmax, p.wck);
+    (void)gp_build_walkmap(p.nx, p.ny, p.nz, p.dens.data(), p.wmap);
+    p.wbx = gp_walkmap_bricks(p.nx);
+    p.wby = gp_walkmap_bricks(p.ny);
     c->slots[slot] = std::move(p);
     c->slot_set[slot] = 1;
     if (slot == 0) c->has_ph = true;
@@ -1644,6 +1738,7 @@ gpet_status gpet_cpu_get_tallies(gpet_ctx *c, gpet_tallies *t)
     t->majorant_violations = R.c.majorant;
     t->phantom_steps = R.c.phantom_steps;
     t->range_steps = R.c.range_steps;
+    t->range_jumps = R.c.range_jumps;
     t->panel_steps = R.c.panel_steps;
     t->seconds_transport = R.sec_transport;
     t->seconds_digitizer = R.sec_digi;

@@ -423,6 +423,65 @@ def test_walk_heterogeneous_brute_force(tables, seed):
         assert wrong.sum() > 20, mut
 
 
+@pytest.mark.parametrize("seed", [3, 4])
+def test_walk_jumps_over_uniform_boxes(tables, seed):
+    """A 44 x 40 x 36 phantom of uniform 8^3 blocks (densities 0.0012, 0.3,
+    1, 1.85) has many uniform bricks, so the walk crosses whole boxes of
+    equal density in one step (range_jump); its end points must still be
+    the brute-force crossing list's (every segment charged at the density
+    of the voxel it enters), for walks that end inside boxes, at box edges,
+    in other blocks and in the air outside the grid.  Planted mistakes
+    fail as in the voxel-by-voxel test."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = 44, 40, 36
+    blocks = rng.choice(np.array([0.0012, 0.3, 1.0, 1.85], np.float32), size=(nz // 8 + 1, ny // 8 + 1, nx // 8 + 1))
+    # blocks aligned with the walk map's bricks (padded voxel i + 1 in brick
+    # (i + 1) // 4), so that box exits meet density changes
+    iz, iy, ix = np.meshgrid((np.arange(nz) + 1) // 8, (np.arange(ny) + 1) // 8, (np.arange(nx) + 1) // 8,
+                             indexing="ij")
+    dens = blocks[iz, iy, ix].astype(np.float32)
+    vox = (0.02, 0.025, 0.03)
+    size = (nx * vox[0], ny * vox[1], nz * vox[2])
+    off = tuple(-v / 2 for v in size)
+    eng = _walk_engine(tables, dens, off, size)
+    n = 3000
+    p0 = np.column_stack([rng.uniform(off[a] + 0.01, off[a] + size[a] - 0.01, n) for a in range(3)])
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    # some walks along the axes and the diagonals (ties between axes)
+    d[:200] = np.eye(3)[rng.integers(0, 3, 200)] * rng.choice([-1.0, 1.0], (200, 1))
+    d[200:300] = np.array([1.0, 1.0, 1.0]) / np.sqrt(3.0) * rng.choice([-1.0, 1.0], (100, 3))
+    p0 = p0.astype(np.float32)
+    d = d.astype(np.float32)
+    rr = rng.uniform(0.0, 0.8, n).astype(np.float32)
+    out, steps = _oracle_walk(eng, p0, d, rr)
+    eng.close()
+    want = np.array([_brute_walk(dens, off, vox, p0[i], d[i], float(rr[i])) for i in range(n)])
+    dist = np.linalg.norm(out - want, axis=1)
+    scale = 1.0 + np.linalg.norm(want - p0, axis=1)
+    assert np.max(dist / scale) < 1e-4, np.argmax(dist / scale)
+    # the walk crossed far fewer voxels one by one than the brute-force lists hold
+    crossings = np.array([_crossings_inside(off, vox, (nx, ny, nz), p0[i], d[i], want[i]) for i in range(n)])
+    assert steps.sum() < 0.5 * crossings.sum()
+    for mut in ("current", "stop_at_edge", "air_0012"):
+        bad = np.array([_brute_walk(dens, off, vox, p0[i], d[i], float(rr[i]), mut) for i in range(n)])
+        wrong = np.linalg.norm(out - bad, axis=1) / scale > 1e-3
+        assert wrong.sum() > 20, mut
+
+
+def _crossings_inside(off, vox, n, p0, d, end):
+    """Voxel boundaries inside the grid between p0 and end (brute force)."""
+    L = float(np.dot(np.asarray(end, float) - p0, d))
+    c = 0
+    for a in range(3):
+        if d[a] == 0:
+            continue
+        planes = off[a] + vox[a] * np.arange(n[a] + 1)
+        t = (planes - p0[a]) / d[a]
+        c += int(np.sum((t > 0) & (t <= L)))
+    return c
+
+
 # --------------------------------------------------------- adder / readout --
 
 def _module_scanner():
