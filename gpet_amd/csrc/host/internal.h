@@ -295,9 +295,13 @@ int gpet_clip_factor(const float *g, int64_t n, float clip, float *part, float *
 int gpet_pack_train(const float *set, int64_t n, uint64_t seed, int64_t step, int64_t M, uint16_t *X, float *Y,
                     hipStream_t s);
 int gpet_loss_reduce(const float *part, int n, float *out, hipStream_t s);
-int gpet_pack_infer(const DevQueue &sq, int64_t row0, int64_t n, const DevDetector &det, uint16_t *X, hipStream_t s);
-int gpet_surr_sample(const float *H, int64_t ldh, const DevQueue &sq, int64_t row0, int64_t n, const DevDetector &det,
-                     const PanelParams &pp, hipStream_t s);
+/* The whole network in one kernel (layer l's bf16 weights w[l]): with X
+ * (packed bf16 [n x 8] inputs) the 145 head outputs of each row go to H
+ * (row stride ldh); without X the inputs are surrogate-queue rows
+ * [row0, row0 + n) of sq and each row's event is sampled and appended
+ * (pp.events / pp.n_events). */
+int gpet_mlp_infer(const uint16_t *const w[5], const uint16_t *X, const DevQueue &sq, int64_t row0, int64_t n,
+                   const DevDetector &det, const PanelParams &pp, float *H, int64_t ldh, hipStream_t s);
 /* Sets the first `n` entries of `p` to zero (graph-safe: a kernel, not a
  * memset node). */
 int gpet_launch_zero(unsigned *p, int n, hipStream_t s);

@@ -27,7 +27,6 @@ constexpr int NL = 5;
 constexpr int LN[NL] = {256, 256, 256, 256, 192};      /* outputs (stored)       */
 constexpr int LK[NL] = {SURR_NX, 256, 256, 256, 256};  /* inputs                 */
 constexpr int HID = 256, HLD = HID + 8, HEAD_LD = 192;
-constexpr int64_t WORK_ROWS_INFER = (int64_t)1 << 20;
 constexpr int64_t SPLIT_ROWS = 4096;                   /* split-K chunk (rows)   */
 
 int kp(int l) { return l == 0 ? SURR_LDX : LK[l] + 8; }
@@ -271,18 +270,20 @@ gpet_status surr_run_prepare(gpet_ctx *c, int64_t n_hist, int64_t qcap, DevQueue
 }
 
 /* Network over the n listed entries in chunks; events appended via pp. */
+/* Layer weights (bf16) of the fused inference kernel. */
+void layer_weights(const SurrState &S, const uint16_t *w[NL])
+{
+    for (int l = 0; l < NL; ++l) w[l] = S.wb.p + param_offset(l);
+}
+
 gpet_status surr_run_infer(gpet_ctx *c, int64_t n, const PanelParams &pp)
 {
     SurrState &S = c->surr;
-    gpet_status st;
     auto t0 = std::chrono::steady_clock::now();
-    if ((st = alloc_work(c, std::min<int64_t>(std::max<int64_t>(n, 1), WORK_ROWS_INFER), false))) return st;
-    for (int64_t r0 = 0; r0 < n; r0 += S.work_rows) {
-        const int64_t m = std::min<int64_t>(S.work_rows, n - r0);
-        KOK(c, gpet_pack_infer(pp.sq, r0, m, c->ddet, S.X.p, c->stream), "pack");
-        if ((st = forward(c, m, false))) return st;
-        KOK(c, gpet_surr_sample(S.H.p, HEAD_LD, pp.sq, r0, m, c->ddet, pp, c->stream), "sample");
-    }
+    /* the whole network and the sampling in one kernel: no work buffers */
+    const uint16_t *w[NL];
+    layer_weights(S, w);
+    KOK(c, gpet_mlp_infer(w, nullptr, pp.sq, 0, n, c->ddet, pp, nullptr, 0, c->stream), "fused inference");
     HIPOK(c, hipStreamSynchronize(c->stream));
     c->last_surr_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return GPET_OK;
@@ -487,7 +488,14 @@ gpet_status gpet_surrogate_forward(gpet_ctx *c, const float *x, int64_t n, float
     std::vector<uint16_t> X;
     pack_host_rows(x, n, X);
     HIPOK(c, hipMemcpy(c->surr.X.p, X.data(), sizeof(uint16_t) * X.size(), hipMemcpyHostToDevice));
-    if ((st = forward(c, n, false))) return st;
+    /* the inference kernel (the one surrogate runs use), head outputs to H */
+    const uint16_t *w[NL];
+    layer_weights(c->surr, w);
+    DevQueue none;
+    memset(&none, 0, sizeof(none));
+    PanelParams pp;
+    memset(&pp, 0, sizeof(pp));
+    KOK(c, gpet_mlp_infer(w, c->surr.X.p, none, 0, n, c->ddet, pp, c->surr.H.p, HEAD_LD, c->stream), "fused forward");
     std::vector<float> H((size_t)n * HEAD_LD);
     HIPOK(c, hipStreamSynchronize(c->stream));
     HIPOK(c, hipMemcpy(H.data(), c->surr.H.p, sizeof(float) * H.size(), hipMemcpyDeviceToHost));

@@ -319,89 +319,72 @@ __global__ void k_pack_train(const float *set, int64_t n, uint64_t seed, int64_t
     for (int c = 0; c < SURR_NY; ++c) Y[i * SURR_NY + c] = r[SURR_NX + c];
 }
 
-/* Inference inputs of surrogate-queue rows [row0, row0 + n). */
-__global__ void k_pack_infer(DevQueue q, int64_t row0, int64_t n, const DevDetector det, uint16_t *X)
+/* Sampling from the head outputs h of surrogate-queue row j: P(event) on
+ * the photon's own stream, a component by its weight, four normals; the
+ * event at the first deposit point, with the readout site of the digitizer
+ * settings (one event per photon).  Returns whether the photon gives an
+ * event (ev filled). */
+__device__ __forceinline__ bool d_surr_sample_row(const float *h, const DevQueue &q, int64_t j, const DevDetector &det,
+                                                  const PanelParams &pp, gpet_event &ev)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int64_t j = row0 + i;
-    const DevPanel &P = det.p[q.info[j].x];
-    float f[SURR_NX];
-    surr_features(P, q.E[j], q.vx[j], q.vy[j], q.vz[j], q.y[j], q.z[j], f);
-#pragma unroll
-    for (int c = 0; c < SURR_NX; ++c) X[i * SURR_LDX + c] = f2bf(f[c]);
-    X[i * SURR_LDX + SURR_NX] = 0x3f80;
-    for (int c = SURR_NX + 1; c < SURR_LDX; ++c) X[i * SURR_LDX + c] = 0;
+    const int4 info = q.info[j];
+    const DevPanel &P = det.p[info.x];
+    const unsigned draws = (unsigned)info.w & 0x7fffffffu;
+    const unsigned k = (unsigned)info.w >> 31;
+    gp_rng r;
+    gp_rng_init(&r, pp.seed, (uint64_t)(int64_t)info.z, GP_TAG_PHOTON + k, draws);
+    gp_u4 b = gp_rng_block(&r);
+    const float pe = 1.0f / (1.0f + gp_expf(-h[0]));
+    if (!(gp_u01(b.x) <= pe)) return false;
+    float mw = -1e30f;
+    for (int c = 0; c < SURR_K; ++c) mw = fmaxf(mw, h[1 + c]);
+    float sw = 0.0f;
+    for (int c = 0; c < SURR_K; ++c) sw += gp_expf(h[1 + c] - mw);
+    float u = gp_u01(b.y) * sw, acc = 0.0f;
+    int kc = SURR_K - 1;
+    for (int c = 0; c < SURR_K; ++c) {
+        acc += gp_expf(h[1 + c] - mw);
+        if (u <= acc) { kc = c; break; }
+    }
+    gp_u4 b2 = gp_rng_block(&r);
+    float z[4];
+    float s0, c0, s1, c1;
+    gp_sincos2pi(gp_u01(b2.y), &s0, &c0);
+    gp_sincos2pi(gp_u01(b2.w), &s1, &c1);
+    float r0 = sqrtf(-2.0f * gp_logf(gp_u01(b2.x))), r1 = sqrtf(-2.0f * gp_logf(gp_u01(b2.z)));
+    z[0] = r0 * c0; z[1] = r0 * s0; z[2] = r1 * c1; z[3] = r1 * s1;
+    float y[4];
+    for (int d = 0; d < 4; ++d) {
+        float ls = fminf(fmaxf(h[81 + 4 * kc + d], SURR_LOGSIG_MIN), SURR_LOGSIG_MAX);
+        y[d] = h[17 + 4 * kc + d] + gp_expf(ls) * z[d];
+    }
+    float lx, ly, lz;
+    const float E = q.E[j];
+    const float fe = fminf(y[0], 1.0f);
+    bool has = fe > 0.0f && surr_position(P, q.vx[j], q.vy[j], q.vz[j], q.y[j], q.z[j], y[1], y[2], y[3], lx, ly, lz);
+    int M_id = -1, L_id = -1;
+    if (has) has = d_crystal(det, P, lx, ly, lz, M_id, L_id) == 0;
+    if (!has) return false;
+    const int pann = P.panel;
+    float dx = lx, dy = ly - q.y[j], dz = lz - q.z[j];
+    ev.parn = info.y;
+    ev.pann = pann;
+    ev.modn = M_id;
+    ev.cryn = L_id;
+    ev.siten = surr_site(pp.readout_depth, pp.readout_policy, pann, M_id, L_id, det.moduleN, det.crystalN);
+    ev.eventid = info.z;
+    ev.t = q.t[j] + (double)sqrtf(dx * dx + dy * dy + dz * dz) * D_INV_C;
+    ev.E = fe * E;
+    ev.x = lx;
+    ev.y = ly;
+    ev.z = lz;
+    return true;
 }
 
-/* Sampling from the head: P(event) on the photon's own stream, a component
- * by its weight, four normals; the event at the first deposit point, with
- * the readout site of the digitizer settings (one event per photon). */
-__global__ __launch_bounds__(256) void k_surr_sample(const float *H, int64_t ldh, DevQueue q, int64_t row0, int64_t n,
-                                                     DevDetector det, PanelParams pp)
+/* Wave-aggregated append of the lanes' events (every lane of the wave
+ * calls it). */
+__device__ __forceinline__ void d_append_event(const PanelParams &pp, bool has, const gpet_event &ev)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool in = i < n;
-    bool has = false;
-    gpet_event ev;
-    if (in) {
-        const int64_t j = row0 + i;
-        const float *h = H + i * ldh;
-        const int4 info = q.info[j];
-        const DevPanel &P = det.p[info.x];
-        const unsigned draws = (unsigned)info.w & 0x7fffffffu;
-        const unsigned k = (unsigned)info.w >> 31;
-        gp_rng r;
-        gp_rng_init(&r, pp.seed, (uint64_t)(int64_t)info.z, GP_TAG_PHOTON + k, draws);
-        gp_u4 b = gp_rng_block(&r);
-        const float pe = 1.0f / (1.0f + gp_expf(-h[0]));
-        if (gp_u01(b.x) <= pe) {
-            float mw = -1e30f;
-            for (int c = 0; c < SURR_K; ++c) mw = fmaxf(mw, h[1 + c]);
-            float sw = 0.0f;
-            for (int c = 0; c < SURR_K; ++c) sw += gp_expf(h[1 + c] - mw);
-            float u = gp_u01(b.y) * sw, acc = 0.0f;
-            int kc = SURR_K - 1;
-            for (int c = 0; c < SURR_K; ++c) {
-                acc += gp_expf(h[1 + c] - mw);
-                if (u <= acc) { kc = c; break; }
-            }
-            gp_u4 b2 = gp_rng_block(&r);
-            float z[4];
-            float s0, c0, s1, c1;
-            gp_sincos2pi(gp_u01(b2.y), &s0, &c0);
-            gp_sincos2pi(gp_u01(b2.w), &s1, &c1);
-            float r0 = sqrtf(-2.0f * gp_logf(gp_u01(b2.x))), r1 = sqrtf(-2.0f * gp_logf(gp_u01(b2.z)));
-            z[0] = r0 * c0; z[1] = r0 * s0; z[2] = r1 * c1; z[3] = r1 * s1;
-            float y[4];
-            for (int d = 0; d < 4; ++d) {
-                float ls = fminf(fmaxf(h[81 + 4 * kc + d], SURR_LOGSIG_MIN), SURR_LOGSIG_MAX);
-                y[d] = h[17 + 4 * kc + d] + gp_expf(ls) * z[d];
-            }
-            float lx, ly, lz;
-            const float E = q.E[j];
-            const float fe = fminf(y[0], 1.0f);
-            has = fe > 0.0f && surr_position(P, q.vx[j], q.vy[j], q.vz[j], q.y[j], q.z[j], y[1], y[2], y[3], lx, ly, lz);
-            int M_id = -1, L_id = -1;
-            if (has) has = d_crystal(det, P, lx, ly, lz, M_id, L_id) == 0;
-            if (has) {
-                const int pann = P.panel;
-                float dx = lx, dy = ly - q.y[j], dz = lz - q.z[j];
-                ev.parn = info.y;
-                ev.pann = pann;
-                ev.modn = M_id;
-                ev.cryn = L_id;
-                ev.siten = surr_site(pp.readout_depth, pp.readout_policy, pann, M_id, L_id, det.moduleN, det.crystalN);
-                ev.eventid = info.z;
-                ev.t = q.t[j] + (double)sqrtf(dx * dx + dy * dy + dz * dz) * D_INV_C;
-                ev.E = fe * E;
-                ev.x = lx;
-                ev.y = ly;
-                ev.z = lz;
-            }
-        }
-    }
-    /* wave-aggregated append */
     unsigned long long m = __ballot(has);
     unsigned base = 0;
     if (m) {
@@ -413,6 +396,183 @@ __global__ __launch_bounds__(256) void k_surr_sample(const float *H, int64_t ldh
     if (has) {
         int64_t dst = (int64_t)base + __popcll(m & lt);
         if (dst < pp.events_cap) pp.events[dst] = ev;
+    }
+}
+
+/* ---- fused inference ------------------------------------------------------
+ * One block of 256 threads (4 waves) runs the whole network over IB = 128
+ * rows: the activations stay in LDS (bf16 [128][264]; column 256 holds the
+ * constant 1 that carries the bias, 257..263 zero), the weights stream
+ * through the L2 into registers, and the products run on
+ * v_mfma_f32_32x32x16_bf16 with the weights as the first operand (32
+ * neurons) and the activations as the second (32 rows).  A lane's
+ * accumulators are then 4 consecutive neurons of one row, so the SiLU and
+ * the bf16 rounding happen in registers and every 4 outputs go to LDS in
+ * one 8-byte write.  Hidden layers: wave w computes neurons [64w, 64w + 64)
+ * of all 128 rows (2 x 4 tiles, 128 accumulators).  Head: wave w computes
+ * the first 160 outputs (145 used) of rows [32w, 32w + 32), which replace
+ * the activations in LDS as fp32; then one thread per row samples the
+ * row's event (SAMPLE) or copies its 145 outputs to H.  Inputs: the
+ * surrogate queue (features packed in the prologue) or, without SAMPLE, a
+ * packed bf16 [n x 8] matrix.  The SiLU here is z / (1 + 2^(-z log2 e))
+ * with the hardware exp2 and reciprocal (training uses the exact form; the
+ * difference is below the bf16 rounding of the activations). */
+constexpr int IB = 128;
+constexpr int ILD = 264;
+constexpr int IHLD = 148;
+constexpr int IHT = 5;            /* head tiles of 32 outputs: 160 >= 145 */
+constexpr int INFER_LDS = IB * IHLD * 4 > IB * ILD * 2 ? IB * IHLD * 4 : IB * ILD * 2;
+
+struct InferWeights {
+    const uint16_t *w[5];      /* layer l: [outputs][kp(l)] bf16              */
+};
+
+__device__ __forceinline__ float silu_fast(float z)
+{
+    return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.44269504f * z));
+}
+
+__device__ __forceinline__ bf16x8_t load_w(const uint16_t *p, bool ok)
+{
+    uint4 v = ok ? *reinterpret_cast<const uint4 *>(p) : make_uint4(0, 0, 0, 0);
+    return __builtin_bit_cast(bf16x8_t, v);
+}
+
+template <bool SAMPLE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_mlp_infer(InferWeights W, const uint16_t *X, DevQueue q, int64_t row0,
+                                                   int64_t n, DevDetector det, PanelParams pp, float *H, int64_t ldh)
+{
+    __shared__ __attribute__((aligned(16))) unsigned char s_buf[INFER_LDS];
+    uint16_t *act = reinterpret_cast<uint16_t *>(s_buf);
+    float *head = reinterpret_cast<float *>(s_buf);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t b0 = (int64_t)blockIdx.x * IB;
+
+    /* prologue: inputs in columns 0..7, the bias column 256 and zeros */
+    if (threadIdx.x < IB) {
+        const int m = threadIdx.x;
+        const int64_t i = b0 + m;
+        uint16_t in[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (i < n) {
+            if (SAMPLE) {
+                const int64_t j = row0 + i;
+                const DevPanel &P = det.p[q.info[j].x];
+                float f[SURR_NX];
+                surr_features(P, q.E[j], q.vx[j], q.vy[j], q.vz[j], q.y[j], q.z[j], f);
+#pragma unroll
+                for (int c = 0; c < SURR_NX; ++c) in[c] = f2bf(f[c]);
+                in[SURR_NX] = 0x3f80;
+            } else {
+#pragma unroll
+                for (int c = 0; c < 8; ++c) in[c] = X[i * SURR_LDX + c];
+            }
+        }
+        uint16_t *row = act + m * ILD;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) row[c] = in[c];
+        row[256] = 0x3f80;
+#pragma unroll
+        for (int c = 257; c < ILD; ++c) row[c] = 0;
+    }
+    __syncthreads();
+
+    const int lr = lane & 31, kh = (lane >> 5) * 8;
+    /* hidden layers */
+#pragma unroll 1
+    for (int l = 0; l < 4; ++l) {
+        const int kp = l == 0 ? SURR_LDX : ILD;
+        const int nsteps = l == 0 ? 1 : (ILD + 15) / 16;
+        const uint16_t *w = W.w[l];
+        f32x16_t acc[2][4];
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) acc[t][mt] = f32x16_t{};
+        const int nrow0 = 64 * wave + lr, nrow1 = nrow0 + 32;
+        bf16x8_t a0 = load_w(w + (int64_t)nrow0 * kp + kh, kh < kp);
+        bf16x8_t a1 = load_w(w + (int64_t)nrow1 * kp + kh, kh < kp);
+#pragma unroll 1
+        for (int st = 0; st < nsteps; ++st) {
+            const int k = st * 16 + kh;
+            const int kn = k + 16;
+            /* next step's weights while this step multiplies */
+            const bf16x8_t na0 = load_w(w + (int64_t)nrow0 * kp + kn, kn < kp);
+            const bf16x8_t na1 = load_w(w + (int64_t)nrow1 * kp + kn, kn < kp);
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                bf16x8_t bv = *reinterpret_cast<const bf16x8_t *>(act + (mt * 32 + lr) * ILD + (k < kp ? k : 0));
+                if (k >= kp) bv = bf16x8_t{};
+                acc[0][mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bv, acc[0][mt], 0, 0, 0);
+                acc[1][mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bv, acc[1][mt], 0, 0, 0);
+            }
+            a0 = na0;
+            a1 = na1;
+        }
+        __syncthreads();
+        /* accumulator i of lane: neuron (i/4)*8 + (lane/32)*4 + i%4 of the
+         * tile, row lane%32 of the tile */
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const int nb = 64 * wave + 32 * t + g * 8 + (lane >> 5) * 4;
+                    const int m = mt * 32 + lr;
+                    const uint32_t lo = (uint32_t)f2bf(silu_fast(acc[t][mt][4 * g])) |
+                                        ((uint32_t)f2bf(silu_fast(acc[t][mt][4 * g + 1])) << 16);
+                    const uint32_t hi = (uint32_t)f2bf(silu_fast(acc[t][mt][4 * g + 2])) |
+                                        ((uint32_t)f2bf(silu_fast(acc[t][mt][4 * g + 3])) << 16);
+                    *reinterpret_cast<uint2 *>(act + m * ILD + nb) = make_uint2(lo, hi);
+                }
+        __syncthreads();
+    }
+
+    /* head: rows [32 wave, 32 wave + 32), outputs [0, 160) */
+    {
+        const uint16_t *w = W.w[4];
+        f32x16_t acc[IHT];
+#pragma unroll
+        for (int t = 0; t < IHT; ++t) acc[t] = f32x16_t{};
+        const int m = 32 * wave + lr;
+#pragma unroll 1
+        for (int st = 0; st < (ILD + 15) / 16; ++st) {
+            const int k = st * 16 + kh;
+            const bool ok = k < ILD;
+            bf16x8_t bv = *reinterpret_cast<const bf16x8_t *>(act + m * ILD + (ok ? k : 0));
+            if (!ok) bv = bf16x8_t{};
+#pragma unroll
+            for (int t = 0; t < IHT; ++t) {
+                const bf16x8_t a = load_w(w + (int64_t)(32 * t + lr) * ILD + k, ok);
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bv, acc[t], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < IHT; ++t)
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int nb = 32 * t + g * 8 + (lane >> 5) * 4;
+                if (nb < IHLD)
+                    *reinterpret_cast<float4 *>(head + m * IHLD + nb) =
+                        make_float4(acc[t][4 * g], acc[t][4 * g + 1], acc[t][4 * g + 2], acc[t][4 * g + 3]);
+            }
+        __syncthreads();
+    }
+
+    /* one thread per row */
+    if (threadIdx.x < IB) {
+        const int m = threadIdx.x;
+        const int64_t i = b0 + m;
+        const float *h = head + m * IHLD;
+        if (SAMPLE) {
+            bool has = false;
+            gpet_event ev;
+            if (i < n) has = d_surr_sample_row(h, q, row0 + i, det, pp, ev);
+            d_append_event(pp, has, ev);
+        } else if (i < n) {
+            for (int c = 0; c < SURR_NH; ++c) H[i * ldh + c] = h[c];
+        }
     }
 }
 
@@ -490,18 +650,17 @@ int gpet_loss_reduce(const float *part, int n, float *out, hipStream_t s)
     return (int)hipGetLastError();
 }
 
-int gpet_pack_infer(const DevQueue &sq, int64_t row0, int64_t n, const DevDetector &det, uint16_t *X, hipStream_t s)
+int gpet_mlp_infer(const uint16_t *const w[5], const uint16_t *X, const DevQueue &sq, int64_t row0, int64_t n,
+                   const DevDetector &det, const PanelParams &pp, float *H, int64_t ldh, hipStream_t s)
 {
     if (n <= 0) return 0;
-    hipLaunchKernelGGL(k_pack_infer, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, sq, row0, n, det, X);
+    InferWeights W;
+    for (int l = 0; l < 5; ++l) W.w[l] = w[l];
+    const dim3 grid((unsigned)((n + IB - 1) / IB));
+    if (X)
+        hipLaunchKernelGGL(k_mlp_infer<false>, grid, dim3(256), 0, s, W, X, sq, row0, n, det, pp, H, ldh);
+    else
+        hipLaunchKernelGGL(k_mlp_infer<true>, grid, dim3(256), 0, s, W, X, sq, row0, n, det, pp, H, ldh);
     return (int)hipGetLastError();
 }
 
-int gpet_surr_sample(const float *H, int64_t ldh, const DevQueue &sq, int64_t row0, int64_t n, const DevDetector &det,
-                     const PanelParams &pp, hipStream_t s)
-{
-    if (n <= 0) return 0;
-    hipLaunchKernelGGL(k_surr_sample, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, H, ldh, sq, row0, n, det,
-                       pp);
-    return (int)hipGetLastError();
-}
