@@ -923,19 +923,29 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
         if (sink->adder)
             memcpy(sink->adder, adder.data(), sizeof(gpet_event) * (size_t)std::min<int64_t>(sink->adder_cap, c->n_events));
     }
-    if ((sink->singles && sink->singles_cap > 0) || to_files) {
-        int64_t m = to_files ? c->dout.n_singles : std::min<int64_t>(sink->singles_cap, c->dout.n_singles);
-        singles.resize((size_t)m);
-        if (m) HIPOK(c, hipMemcpy(singles.data(), c->dout.singles, sizeof(gpet_event) * (size_t)m, hipMemcpyDeviceToHost));
-        if (sink->singles)
-            memcpy(sink->singles, singles.data(), sizeof(gpet_event) * (size_t)std::min<int64_t>(sink->singles_cap, m));
+    /* singles and pairs go straight into the caller's buffers (one copy;
+     * with pinned buffers at full PCIe rate); files need a host copy */
+    if (to_files) {
+        singles.resize((size_t)c->dout.n_singles);
+        if (!singles.empty())
+            HIPOK(c, hipMemcpy(singles.data(), c->dout.singles, sizeof(gpet_event) * singles.size(), hipMemcpyDeviceToHost));
+        if (sink->singles && sink->singles_cap > 0)
+            memcpy(sink->singles, singles.data(),
+                   sizeof(gpet_event) * (size_t)std::min<int64_t>(sink->singles_cap, c->dout.n_singles));
+    } else if (sink->singles && sink->singles_cap > 0) {
+        const int64_t m = std::min<int64_t>(sink->singles_cap, c->dout.n_singles);
+        if (m) HIPOK(c, hipMemcpy(sink->singles, c->dout.singles, sizeof(gpet_event) * (size_t)m, hipMemcpyDeviceToHost));
     }
-    if ((sink->coinc && sink->coinc_cap > 0) || to_files) {
-        int64_t m = to_files ? c->dout.n_coinc : std::min<int64_t>(sink->coinc_cap, c->dout.n_coinc);
-        coinc.resize((size_t)(2 * m));
-        if (m) HIPOK(c, hipMemcpy(coinc.data(), c->dout.coinc, sizeof(gpet_event) * (size_t)(2 * m), hipMemcpyDeviceToHost));
-        if (sink->coinc)
-            memcpy(sink->coinc, coinc.data(), sizeof(gpet_event) * (size_t)(2 * std::min<int64_t>(sink->coinc_cap, m)));
+    if (to_files) {
+        coinc.resize((size_t)(2 * c->dout.n_coinc));
+        if (!coinc.empty())
+            HIPOK(c, hipMemcpy(coinc.data(), c->dout.coinc, sizeof(gpet_event) * coinc.size(), hipMemcpyDeviceToHost));
+        if (sink->coinc && sink->coinc_cap > 0)
+            memcpy(sink->coinc, coinc.data(),
+                   sizeof(gpet_event) * (size_t)(2 * std::min<int64_t>(sink->coinc_cap, c->dout.n_coinc)));
+    } else if (sink->coinc && sink->coinc_cap > 0) {
+        const int64_t m = std::min<int64_t>(sink->coinc_cap, c->dout.n_coinc);
+        if (m) HIPOK(c, hipMemcpy(sink->coinc, c->dout.coinc, sizeof(gpet_event) * (size_t)(2 * m), hipMemcpyDeviceToHost));
     }
     if ((sink->delayed && sink->delayed_cap > 0) || to_files) {
         int64_t m = to_files ? c->dout.n_delayed : std::min<int64_t>(sink->delayed_cap, c->dout.n_delayed);

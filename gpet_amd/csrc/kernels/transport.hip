@@ -269,12 +269,12 @@ struct RangeWalk {
      * kept inside the box.  (x0, y0, z0): the walk's start.  UNIFORM: one
      * phantom per wave (buffer load bounded by the map's size); otherwise a
      * plain load (the brick index is always inside the map). */
+    /* walk-map record of the brick holding byte offset O of the bricked
+     * densities (UNIFORM: a buffer load bounded by the map's size; else the
+     * brick index clamped into the map) */
     template <bool UNIFORM>
-    __device__ __forceinline__ void jump(const DevPhantom &p, float x0, float y0, float z0, unsigned &njump)
+    __device__ __forceinline__ uint4 map_at(const DevPhantom &p, unsigned O) const
     {
-        if (done || !p.wmap) return;
-        ++njump;
-        uint4 m;
         if (UNIFORM) {
             const uint64_t base = (uint64_t)(uintptr_t)p.wmap;
             const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)base);
@@ -282,11 +282,23 @@ struct RangeWalk {
             const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
                 (void *)(uintptr_t)(((uint64_t)hi << 32) | lo), (short)0,
                 (int)__builtin_amdgcn_readfirstlane(p.wmap_bytes), 0x00020000);
-            const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, (o >> 8) << 4, 0, 0);
-            m = make_uint4(v[0], v[1], v[2], v[3]);
-        } else {
-            m = p.wmap[o >> 8];
+            const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, (O >> 8) << 4, 0, 0);
+            return make_uint4(v[0], v[1], v[2], v[3]);
         }
+        return p.wmap[min(O >> 8, (p.wmap_bytes >> 4) - 1u)];
+    }
+
+    template <bool UNIFORM>
+    __device__ __forceinline__ void jump(const DevPhantom &p, float x0, float y0, float z0, unsigned &njump)
+    {
+        if (done || !p.wmap) return;
+        ++njump;
+        jump_with(p, map_at<UNIFORM>(p, o), x0, y0, z0);
+    }
+
+    /* the jump with the record m of the current brick */
+    __device__ __forceinline__ void jump_with(const DevPhantom &p, const uint4 m, float x0, float y0, float z0)
+    {
         const float rho = __uint_as_float(m.w);
         if (!(rho >= 0.0f)) return;
         const int lx = (int)(m.x & 0xffffu), hx = (int)(m.x >> 16);
@@ -358,13 +370,14 @@ struct RangeWalk {
      * reads the border's -D_RHO_OUTSIDE; there t = tn + (rr - s)/rho, or tn
      * when s == rr exactly (rho = |density|, so one rule covers both ends).
      * Crossings after the end in the same call read whatever lies there and
-     * are ignored. */
+     * are ignored.  If the walk goes on, the group ends with the jump across
+     * the uniform box it now sits in, whose walk-map record is loaded with
+     * the densities (the walk starts with one jump, so the order is that of
+     * the oracle: jump, four crossings, jump, ...). */
     template <bool UNIFORM = true, int K = WALK_K>
     __device__ __forceinline__ void step4(const DevPhantom &p, unsigned &nsteps, unsigned &njump, float x0, float y0,
                                           float z0)
     {
-        jump<UNIFORM>(p, x0, y0, z0, njump);
-        if (done) return;
         float tn[K];
         float g[K];
         /* UNIFORM (one phantom per wave): the densities are read through a
@@ -409,6 +422,11 @@ struct RangeWalk {
             }
         }
         o = O;
+        /* the record for the jump after these crossings, loaded with their
+         * densities (used only if the walk goes on, when O is inside) */
+        const bool use_map = p.wmap != nullptr;
+        uint4 mnext = make_uint4(0u, 0u, 0u, 0u);
+        if (use_map) mnext = map_at<UNIFORM>(p, O);
         const bool was = done;
         float ftn = 0.0f, fs = 0.0f, frho = 1.0f;
 #pragma unroll
@@ -429,6 +447,10 @@ struct RangeWalk {
         if (done & !was) t = (fs == rr) ? ftn : ftn + (rr - fs) / frho;
         it += K;
         if (it >= D_STEP_CAP) done = true;
+        if (use_map && !done) {
+            ++njump;
+            jump_with(p, mnext, x0, y0, z0);
+        }
     }
 
     __device__ __forceinline__ void end(float &x, float &y, float &z) const
@@ -447,6 +469,7 @@ __device__ void d_positron_range(const DevPhantom &p, float &x, float &y, float 
     RangeWalk w;
     w.begin(p, x, y, z, ek_ev, usedir, vx, vy, vz, r);
     unsigned n = 0, nj = 0;
+    w.jump<true>(p, x, y, z, nj);
     while (!w.done) w.step4(p, n, nj, x, y, z);
     nsteps += n;
     njumps += nj;
@@ -641,6 +664,7 @@ __device__ __forceinline__ void d_history(const RunParams &rp, const DevTables &
                 RangeWalk w;
                 w.begin(ph, px, py, pz, ek, 0, 0.f, 0.f, 0.f, r);
                 unsigned ns = 0, nj = 0;
+                w.jump<true>(ph, px, py, pz, nj);
                 if (q.lw) {
                     for (int k = 0; k < LW_STEPS / WALK_K && !w.done; ++k) w.step4(ph, ns, nj, px, py, pz);
                     if (!w.done) {
