@@ -126,6 +126,7 @@ struct gpet_ctx {
     DBuf<int> q_slot;
     int64_t q_cap = 0;
     DBuf<unsigned> d_small;      /* [0] queue count, [1] events, [2] hits */
+    void *na_scratch = nullptr;  /* device address of gpet_run_psf_noalloc's scratch */
     DBuf<unsigned long long> d_cnt;
     DBuf<gpet_event> d_events;
     DBuf<gpet_trace> d_trace;

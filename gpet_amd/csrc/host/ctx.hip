@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1001,6 +1002,50 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
     return GPET_OK;
 }
 
+/* ---- scratch memory of gpet_run_psf_noalloc ------------------------------
+ * A file-scope device array holds everything that entry point's kernels need
+ * beyond its inputs and outputs: the transport counters, the panel queue's
+ * region counts and the panel queue itself.  Each device that loads the
+ * library holds one copy until the process ends; every call on a device
+ * shares it (calls that overlap on one device race).  Host code uses the
+ * address that hipGetSymbolAddress returns for the current device, never
+ * the array's name. */
+constexpr int64_t NA_MAX = GPET_NOALLOC_MAX_RECORDS;
+/* regions: one per phantom-kernel block, at most ceil(n / 64) + 1 */
+constexpr int64_t NA_REGIONS = NA_MAX / 64 + 2;
+/* queue slots: G x region < 2 (n + G x block) <= 4 n + 2 x 256 */
+constexpr int64_t NA_QCAP = 4 * NA_MAX + 2 * 256;
+constexpr size_t na_round(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t NA_OFF_CNT = 0;
+constexpr size_t NA_OFF_SMALL = NA_OFF_CNT + na_round(sizeof(unsigned long long) * CNT_NUM * CNT_STRIPES);
+constexpr size_t NA_OFF_RCOUNT = NA_OFF_SMALL + 256;
+constexpr size_t NA_OFF_QF = NA_OFF_RCOUNT + na_round(sizeof(unsigned) * (size_t)NA_REGIONS);
+constexpr size_t NA_OFF_QT = NA_OFF_QF + 6 * na_round(sizeof(float) * (size_t)NA_QCAP);
+constexpr size_t NA_OFF_QINFO = NA_OFF_QT + na_round(sizeof(double) * (size_t)NA_QCAP);
+constexpr size_t NA_BYTES = NA_OFF_QINFO + na_round(sizeof(int4) * (size_t)NA_QCAP);
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex na_mutex;
+void *na_ptr[NA_MAX_DEVICES] = {nullptr};
+
+/* Device address of `scratch` on the current device `dev`, looked up once
+ * per device. */
+hipError_t na_scratch_address(int dev, void **out)
+{
+    if (dev < 0 || dev >= NA_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(na_mutex);
+    if (!na_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        na_ptr[dev] = p;
+    }
+    *out = na_ptr[dev];
+    return hipSuccess;
+}
+
 } // namespace
 
 extern "C" {
@@ -1034,6 +1079,11 @@ gpet_status gpet_create(gpet_ctx **out, const gpet_options *opt)
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess && ncu > 0)
         c->n_cu = ncu;
     c->dw = gpet_digi_work_create();
+    /* the allocation-free entry point's scratch address on this device */
+    if (na_scratch_address(c->device, &c->na_scratch) != hipSuccess) {
+        (void)hipGetLastError();
+        c->na_scratch = nullptr;
+    }
     *out = c;
     return GPET_OK;
 }
@@ -1456,6 +1506,90 @@ gpet_status gpet_run_psf(gpet_ctx *c, const double *rec8, int64_t n, int32_t pty
     rp.mode = ptype == 0 ? RUN_PSF_POSITRON : RUN_PSF_PHOTON;
     rp.psf = c->d_psf.p;
     return run_common(c, rp, n, sink, nullptr, nullptr);
+}
+
+gpet_status gpet_run_psf_noalloc(gpet_ctx *c, const double *d_rec8, int64_t n, int32_t ptype, gpet_event *d_events,
+                                 int64_t events_cap, uint32_t *d_n_events, void *stream)
+{
+    gpet_status st = check_ready(c);
+    if (st) return st;
+    if (!d_rec8 || !d_events || !d_n_events || n <= 0 || events_cap <= 0 || (ptype != 0 && ptype != 1))
+        return fail(c, GPET_ERR_ARG, "psf_noalloc arguments");
+    if (n > NA_MAX)
+        return fail(c, GPET_ERR_CAPACITY, "psf_noalloc: more than GPET_NOALLOC_MAX_RECORDS (65536) records");
+    if (events_cap > (int64_t)UINT32_MAX) events_cap = (int64_t)UINT32_MAX;
+    HIPOK(c, hipSetDevice(c->device));
+    if (!c->na_scratch) {
+        /* the first context on this device could not look it up */
+        const hipError_t e = na_scratch_address(c->device, &c->na_scratch);
+        if (e != hipSuccess) return fail(c, GPET_ERR_DEVICE, std::string("hipGetSymbolAddress: ") + hipGetErrorString(e));
+    }
+    unsigned char *base = static_cast<unsigned char *>(c->na_scratch);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+
+    /* launch shape: the context's, without photon regrouping; the queue
+     * shape it implies must fit the scratch queue */
+    gpet_launch L = c->launch;
+    memset(L.transit, 0, sizeof(L.transit));
+    int G = 0;
+    int64_t region = 0;
+    queue_shape(n, L, c->n_cu, G, region);
+    if ((int64_t)G + 1 > NA_REGIONS || (int64_t)G * region > NA_QCAP)
+        return fail(c, GPET_ERR_CAPACITY, "psf_noalloc: queue shape exceeds the scratch queue");
+
+    unsigned long long *cnt = reinterpret_cast<unsigned long long *>(base + NA_OFF_CNT);
+    unsigned *small = reinterpret_cast<unsigned *>(base + NA_OFF_SMALL);
+    DevQueue q;
+    memset(&q, 0, sizeof(q));
+    float *qf = reinterpret_cast<float *>(base + NA_OFF_QF);
+    const size_t fstride = na_round(sizeof(float) * (size_t)NA_QCAP) / sizeof(float);
+    q.y = qf; q.z = qf + fstride; q.vx = qf + 2 * fstride; q.vy = qf + 3 * fstride; q.vz = qf + 4 * fstride;
+    q.E = qf + 5 * fstride;
+    q.t = reinterpret_cast<double *>(base + NA_OFF_QT);
+    q.info = reinterpret_cast<int4 *>(base + NA_OFF_QINFO);
+    q.count = reinterpret_cast<int *>(small);
+    q.cap = NA_QCAP;
+    q.rcount = reinterpret_cast<unsigned *>(base + NA_OFF_RCOUNT);
+    q.nregions = G;
+    q.region = region;
+
+    RunParams rp;
+    memset(&rp, 0, sizeof(rp));
+    rp.mode = ptype == 0 ? RUN_PSF_POSITRON : RUN_PSF_PHOTON;
+    rp.psf = d_rec8;
+    rp.h0 = 0;
+    rp.nh = n;
+    rp.seed = c->opt.seed;
+    rp.eabs = c->opt.eabs;
+    rp.sigma = c->opt.noncol_sigma;
+    rp.prange = c->opt.positron_range;
+    rp.cnt = cnt;
+    for (int k = 0; k < 4; ++k) rp.sphere[k] = c->opt.record_sphere[k];
+
+    PanelParams pp;
+    memset(&pp, 0, sizeof(pp));
+    pp.seed = c->opt.seed;
+    pp.eabs = c->opt.eabs;
+    pp.readout_depth = c->dig.readout_depth;
+    pp.readout_policy = c->dig.readout_policy;
+    pp.record_rayleigh = c->dig.record_rayleigh;
+    pp.events = d_events;
+    pp.events_cap = events_cap;
+    pp.n_events = d_n_events;
+    pp.n_hits = small + 2;
+    pp.cnt = cnt;
+
+    /* clear the counters with kernels (no memset nodes if the caller
+     * captures the stream) */
+    int e = gpet_launch_zero(reinterpret_cast<unsigned *>(cnt), 2 * CNT_NUM * CNT_STRIPES, s);
+    if (!e) e = gpet_launch_zero(small, 4, s);
+    if (!e) e = gpet_launch_zero(d_n_events, 1, s);
+    if (e) return fail(c, GPET_ERR_DEVICE, "psf_noalloc: zero kernel");
+    e = gpet_launch_phantom(rp, c->dtab, c->dph, c->ddet, q, false, cfg_of(L, 0), c->n_cu, s);
+    if (e) return fail(c, GPET_ERR_DEVICE, std::string("phantom kernel: ") + hipGetErrorString((hipError_t)e));
+    e = gpet_launch_panel(pp, c->dtab, c->ddet, q, 2 * n, cfg_of(L, 1), c->n_cu, s, PANEL_PLAIN);
+    if (e) return fail(c, GPET_ERR_DEVICE, std::string("panel kernel: ") + hipGetErrorString((hipError_t)e));
+    return GPET_OK;
 }
 
 gpet_status gpet_get_tallies(gpet_ctx *c, gpet_tallies *t)

@@ -461,6 +461,26 @@ gpet_status gpet_run_source(gpet_ctx *ctx, const gpet_source *src, int32_t n_src
 gpet_status gpet_run_psf(gpet_ctx *ctx, const double *rec8, int64_t n, int32_t ptype,
                          gpet_sink *sink);
 
+/* Allocation-free phase-space transport.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  The transport of gpet_run_psf (phantom and
+ * panel stages, adder and readout) over n <= GPET_NOALLOC_MAX_RECORDS records
+ * already in device memory (d_rec8, 8 doubles each); the events it produces
+ * (gpet_run_psf's sink->adder events, as a set: in no fixed order) go to
+ * d_events (device memory, events_cap records) and their number to
+ * *d_n_events (device memory).  A count above events_cap means the buffer
+ * was too small; the first events_cap events are written.  No digitizer, no
+ * hit, trace or phase-space outputs, no tallies.  The call only enqueues
+ * work on `stream` (a hipStream_t; NULL: the context's stream) and returns;
+ * it calls no allocator.  Working memory is a fixed device array of 12.6 MB
+ * inside the library, one per device, shared by every call on that
+ * device: calls that overlap on one device race.  n above the maximum is
+ * rejected with GPET_ERR_CAPACITY before anything is launched. */
+#define GPET_NOALLOC_MAX_RECORDS 65536
+gpet_status gpet_run_psf_noalloc(gpet_ctx *ctx, const double *d_rec8, int64_t n, int32_t ptype,
+                                 gpet_event *d_events, int64_t events_cap, uint32_t *d_n_events,
+                                 void *stream);
+
 /* Batched sampleParticle (new): n_jobs acquisitions of different phantoms
  * in one run.  Histories are numbered job after job, each job's sources in
  * order, so a run of one job equals gpet_run_source on its sources.  Every
