@@ -13,15 +13,21 @@
  * outside the phantom), such a box:
  *
  *   word 0..2: lo | hi << 16, the box's first and last padded voxel along
- *              x, y, z (a cube of bricks centred on this brick);
+ *              x, y, z (whole bricks, including this one);
  *   word 3:    the float bits of the box's density, or of -1.0f when the
  *              brick is not uniform (no box).
  *
  * A brick is uniform when its 64 padded voxels all lie in the phantom and
- * hold the same density (bit for bit).  Its box is the largest cube of
- * bricks of Chebyshev radius r <= GP_WALKMAP_RMAX around it whose bricks are
- * all uniform with the same density (found by r rounds of erosion over the
- * 26 neighbours).
+ * hold the same density (bit for bit).  Its box starts as the largest cube
+ * of bricks of Chebyshev radius r <= GP_WALKMAP_RMAX around it whose bricks
+ * are all uniform with the same density (r rounds of erosion over the 26
+ * neighbours), and then grows face by face, in turn -x, +x, -y, +y, -z, +z,
+ * by one brick layer at a time while the grown box still holds only such
+ * bricks (checked on a summed-volume table of the bricks of that density),
+ * by at most GP_WALKMAP_GROW layers per face: a slab of air beside a
+ * cylinder becomes a slab-shaped box, which a walk crossing it at a grazing
+ * angle leaves in a few jumps instead of many.  Densities held by fewer
+ * than 64 uniform bricks keep the cubes.
  */
 #ifndef GP_WALKMAP_H
 #define GP_WALKMAP_H
@@ -29,9 +35,11 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <map>
 #include <vector>
 
 #define GP_WALKMAP_RMAX 15
+#define GP_WALKMAP_GROW 24
 
 /* Bricks of the padded grid along an axis of n voxels (n + 2 padded voxels). */
 inline int gp_walkmap_bricks(int n) { return (n + 2 + 3) / 4; }
@@ -99,6 +107,73 @@ inline bool gp_build_walkmap(int nx, int ny, int nz, const float *dens, std::vec
         rad = next;
         if (!grew) break;
     }
+    /* box growth on summed-volume tables, one per density held by at least
+     * 64 uniform bricks */
+    std::vector<int> lo3(3 * nb), hi3(3 * nb);
+    for (int kz = 0; kz < bz; ++kz)
+        for (int ky = 0; ky < by; ++ky)
+            for (int kx = 0; kx < bx; ++kx) {
+                const size_t b = ((size_t)kz * by + ky) * bx + kx;
+                const int r = rad[b] < 0 ? 0 : rad[b];
+                const int k[3] = {kx, ky, kz};
+                for (int a = 0; a < 3; ++a) {
+                    lo3[3 * b + a] = k[a] - r;
+                    hi3[3 * b + a] = k[a] + r;
+                }
+            }
+    std::map<uint32_t, size_t> count;
+    for (size_t b = 0; b < nb; ++b)
+        if (rad[b] >= 0) count[rho[b]]++;
+    const size_t sx = (size_t)bx + 1, sy = (size_t)by + 1, sz = (size_t)bz + 1;
+    std::vector<int32_t> sat(sx * sy * sz);
+    for (const auto &kv : count) {
+        if (kv.second < 64) continue;
+        const uint32_t v = kv.first;
+        /* sat[z][y][x] = number of v-bricks with indices below (x, y, z) */
+        for (size_t z = 0; z < sz; ++z)
+            for (size_t y = 0; y < sy; ++y)
+                for (size_t x = 0; x < sx; ++x) {
+                    int32_t c = 0;
+                    if (x && y && z) {
+                        const size_t b = ((z - 1) * by + (y - 1)) * bx + (x - 1);
+                        c = (rad[b] >= 0 && rho[b] == v) ? 1 : 0;
+                        c += sat[(z * sy + y) * sx + x - 1] + sat[(z * sy + y - 1) * sx + x] +
+                             sat[((z - 1) * sy + y) * sx + x] - sat[(z * sy + y - 1) * sx + x - 1] -
+                             sat[((z - 1) * sy + y) * sx + x - 1] - sat[((z - 1) * sy + y - 1) * sx + x] +
+                             sat[((z - 1) * sy + y - 1) * sx + x - 1];
+                    }
+                    sat[(z * sy + y) * sx + x] = c;
+                }
+        auto S = [&](int x, int y, int z) { return sat[((size_t)z * sy + y) * sx + x]; };
+        /* v-bricks in the box [l, h] (inclusive brick indices) */
+        auto inside = [&](const int *l, const int *h) {
+            const int x0 = l[0], y0 = l[1], z0 = l[2], x1 = h[0] + 1, y1 = h[1] + 1, z1 = h[2] + 1;
+            return (int64_t)S(x1, y1, z1) - S(x0, y1, z1) - S(x1, y0, z1) - S(x1, y1, z0) + S(x0, y0, z1) +
+                   S(x0, y1, z0) + S(x1, y0, z0) - S(x0, y0, z0);
+        };
+        const int n3[3] = {bx, by, bz};
+        for (size_t b = 0; b < nb; ++b) {
+            if (rad[b] < 0 || rho[b] != v) continue;
+            int *l = &lo3[3 * b], *h = &hi3[3 * b];
+            int grown[6] = {0, 0, 0, 0, 0, 0};
+            for (bool more = true; more;) {
+                more = false;
+                for (int f = 0; f < 6; ++f) {
+                    const int a = f >> 1;
+                    if (grown[f] >= GP_WALKMAP_GROW) continue;
+                    int tl[3] = {l[0], l[1], l[2]}, th[3] = {h[0], h[1], h[2]};
+                    if (f & 1) th[a] += 1; else tl[a] -= 1;
+                    if (tl[a] < 0 || th[a] >= n3[a]) continue;
+                    const int64_t vol = (int64_t)(th[0] - tl[0] + 1) * (th[1] - tl[1] + 1) * (th[2] - tl[2] + 1);
+                    if (inside(tl, th) != vol) continue;
+                    l[a] = tl[a];
+                    h[a] = th[a];
+                    grown[f]++;
+                    more = true;
+                }
+            }
+        }
+    }
     out.assign(4 * nb, 0u);
     const float none = -1.0f;
     uint32_t none_bits;
@@ -113,9 +188,8 @@ inline bool gp_build_walkmap(int nx, int ny, int nz, const float *dens, std::vec
                     w[3] = none_bits;
                     continue;
                 }
-                const int k[3] = {kx, ky, kz};
                 for (int a = 0; a < 3; ++a) {
-                    const uint32_t lo = (uint32_t)(4 * (k[a] - r)), hi = (uint32_t)(4 * (k[a] + r) + 3);
+                    const uint32_t lo = (uint32_t)(4 * lo3[3 * b + a]), hi = (uint32_t)(4 * hi3[3 * b + a] + 3);
                     w[a] = lo | (hi << 16);
                 }
                 w[3] = rho[b];

@@ -10,7 +10,7 @@
  *              the panel entry test (photonde, :958-1009).  Only photons that
  *              enter a panel (about a tenth) are appended to the block's
  *              queue region (LDS counter, one atomic per wavefront).
- *              Positron walks longer than LW_STEPS crossings go on in
+ *              Positron walks longer than a few crossing groups go on in
  *              k_longwalk; optionally, photons still in the phantom after a
  *              step budget go on in k_transit rounds.
  *   k_panel    one lane per queued photon: Woodcock tracking in the crystal
@@ -630,16 +630,22 @@ __device__ __forceinline__ void d_photons(const RunParams &rp, const DevTables &
 
 /* One history: decay (or PSF record i), positron range, pair, phantom walk
  * of both photons, panel entry and queue append. */
-/* Crossings a positron walk may take in the phantom kernel before it is
- * handed to k_longwalk (multiple of 4).  Walk lengths have a long tail (a
- * positron that leaves the water crosses the air voxels to the grid's
- * edge), and a wave runs until its longest walk ends; deferring the tail
- * lets the short walks' waves move on, and k_longwalk runs the long ones
- * together. */
-#ifndef LW_STEPS
-#define LW_STEPS 32
+/* Crossing groups a positron walk may take in the phantom kernel before it
+ * is handed to k_longwalk.  Walk lengths have a long tail (a positron that
+ * leaves the water crosses the air to the grid's edge), and a wave runs
+ * until its longest walk ends; deferring the tail lets the short walks'
+ * waves move on, and k_longwalk runs the long ones together.  With uniform
+ * boxes (a walk map) half the walks end in their first jump and the wave
+ * maxima of the rest are long, so the hand-off comes after two groups;
+ * without a map after eight (32 crossings). */
+#ifndef LW_GROUPS_MAP
+#define LW_GROUPS_MAP 2
 #endif
-/* crossings per walk step in k_longwalk */
+#ifndef LW_GROUPS_NOMAP
+#define LW_GROUPS_NOMAP 8
+#endif
+/* crossings per walk step in k_longwalk when no walk of the wave has a map
+ * (without jumps the grouping does not change results) */
 #ifndef LW_WALK_K
 #define LW_WALK_K 8
 #endif
@@ -666,7 +672,8 @@ __device__ __forceinline__ void d_history(const RunParams &rp, const DevTables &
                 unsigned ns = 0, nj = 0;
                 w.jump<true>(ph, px, py, pz, nj);
                 if (q.lw) {
-                    for (int k = 0; k < LW_STEPS / WALK_K && !w.done; ++k) w.step4(ph, ns, nj, px, py, pz);
+                    const int groups = ph.wmap ? LW_GROUPS_MAP : LW_GROUPS_NOMAP;
+                    for (int k = 0; k < groups && !w.done; ++k) w.step4(ph, ns, nj, px, py, pz);
                     if (!w.done) {
                         const unsigned lcap = (unsigned)(q.region >> 3);
                         const unsigned li = atomicAdd(s_ln, 1u);
@@ -1064,7 +1071,11 @@ __global__ __launch_bounds__(256) void k_longwalk(RunParams rp, DevTables tab, D
         w.done = false;
         unsigned ns = 0, nj = 0;
         float px = L.px, py = L.py, pz = L.pz;
-        while (!w.done) w.step4<!BATCH, WALK_K>(P, ns, nj, px, py, pz);
+        if (__ballot(P.wmap != nullptr) != 0ull) {
+            while (!w.done) w.step4<!BATCH, WALK_K>(P, ns, nj, px, py, pz);
+        } else {
+            while (!w.done) w.step4<!BATCH, LW_WALK_K>(P, ns, nj, px, py, pz);
+        }
         C.rs += ns;
         C.rj += nj;
         w.end(px, py, pz);
