@@ -130,7 +130,7 @@ class SurrogateReport(C.Structure):
 
 SURR_NX = 6
 SURR_NY = 5
-SURR_NH = 145
+SURR_NH = 161
 
 
 class Job(C.Structure):

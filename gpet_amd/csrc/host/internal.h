@@ -129,7 +129,7 @@ struct DevSegment {
  * region r holds slots [r * region, r * region + rcount[r]).  The run-wide
  * surrogate queue uses the single counter `count` instead (region = 0). */
 /* A positron-range walk handed from the phantom kernel to k_longwalk: the
- * walk state after LW_STEPS crossings and what the history needs after it. */
+ * walk state after its first crossing groups and what the history needs after it. */
 struct LongWalkItem {
     float ux, uy, uz, rr, tmx, tmy, tmz, tdx, tdy, tdz, tcur, s;
     unsigned o;                /* byte offset of the current voxel in dens_b  */
@@ -296,7 +296,7 @@ int gpet_pack_train(const float *set, int64_t n, uint64_t seed, int64_t step, in
                     hipStream_t s);
 int gpet_loss_reduce(const float *part, int n, float *out, hipStream_t s);
 /* The whole network in one kernel (layer l's bf16 weights w[l]): with X
- * (packed bf16 [n x 8] inputs) the 145 head outputs of each row go to H
+ * (packed bf16 [n x 8] inputs) the SURR_NH head outputs of each row go to H
  * (row stride ldh); without X the inputs are surrogate-queue rows
  * [row0, row0 + n) of sq and each row's event is sampled and appended
  * (pp.events / pp.n_events). */

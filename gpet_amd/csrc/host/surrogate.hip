@@ -4,9 +4,9 @@
  * (mlp.hip), Adam training on harvested panel crossings, inference inside a
  * run, save/load, and the C ABI of gpet.h's surrogate section.
  *
- * Network: 6 -> 256 -> 256 -> 256 -> 256 -> 145 (SiLU).  Layer l keeps its
+ * Network: 6 -> 256 -> 256 -> 256 -> 256 -> 161 (SiLU).  Layer l keeps its
  * weights as [N_l x Kp_l] with Kp = K + 8 (bias in column K, zeros after);
- * the head is stored with 192 rows (rows 145.. stay zero) so every GEMM
+ * the head is stored with 192 rows (rows 161.. stay zero) so every GEMM
  * dimension is a multiple of 8.
  */
 #include <hip/hip_runtime.h>

@@ -3,10 +3,11 @@
  * with FP32 accumulation and fused epilogues, the mixture-density loss and
  * its gradient, Adam, and the inference-side input packing and sampling.
  *
- * The network (6 -> 256 -> 256 -> 256 -> 256 -> 145) maps a photon's panel
+ * The network (6 -> 256 -> 256 -> 256 -> 256 -> 161) maps a photon's panel
  * entry (energy, local direction, entry point) to P(event) and a 16-component
  * diagonal Gaussian mixture over (deposited-energy fraction, depth of the
- * first deposit, its lateral offsets from the straight entry ray).  It stands
+ * first deposit, its lateral offsets from the straight entry ray), each
+ * component with its own full-absorption probability.  It stands
  * in for the panel transport of photonde (gPET_kernals.cu:1011-1193) plus the
  * adder/readout (:737-813) inside its validity range; nothing in the reference
  * corresponds to the network itself.
@@ -158,11 +159,17 @@ __global__ void k_bias_columns(uint16_t *a, int64_t M, int64_t ld, int c0)
  * h[1 .. 16]      component logits
  * h[17 + 4k + d]  mean of component k, target d
  * h[81 + 4k + d]  log sigma, clamped to [SURR_LOGSIG_MIN, SURR_LOGSIG_MAX]
- * Targets y: {event, fE, depth, dy, dz}. */
+ * h[145 + k]      photopeak logit of component k
+ * Targets y: {event, fE, depth, dy, dz}.  The deposited-energy fraction has
+ * a point mass at 1 (full absorption, ~70% of events): component k gives a
+ * photopeak with probability q_k = sigmoid(h[145 + k]), and a continuum
+ * fraction from its Gaussian otherwise, so an event's likelihood under k
+ * is  q_k N3(depth, dy, dz)  if fE >= SURR_PP_MIN, else
+ * (1 - q_k) N(fE) N3(depth, dy, dz). */
 
 __device__ __forceinline__ float softplus(float x) { return x > 0.0f ? x + gp_logf(1.0f + gp_expf(-x)) : gp_logf(1.0f + gp_expf(x)); }
 
-/* Per-sample loss and gradient dL/dh / n_total (bf16, cols >= 145 zero);
+/* Per-sample loss and gradient dL/dh / n_total (bf16, cols >= SURR_NH zero);
  * loss partial sums per block to part[blockIdx.x] (fixed order). */
 __global__ __launch_bounds__(256) void k_mdn_loss_grad(const float *H, int64_t ldh, const float *Y, int64_t M,
                                                        float inv_n, uint16_t *dH, int64_t lddh, float *part)
@@ -191,11 +198,15 @@ __global__ __launch_bounds__(256) void k_mdn_loss_grad(const float *H, int64_t l
         for (int k = 0; k < SURR_K; ++k) sw += gp_expf(lw[k] - mw);
         const float lse_w = mw + gp_logf(sw);
         if (ev > 0.5f) {
+            const bool pp = y[1] >= SURR_PP_MIN;
 #pragma unroll
             for (int k = 0; k < SURR_K; ++k) {
-                float s = lw[k] - lse_w;
+                const float hq = h[SURR_PP + k];
+                /* log q_k or log (1 - q_k) */
+                float s = lw[k] - lse_w - (pp ? softplus(-hq) : softplus(hq));
 #pragma unroll
                 for (int d = 0; d < SURR_D; ++d) {
+                    if (d == 0 && pp) continue;
                     float mu = h[17 + 4 * k + d];
                     float ls = fminf(fmaxf(h[81 + 4 * k + d], SURR_LOGSIG_MIN), SURR_LOGSIG_MAX);
                     float zz = (y[1 + d] - mu) * gp_expf(-ls);
@@ -214,8 +225,15 @@ __global__ __launch_bounds__(256) void k_mdn_loss_grad(const float *H, int64_t l
                 const float gam = gp_expf(lp[k] - lse);
                 const float pik = gp_expf(lw[k] - lse_w);
                 g[1 + k] = f2bf((pik - gam) * inv_n);
+                const float qk = 1.0f / (1.0f + gp_expf(-h[SURR_PP + k]));
+                g[SURR_PP + k] = f2bf((pp ? -gam * (1.0f - qk) : gam * qk) * inv_n);
 #pragma unroll
                 for (int d = 0; d < SURR_D; ++d) {
+                    if (d == 0 && pp) {
+                        g[17 + 4 * k] = 0;
+                        g[81 + 4 * k] = 0;
+                        continue;
+                    }
                     float mu = h[17 + 4 * k + d];
                     float raw = h[81 + 4 * k + d];
                     float ls = fminf(fmaxf(raw, SURR_LOGSIG_MIN), SURR_LOGSIG_MAX);
@@ -346,21 +364,30 @@ __device__ __forceinline__ bool d_surr_sample_row(const float *h, const DevQueue
         acc += gp_expf(h[1 + c] - mw);
         if (u <= acc) { kc = c; break; }
     }
-    gp_u4 b2 = gp_rng_block(&r);
-    float z[4];
-    float s0, c0, s1, c1;
-    gp_sincos2pi(gp_u01(b2.y), &s0, &c0);
-    gp_sincos2pi(gp_u01(b2.w), &s1, &c1);
-    float r0 = sqrtf(-2.0f * gp_logf(gp_u01(b2.x))), r1 = sqrtf(-2.0f * gp_logf(gp_u01(b2.z)));
-    z[0] = r0 * c0; z[1] = r0 * s0; z[2] = r1 * c1; z[3] = r1 * s1;
+    /* full absorption with the component's photopeak probability */
+    const bool peak = gp_u01(b.z) <= 1.0f / (1.0f + gp_expf(-h[SURR_PP + kc]));
+    /* the component's Gaussian restricted to the region the training data
+     * cover (continuum fraction in (0, SURR_PP_MIN), depth in [0, 1]): up to
+     * 8 draws of four normals, then the last one clamped */
     float y[4];
-    for (int d = 0; d < 4; ++d) {
-        float ls = fminf(fmaxf(h[81 + 4 * kc + d], SURR_LOGSIG_MIN), SURR_LOGSIG_MAX);
-        y[d] = h[17 + 4 * kc + d] + gp_expf(ls) * z[d];
+    for (int tries = 0; tries < 8; ++tries) {
+        gp_u4 b2 = gp_rng_block(&r);
+        float z[4];
+        float s0, c0, s1, c1;
+        gp_sincos2pi(gp_u01(b2.y), &s0, &c0);
+        gp_sincos2pi(gp_u01(b2.w), &s1, &c1);
+        float r0 = sqrtf(-2.0f * gp_logf(gp_u01(b2.x))), r1 = sqrtf(-2.0f * gp_logf(gp_u01(b2.z)));
+        z[0] = r0 * c0; z[1] = r0 * s0; z[2] = r1 * c1; z[3] = r1 * s1;
+        for (int d = 0; d < 4; ++d) {
+            float ls = fminf(fmaxf(h[81 + 4 * kc + d], SURR_LOGSIG_MIN), SURR_LOGSIG_MAX);
+            y[d] = h[17 + 4 * kc + d] + gp_expf(ls) * z[d];
+        }
+        const bool fe_ok = peak || (y[0] > 0.0f && y[0] < SURR_PP_MIN);
+        if (fe_ok && y[1] >= 0.0f && y[1] <= 1.0f) break;
     }
     float lx, ly, lz;
     const float E = q.E[j];
-    const float fe = fminf(y[0], 1.0f);
+    const float fe = peak ? 1.0f : fminf(y[0], SURR_PP_MIN);
     bool has = fe > 0.0f && surr_position(P, q.vx[j], q.vy[j], q.vz[j], q.y[j], q.z[j], y[1], y[2], y[3], lx, ly, lz);
     int M_id = -1, L_id = -1;
     if (has) has = d_crystal(det, P, lx, ly, lz, M_id, L_id) == 0;
@@ -410,18 +437,19 @@ __device__ __forceinline__ void d_append_event(const PanelParams &pp, bool has, 
  * the bf16 rounding happen in registers and every 4 outputs go to LDS in
  * one 8-byte write.  Hidden layers: wave w computes neurons [64w, 64w + 64)
  * of all 128 rows (2 x 4 tiles, 128 accumulators).  Head: wave w computes
- * the first 160 outputs (145 used) of rows [32w, 32w + 32), which replace
- * the activations in LDS as fp32; then one thread per row samples the
- * row's event (SAMPLE) or copies its 145 outputs to H.  Inputs: the
+ * the 192 stored outputs (161 used) of rows [32w, 32w + 32); they replace
+ * the activations in LDS as fp32, half the rows at a time, and one lane per
+ * row samples the row's event (SAMPLE) or copies its outputs to H.  Inputs: the
  * surrogate queue (features packed in the prologue) or, without SAMPLE, a
  * packed bf16 [n x 8] matrix.  The SiLU here is z / (1 + 2^(-z log2 e))
  * with the hardware exp2 and reciprocal (training uses the exact form; the
  * difference is below the bf16 rounding of the activations). */
 constexpr int IB = 128;
 constexpr int ILD = 264;
-constexpr int IHLD = 148;
-constexpr int IHT = 5;            /* head tiles of 32 outputs: 160 >= 145 */
-constexpr int INFER_LDS = IB * IHLD * 4 > IB * ILD * 2 ? IB * IHLD * 4 : IB * ILD * 2;
+constexpr int IHT = 6;            /* head tiles of 32 outputs: 192 >= SURR_NH */
+constexpr int IHLD = 164;         /* head row in LDS (fp32): >= SURR_NH, 16-byte rows */
+/* the head outputs of half the rows at a time replace the activations */
+constexpr int INFER_LDS = (IB / 2) * IHLD * 4 > IB * ILD * 2 ? (IB / 2) * IHLD * 4 : IB * ILD * 2;
 
 struct InferWeights {
     const uint16_t *w[5];      /* layer l: [outputs][kp(l)] bf16              */
@@ -547,31 +575,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bv, acc[t], 0, 0, 0);
             }
         }
-        __syncthreads();
+        /* rows [0, 64) (waves 0, 1), then rows [64, 128) (waves 2, 3): a
+         * half's outputs go to LDS and wave 0 handles its 64 rows, one per
+         * lane */
+#pragma unroll 1
+        for (int half = 0; half < 2; ++half) {
+            __syncthreads();
+            if ((wave >> 1) == half) {
+                const int mh = m - 64 * half;
 #pragma unroll
-        for (int t = 0; t < IHT; ++t)
+                for (int t = 0; t < IHT; ++t)
 #pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                const int nb = 32 * t + g * 8 + (lane >> 5) * 4;
-                if (nb < IHLD)
-                    *reinterpret_cast<float4 *>(head + m * IHLD + nb) =
-                        make_float4(acc[t][4 * g], acc[t][4 * g + 1], acc[t][4 * g + 2], acc[t][4 * g + 3]);
+                    for (int g = 0; g < 4; ++g) {
+                        const int nb = 32 * t + g * 8 + (lane >> 5) * 4;
+                        if (nb < IHLD)
+                            *reinterpret_cast<float4 *>(head + mh * IHLD + nb) =
+                                make_float4(acc[t][4 * g], acc[t][4 * g + 1], acc[t][4 * g + 2], acc[t][4 * g + 3]);
+                    }
             }
-        __syncthreads();
-    }
-
-    /* one thread per row */
-    if (threadIdx.x < IB) {
-        const int m = threadIdx.x;
-        const int64_t i = b0 + m;
-        const float *h = head + m * IHLD;
-        if (SAMPLE) {
-            bool has = false;
-            gpet_event ev;
-            if (i < n) has = d_surr_sample_row(h, q, row0 + i, det, pp, ev);
-            d_append_event(pp, has, ev);
-        } else if (i < n) {
-            for (int c = 0; c < SURR_NH; ++c) H[i * ldh + c] = h[c];
+            __syncthreads();
+            if (wave == 0) {
+                const int64_t i = b0 + 64 * half + lane;
+                const float *h = head + lane * IHLD;
+                if (SAMPLE) {
+                    bool has = false;
+                    gpet_event ev;
+                    if (i < n) has = d_surr_sample_row(h, q, row0 + i, det, pp, ev);
+                    d_append_event(pp, has, ev);
+                } else if (i < n) {
+                    for (int c = 0; c < SURR_NH; ++c) H[i * ldh + c] = h[c];
+                }
+            }
         }
     }
 }

@@ -15,7 +15,10 @@
 #define SURR_REC 12        /* harvested record: features, targets, padding          */
 #define SURR_K 16          /* mixture components                                   */
 #define SURR_D 4           /* mixture dimensions                                   */
-#define SURR_NH 145        /* head outputs: 1 + 16 + 16 * 4 + 16 * 4               */
+#define SURR_NH 161        /* head outputs: 1 + 16 + 16 * 4 + 16 * 4 + 16          */
+#define SURR_PP 145        /* first of the 16 per-component photopeak logits       */
+#define SURR_PP_MIN 0.999f /* a deposited-energy fraction at or above this is a
+                              full absorption (photopeak)                          */
 #define SURR_LOGSIG_MIN (-5.0f)
 #define SURR_LOGSIG_MAX (2.0f)
 #define SURR_E_MIN 100.0e3f /* validity: entry energy (eV) ...                     */

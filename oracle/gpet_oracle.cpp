@@ -1964,17 +1964,22 @@ void gpet_cpu_stream(uint64_t seed, uint64_t id, uint32_t aux, int32_t kind, int
 
 /* ======================= surrogate network (CPU restatement) =============
  * The detector-response MLP of the HIP library restated in FP64 for the
- * gradient check: 6 -> 256 -> 256 -> 256 -> 256 -> 145, SiLU, weights
+ * gradient check: 6 -> 256 -> 256 -> 256 -> 256 -> 161, SiLU, weights
  * [outputs x (inputs + 8)] with the bias in column `inputs`, head stored
  * with 192 rows.  By default it rounds what the GPU path stores in BF16
  * (weights, inputs, activations, dL/dh, dL/dz) to BF16 and accumulates in
  * FP64; exact mode (gpet_cpu_surrogate_set_exact) rounds nothing, for
  * finite-difference checks.  Loss: BCE on the event logit plus, for events,
  * the negative log-likelihood of the 16-component diagonal Gaussian mixture
- * over the 4 targets (log sigma clamped to [-5, 2]), averaged over rows. */
+ * over the 4 targets (log sigma clamped to [-5, 2]), averaged over rows;
+ * component k gives a full absorption (fE >= 0.999, whose fE term is then
+ * left out) with probability sigmoid(h[145 + k]) and a continuum fE from its
+ * Gaussian otherwise. */
 namespace {
 
-constexpr int M_NL = 5, M_HID = 256, M_NH = 145, M_K = 16, M_D = 4, M_NX = 6, M_NY = 5;
+constexpr int M_NL = 5, M_HID = 256, M_NH = 161, M_K = 16, M_D = 4, M_NX = 6, M_NY = 5;
+constexpr int M_PP = 145;           /* per-component photopeak logits h[145 + k] */
+constexpr double M_PP_MIN = 0.999;  /* fE >= this: full absorption                */
 constexpr int M_LN[M_NL] = {256, 256, 256, 256, 192};
 constexpr int M_LK[M_NL] = {6, 256, 256, 256, 256};
 constexpr double M_LSMIN = -5.0, M_LSMAX = 2.0;
@@ -2041,6 +2046,7 @@ double mlp_loss_row(const double *h, const float *y, double *g)
     for (int c = 0; c < M_NH; ++c) g[c] = 0.0;
     g[0] = m_sig(h[0]) - ev;
     if (ev > 0.5) {
+        const bool pp = (double)y[1] >= M_PP_MIN;
         double mw = -1e300;
         for (int k = 0; k < M_K; ++k) mw = std::max(mw, h[1 + k]);
         double sw = 0.0;
@@ -2048,8 +2054,10 @@ double mlp_loss_row(const double *h, const float *y, double *g)
         const double lse_w = mw + std::log(sw);
         double lp[M_K], mx = -1e300;
         for (int k = 0; k < M_K; ++k) {
-            double s = h[1 + k] - lse_w;
+            /* log q_k for a full absorption, log(1 - q_k) otherwise */
+            double s = h[1 + k] - lse_w - (pp ? softplus_d(-h[M_PP + k]) : softplus_d(h[M_PP + k]));
             for (int d = 0; d < M_D; ++d) {
+                if (d == 0 && pp) continue;
                 double ls = std::min(std::max(h[81 + 4 * k + d], M_LSMIN), M_LSMAX);
                 double zz = (y[1 + d] - h[17 + 4 * k + d]) * std::exp(-ls);
                 s += -0.5 * zz * zz - ls - 0.91893853320467274;
@@ -2064,7 +2072,10 @@ double mlp_loss_row(const double *h, const float *y, double *g)
         for (int k = 0; k < M_K; ++k) {
             const double gam = std::exp(lp[k] - lse), pik = std::exp(h[1 + k] - lse_w);
             g[1 + k] = pik - gam;
+            const double qk = m_sig(h[M_PP + k]);
+            g[M_PP + k] = pp ? -gam * (1.0 - qk) : gam * qk;
             for (int d = 0; d < M_D; ++d) {
+                if (d == 0 && pp) continue;
                 const double raw = h[81 + 4 * k + d];
                 const double ls = std::min(std::max(raw, M_LSMIN), M_LSMAX);
                 const double is = std::exp(-ls);

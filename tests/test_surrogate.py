@@ -26,13 +26,13 @@ def layer_slices():
 
 
 def masked_params(w):
-    """Zero the padding (columns after the bias, head rows >= 145)."""
+    """Zero the padding (columns after the bias, head rows >= 161)."""
     w = w.copy()
     for l, (o, n, kp, k) in enumerate(layer_slices()[0]):
         blk = w[o:o + n * kp].reshape(n, kp)
         blk[:, k + 1:] = 0
         if l == 4:
-            blk[145:] = 0
+            blk[161:] = 0
     return w
 
 
@@ -40,7 +40,9 @@ def sample_rows(n, seed=1):
     rng = np.random.default_rng(seed)
     x = np.stack([rng.uniform(0.2, 1.0, n), rng.uniform(0.5, 1.0, n), rng.uniform(-0.6, 0.6, n),
                   rng.uniform(-0.6, 0.6, n), rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)], 1).astype(np.float32)
-    y = np.stack([(rng.uniform(size=n) < 0.7).astype(float), rng.uniform(0, 1, n), rng.uniform(0, 1, n),
+    # deposited fractions: a third are full absorptions (1.0)
+    fe = np.where(rng.uniform(size=n) < 1.0 / 3.0, 1.0, rng.uniform(0, 0.99, n))
+    y = np.stack([(rng.uniform(size=n) < 0.7).astype(float), fe, rng.uniform(0, 1, n),
                   rng.normal(0, 0.3, n), rng.normal(0, 0.3, n)], 1).astype(np.float32)
     return x, y
 
@@ -75,14 +77,19 @@ def test_layout():
 
 def test_zero_network_closed_form(eng):
     """h = 0: BCE = log 2 per row; the mixture is 16 equal unit Gaussians at
-    0, so the event NLL is sum_d (y_d^2 / 2 + log sqrt(2 pi))."""
+    0 with full-absorption probability 1/2 each, so the event NLL is
+    log 2 + sum_d (y_d^2 / 2 + log sqrt(2 pi)) over the four targets, or over
+    the three positional ones for a full absorption (fE >= 0.999)."""
     eng.surrogate_set_params(np.zeros(layer_slices()[1], np.float32))
     x, y = sample_rows(64)
     h = eng.surrogate_forward(x)
     assert np.all(h == 0)
     loss, g = grad_d(eng, x, y)
     ev = y[:, 0] > 0.5
-    nll = (0.5 * y[:, 1:].astype(float) ** 2 + 0.5 * np.log(2 * np.pi)).sum(1)
+    terms = 0.5 * y[:, 1:].astype(float) ** 2 + 0.5 * np.log(2 * np.pi)
+    pp = y[:, 1] >= np.float32(0.999)
+    assert pp[ev].any() and (~pp[ev]).any()
+    nll = np.log(2.0) + terms[:, 1:].sum(1) + np.where(pp, 0.0, terms[:, 0])
     expect = np.log(2.0) + np.where(ev, nll, 0.0).mean()
     assert loss == pytest.approx(expect, rel=1e-12)
     # dL/d(head bias of the event logit) = mean(sigmoid(0) - e)
@@ -100,11 +107,11 @@ def test_gradient_finite_differences(eng):
     rng = np.random.default_rng(5)
     picks = []
     for l, (o, n, kp, k) in enumerate(layer_slices()[0]):
-        rows = 145 if l == 4 else n
+        rows = 161 if l == 4 else n
         for _ in range(8):
             picks.append(o + int(rng.integers(rows)) * kp + int(rng.integers(k + 1)))
     o4, _, kp4, _ = layer_slices()[0][4]
-    for r in (0, 3, 20, 90):           # event logit, a weight logit, a mean, a log sigma
+    for r in (0, 3, 20, 90, 150):      # event logit, a weight logit, a mean, a log sigma, a photopeak logit
         picks.append(o4 + r * kp4 + 256)
     for i in picks:
         eps = np.float32(1e-3)
@@ -120,7 +127,7 @@ def test_gradient_finite_differences(eng):
     # padding gets no gradient
     o, n, kp, k = layer_slices()[0][1]
     assert np.all(g[o:o + n * kp].reshape(n, kp)[:, k + 1:] == 0)
-    assert np.all(g[o4:].reshape(192, kp4)[145:] == 0)
+    assert np.all(g[o4:].reshape(192, kp4)[161:] == 0)
 
 
 def test_bf16_mode_close_to_exact(eng):
