@@ -447,6 +447,8 @@ __device__ __forceinline__ void d_append_event(const PanelParams &pp, bool has, 
 constexpr int IB = 128;
 constexpr int ILD = 264;
 constexpr int IHT = 6;            /* head tiles of 32 outputs: 192 >= SURR_NH */
+constexpr int IPD = 3;            /* hidden layers: weight loads this many steps ahead */
+constexpr int IHPD = 2;           /* head: the same */
 constexpr int IHLD = 164;         /* head row in LDS (fp32): >= SURR_NH, 16-byte rows */
 /* the head outputs of half the rows at a time replace the activations */
 constexpr int INFER_LDS = (IB / 2) * IHLD * 4 > IB * ILD * 2 ? (IB / 2) * IHLD * 4 : IB * ILD * 2;
@@ -508,8 +510,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     /* hidden layers */
 #pragma unroll 1
     for (int l = 0; l < 4; ++l) {
-        const int kp = l == 0 ? SURR_LDX : ILD;
-        const int nsteps = l == 0 ? 1 : (ILD + 15) / 16;
         const uint16_t *w = W.w[l];
         f32x16_t acc[2][4];
 #pragma unroll
@@ -517,24 +517,48 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt) acc[t][mt] = f32x16_t{};
         const int nrow0 = 64 * wave + lr, nrow1 = nrow0 + 32;
-        bf16x8_t a0 = load_w(w + (int64_t)nrow0 * kp + kh, kh < kp);
-        bf16x8_t a1 = load_w(w + (int64_t)nrow1 * kp + kh, kh < kp);
-#pragma unroll 1
-        for (int st = 0; st < nsteps; ++st) {
-            const int k = st * 16 + kh;
-            const int kn = k + 16;
-            /* next step's weights while this step multiplies */
-            const bf16x8_t na0 = load_w(w + (int64_t)nrow0 * kp + kn, kn < kp);
-            const bf16x8_t na1 = load_w(w + (int64_t)nrow1 * kp + kn, kn < kp);
+        if (l == 0) {
+            /* 8 input columns: one step, lanes 32..63 (k 8..15) zero */
+            const bool ok = kh < SURR_LDX;
+            const bf16x8_t a0 = load_w(w + (int64_t)nrow0 * SURR_LDX + kh, ok);
+            const bf16x8_t a1 = load_w(w + (int64_t)nrow1 * SURR_LDX + kh, ok);
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt) {
-                bf16x8_t bv = *reinterpret_cast<const bf16x8_t *>(act + (mt * 32 + lr) * ILD + (k < kp ? k : 0));
-                if (k >= kp) bv = bf16x8_t{};
+                bf16x8_t bv = *reinterpret_cast<const bf16x8_t *>(act + (mt * 32 + lr) * ILD);
+                if (!ok) bv = bf16x8_t{};
                 acc[0][mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bv, acc[0][mt], 0, 0, 0);
                 acc[1][mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bv, acc[1][mt], 0, 0, 0);
             }
-            a0 = na0;
-            a1 = na1;
+        } else {
+            /* 17 steps of 16 columns (the last: the bias column, zeros, and
+             * lanes 32..63 past the row set to zero); the weights of step
+             * st + IPD are loaded while step st multiplies */
+            constexpr int NS = (ILD + 15) / 16;
+            bf16x8_t ab[IPD][2];
+#pragma unroll
+            for (int p = 0; p < IPD; ++p) {
+                const int k = p * 16 + kh;
+                ab[p][0] = load_w(w + (int64_t)nrow0 * ILD + k, k < ILD);
+                ab[p][1] = load_w(w + (int64_t)nrow1 * ILD + k, k < ILD);
+            }
+#pragma unroll
+            for (int st = 0; st < NS; ++st) {
+                const int sl = st % IPD;
+                const bf16x8_t a0 = ab[sl][0], a1 = ab[sl][1];
+                if (st + IPD < NS) {
+                    const int kf = (st + IPD) * 16 + kh;
+                    ab[sl][0] = load_w(w + (int64_t)nrow0 * ILD + kf, kf < ILD);
+                    ab[sl][1] = load_w(w + (int64_t)nrow1 * ILD + kf, kf < ILD);
+                }
+                const int k = st * 16 + kh;
+#pragma unroll
+                for (int mt = 0; mt < 4; ++mt) {
+                    bf16x8_t bv = *reinterpret_cast<const bf16x8_t *>(act + (mt * 32 + lr) * ILD + (k < ILD ? k : 0));
+                    if (k >= ILD) bv = bf16x8_t{};
+                    acc[0][mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bv, acc[0][mt], 0, 0, 0);
+                    acc[1][mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bv, acc[1][mt], 0, 0, 0);
+                }
+            }
         }
         __syncthreads();
         /* accumulator i of lane: neuron (i/4)*8 + (lane/32)*4 + i%4 of the
@@ -556,22 +580,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         __syncthreads();
     }
 
-    /* head: rows [32 wave, 32 wave + 32), outputs [0, 160) */
+    /* head: rows [32 wave, 32 wave + 32), outputs [0, 192) */
     {
         const uint16_t *w = W.w[4];
         f32x16_t acc[IHT];
 #pragma unroll
         for (int t = 0; t < IHT; ++t) acc[t] = f32x16_t{};
         const int m = 32 * wave + lr;
-#pragma unroll 1
-        for (int st = 0; st < (ILD + 15) / 16; ++st) {
+        constexpr int NS = (ILD + 15) / 16;
+        bf16x8_t ab[IHPD][IHT];
+#pragma unroll
+        for (int p = 0; p < IHPD; ++p)
+#pragma unroll
+            for (int t = 0; t < IHT; ++t) {
+                const int k = p * 16 + kh;
+                ab[p][t] = load_w(w + (int64_t)(32 * t + lr) * ILD + k, k < ILD);
+            }
+#pragma unroll
+        for (int st = 0; st < NS; ++st) {
+            const int sl = st % IHPD;
             const int k = st * 16 + kh;
             const bool ok = k < ILD;
             bf16x8_t bv = *reinterpret_cast<const bf16x8_t *>(act + m * ILD + (ok ? k : 0));
             if (!ok) bv = bf16x8_t{};
 #pragma unroll
             for (int t = 0; t < IHT; ++t) {
-                const bf16x8_t a = load_w(w + (int64_t)(32 * t + lr) * ILD + k, ok);
+                const bf16x8_t a = ab[sl][t];
+                if (st + IHPD < NS) {
+                    const int kf = (st + IHPD) * 16 + kh;
+                    ab[sl][t] = load_w(w + (int64_t)(32 * t + lr) * ILD + kf, kf < ILD);
+                }
                 acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bv, acc[t], 0, 0, 0);
             }
         }
