@@ -365,6 +365,11 @@ def run_workload(name, args, tabs, tune=True, want_engine=False):
         n_full = n_dec
         step_arg = n_dec
     run = setup(eng, wl, tabs, batch_data)
+    if args.shape and name == args.workload:
+        wp, wq, bp, bq = (int(v) for v in args.shape.split(","))
+        cur = eng.get_launch()
+        eng.set_launch(block=tuple(cur["block"]), wpe=(wp, wq), blocks_per_cu=(bp, bq), batch=cur["batch"],
+                       transit=tuple(cur["transit"]))
     M = timed(run, step_arg, args.warmup, args.steps)
     M["steps"] = args.steps
     launch = eng.get_launch()
@@ -389,6 +394,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="length of the CPU baseline sample (all cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the library's default launch shape")
+    ap.add_argument("--shape", default="", help="launch shape of the headline workload instead of the tuner's: "
+                    "wpe_phantom,wpe_panel,blocks_per_cu_phantom,blocks_per_cu_panel (implies --no-tune)")
     ap.add_argument("--no-graph", action="store_true", help="launch batches one by one (profiling counters)")
     ap.add_argument("--no-configs", action="store_true", help="measure the headline workload only")
     ap.add_argument("--no-surrogate", action="store_true")
@@ -410,7 +417,8 @@ def main():
     name = args.workload
     wl = WORKLOADS[name]
     log("headline workload %s" % name)
-    S, M, eng, run, step_arg, n_full, batch_data = run_workload(name, args, tabs, tune=not args.no_tune,
+    S, M, eng, run, step_arg, n_full, batch_data = run_workload(name, args, tabs,
+                                                                tune=not (args.no_tune or args.shape),
                                                                 want_engine=True)
     out = {
         "metric": METRIC,

@@ -39,6 +39,21 @@ __device__ __forceinline__ float d_lam(const DevWck &w, float E)
     return d_lerp(w.lam, GPET_NWCK, w.e0, w.idl, E);
 }
 
+/* d_lam with the first n_lds nodes of the majorant copied to LDS (s_lam):
+ * the same index and arithmetic, the nodes read from LDS when both lie
+ * there. */
+__device__ __forceinline__ float d_lam_lds(const float *s_lam, int n_lds, const DevWck &w, float E)
+{
+    float x = (E - w.e0) * w.idl;
+    x = fminf(fmaxf(x, 0.0f), (float)(GPET_NWCK - 1));
+    int j = (int)x;
+    if (j > GPET_NWCK - 2) j = GPET_NWCK - 2;
+    float f = x - (float)j;
+    const bool in = j + 1 < n_lds;
+    float a = in ? s_lam[j] : w.lam[j], b = in ? s_lam[j + 1] : w.lam[j + 1];
+    return a + f * (b - a);
+}
+
 /* mu/rho of (total, Compton, Rayleigh) for one material at E: two 16-byte
  * nodes, three lerps (same arithmetic as three separate table reads). */
 __device__ __forceinline__ float3 d_mu3(const DevTables &t, int mat, float E)

@@ -337,6 +337,15 @@ __global__ void k_pack_train(const float *set, int64_t n, uint64_t seed, int64_t
     for (int c = 0; c < SURR_NY; ++c) Y[i * SURR_NY + c] = r[SURR_NX + c];
 }
 
+/* Elementary functions of the sampling: the hardware approximations
+ * (v_exp_f32 = 2^x, v_log_f32 = log2, v_sin/v_cos_f32 of x turns,
+ * v_sqrt_f32, v_rcp_f32; about 1 ulp).  The sampling is a statistical
+ * model checked against plain MC, not restated by the oracle, so it needs
+ * no correctly rounded library routine. */
+__device__ __forceinline__ float s_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504f); }
+__device__ __forceinline__ float s_log(float x) { return __builtin_amdgcn_logf(x) * 0.693147181f; }
+__device__ __forceinline__ float s_sigmoid(float x) { return __builtin_amdgcn_rcpf(1.0f + s_exp(-x)); }
+
 /* Sampling from the head outputs h of surrogate-queue row j: P(event) on
  * the photon's own stream, a component by its weight, four normals; the
  * event at the first deposit point, with the readout site of the digitizer
@@ -352,20 +361,37 @@ __device__ __forceinline__ bool d_surr_sample_row(const float *h, const DevQueue
     gp_rng r;
     gp_rng_init(&r, pp.seed, (uint64_t)(int64_t)info.z, GP_TAG_PHOTON + k, draws);
     gp_u4 b = gp_rng_block(&r);
-    const float pe = 1.0f / (1.0f + gp_expf(-h[0]));
+    const float pe = s_sigmoid(h[0]);
     if (!(gp_u01(b.x) <= pe)) return false;
+    float lw[SURR_K];
     float mw = -1e30f;
-    for (int c = 0; c < SURR_K; ++c) mw = fmaxf(mw, h[1 + c]);
-    float sw = 0.0f;
-    for (int c = 0; c < SURR_K; ++c) sw += gp_expf(h[1 + c] - mw);
-    float u = gp_u01(b.y) * sw, acc = 0.0f;
-    int kc = SURR_K - 1;
+#pragma unroll
     for (int c = 0; c < SURR_K; ++c) {
-        acc += gp_expf(h[1 + c] - mw);
-        if (u <= acc) { kc = c; break; }
+        lw[c] = h[1 + c];
+        mw = fmaxf(mw, lw[c]);
+    }
+    float sw = 0.0f;
+#pragma unroll
+    for (int c = 0; c < SURR_K; ++c) {
+        lw[c] = s_exp(lw[c] - mw);
+        sw += lw[c];
+    }
+    /* the first component whose running weight reaches u (the last if
+     * rounding leaves u above the total) */
+    const float u = gp_u01(b.y) * sw;
+    float acc = 0.0f;
+    int kc = SURR_K - 1;
+    bool found = false;
+#pragma unroll
+    for (int c = 0; c < SURR_K; ++c) {
+        acc += lw[c];
+        if (!found && u <= acc) {
+            kc = c;
+            found = true;
+        }
     }
     /* full absorption with the component's photopeak probability */
-    const bool peak = gp_u01(b.z) <= 1.0f / (1.0f + gp_expf(-h[SURR_PP + kc]));
+    const bool peak = gp_u01(b.z) <= s_sigmoid(h[SURR_PP + kc]);
     /* the component's Gaussian restricted to the region the training data
      * cover (continuum fraction in (0, SURR_PP_MIN), depth in [0, 1]): up to
      * 8 draws of four normals, then the last one clamped */
@@ -373,14 +399,16 @@ __device__ __forceinline__ bool d_surr_sample_row(const float *h, const DevQueue
     for (int tries = 0; tries < 8; ++tries) {
         gp_u4 b2 = gp_rng_block(&r);
         float z[4];
-        float s0, c0, s1, c1;
-        gp_sincos2pi(gp_u01(b2.y), &s0, &c0);
-        gp_sincos2pi(gp_u01(b2.w), &s1, &c1);
-        float r0 = sqrtf(-2.0f * gp_logf(gp_u01(b2.x))), r1 = sqrtf(-2.0f * gp_logf(gp_u01(b2.z)));
+        const float a0 = gp_u01(b2.y), a1 = gp_u01(b2.w);
+        const float s0 = __builtin_amdgcn_sinf(a0), c0 = __builtin_amdgcn_cosf(a0);
+        const float s1 = __builtin_amdgcn_sinf(a1), c1 = __builtin_amdgcn_cosf(a1);
+        const float r0 = __builtin_amdgcn_sqrtf(-2.0f * s_log(gp_u01(b2.x)));
+        const float r1 = __builtin_amdgcn_sqrtf(-2.0f * s_log(gp_u01(b2.z)));
         z[0] = r0 * c0; z[1] = r0 * s0; z[2] = r1 * c1; z[3] = r1 * s1;
+#pragma unroll
         for (int d = 0; d < 4; ++d) {
             float ls = fminf(fmaxf(h[81 + 4 * kc + d], SURR_LOGSIG_MIN), SURR_LOGSIG_MAX);
-            y[d] = h[17 + 4 * kc + d] + gp_expf(ls) * z[d];
+            y[d] = h[17 + 4 * kc + d] + s_exp(ls) * z[d];
         }
         const bool fe_ok = peak || (y[0] > 0.0f && y[0] < SURR_PP_MIN);
         if (fe_ok && y[1] >= 0.0f && y[1] <= 1.0f) break;
@@ -460,6 +488,16 @@ struct InferWeights {
 __device__ __forceinline__ float silu_fast(float z)
 {
     return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.44269504f * z));
+}
+
+/* two floats to packed bf16 (round to nearest even, a in the low half):
+ * one v_cvt_pk_bf16_f32 */
+__device__ __forceinline__ uint32_t f2bf2(float a, float b)
+{
+    typedef float f32x2_t __attribute__((ext_vector_type(2)));
+    typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+    const f32x2_t v = {a, b};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2_t));
 }
 
 __device__ __forceinline__ bf16x8_t load_w(const uint16_t *p, bool ok)
@@ -571,10 +609,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 for (int g = 0; g < 4; ++g) {
                     const int nb = 64 * wave + 32 * t + g * 8 + (lane >> 5) * 4;
                     const int m = mt * 32 + lr;
-                    const uint32_t lo = (uint32_t)f2bf(silu_fast(acc[t][mt][4 * g])) |
-                                        ((uint32_t)f2bf(silu_fast(acc[t][mt][4 * g + 1])) << 16);
-                    const uint32_t hi = (uint32_t)f2bf(silu_fast(acc[t][mt][4 * g + 2])) |
-                                        ((uint32_t)f2bf(silu_fast(acc[t][mt][4 * g + 3])) << 16);
+                    const uint32_t lo = f2bf2(silu_fast(acc[t][mt][4 * g]), silu_fast(acc[t][mt][4 * g + 1]));
+                    const uint32_t hi = f2bf2(silu_fast(acc[t][mt][4 * g + 2]), silu_fast(acc[t][mt][4 * g + 3]));
                     *reinterpret_cast<uint2 *>(act + m * ILD + nb) = make_uint2(lo, hi);
                 }
         __syncthreads();

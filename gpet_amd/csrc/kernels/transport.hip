@@ -970,6 +970,31 @@ __device__ __forceinline__ void d_photons(const RunParams &rp, const DevTables &
     }
 }
 
+/* The phantom's Woodcock majorant (GPET_NWCK floats, 16 KB) staged in LDS
+ * per block for single-phantom launches when GPET_PHANTOM_LDS_WCK is set:
+ * the kernels then read it through phl, a copy of the phantom descriptor
+ * whose majorant pointer is the LDS copy (same nodes, same arithmetic).
+ * Off by default: on MI355X the phantom stage ran 1-3% slower with it (body
+ * phantom 3.11-3.20 -> 3.21 ms per 2^23-history launch, cylinder 4.27 ->
+ * 4.30-4.33 ms per 2^25), since the table is L1/L2-resident anyway and the
+ * stage is bound by instruction issue, not by these lookups.  The material
+ * tables (16 B per node and material) do not fit: the nodes up to 530 keV
+ * of the body phantom's four materials take 139 KB, one block per CU. */
+#ifndef GPET_PHANTOM_LDS_WCK
+#define GPET_PHANTOM_LDS_WCK 0
+#endif
+#if GPET_PHANTOM_LDS_WCK
+#define PHANTOM_LDS_WCK(BATCH, ph, phl)                                                            \
+    __shared__ float s_wck[GPET_NWCK];                                                             \
+    DevPhantom phl = ph;                                                                           \
+    if (!(BATCH) && ph.wck.lam) {                                                                  \
+        for (int t_ = threadIdx.x; t_ < GPET_NWCK; t_ += blockDim.x) s_wck[t_] = ph.wck.lam[t_];   \
+        phl.wck.lam = s_wck;                                                                       \
+    }
+#else
+#define PHANTOM_LDS_WCK(BATCH, ph, phl) const DevPhantom &phl = ph;
+#endif
+
 /* The phantom kernel, grid-stride over histories.
  *
  * BATCH: the launch covers a list of segments, each a history range of one
@@ -994,13 +1019,14 @@ void k_phantom(RunParams rp, DevTables tab, DevPhantom ph, DevDetector det, DevQ
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     const float4 *s_pf = reinterpret_cast<const float4 *>(smem);
     const DevPanel *s_p = d_panel_lds(smem, det);
+    PHANTOM_LDS_WCK(BATCH, ph, phl)
     __syncthreads();
     if (!BATCH) {
         const int64_t stride = (int64_t)gridDim.x * blockDim.x;
         for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < rp.nh; i += stride) {
             const int64_t h = rp.h0 + i;
             const int si = (MODE == RUN_DECAY) ? d_source_of(rp, h) : 0;
-            d_history<MODE, TQ>(rp, tab, ph, det, s_pf, s_p, q, &s_qn, &s_ln, &s_tn, i, h, si, 0, C);
+            d_history<MODE, TQ>(rp, tab, phl, det, s_pf, s_p, q, &s_qn, &s_ln, &s_tn, i, h, si, 0, C);
         }
     } else {
         for (int64_t vb = blockIdx.x; vb < rp.n_vblocks; vb += gridDim.x) {
@@ -1052,12 +1078,13 @@ __global__ __launch_bounds__(256) void k_longwalk(RunParams rp, DevTables tab, D
         s_tn = TQ ? q.tq_count[blockIdx.x] : 0u;
     }
     const DevPanel *s_p = d_panel_lds(smem, det);
+    PHANTOM_LDS_WCK(BATCH, ph, phl)
     __syncthreads();
     const unsigned lcap = (unsigned)(q.region >> 3);
     const unsigned n = min(q.lw_count[blockIdx.x], lcap);
     for (unsigned j = threadIdx.x; j < n; j += blockDim.x) {
         const LongWalkItem L = q.lw[(int64_t)blockIdx.x * lcap + j];
-        const DevPhantom P = BATCH ? rp.phs[L.slot] : ph;
+        const DevPhantom P = BATCH ? rp.phs[L.slot] : phl;
         RangeWalk w;
         w.ux = L.ux; w.uy = L.uy; w.uz = L.uz; w.rr = L.rr;
         w.tmx = L.tmx; w.tmy = L.tmy; w.tmz = L.tmz;
@@ -1123,11 +1150,12 @@ __global__ __launch_bounds__(256) void k_transit(RunParams rp, DevTables tab, De
         s_tn = 0u;
     }
     const DevPanel *s_p = d_panel_lds(smem, det);
+    PHANTOM_LDS_WCK(BATCH, ph, phl)
     __syncthreads();
     const unsigned n = min(in_count[blockIdx.x], (unsigned)q.region);
     for (unsigned j = threadIdx.x; j < n; j += blockDim.x) {
         const TransitItem T = in[(int64_t)blockIdx.x * q.region + j];
-        const DevPhantom P = BATCH ? rp.phs[T.slot] : ph;
+        const DevPhantom P = BATCH ? rp.phs[T.slot] : phl;
         const int k = (int)(T.n >> 31);
         gp_rng r;
         gp_rng_init(&r, rp.seed, (uint64_t)T.h, (uint32_t)(GP_TAG_PHOTON + k), T.n & 0x7fffffffu);
@@ -1164,19 +1192,23 @@ __global__ __launch_bounds__(256) void k_transit(RunParams rp, DevTables tab, De
 
 /* ---- panel -------------------------------------------------------------- */
 
+/* A photon's adder list in registers.  cs is the crystal site,
+ * (panel * moduleN + module) * crystalN + crystal: the adder's merge key,
+ * from which the module, the crystal and the readout site follow, so the
+ * list carries one word for the three. */
 struct EvList {
-    int site[GPET_MAX_PH_EVENTS], mod[GPET_MAX_PH_EVENTS], cry[GPET_MAX_PH_EVENTS];
+    int cs[GPET_MAX_PH_EVENTS];
     double t[GPET_MAX_PH_EVENTS];
     float E[GPET_MAX_PH_EVENTS], x[GPET_MAX_PH_EVENTS], y[GPET_MAX_PH_EVENTS], z[GPET_MAX_PH_EVENTS];
 };
 
-__device__ __forceinline__ void d_adder(EvList &L, int &n, int site, int mod, int cry, double t, float E, float x,
-                                        float y, float z, unsigned long long &dropped)
+__device__ __forceinline__ void d_adder(EvList &L, int &n, int site, double t, float E, float x, float y, float z,
+                                        unsigned long long &dropped)
 {
     bool merged = false;
 #pragma unroll
     for (int i = 0; i < GPET_MAX_PH_EVENTS; ++i) {
-        if (!merged && i < n && L.site[i] == site) {
+        if (!merged && i < n && L.cs[i] == site) {
             float sE = L.E[i] + E;
             L.x[i] = (L.x[i] * L.E[i] + x * E) / sE;
             L.y[i] = (L.y[i] * L.E[i] + y * E) / sE;
@@ -1193,7 +1225,7 @@ __device__ __forceinline__ void d_adder(EvList &L, int &n, int site, int mod, in
 #pragma unroll
     for (int i = 0; i < GPET_MAX_PH_EVENTS; ++i) {
         if (i == n) {
-            L.site[i] = site; L.mod[i] = mod; L.cry[i] = cry; L.t[i] = t;
+            L.cs[i] = site; L.t[i] = t;
             L.E[i] = E; L.x[i] = x; L.y[i] = y; L.z[i] = z;
         }
     }
@@ -1202,22 +1234,26 @@ __device__ __forceinline__ void d_adder(EvList &L, int &n, int site, int mod, in
 
 /* readout (gPET_kernals.cu:756-813) in place on the register list: merged
  * entries are flagged dead; survivors keep their order. */
-__device__ __forceinline__ void d_readout(EvList &L, int n, int depth, int policy, int pann, int moduleN,
-                                          bool (&alive)[GPET_MAX_PH_EVENTS])
+__device__ __forceinline__ void d_readout(EvList &L, int n, int depth, int policy, int pann, int crystalN,
+                                          bool (&alive)[GPET_MAX_PH_EVENTS], int (&site)[GPET_MAX_PH_EVENTS])
 {
 #pragma unroll
-    for (int i = 0; i < GPET_MAX_PH_EVENTS; ++i) alive[i] = i < n;
+    for (int i = 0; i < GPET_MAX_PH_EVENTS; ++i) {
+        alive[i] = i < n;
+        site[i] = L.cs[i];
+    }
     if (depth == 3) return;
     if (policy == 1) depth = 2;
+    /* panel * moduleN + module = cs / crystalN */
 #pragma unroll
     for (int i = 0; i < GPET_MAX_PH_EVENTS; ++i)
-        L.site[i] = (depth == 0) ? 0 : (depth == 1 ? pann : pann * moduleN + L.mod[i]);
+        site[i] = (depth == 0) ? 0 : (depth == 1 ? pann : L.cs[i] / crystalN);
 #pragma unroll
     for (int i = 0; i < GPET_MAX_PH_EVENTS; ++i) {
         if (alive[i]) {
 #pragma unroll
             for (int j = i + 1; j < GPET_MAX_PH_EVENTS; ++j) {
-                if (alive[j] && L.site[j] == L.site[i]) {
+                if (alive[j] && site[j] == site[i]) {
                     if (policy == 1) {
                         float sE = L.E[i] + L.E[j];
                         L.x[i] = (L.x[i] * L.E[i] + L.x[j] * L.E[j]) / sE;
@@ -1225,7 +1261,7 @@ __device__ __forceinline__ void d_readout(EvList &L, int n, int depth, int polic
                         L.z[i] = (L.z[i] * L.E[i] + L.z[j] * L.E[j]) / sE;
                         L.E[i] = sE;
                     } else if (!(L.E[i] > L.E[j])) {
-                        L.mod[i] = L.mod[j]; L.cry[i] = L.cry[j]; L.t[i] = L.t[j];
+                        L.cs[i] = L.cs[j]; L.t[i] = L.t[j];
                         L.E[i] = L.E[j]; L.x[i] = L.x[j]; L.y[i] = L.y[j]; L.z[i] = L.z[j];
                     }
                     alive[j] = false;
@@ -1237,13 +1273,28 @@ __device__ __forceinline__ void d_readout(EvList &L, int n, int depth, int polic
 
 /* events staged in LDS per panel-kernel block (48 B each) */
 #define EV_STAGE 512u
+/* detector majorant in LDS for the panel kernel (nodes; 8.7 KB).  Off by
+ * default: the body workload's panel kernel ran 5% slower with it (1.65 ->
+ * 1.74 ms per launch; the cylinder's unchanged), the extra LDS leaving
+ * fewer blocks per CU for no saving on L1-resident loads. */
+#ifndef GPET_PANEL_LDS_WCK
+#define GPET_PANEL_LDS_WCK 0
+#endif
+#define WCK_LDS_NODES 2176
 
 /* MODE: PANEL_PLAIN transports every queued photon; PANEL_SURROGATE lists
  * the photons inside the network's validity range for the surrogate and
  * transports the rest; PANEL_HARVEST transports every photon and also writes
  * a training record (features, first-deposit targets) per in-range photon. */
+/* register budget of the default panel-kernel variant (waves per SIMD the
+ * compiler must allow).  The kernel needs 134 VGPRs (3 waves per SIMD); a
+ * budget of 4 waves (127 VGPRs, 24 spilled to scratch) ran no faster on
+ * either workload, so the compiler's own allocation is kept. */
+#ifndef GPET_PANEL_WPE_DEFAULT
+#define GPET_PANEL_WPE_DEFAULT 1
+#endif
 template <int WPE, int MODE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : GPET_PANEL_WPE_DEFAULT)))
 void k_panel(PanelParams pp, DevTables tab, DevDetector det, DevQueue q)
 {
     unsigned long long c_hits = 0, c_drop = 0, c_droph = 0, c_maj = 0, c_steps = 0;
@@ -1254,6 +1305,12 @@ void k_panel(PanelParams pp, DevTables tab, DevDetector det, DevQueue q)
     __shared__ gpet_event s_ev[EV_STAGE];
     __shared__ unsigned s_evn, s_evbase;
     if (threadIdx.x == 0) s_evn = 0u;
+#if GPET_PANEL_LDS_WCK
+    /* the detector majorant's nodes up to ~530 keV (every photon that
+     * enters a panel is below) in LDS */
+    __shared__ float s_lam[WCK_LDS_NODES];
+    for (int t = threadIdx.x; t < WCK_LDS_NODES; t += blockDim.x) s_lam[t] = det.wck.lam[t];
+#endif
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     DevPanel *s_p = reinterpret_cast<DevPanel *>(smem);
     unsigned *s_pre = reinterpret_cast<unsigned *>(smem + sizeof(DevPanel) * (size_t)det.npanels);
@@ -1330,7 +1387,11 @@ void k_panel(PanelParams pp, DevTables tab, DevDetector det, DevQueue q)
             /* block per step: length, selector, Rayleigh CDF, azimuth; the KN
              * rejection loop draws after the block */
             gp_u4 b = gp_rng_block(&r);
+#if GPET_PANEL_LDS_WCK
+            float lammin = d_lam_lds(s_lam, WCK_LDS_NODES, det.wck, E);
+#else
             float lammin = d_lam(det.wck, E);
+#endif
             float s = -lammin * gp_logf(gp_u01(b.x));
             x2 = x2 + s * vx2;
             y2 = y2 + s * vy2;
@@ -1408,7 +1469,7 @@ void k_panel(PanelParams pp, DevTables tab, DevDetector det, DevQueue q)
                             fx = x2; fy = y2; fz = z2;
                             got = true;
                         }
-                        d_adder(L, nev, site, M_id, L_id, tof, ed, x2, y2, z2, c_drop);
+                        d_adder(L, nev, site, tof, ed, x2, y2, z2, c_drop);
                     }
                 }
             }
@@ -1416,7 +1477,8 @@ void k_panel(PanelParams pp, DevTables tab, DevDetector det, DevQueue q)
             if (rot) d_rotate(vx2, vy2, vz2, cth, uphi);
         }
         bool alive[GPET_MAX_PH_EVENTS];
-        d_readout(L, nev, pp.readout_depth, pp.readout_policy, pann, det.moduleN, alive);
+        int rsite[GPET_MAX_PH_EVENTS];
+        d_readout(L, nev, pp.readout_depth, pp.readout_policy, pann, det.crystalN, alive, rsite);
         int nout = 0;
 #pragma unroll
         for (int j = 0; j < GPET_MAX_PH_EVENTS; ++j) nout += alive[j] ? 1 : 0;
@@ -1449,8 +1511,9 @@ void k_panel(PanelParams pp, DevTables tab, DevDetector det, DevQueue q)
         for (int j = 0; j < GPET_MAX_PH_EVENTS; ++j) {
             if (alive[j]) {
                 gpet_event ev;
-                ev.parn = parn; ev.pann = pann; ev.modn = L.mod[j]; ev.cryn = L.cry[j];
-                ev.siten = L.site[j]; ev.eventid = eid; ev.t = L.t[j];
+                const int pm = L.cs[j] / det.crystalN;   /* panel * moduleN + module */
+                ev.parn = parn; ev.pann = pann; ev.modn = pm - pann * det.moduleN; ev.cryn = L.cs[j] - pm * det.crystalN;
+                ev.siten = rsite[j]; ev.eventid = eid; ev.t = L.t[j];
                 ev.E = L.E[j]; ev.x = L.x[j]; ev.y = L.y[j]; ev.z = L.z[j];
                 const unsigned e = (unsigned)(off + o);
                 if (e < nl) {

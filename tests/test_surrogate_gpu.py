@@ -155,14 +155,15 @@ def acceptance(cs_p, eh_p, cs_s, eh_s):
 def test_surrogate_acceptance_bar(trained, tables):
     """The stated bar for surrogate mode: per-crystal singles and the energy
     spectrum within 2% of plain MC with KS p > 0.01, checked on independent
-    seeds (plain MC with one seed, surrogate mode with another; 1e8 decays
-    each, ~5.4e6 singles).  Reported as an expected failure, with the
+    seeds (plain MC with one seed, surrogate mode with another; 1e9 decays
+    each, ~5.4e7 singles, ~1.7e4 per crystal, where the counting noise of a
+    crystal's ratio is ~1.1%).  Reported as an expected failure, with the
     measured figures, while the network does not meet it."""
     from gpet_amd import engine
     from gpet_amd import workloads as W
 
     e, _, (src, iso, t0, t1) = trained
-    n = int(1e8)
+    n = int(1e9)
     path = os.path.join(tempfile.mkdtemp(), "surrogate.bin")
     e.surrogate_save(path)
     e.surrogate_enable(False)
