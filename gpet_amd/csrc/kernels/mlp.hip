@@ -37,6 +37,7 @@ namespace {
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8_t;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16_t;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4_t;
 
 constexpr int BM = 64, BN = 64, BK = 32, LDK = BK + 8; /* +8: spread LDS banks */
 
@@ -465,16 +466,18 @@ __device__ __forceinline__ void d_append_event(const PanelParams &pp, bool has, 
  * the bf16 rounding happen in registers and every 4 outputs go to LDS in
  * one 8-byte write.  Hidden layers: wave w computes neurons [64w, 64w + 64)
  * of all 128 rows (2 x 4 tiles, 128 accumulators).  Head: wave w computes
- * the 192 stored outputs (161 used) of rows [32w, 32w + 32); they replace
- * the activations in LDS as fp32, half the rows at a time, and one lane per
- * row samples the row's event (SAMPLE) or copies its outputs to H.  Inputs: the
+ * outputs [48w, 48w + 48) of the 192 stored (161 used) for all 128 rows on
+ * v_mfma_f32_16x16x32_bf16, so each head weight is loaded once per tile;
+ * the outputs replace the activations in LDS as fp32, half the rows at a
+ * time, and one lane per row samples the row's event (SAMPLE) or copies
+ * its outputs to H.  Inputs: the
  * surrogate queue (features packed in the prologue) or, without SAMPLE, a
  * packed bf16 [n x 8] matrix.  The SiLU here is z / (1 + 2^(-z log2 e))
  * with the hardware exp2 and reciprocal (training uses the exact form; the
  * difference is below the bf16 rounding of the activations). */
 constexpr int IB = 128;
 constexpr int ILD = 264;
-constexpr int IHT = 6;            /* head tiles of 32 outputs: 192 >= SURR_NH */
+constexpr int IHO = 3;            /* head: 16-output tiles per wave (4 x 48 = 192 >= SURR_NH) */
 constexpr int IPD = 3;            /* hidden layers: weight loads this many steps ahead */
 constexpr int IHPD = 2;           /* head: the same */
 constexpr int IHLD = 164;         /* head row in LDS (fp32): >= SURR_NH, 16-byte rows */
@@ -616,56 +619,69 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         __syncthreads();
     }
 
-    /* head: rows [32 wave, 32 wave + 32), outputs [0, 192) */
+    /* head: outputs [48 wave, 48 wave + 48) of all 128 rows on
+     * v_mfma_f32_16x16x32_bf16 (3 output tiles x 8 row tiles of 16), so
+     * each head weight is loaded by one wave; lanes hold 8 consecutive k of
+     * output (or row) lane % 16, k block lane / 16 */
     {
         const uint16_t *w = W.w[4];
-        f32x16_t acc[IHT];
+        const int l16 = lane & 15, kq = (lane >> 4) * 8;
+        f32x4_t acc[IHO][8];
 #pragma unroll
-        for (int t = 0; t < IHT; ++t) acc[t] = f32x16_t{};
-        const int m = 32 * wave + lr;
-        constexpr int NS = (ILD + 15) / 16;
-        bf16x8_t ab[IHPD][IHT];
+        for (int t = 0; t < IHO; ++t)
+#pragma unroll
+            for (int rt = 0; rt < 8; ++rt) acc[t][rt] = f32x4_t{};
+        constexpr int NS = (ILD + 31) / 32;
+        const uint16_t *wr = w + (int64_t)(48 * wave + l16) * ILD;
+        bf16x8_t ab[IHPD][IHO];
 #pragma unroll
         for (int p = 0; p < IHPD; ++p)
 #pragma unroll
-            for (int t = 0; t < IHT; ++t) {
-                const int k = p * 16 + kh;
-                ab[p][t] = load_w(w + (int64_t)(32 * t + lr) * ILD + k, k < ILD);
+            for (int t = 0; t < IHO; ++t) {
+                const int k = p * 32 + kq;
+                ab[p][t] = load_w(wr + 16 * t * ILD + k, k < ILD);
             }
 #pragma unroll
         for (int st = 0; st < NS; ++st) {
             const int sl = st % IHPD;
-            const int k = st * 16 + kh;
+            const int k = st * 32 + kq;
             const bool ok = k < ILD;
-            bf16x8_t bv = *reinterpret_cast<const bf16x8_t *>(act + m * ILD + (ok ? k : 0));
-            if (!ok) bv = bf16x8_t{};
+            bf16x8_t a[IHO];
 #pragma unroll
-            for (int t = 0; t < IHT; ++t) {
-                const bf16x8_t a = ab[sl][t];
+            for (int t = 0; t < IHO; ++t) {
+                a[t] = ab[sl][t];
                 if (st + IHPD < NS) {
-                    const int kf = (st + IHPD) * 16 + kh;
-                    ab[sl][t] = load_w(w + (int64_t)(32 * t + lr) * ILD + kf, kf < ILD);
+                    const int kf = (st + IHPD) * 32 + kq;
+                    ab[sl][t] = load_w(wr + 16 * t * ILD + kf, kf < ILD);
                 }
-                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bv, acc[t], 0, 0, 0);
+            }
+#pragma unroll
+            for (int rt = 0; rt < 8; ++rt) {
+                bf16x8_t bv = *reinterpret_cast<const bf16x8_t *>(act + (rt * 16 + l16) * ILD + (ok ? k : 0));
+                if (!ok) bv = bf16x8_t{};
+#pragma unroll
+                for (int t = 0; t < IHO; ++t)
+                    acc[t][rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], bv, acc[t][rt], 0, 0, 0);
             }
         }
-        /* rows [0, 64) (waves 0, 1), then rows [64, 128) (waves 2, 3): a
-         * half's outputs go to LDS and wave 0 handles its 64 rows, one per
-         * lane */
+        /* rows [0, 64) (row tiles 0-3), then rows [64, 128) (4-7): every
+         * wave writes its outputs of the half's rows to LDS (accumulator i:
+         * output 4 (lane / 16) + i of the tile, row lane % 16) and wave 0
+         * handles the 64 rows, one per lane */
 #pragma unroll 1
         for (int half = 0; half < 2; ++half) {
             __syncthreads();
-            if ((wave >> 1) == half) {
-                const int mh = m - 64 * half;
 #pragma unroll
-                for (int t = 0; t < IHT; ++t)
+            for (int t = 0; t < IHO; ++t) {
+                const int nb = 48 * wave + 16 * t + (lane >> 4) * 4;
+                if (nb < IHLD) {
 #pragma unroll
-                    for (int g = 0; g < 4; ++g) {
-                        const int nb = 32 * t + g * 8 + (lane >> 5) * 4;
-                        if (nb < IHLD)
-                            *reinterpret_cast<float4 *>(head + mh * IHLD + nb) =
-                                make_float4(acc[t][4 * g], acc[t][4 * g + 1], acc[t][4 * g + 2], acc[t][4 * g + 3]);
+                    for (int r4 = 0; r4 < 4; ++r4) {
+                        const f32x4_t v = half ? acc[t][4 + r4] : acc[t][r4];
+                        *reinterpret_cast<float4 *>(head + (r4 * 16 + l16) * IHLD + nb) =
+                            make_float4(v[0], v[1], v[2], v[3]);
                     }
+                }
             }
             __syncthreads();
             if (wave == 0) {
