@@ -66,6 +66,11 @@ struct SurrState {
     int64_t work_rows = 0;
     DBuf<uint16_t> X, A[4], dH, dZ[2];
     DBuf<float> Z[4], H, Y, part, lpart;
+    /* training: the inputs, activations and gradients transposed ([features][ldt],
+     * ldt = work rows rounded up to 8) and the weights transposed per layer */
+    DBuf<uint16_t> XT, AT[4], dHT, dZT[2], wbT;
+    DBuf<float> HT;
+    int64_t ldt = 0;
     DBuf<int64_t> idx;
 };
 

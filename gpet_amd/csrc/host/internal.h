@@ -279,21 +279,39 @@ struct GemmArgs {
     int64_t ldcb;
     const float *Z;
     int64_t ldz;
+    uint16_t *CbT;      /* EPI_SILU / EPI_DSILU: bf16 output transposed, [N][ldt] */
+    int64_t ldt;
+    float *CT;          /* EPI_F32: output transposed, [N][ldt] */
 };
 /* EPI_F32: C fp32; EPI_SILU: Z = C fp32 and silu(C) bf16; EPI_ACT: silu(C)
  * bf16 only; EPI_DSILU: C * silu'(Z) bf16; EPI_PARTIAL: split-K partial. */
 enum { EPI_F32 = 0, EPI_SILU = 1, EPI_DSILU = 2, EPI_PARTIAL = 3, EPI_ACT = 4 };
-int gpet_gemm(int kind, bool ta, bool tb, const GemmArgs &g, int splits, hipStream_t s);
+/* C[m][n] = sum_k A[m][k] B[n][k] (k contiguous in both; lda, ldb and
+ * k_chunk multiples of 8 and 64); splits > 1: k in chunks of k_chunk, one
+ * partial per chunk (EPI_PARTIAL). */
+int gpet_gemm(int kind, const GemmArgs &g, int splits, hipStream_t s);
+/* wbT = the bf16 weights with every layer transposed ([kp][LN] instead of
+ * [LN][kp], same offsets). */
+struct LayerShapes {
+    int64_t off[6];     /* parameter offset of layer l; off[5] = total */
+    int ln[5], kp[5];   /* stored outputs, stored inputs (row length) */
+};
+int gpet_transpose_params(const uint16_t *wb, uint16_t *wbT, const LayerShapes &L, hipStream_t s);
+/* rows [r0, r1) of a [rows][ld] bf16 matrix: row r0 = 1, the rest 0 (the
+ * transposed activations' bias row and padding) */
+int gpet_bias_rows(uint16_t *a, int64_t ld, int r0, int r1, hipStream_t s);
 int gpet_sum_partials(const float *part, int nz, int64_t n, float *out, hipStream_t s);
 int gpet_bias_columns(uint16_t *a, int64_t M, int64_t ld, int c0, hipStream_t s);
-int gpet_mdn_loss_grad(const float *H, int64_t ldh, const float *Y, int64_t M, float inv_n, uint16_t *dH,
-                       int64_t lddh, float *part, hipStream_t s);
+/* HT: head outputs transposed ([output][ldt]); dH row-major (lddh = 192),
+ * dH^T [output][ldt]; one loss partial per 256 rows in part. */
+int gpet_mdn_loss_grad(const float *HT, int64_t ldt, const float *Y, int64_t M, float inv_n, uint16_t *dH,
+                       int64_t lddh, uint16_t *dHT, float *part, hipStream_t s);
 int gpet_adam(float *w, float *m, float *v, const float *g, const float *gscale, uint16_t *wb, int64_t n, float lr,
               float b1, float b2, float c1, float c2, float eps, hipStream_t s);
 /* factor[0] = min(1, clip / |g|) (240 partials in part). */
 int gpet_clip_factor(const float *g, int64_t n, float clip, float *part, float *factor, hipStream_t s);
 int gpet_pack_train(const float *set, int64_t n, uint64_t seed, int64_t step, int64_t M, uint16_t *X, float *Y,
-                    hipStream_t s);
+                    uint16_t *XT, int64_t ldt, hipStream_t s);
 int gpet_loss_reduce(const float *part, int n, float *out, hipStream_t s);
 /* The whole network in one kernel (layer l's bf16 weights w[l]): with X
  * (packed bf16 [n x 8] inputs) the SURR_NH head outputs of each row go to H

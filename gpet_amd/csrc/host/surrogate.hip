@@ -27,7 +27,7 @@ constexpr int NL = 5;
 constexpr int LN[NL] = {256, 256, 256, 256, 192};      /* outputs (stored)       */
 constexpr int LK[NL] = {SURR_NX, 256, 256, 256, 256};  /* inputs                 */
 constexpr int HID = 256, HLD = HID + 8, HEAD_LD = 192;
-constexpr int64_t SPLIT_ROWS = 4096;                   /* split-K chunk (rows)   */
+constexpr int64_t SPLIT_ROWS = 1024;                   /* split-K chunk (rows)   */
 
 int kp(int l) { return l == 0 ? SURR_LDX : LK[l] + 8; }
 
@@ -39,6 +39,17 @@ int64_t param_offset(int l)
 }
 
 int64_t n_params() { return param_offset(NL); }
+
+LayerShapes layer_shapes()
+{
+    LayerShapes L;
+    for (int l = 0; l <= NL; ++l) L.off[l] = param_offset(l);
+    for (int l = 0; l < NL; ++l) {
+        L.ln[l] = LN[l];
+        L.kp[l] = kp(l);
+    }
+    return L;
+}
 
 /* algorithmic FLOPs of one forward pass per row (stored shapes) */
 double fwd_flops_per_row()
@@ -69,7 +80,7 @@ gpet_status alloc_params(gpet_ctx *c)
     const int64_t n = n_params();
     gpet_status st;
     if ((st = dalloc(c, S.w, n)) || (st = dalloc(c, S.m, n)) || (st = dalloc(c, S.v, n)) || (st = dalloc(c, S.g, n)) ||
-        (st = dalloc(c, S.wb, n)))
+        (st = dalloc(c, S.wb, n)) || (st = dalloc(c, S.wbT, n)))
         return st;
     return GPET_OK;
 }
@@ -84,6 +95,7 @@ gpet_status put_params(gpet_ctx *c, const std::vector<float> &w)
     for (size_t i = 0; i < w.size(); ++i) wb[i] = host_bf16(w[i]);
     HIPOK(c, hipMemcpyAsync(S.w.p, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice, c->stream));
     HIPOK(c, hipMemcpyAsync(S.wb.p, wb.data(), sizeof(uint16_t) * wb.size(), hipMemcpyHostToDevice, c->stream));
+    KOK(c, gpet_transpose_params(S.wb.p, S.wbT.p, layer_shapes(), c->stream), "transpose weights");
     HIPOK(c, hipMemsetAsync(S.m.p, 0, sizeof(float) * w.size(), c->stream));
     HIPOK(c, hipMemsetAsync(S.v.p, 0, sizeof(float) * w.size(), c->stream));
     HIPOK(c, hipStreamSynchronize(c->stream));
@@ -130,8 +142,13 @@ gpet_status alloc_work(gpet_ctx *c, int64_t rows, bool train)
             KOK(c, gpet_bias_columns(S.A[l].p, rows, HLD, HID, c->stream), "bias columns");
         }
         S.work_rows = rows;
-        for (int l = 0; l < 4; ++l) dfree(S.Z[l]);
+        S.ldt = (rows + 7) / 8 * 8;
+        for (int l = 0; l < 4; ++l) {
+            dfree(S.Z[l]);
+            dfree(S.AT[l]);
+        }
         dfree(S.dH); dfree(S.dZ[0]); dfree(S.dZ[1]); dfree(S.Y); dfree(S.part); dfree(S.lpart);
+        dfree(S.XT); dfree(S.dHT); dfree(S.dZT[0]); dfree(S.dZT[1]); dfree(S.HT);
     }
     if (train && !S.Y.p) {
         for (int l = 0; l < 4; ++l)
@@ -141,6 +158,14 @@ gpet_status alloc_work(gpet_ctx *c, int64_t rows, bool train)
             (st = dalloc(c, S.dZ[1], S.work_rows * HID)) || (st = dalloc(c, S.Y, S.work_rows * SURR_NY)) ||
             (st = dalloc(c, S.part, splits * (int64_t)HID * HLD)) || (st = dalloc(c, S.lpart, (S.work_rows + 255) / 256 + 1)))
             return st;
+        if ((st = dalloc(c, S.XT, SURR_LDX * S.ldt)) || (st = dalloc(c, S.dHT, HEAD_LD * S.ldt)) ||
+            (st = dalloc(c, S.HT, HEAD_LD * S.ldt)) ||
+            (st = dalloc(c, S.dZT[0], HID * S.ldt)) || (st = dalloc(c, S.dZT[1], HID * S.ldt)))
+            return st;
+        for (int l = 0; l < 4; ++l) {
+            if ((st = dalloc(c, S.AT[l], HLD * S.ldt))) return st;
+            KOK(c, gpet_bias_rows(S.AT[l].p, S.ldt, HID, HLD, c->stream), "bias rows");
+        }
     }
     return GPET_OK;
 }
@@ -166,28 +191,35 @@ gpet_status forward(gpet_ctx *c, int64_t M, bool train)
         if (train) {
             g.C = S.Z[l].p;
             g.ldc = HID;
+            g.CbT = S.AT[l].p;
+            g.ldt = S.ldt;
         }
-        KOK(c, gpet_gemm(train ? EPI_SILU : EPI_ACT, false, false, g, 1, c->stream), "gemm forward");
+        KOK(c, gpet_gemm(train ? EPI_SILU : EPI_ACT, g, 1, c->stream), "gemm forward");
         in = S.A[l].p;
         ldin = HLD;
     }
     GemmArgs g = gargs(in, HLD, S.wb.p + param_offset(4), kp(4), M, HEAD_LD, kp(4));
-    g.C = S.H.p;
-    g.ldc = HEAD_LD;
-    KOK(c, gpet_gemm(EPI_F32, false, false, g, 1, c->stream), "gemm head");
+    if (train) {
+        g.CT = S.HT.p;   /* the loss reads the head outputs transposed */
+        g.ldt = S.ldt;
+    } else {
+        g.C = S.H.p;
+        g.ldc = HEAD_LD;
+    }
+    KOK(c, gpet_gemm(EPI_F32, g, 1, c->stream), "gemm head");
     return GPET_OK;
 }
 
-/* dW_l = dZ^T . A_in over M rows (split-K partials, fixed-order sum). */
-gpet_status weight_grad(gpet_ctx *c, int l, const uint16_t *dz, int64_t lddz, const uint16_t *a_in, int64_t ldin,
-                        int64_t M)
+/* dW_l = dZ^T . A_in over M rows from the transposed copies dZ^T
+ * ([LN][ldt]) and A_in^T ([kp][ldt]) (split-K partials, fixed-order sum). */
+gpet_status weight_grad(gpet_ctx *c, int l, const uint16_t *dzT, const uint16_t *ainT, int64_t M)
 {
     SurrState &S = c->surr;
     const int64_t splits = (M + SPLIT_ROWS - 1) / SPLIT_ROWS;
-    GemmArgs g = gargs(dz, lddz, a_in, ldin, LN[l], kp(l), M);
+    GemmArgs g = gargs(dzT, S.ldt, ainT, S.ldt, LN[l], kp(l), M);
     g.k_chunk = SPLIT_ROWS;
     g.C = S.part.p;
-    KOK(c, gpet_gemm(EPI_PARTIAL, true, true, g, (int)splits, c->stream), "gemm weight grad");
+    KOK(c, gpet_gemm(EPI_PARTIAL, g, (int)splits, c->stream), "gemm weight grad");
     KOK(c, gpet_sum_partials(S.part.p, (int)splits, (int64_t)LN[l] * kp(l), S.g.p + param_offset(l), c->stream),
         "partials");
     return GPET_OK;
@@ -199,21 +231,25 @@ gpet_status backward(gpet_ctx *c, int64_t M)
     SurrState &S = c->surr;
     gpet_status st;
     /* head */
-    if ((st = weight_grad(c, 4, S.dH.p, HEAD_LD, S.A[3].p, HLD, M))) return st;
+    if ((st = weight_grad(c, 4, S.dHT.p, S.AT[3].p, M))) return st;
     const uint16_t *dnext = S.dH.p;
     int64_t ldnext = HEAD_LD;
     int knext = HEAD_LD; /* reduction length of the input-gradient GEMM */
     for (int l = 3; l >= 0; --l) {
         uint16_t *dz = S.dZ[l & 1].p;
-        /* dZ_l = (dnext . W_{l+1})[:, :256] * silu'(Z_l) */
-        GemmArgs g = gargs(dnext, ldnext, S.wb.p + param_offset(l + 1), kp(l + 1), M, HID, knext);
-        g.Cb = dz;
+        /* dZ_l = (dnext . W_{l+1})[:, :256] * silu'(Z_l), with W_{l+1}
+         * transposed ([kp][LN]: row k holds input k's weights); the
+         * row-major copy only where another input gradient reads it */
+        GemmArgs g = gargs(dnext, ldnext, S.wbT.p + param_offset(l + 1), LN[l + 1], M, HID, knext);
+        g.Cb = l > 0 ? dz : nullptr;
         g.ldcb = HID;
         g.Z = S.Z[l].p;
         g.ldz = HID;
-        KOK(c, gpet_gemm(EPI_DSILU, false, true, g, 1, c->stream), "gemm input grad");
-        const uint16_t *a_in = l == 0 ? S.X.p : S.A[l - 1].p;
-        if ((st = weight_grad(c, l, dz, HID, a_in, l == 0 ? SURR_LDX : HLD, M))) return st;
+        g.CbT = S.dZT[l & 1].p;
+        g.ldt = S.ldt;
+        KOK(c, gpet_gemm(EPI_DSILU, g, 1, c->stream), "gemm input grad");
+        const uint16_t *a_inT = l == 0 ? S.XT.p : S.AT[l - 1].p;
+        if ((st = weight_grad(c, l, S.dZT[l & 1].p, a_inT, M))) return st;
         dnext = dz;
         ldnext = HID;
         knext = HID;
@@ -292,7 +328,7 @@ gpet_status surr_run_infer(gpet_ctx *c, int64_t n, const PanelParams &pp)
 void surr_free(gpet_ctx *c)
 {
     SurrState &S = c->surr;
-    dfree(S.w); dfree(S.m); dfree(S.v); dfree(S.g); dfree(S.wb); dfree(S.set); dfree(S.set_count);
+    dfree(S.w); dfree(S.m); dfree(S.v); dfree(S.g); dfree(S.wb); dfree(S.wbT); dfree(S.set); dfree(S.set_count);
     for (auto &b : S.sq_f) dfree(b);
     dfree(S.sq_t); dfree(S.sq_info); dfree(S.sq_count);
     dfree(S.X); dfree(S.dH); dfree(S.dZ[0]); dfree(S.dZ[1]); dfree(S.H); dfree(S.Y); dfree(S.part); dfree(S.lpart);
@@ -355,9 +391,11 @@ gpet_status gpet_surrogate_train(gpet_ctx *c, const gpet_surrogate_config *cfg, 
             double p = (double)(step - warm) / (double)std::max<int64_t>(1, cfg->steps - warm);
             lr = (float)(cfg->lr * (0.05 + 0.95 * 0.5 * (1.0 + std::cos(3.141592653589793 * p))));
         }
-        KOK(c, gpet_pack_train(S.set.p, S.set_n, cfg->seed, S.adam_t, M, S.X.p, S.Y.p, c->stream), "pack");
+        KOK(c, gpet_pack_train(S.set.p, S.set_n, cfg->seed, S.adam_t, M, S.X.p, S.Y.p, S.XT.p, S.ldt, c->stream),
+            "pack");
         if ((st = forward(c, M, true))) return st;
-        KOK(c, gpet_mdn_loss_grad(S.H.p, HEAD_LD, S.Y.p, M, 1.0f / (float)M, S.dH.p, HEAD_LD, S.lpart.p, c->stream),
+        KOK(c, gpet_mdn_loss_grad(S.HT.p, S.ldt, S.Y.p, M, 1.0f / (float)M, S.dH.p, HEAD_LD, S.dHT.p, S.lpart.p,
+                                  c->stream),
             "loss");
         KOK(c, gpet_loss_reduce(S.lpart.p, nblk, losses.p + step, c->stream), "loss reduce");
         if ((st = backward(c, M))) return st;
@@ -369,6 +407,7 @@ gpet_status gpet_surrogate_train(gpet_ctx *c, const gpet_surrogate_config *cfg, 
         KOK(c, gpet_adam(S.w.p, S.m.p, S.v.p, S.g.p, clip.p + 255, S.wb.p, n_params(), lr, b1, b2, c1, c2, eps,
                          c->stream),
             "adam");
+        KOK(c, gpet_transpose_params(S.wb.p, S.wbT.p, layer_shapes(), c->stream), "transpose weights");
     }
     HIPOK(c, hipStreamSynchronize(c->stream));
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -515,11 +554,17 @@ gpet_status gpet_surrogate_grad(gpet_ctx *c, const float *x, const float *y, int
     }
     std::vector<uint16_t> X;
     pack_host_rows(x, n, X);
+    std::vector<uint16_t> XT((size_t)(SURR_LDX * S.ldt), 0);
+    for (int64_t i = 0; i < n; ++i)
+        for (int k = 0; k < SURR_LDX; ++k) XT[(size_t)(k * S.ldt + i)] = X[(size_t)(i * SURR_LDX + k)];
     HIPOK(c, hipMemcpy(S.X.p, X.data(), sizeof(uint16_t) * X.size(), hipMemcpyHostToDevice));
+    HIPOK(c, hipMemcpy(S.XT.p, XT.data(), sizeof(uint16_t) * XT.size(), hipMemcpyHostToDevice));
     HIPOK(c, hipMemcpy(S.Y.p, y, sizeof(float) * (size_t)(n * SURR_NY), hipMemcpyHostToDevice));
     if ((st = forward(c, n, true))) return st;
     const int nblk = (int)((n + 255) / 256);
-    KOK(c, gpet_mdn_loss_grad(S.H.p, HEAD_LD, S.Y.p, n, 1.0f / (float)n, S.dH.p, HEAD_LD, S.lpart.p, c->stream), "loss");
+    KOK(c, gpet_mdn_loss_grad(S.HT.p, S.ldt, S.Y.p, n, 1.0f / (float)n, S.dH.p, HEAD_LD, S.dHT.p, S.lpart.p,
+                              c->stream),
+        "loss");
     DBuf<float> l1;
     if ((st = dalloc(c, l1, 1))) return st;
     KOK(c, gpet_loss_reduce(S.lpart.p, nblk, l1.p, c->stream), "loss reduce");

@@ -39,7 +39,6 @@ typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8_t;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16_t;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4_t;
 
-constexpr int BM = 64, BN = 64, BK = 32, LDK = BK + 8; /* +8: spread LDS banks */
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
@@ -59,89 +58,236 @@ __device__ __forceinline__ float silu_grad(float z)
     return s * (1.0f + z * (1.0f - s));
 }
 
-/* Stage a BM x BK (or BN x BK) tile into LDS as [row][k].
- * TRANS = false: element (r, k) at src[r * ld + k] (k contiguous);
- * TRANS = true:  element (r, k) at src[k * ld + r] (r contiguous). */
-template <bool TRANS>
-__device__ __forceinline__ void stage(uint16_t (*dst)[LDK], const uint16_t *src, int64_t ld, int64_t r0, int64_t nr,
-                                      int64_t k0, int64_t nk)
-{
-    const int t = threadIdx.x;
-    if (!TRANS) {
-        /* 64 rows x 32 k = 64 x 4 vectors of 8 */
-        const int r = t >> 2, kv = (t & 3) * 8;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (r0 + r < nr && k0 + kv < nk) v = *reinterpret_cast<const uint4 *>(src + (r0 + r) * ld + k0 + kv);
-        *reinterpret_cast<uint4 *>(&dst[r][kv]) = v;
-    } else {
-        /* 32 k x 64 rows = 32 x 8 vectors of 8 rows */
-        const int k = t >> 3, rv = (t & 7) * 8;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (k0 + k < nk && r0 + rv < nr) v = *reinterpret_cast<const uint4 *>(src + (k0 + k) * ld + r0 + rv);
-        const uint16_t *e = reinterpret_cast<const uint16_t *>(&v);
-#pragma unroll
-        for (int j = 0; j < 8; ++j) dst[rv + j][k] = e[j];
-    }
-}
-
 } // namespace
 
-template <bool TA, bool TB, int EPI>
-__global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g)
+/* ---- training GEMMs ----------------------------------------------------------
+ * C[m][n] = sum_k A[m][k] B[n][k] with k contiguous in both operands (rows
+ * of lda / ldb elements).  Every product of a training step has this form
+ * because the forward pass also writes its activations transposed, the
+ * backward pass its pre-activation gradients transposed, and the weights
+ * are kept in both layouts: no operand is transposed on the way into the
+ * MFMAs.  One block of 256 threads (4 waves, 2 x 2 sub-tiles of 64 x 64,
+ * four v_mfma_f32_32x32x16_bf16 accumulators each) per 128 x 128 tile of C;
+ * the k loop stages 64-wide slices of both operands in LDS, double
+ * buffered: the next slice's global loads are in flight while the current
+ * one multiplies, one barrier per slice.  blockIdx.z selects a chunk of k
+ * (split-K partials, summed in a fixed order by k_sum_partials).  k must
+ * be a multiple of 8 at every row start; a vector that straddles the end of
+ * k is masked element by element.  Epilogues: EPI_F32 C; EPI_SILU Z = C
+ * (fp32, optional), silu(C) bf16 (optional) and transposed (optional);
+ * EPI_ACT silu(C) bf16; EPI_DSILU C * silu'(Z) bf16 (optional) and
+ * transposed (optional); EPI_F32 C fp32 and/or transposed (CT); EPI_PARTIAL
+ * the chunk's partial.  Transposed copies go through LDS so that their rows
+ * are written 16 bytes per lane. */
+constexpr int TM = 128, TN = 128, TK = 64;
+constexpr int TLD = TK + 8;   /* staged row: 144 B, 16-B fragment reads without bank conflicts */
+constexpr int TTL = TM + 8;   /* transposed epilogue row (bf16) */
+constexpr int TTL32 = TM + 4; /* transposed epilogue row (fp32; 128 x 132 x 4 B fits the staging buffers) */
+
+template <int EPI>
+__global__ __launch_bounds__(256) void k_gemm_nt(GemmArgs g)
 {
-    __shared__ __attribute__((aligned(16))) uint16_t As[BM][LDK];
-    __shared__ __attribute__((aligned(16))) uint16_t Bs[BN][LDK];
-    const int64_t m0 = (int64_t)blockIdx.x * BM, n0 = (int64_t)blockIdx.y * BN;
+    __shared__ __attribute__((aligned(16))) uint16_t sm[2][TM + TN][TLD];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int64_t m0 = (int64_t)blockIdx.x * TM, n0 = (int64_t)blockIdx.y * TN;
     const int64_t kb = (int64_t)blockIdx.z * g.k_chunk;
     const int64_t ke = min(g.K, kb + g.k_chunk);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-    f32x16_t acc = {};
-    for (int64_t k0 = kb; k0 < ke; k0 += BK) {
-        stage<TA>(As, g.A, g.lda, m0, g.M, k0, ke);
-        stage<TB>(Bs, g.B, g.ldb, n0, g.N, k0, ke);
-        __syncthreads();
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    f32x16_t acc[2][2];
 #pragma unroll
-        for (int ks = 0; ks < BK; ks += 16) {
-            /* lanes 0-31 hold k 0-7 of the step, lanes 32-63 k 8-15; A and B
-             * use the same k assignment, so the product is the dot product */
-            const int kk = ks + (lane >> 5) * 8;
-            bf16x8_t a = *reinterpret_cast<const bf16x8_t *>(&As[wm + (lane & 31)][kk]);
-            bf16x8_t b = *reinterpret_cast<const bf16x8_t *>(&Bs[wn + (lane & 31)][kk]);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x16_t{};
+    /* vector j of a thread: operand j / 4, element v = t + 256 (j % 4) of
+     * the 128 rows x 8 vectors of the slice */
+    uint4 st[8];
+    auto load = [&](int64_t k0) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int v = t + 256 * (j & 3);
+            const int r = v >> 3, kv = (v & 7) * 8;
+            const bool isb = j >= 4;
+            const int64_t row = (isb ? n0 : m0) + r;
+            const int64_t k = k0 + kv;
+            uint4 x = make_uint4(0u, 0u, 0u, 0u);
+            if (row < (isb ? g.N : g.M) && k < ke) {
+                x = *reinterpret_cast<const uint4 *>((isb ? g.B : g.A) + row * (isb ? g.ldb : g.lda) + k);
+                if (k + 8 > ke) {
+                    const int nv = (int)(ke - k);   /* 1..7 elements inside */
+                    uint32_t w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+                    for (int e = 0; e < 8; ++e)
+                        if (e >= nv) w[e >> 1] &= (e & 1) ? 0x0000ffffu : 0xffff0000u;
+                    x = make_uint4(w[0], w[1], w[2], w[3]);
+                }
+            }
+            st[j] = x;
         }
+    };
+    auto store = [&](int b) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int v = t + 256 * (j & 3);
+            *reinterpret_cast<uint4 *>(&sm[b][(j >= 4 ? TM : 0) + (v >> 3)][(v & 7) * 8]) = st[j];
+        }
+    };
+    /* EPI_DSILU: the pre-activations of the tile, loaded before the k loop
+     * so that their latency hides behind it */
+    float zpre[2][2][16];
+    if (EPI == EPI_DSILU) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int64_t col = n0 + wn + 32 * j + (lane & 31);
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int64_t row = m0 + wm + 32 * i + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+                    zpre[i][j][r] = (row < g.M && col < g.N) ? g.Z[row * g.ldz + col] : 0.0f;
+                }
+            }
+    }
+    const int nk = ke > kb ? (int)((ke - kb + TK - 1) / TK) : 0;
+    if (nk > 0) {
+        load(kb);
+        store(0);
+    }
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        const int b = kt & 1;
+        if (kt + 1 < nk) load(kb + (int64_t)(kt + 1) * TK);
+#pragma unroll
+        for (int ks = 0; ks < TK; ks += 16) {
+            /* lanes 0-31 hold k 0-7 of the step, lanes 32-63 k 8-15 */
+            const int kk = ks + (lane >> 5) * 8;
+            bf16x8_t a[2], bv[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                a[i] = *reinterpret_cast<const bf16x8_t *>(&sm[b][wm + 32 * i + (lane & 31)][kk]);
+                bv[i] = *reinterpret_cast<const bf16x8_t *>(&sm[b][TM + wn + 32 * i + (lane & 31)][kk]);
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], bv[j], acc[i][j], 0, 0, 0);
+        }
+        if (kt + 1 < nk) store(b ^ 1);
         __syncthreads();
     }
-    /* accumulator i of lane l: row (i/4)*8 + (l/32)*4 + i%4, column l%32 */
-    const int64_t col = n0 + wn + (lane & 31);
+    /* accumulator r of lane l in sub-tile (i, j): row wm + 32 i + (r / 4) 8 +
+     * (l / 32) 4 + r % 4, column wn + 32 j + l % 32 */
+    constexpr bool TR = EPI == EPI_SILU || EPI == EPI_DSILU;
+    uint16_t (*T)[TTL] = reinterpret_cast<uint16_t (*)[TTL]>(&sm[0][0][0]);
+    float (*T32)[TTL32] = reinterpret_cast<float (*)[TTL32]>(&sm[0][0][0]);
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int64_t row = m0 + wm + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
-        if (row >= g.M || col >= g.N) continue;
-        const float v = acc[i];
-        if (EPI == EPI_F32) {
-            g.C[row * g.ldc + col] = v;
-        } else if (EPI == EPI_SILU) {
-            g.C[row * g.ldc + col] = v;
-            g.Cb[row * g.ldcb + col] = f2bf(silu(v));
-        } else if (EPI == EPI_ACT) {
-            g.Cb[row * g.ldcb + col] = f2bf(silu(v));
-        } else if (EPI == EPI_DSILU) {
-            g.Cb[row * g.ldcb + col] = f2bf(v * silu_grad(g.Z[row * g.ldz + col]));
-        } else {
-            g.C[(int64_t)blockIdx.z * g.M * g.N + row * g.N + col] = v;
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int cl = wn + 32 * j + (lane & 31);
+            const int64_t col = n0 + cl;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int rl = wm + 32 * i + q * 8 + (lane >> 5) * 4;
+                uint16_t o[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int64_t row = m0 + rl + e;
+                    const float v = acc[i][j][4 * q + e];
+                    const bool in = row < g.M && col < g.N;
+                    if (EPI == EPI_F32) {
+                        if (in && g.C) g.C[row * g.ldc + col] = v;
+                    } else if (EPI == EPI_PARTIAL) {
+                        if (in) g.C[(int64_t)blockIdx.z * g.M * g.N + row * g.N + col] = v;
+                    } else if (EPI == EPI_SILU || EPI == EPI_ACT) {
+                        o[e] = f2bf(silu(v));
+                        if (in) {
+                            if (EPI == EPI_SILU && g.C) g.C[row * g.ldc + col] = v;
+                            if (g.Cb) g.Cb[row * g.ldcb + col] = o[e];
+                        }
+                    } else {
+                        o[e] = in ? f2bf(v * silu_grad(zpre[i][j][4 * q + e])) : (uint16_t)0;
+                        if (in && g.Cb) g.Cb[row * g.ldcb + col] = o[e];
+                    }
+                }
+                if (TR && g.CbT)
+                    *reinterpret_cast<uint2 *>(&T[cl][rl]) =
+                        make_uint2((uint32_t)o[0] | ((uint32_t)o[1] << 16), (uint32_t)o[2] | ((uint32_t)o[3] << 16));
+                if (EPI == EPI_F32 && g.CT)
+                    *reinterpret_cast<float4 *>(&T32[cl][rl]) =
+                        make_float4(acc[i][j][4 * q], acc[i][j][4 * q + 1], acc[i][j][4 * q + 2], acc[i][j][4 * q + 3]);
+            }
+        }
+    if (EPI == EPI_F32 && g.CT) {
+        __syncthreads();
+        for (int v = t; v < TN * (TM / 4); v += 256) {
+            const int cl = v >> 5, rv = (v & 31) * 4;
+            const int64_t col = n0 + cl, row = m0 + rv;
+            if (col >= g.N || row >= g.M) continue;
+            const float4 x = *reinterpret_cast<const float4 *>(&T32[cl][rv]);
+            float *dst = g.CT + col * g.ldt + row;
+            if (row + 4 <= g.M) {
+                *reinterpret_cast<float4 *>(dst) = x;
+            } else {
+                const float w[4] = {x.x, x.y, x.z, x.w};
+                for (int e = 0; e < (int)(g.M - row); ++e) dst[e] = w[e];
+            }
+        }
+    }
+    if (TR && g.CbT) {
+        __syncthreads();
+        /* row n of the transposed tile: the block's 128 m values of column n */
+        for (int v = t; v < TN * (TM / 8); v += 256) {
+            const int cl = v >> 4, rv = (v & 15) * 8;
+            const int64_t col = n0 + cl, row = m0 + rv;
+            if (col >= g.N || row >= g.M) continue;
+            const uint4 x = *reinterpret_cast<const uint4 *>(&T[cl][rv]);
+            uint16_t *dst = g.CbT + col * g.ldt + row;
+            if (row + 8 <= g.M) {
+                *reinterpret_cast<uint4 *>(dst) = x;
+            } else {
+                const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+                for (int e = 0; e < (int)(g.M - row); ++e) dst[e] = (uint16_t)(w[e >> 1] >> ((e & 1) * 16));
+            }
         }
     }
 }
 
-/* Sum of split-K partials: out[i] = sum_z part[z][i] (fixed order). */
-__global__ void k_sum_partials(const float *part, int nz, int64_t n, float *out)
+/* Sum of split-K partials: out[i] = sum_z part[z][i].  Block: 64 elements
+ * x 4 groups; group q sums z = q, q + 4, ... in order, then the four group
+ * sums are added in order q = 0..3 (a fixed order: deterministic). */
+__global__ __launch_bounds__(256) void k_sum_partials(const float *part, int nz, int64_t n, float *out)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    __shared__ float red[4][64];
+    const int e = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int64_t i = (int64_t)blockIdx.x * 64 + e;
     float s = 0.0f;
-    for (int z = 0; z < nz; ++z) s += part[(int64_t)z * n + i];
-    out[i] = s;
+    if (i < n)
+        for (int z = q; z < nz; z += 4) s += part[(int64_t)z * n + i];
+    red[q][e] = s;
+    __syncthreads();
+    if (q == 0 && i < n) out[i] = ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
+}
+
+/* Every layer's bf16 weights transposed: [kp][LN] at the same offset. */
+__global__ void k_transpose_params(const uint16_t *wb, uint16_t *wbT, LayerShapes L)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L.off[5]) return;
+    int l = 0;
+    while (l < 4 && i >= L.off[l + 1]) ++l;
+    const int64_t r = i - L.off[l];
+    const int n = (int)(r / L.kp[l]), k = (int)(r % L.kp[l]);
+    wbT[L.off[l] + (int64_t)k * L.ln[l] + n] = wb[i];
+}
+
+/* Rows [r0, r1) of a [rows][ld] bf16 matrix: 1 in row r0, 0 after. */
+__global__ void k_bias_rows(uint16_t *a, int64_t ld, int r0, int r1)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)(r1 - r0) * ld) return;
+    const int r = r0 + (int)(i / ld);
+    a[(int64_t)r * ld + i % ld] = r == r0 ? (uint16_t)0x3f80 : (uint16_t)0;
 }
 
 /* Columns [c0, ld) of an activation buffer: 1 in column c0, 0 after. */
@@ -170,28 +316,43 @@ __global__ void k_bias_columns(uint16_t *a, int64_t M, int64_t ld, int c0)
 
 __device__ __forceinline__ float softplus(float x) { return x > 0.0f ? x + gp_logf(1.0f + gp_expf(-x)) : gp_logf(1.0f + gp_expf(x)); }
 
-/* Per-sample loss and gradient dL/dh / n_total (bf16, cols >= SURR_NH zero);
- * loss partial sums per block to part[blockIdx.x] (fixed order). */
-__global__ __launch_bounds__(256) void k_mdn_loss_grad(const float *H, int64_t ldh, const float *Y, int64_t M,
-                                                       float inv_n, uint16_t *dH, int64_t lddh, float *part)
+/* Per-sample loss and gradient dL/dh / n_total (bf16, cols >= SURR_NH zero)
+ * from the head outputs transposed (HT: [output][ldt], a wave reads 64
+ * consecutive rows of one output); one thread per row.  The row's gradient
+ * is built in registers (LOSS_LD bf16 packed in pairs) and leaves as dH
+ * (row-major, 16-byte stores) and dH^T ([output][ldt], a wave's stores
+ * cover 64 consecutive rows).  Loss partial sums per block to
+ * part[blockIdx.x] (fixed order). */
+constexpr int LOSS_LD = 192;
+struct GradRow {
+    uint32_t w[LOSS_LD / 2];
+    __device__ __forceinline__ void set(int c, uint16_t v)
+    {
+        w[c >> 1] = (c & 1) ? ((w[c >> 1] & 0x0000ffffu) | ((uint32_t)v << 16)) : ((w[c >> 1] & 0xffff0000u) | v);
+    }
+};
+__global__ __launch_bounds__(256) void k_mdn_loss_grad(const float *HT, int64_t ldt, const float *Y, int64_t M,
+                                                       float inv_n, uint16_t *dH, uint16_t *dHT, float *part)
 {
     __shared__ float red[256];
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     float loss = 0.0f;
+    GradRow g;
+#pragma unroll
+    for (int c = 0; c < LOSS_LD / 2; ++c) g.w[c] = 0u;
     if (i < M) {
-        const float *h = H + i * ldh;
+        auto hv = [&](int c) -> float { return HT[(int64_t)c * ldt + i]; };
         const float *y = Y + i * SURR_NY;
-        uint16_t *g = dH + i * lddh;
         const float ev = y[0];
-        const float h0 = h[0];
+        const float h0 = hv(0);
         const float p = 1.0f / (1.0f + gp_expf(-h0));
         loss = ev > 0.5f ? softplus(-h0) : softplus(h0);
-        g[0] = f2bf((p - ev) * inv_n);
+        g.set(0, f2bf((p - ev) * inv_n));
         float lw[SURR_K], lp[SURR_K];
         float mx = -1e30f, mw = -1e30f;
 #pragma unroll
         for (int k = 0; k < SURR_K; ++k) {
-            lw[k] = h[1 + k];
+            lw[k] = hv(1 + k);
             mw = fmaxf(mw, lw[k]);
         }
         float sw = 0.0f;
@@ -202,14 +363,14 @@ __global__ __launch_bounds__(256) void k_mdn_loss_grad(const float *H, int64_t l
             const bool pp = y[1] >= SURR_PP_MIN;
 #pragma unroll
             for (int k = 0; k < SURR_K; ++k) {
-                const float hq = h[SURR_PP + k];
+                const float hq = hv(SURR_PP + k);
                 /* log q_k or log (1 - q_k) */
                 float s = lw[k] - lse_w - (pp ? softplus(-hq) : softplus(hq));
 #pragma unroll
                 for (int d = 0; d < SURR_D; ++d) {
                     if (d == 0 && pp) continue;
-                    float mu = h[17 + 4 * k + d];
-                    float ls = fminf(fmaxf(h[81 + 4 * k + d], SURR_LOGSIG_MIN), SURR_LOGSIG_MAX);
+                    float mu = hv(17 + 4 * k + d);
+                    float ls = fminf(fmaxf(hv(81 + 4 * k + d), SURR_LOGSIG_MIN), SURR_LOGSIG_MAX);
                     float zz = (y[1 + d] - mu) * gp_expf(-ls);
                     s += -0.5f * zz * zz - ls - 0.918938533f;
                 }
@@ -225,30 +386,32 @@ __global__ __launch_bounds__(256) void k_mdn_loss_grad(const float *H, int64_t l
             for (int k = 0; k < SURR_K; ++k) {
                 const float gam = gp_expf(lp[k] - lse);
                 const float pik = gp_expf(lw[k] - lse_w);
-                g[1 + k] = f2bf((pik - gam) * inv_n);
-                const float qk = 1.0f / (1.0f + gp_expf(-h[SURR_PP + k]));
-                g[SURR_PP + k] = f2bf((pp ? -gam * (1.0f - qk) : gam * qk) * inv_n);
+                g.set(1 + k, f2bf((pik - gam) * inv_n));
+                const float qk = 1.0f / (1.0f + gp_expf(-hv(SURR_PP + k)));
+                g.set(SURR_PP + k, f2bf((pp ? -gam * (1.0f - qk) : gam * qk) * inv_n));
 #pragma unroll
                 for (int d = 0; d < SURR_D; ++d) {
                     if (d == 0 && pp) {
-                        g[17 + 4 * k] = 0;
-                        g[81 + 4 * k] = 0;
+                        g.set(17 + 4 * k, 0);
+                        g.set(81 + 4 * k, 0);
                         continue;
                     }
-                    float mu = h[17 + 4 * k + d];
-                    float raw = h[81 + 4 * k + d];
+                    float mu = hv(17 + 4 * k + d);
+                    float raw = hv(81 + 4 * k + d);
                     float ls = fminf(fmaxf(raw, SURR_LOGSIG_MIN), SURR_LOGSIG_MAX);
                     float is = gp_expf(-ls);
                     float zz = (y[1 + d] - mu) * is;
-                    g[17 + 4 * k + d] = f2bf(-gam * zz * is * inv_n);
+                    g.set(17 + 4 * k + d, f2bf(-gam * zz * is * inv_n));
                     bool free_ = raw > SURR_LOGSIG_MIN && raw < SURR_LOGSIG_MAX;
-                    g[81 + 4 * k + d] = f2bf(free_ ? gam * (1.0f - zz * zz) * inv_n : 0.0f);
+                    g.set(81 + 4 * k + d, f2bf(free_ ? gam * (1.0f - zz * zz) * inv_n : 0.0f));
                 }
             }
-        } else {
-            for (int c = 1; c < SURR_NH; ++c) g[c] = 0;
-        }
-        for (int c = SURR_NH; c < lddh; ++c) g[c] = 0;
+        }   /* no event: only the event logit's gradient (the row starts at zero) */
+        uint4 *row = reinterpret_cast<uint4 *>(dH + i * LOSS_LD);
+#pragma unroll
+        for (int v = 0; v < LOSS_LD / 8; ++v) row[v] = make_uint4(g.w[4 * v], g.w[4 * v + 1], g.w[4 * v + 2], g.w[4 * v + 3]);
+#pragma unroll
+        for (int c = 0; c < LOSS_LD; ++c) dHT[(int64_t)c * ldt + i] = (uint16_t)(g.w[c >> 1] >> ((c & 1) * 16));
     }
     red[threadIdx.x] = loss;
     __syncthreads();
@@ -321,7 +484,7 @@ __global__ void k_loss_reduce(const float *part, int n, float *out)
  * records (Philox stream (step, 14), one draw per row) -> input matrix
  * (bf16 [M x 8]: 6 features, 1, 0) and targets (fp32 [M x SURR_NY]). */
 __global__ void k_pack_train(const float *set, int64_t n, uint64_t seed, int64_t step, int64_t M, uint16_t *X,
-                             float *Y)
+                             float *Y, uint16_t *XT, int64_t ldt)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
@@ -331,9 +494,11 @@ __global__ void k_pack_train(const float *set, int64_t n, uint64_t seed, int64_t
     const int64_t j = (int64_t)(u % (uint64_t)n);
     const float *r = set + j * SURR_REC;
 #pragma unroll
-    for (int c = 0; c < SURR_NX; ++c) X[i * SURR_LDX + c] = f2bf(r[c]);
-    X[i * SURR_LDX + SURR_NX] = 0x3f80;
-    for (int c = SURR_NX + 1; c < SURR_LDX; ++c) X[i * SURR_LDX + c] = 0;
+    for (int c = 0; c < SURR_LDX; ++c) {
+        const uint16_t x = c < SURR_NX ? f2bf(r[c]) : (c == SURR_NX ? (uint16_t)0x3f80 : (uint16_t)0);
+        X[i * SURR_LDX + c] = x;
+        if (XT) XT[(int64_t)c * ldt + i] = x;
+    }
 #pragma unroll
     for (int c = 0; c < SURR_NY; ++c) Y[i * SURR_NY + c] = r[SURR_NX + c];
 }
@@ -702,30 +867,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 
 /* ---- launchers ------------------------------------------------------------ */
 
-int gpet_gemm(int kind, bool ta, bool tb, const GemmArgs &g, int splits, hipStream_t s)
+int gpet_gemm(int kind, const GemmArgs &g, int splits, hipStream_t s)
 {
-    dim3 grid((unsigned)((g.M + BM - 1) / BM), (unsigned)((g.N + BN - 1) / BN), (unsigned)(splits > 0 ? splits : 1));
+    dim3 grid((unsigned)((g.M + TM - 1) / TM), (unsigned)((g.N + TN - 1) / TN), (unsigned)(splits > 0 ? splits : 1));
     GemmArgs a = g;
     if (a.k_chunk <= 0) a.k_chunk = a.K;
-#define GEMM_CASE(TA_, TB_, E_)                                                             \
-    if (ta == TA_ && tb == TB_ && kind == E_) {                                             \
-        hipLaunchKernelGGL((k_gemm_bf16<TA_, TB_, E_>), grid, dim3(256), 0, s, a);          \
-        return (int)hipGetLastError();                                                      \
+    if ((a.lda | a.ldb) % 8 != 0 || a.k_chunk % TK != 0 && a.k_chunk < a.K) return (int)hipErrorInvalidValue;
+    switch (kind) {
+    case EPI_F32: hipLaunchKernelGGL(k_gemm_nt<EPI_F32>, grid, dim3(256), 0, s, a); break;
+    case EPI_SILU: hipLaunchKernelGGL(k_gemm_nt<EPI_SILU>, grid, dim3(256), 0, s, a); break;
+    case EPI_ACT: hipLaunchKernelGGL(k_gemm_nt<EPI_ACT>, grid, dim3(256), 0, s, a); break;
+    case EPI_DSILU: hipLaunchKernelGGL(k_gemm_nt<EPI_DSILU>, grid, dim3(256), 0, s, a); break;
+    case EPI_PARTIAL: hipLaunchKernelGGL(k_gemm_nt<EPI_PARTIAL>, grid, dim3(256), 0, s, a); break;
+    default: return (int)hipErrorInvalidValue;
     }
-    GEMM_CASE(false, false, EPI_F32)
-    GEMM_CASE(false, false, EPI_SILU)
-    GEMM_CASE(false, false, EPI_ACT)
-    GEMM_CASE(false, true, EPI_DSILU)
-    GEMM_CASE(false, true, EPI_F32)
-    GEMM_CASE(true, true, EPI_PARTIAL)
-    GEMM_CASE(true, true, EPI_F32)
-#undef GEMM_CASE
-    return (int)hipErrorInvalidValue;
+    return (int)hipGetLastError();
 }
 
 int gpet_sum_partials(const float *part, int nz, int64_t n, float *out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, nz, n, out);
+    hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, s, part, nz, n, out);
     return (int)hipGetLastError();
 }
 
@@ -737,11 +898,26 @@ int gpet_bias_columns(uint16_t *a, int64_t M, int64_t ld, int c0, hipStream_t s)
     return (int)hipGetLastError();
 }
 
-int gpet_mdn_loss_grad(const float *H, int64_t ldh, const float *Y, int64_t M, float inv_n, uint16_t *dH,
-                       int64_t lddh, float *part, hipStream_t s)
+int gpet_mdn_loss_grad(const float *HT, int64_t ldt, const float *Y, int64_t M, float inv_n, uint16_t *dH,
+                       int64_t lddh, uint16_t *dHT, float *part, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_mdn_loss_grad, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, H, ldh, Y, M, inv_n, dH,
-                       lddh, part);
+    if (lddh != LOSS_LD) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_mdn_loss_grad, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, HT, ldt, Y, M, inv_n, dH,
+                       dHT, part);
+    return (int)hipGetLastError();
+}
+
+int gpet_transpose_params(const uint16_t *wb, uint16_t *wbT, const LayerShapes &L, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_transpose_params, dim3((unsigned)((L.off[5] + 255) / 256)), dim3(256), 0, s, wb, wbT, L);
+    return (int)hipGetLastError();
+}
+
+int gpet_bias_rows(uint16_t *a, int64_t ld, int r0, int r1, hipStream_t s)
+{
+    const int64_t n = (int64_t)(r1 - r0) * ld;
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_bias_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, ld, r0, r1);
     return (int)hipGetLastError();
 }
 
@@ -762,9 +938,10 @@ int gpet_adam(float *w, float *m, float *v, const float *g, const float *gscale,
 }
 
 int gpet_pack_train(const float *set, int64_t n, uint64_t seed, int64_t step, int64_t M, uint16_t *X, float *Y,
-                    hipStream_t s)
+                    uint16_t *XT, int64_t ldt, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_pack_train, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, set, n, seed, step, M, X, Y);
+    hipLaunchKernelGGL(k_pack_train, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, set, n, seed, step, M, X, Y,
+                       XT, ldt);
     return (int)hipGetLastError();
 }
 
