@@ -202,8 +202,12 @@ def roofline(M, wl):
            "kernel": "k_phantom + k_longwalk", "bytes_per_launch": bytes_per_launch,
            "avg_launch_ms": 1e3 * avg_launch, "histories_per_launch": h_launch}
     if vpb and avg_launch > 0:
-        # a wave64 VALU instruction occupies its SIMD for 4 cycles
-        out["issue"] = {"valu_insts_per_history": vpb, "frac": vpb * h_launch * 4.0 / (SIMDS * CLOCK_HZ * avg_launch),
+        # the stage's VALU instruction stream at 4 cycles per wave64
+        # instruction on every SIMD, over the measured launch: near or above
+        # 1 means instruction issue, not memory, bounds the stage (above 1:
+        # some instructions issue in fewer than 4 cycles)
+        out["issue"] = {"valu_insts_per_history": vpb,
+                        "valu_cycles_over_launch": vpb * h_launch * 4.0 / (SIMDS * CLOCK_HZ * avg_launch),
                         "from": src}
     return out
 
