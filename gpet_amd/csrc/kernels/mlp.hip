@@ -646,8 +646,22 @@ constexpr int IHO = 3;            /* head: 16-output tiles per wave (4 x 48 = 19
 constexpr int IPD = 3;            /* hidden layers: weight loads this many steps ahead */
 constexpr int IHPD = 2;           /* head: the same */
 constexpr int IHLD = 164;         /* head row in LDS (fp32): >= SURR_NH, 16-byte rows */
-/* the head outputs of half the rows at a time replace the activations */
-constexpr int INFER_LDS = (IB / 2) * IHLD * 4 > IB * ILD * 2 ? (IB / 2) * IHLD * 4 : IB * ILD * 2;
+/* LDS of a block of rg row groups: the activations, which the head outputs
+ * of half of each group's rows at a time replace */
+constexpr int infer_lds(int rg)
+{
+    return rg * (IB / 2) * IHLD * 4 > rg * IB * ILD * 2 ? rg * (IB / 2) * IHLD * 4 : rg * IB * ILD * 2;
+}
+/* Row groups per block (GPET_INFER_RG): 2 gives 256-row tiles on 8 waves,
+ * so the two waves that compute the same neurons for different rows load
+ * the same weights (the second from L1), halving the weight stream from
+ * L2.  Measured on MI355X it is slower (11.1 vs 9.9 ms per cylinder
+ * step, identical outputs): with 135 KB of LDS a CU holds one such block,
+ * and its prologue, barriers and sampling no longer overlap with another
+ * block's products.  One group (two 4-wave blocks per CU) is kept. */
+#ifndef GPET_INFER_RG
+#define GPET_INFER_RG 1
+#endif
 
 struct InferWeights {
     const uint16_t *w[5];      /* layer l: [outputs][kp(l)] bf16              */
@@ -674,18 +688,22 @@ __device__ __forceinline__ bf16x8_t load_w(const uint16_t *p, bool ok)
     return __builtin_bit_cast(bf16x8_t, v);
 }
 
-template <bool SAMPLE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_mlp_infer(InferWeights W, const uint16_t *X, DevQueue q, int64_t row0,
+template <bool SAMPLE, int RG>
+__global__ __launch_bounds__(256 * RG) __attribute__((amdgpu_waves_per_eu(2))) void k_mlp_infer(InferWeights W, const uint16_t *X, DevQueue q, int64_t row0,
                                                    int64_t n, DevDetector det, PanelParams pp, float *H, int64_t ldh)
 {
-    __shared__ __attribute__((aligned(16))) unsigned char s_buf[INFER_LDS];
+    __shared__ __attribute__((aligned(16))) unsigned char s_buf[infer_lds(RG)];
     uint16_t *act = reinterpret_cast<uint16_t *>(s_buf);
     float *head = reinterpret_cast<float *>(s_buf);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t b0 = (int64_t)blockIdx.x * IB;
+    /* row group rg: rows [IB rg, IB rg + IB) of the tile; wv: the wave's
+     * share of the neurons (hidden layers) or outputs (head) */
+    const int rg = wave >> 2, wv = wave & 3;
+    constexpr int IBR = IB * RG;
+    const int64_t b0 = (int64_t)blockIdx.x * IBR;
 
     /* prologue: inputs in columns 0..7, the bias column 256 and zeros */
-    if (threadIdx.x < IB) {
+    if (threadIdx.x < IBR) {
         const int m = threadIdx.x;
         const int64_t i = b0 + m;
         uint16_t in[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -722,7 +740,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         for (int t = 0; t < 2; ++t)
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt) acc[t][mt] = f32x16_t{};
-        const int nrow0 = 64 * wave + lr, nrow1 = nrow0 + 32;
+        const int nrow0 = 64 * wv + lr, nrow1 = nrow0 + 32;
         if (l == 0) {
             /* 8 input columns: one step, lanes 32..63 (k 8..15) zero */
             const bool ok = kh < SURR_LDX;
@@ -730,7 +748,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             const bf16x8_t a1 = load_w(w + (int64_t)nrow1 * SURR_LDX + kh, ok);
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt) {
-                bf16x8_t bv = *reinterpret_cast<const bf16x8_t *>(act + (mt * 32 + lr) * ILD);
+                bf16x8_t bv = *reinterpret_cast<const bf16x8_t *>(act + (IB * rg + mt * 32 + lr) * ILD);
                 if (!ok) bv = bf16x8_t{};
                 acc[0][mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bv, acc[0][mt], 0, 0, 0);
                 acc[1][mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bv, acc[1][mt], 0, 0, 0);
@@ -759,7 +777,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 const int k = st * 16 + kh;
 #pragma unroll
                 for (int mt = 0; mt < 4; ++mt) {
-                    bf16x8_t bv = *reinterpret_cast<const bf16x8_t *>(act + (mt * 32 + lr) * ILD + (k < ILD ? k : 0));
+                    bf16x8_t bv =
+                        *reinterpret_cast<const bf16x8_t *>(act + (IB * rg + mt * 32 + lr) * ILD + (k < ILD ? k : 0));
                     if (k >= ILD) bv = bf16x8_t{};
                     acc[0][mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bv, acc[0][mt], 0, 0, 0);
                     acc[1][mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bv, acc[1][mt], 0, 0, 0);
@@ -775,8 +794,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
                 for (int g = 0; g < 4; ++g) {
-                    const int nb = 64 * wave + 32 * t + g * 8 + (lane >> 5) * 4;
-                    const int m = mt * 32 + lr;
+                    const int nb = 64 * wv + 32 * t + g * 8 + (lane >> 5) * 4;
+                    const int m = IB * rg + mt * 32 + lr;
                     const uint32_t lo = f2bf2(silu_fast(acc[t][mt][4 * g]), silu_fast(acc[t][mt][4 * g + 1]));
                     const uint32_t hi = f2bf2(silu_fast(acc[t][mt][4 * g + 2]), silu_fast(acc[t][mt][4 * g + 3]));
                     *reinterpret_cast<uint2 *>(act + m * ILD + nb) = make_uint2(lo, hi);
@@ -797,7 +816,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
             for (int rt = 0; rt < 8; ++rt) acc[t][rt] = f32x4_t{};
         constexpr int NS = (ILD + 31) / 32;
-        const uint16_t *wr = w + (int64_t)(48 * wave + l16) * ILD;
+        const uint16_t *wr = w + (int64_t)(48 * wv + l16) * ILD;
         bf16x8_t ab[IHPD][IHO];
 #pragma unroll
         for (int p = 0; p < IHPD; ++p)
@@ -822,7 +841,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             }
 #pragma unroll
             for (int rt = 0; rt < 8; ++rt) {
-                bf16x8_t bv = *reinterpret_cast<const bf16x8_t *>(act + (rt * 16 + l16) * ILD + (ok ? k : 0));
+                bf16x8_t bv = *reinterpret_cast<const bf16x8_t *>(act + (IB * rg + rt * 16 + l16) * ILD + (ok ? k : 0));
                 if (!ok) bv = bf16x8_t{};
 #pragma unroll
                 for (int t = 0; t < IHO; ++t)
@@ -838,20 +857,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             __syncthreads();
 #pragma unroll
             for (int t = 0; t < IHO; ++t) {
-                const int nb = 48 * wave + 16 * t + (lane >> 4) * 4;
+                const int nb = 48 * wv + 16 * t + (lane >> 4) * 4;
                 if (nb < IHLD) {
 #pragma unroll
                     for (int r4 = 0; r4 < 4; ++r4) {
                         const f32x4_t v = half ? acc[t][4 + r4] : acc[t][r4];
-                        *reinterpret_cast<float4 *>(head + (r4 * 16 + l16) * IHLD + nb) =
+                        *reinterpret_cast<float4 *>(head + (64 * rg + r4 * 16 + l16) * IHLD + nb) =
                             make_float4(v[0], v[1], v[2], v[3]);
                     }
                 }
             }
             __syncthreads();
-            if (wave == 0) {
-                const int64_t i = b0 + 64 * half + lane;
-                const float *h = head + lane * IHLD;
+            if (wave < RG) {
+                /* wave g (on SIMD g) handles row group g */
+                const int64_t i = b0 + IB * wave + 64 * half + lane;
+                const float *h = head + (64 * wave + lane) * IHLD;
                 if (SAMPLE) {
                     bool has = false;
                     gpet_event ev;
@@ -957,11 +977,12 @@ int gpet_mlp_infer(const uint16_t *const w[5], const uint16_t *X, const DevQueue
     if (n <= 0) return 0;
     InferWeights W;
     for (int l = 0; l < 5; ++l) W.w[l] = w[l];
-    const dim3 grid((unsigned)((n + IB - 1) / IB));
+    constexpr int RG = GPET_INFER_RG;
+    const dim3 grid((unsigned)((n + IB * RG - 1) / (IB * RG)));
     if (X)
-        hipLaunchKernelGGL(k_mlp_infer<false>, grid, dim3(256), 0, s, W, X, sq, row0, n, det, pp, H, ldh);
+        hipLaunchKernelGGL((k_mlp_infer<false, RG>), grid, dim3(256 * RG), 0, s, W, X, sq, row0, n, det, pp, H, ldh);
     else
-        hipLaunchKernelGGL(k_mlp_infer<true>, grid, dim3(256), 0, s, W, X, sq, row0, n, det, pp, H, ldh);
+        hipLaunchKernelGGL((k_mlp_infer<true, RG>), grid, dim3(256 * RG), 0, s, W, X, sq, row0, n, det, pp, H, ldh);
     return (int)hipGetLastError();
 }
 

@@ -159,16 +159,15 @@ __device__ void d_sample_shape(const DevSource &s, gp_rng &r, float &x, float &y
 /* Positron range: 3-D DDA voxel walk (same rule as the oracle, see
  * setPositronRange, gPET_kernals.cu:347-418); crossings gather only the
  * density half of the voxel record. */
-__device__ __forceinline__ void d_dda_axis(float x, float off, float d, int i, float u, float &tmax, float &tdelta,
-                                           int &step)
+__device__ __forceinline__ void d_dda_axis(float x, float off, float d, int i, float u, float inv, float &tmax,
+                                           float &tdelta, int &step)
 {
+    /* inv = 1 / u (correctly rounded), computed once by the caller */
     if (u > 0.0f) {
-        float inv = 1.0f / u;
         tmax = (off + (float)(i + 1) * d - x) * inv;
         tdelta = d * inv;
         step = 1;
     } else if (u < 0.0f) {
-        float inv = 1.0f / u;
         tmax = (off + (float)i * d - x) * inv;
         tdelta = -(d * inv);
         step = -1;
@@ -349,12 +348,12 @@ struct RangeWalk {
             t = rr / D_RHO_OUTSIDE;
             return;
         }
-        d_dda_axis(x, p.offx, p.dx, ix, ux, tmx, tdx, sx);
-        d_dda_axis(y, p.offy, p.dy, iy, uy, tmy, tdy, sy);
-        d_dda_axis(z, p.offz, p.dz, iz, uz, tmz, tdz, sz);
+        inverses();
+        d_dda_axis(x, p.offx, p.dx, ix, ux, ivx, tmx, tdx, sx);
+        d_dda_axis(y, p.offy, p.dy, iy, uy, ivy, tmy, tdy, sy);
+        d_dda_axis(z, p.offz, p.dz, iz, uz, ivz, tmz, tdz, sz);
         o = (unsigned)gpet_brick_offset(ix + 1, iy + 1, iz + 1, p.nbx, p.nby) << 2;
         deltas(p.nbx, p.nby);
-        inverses();
         tcur = 0.0f;
         s = 0.0f;
         it = 0;
