@@ -517,7 +517,8 @@ __device__ __forceinline__ float s_sigmoid(float x) { return __builtin_amdgcn_rc
  * event at the first deposit point, with the readout site of the digitizer
  * settings (one event per photon).  Returns whether the photon gives an
  * event (ev filled). */
-__device__ __forceinline__ bool d_surr_sample_row(const float *h, const DevQueue &q, int64_t j, const DevDetector &det,
+template <typename T>
+__device__ __forceinline__ bool d_surr_sample_row(const T *h, const DevQueue &q, int64_t j, const DevDetector &det,
                                                   const PanelParams &pp, gpet_event &ev)
 {
     const int4 info = q.info[j];
@@ -527,13 +528,13 @@ __device__ __forceinline__ bool d_surr_sample_row(const float *h, const DevQueue
     gp_rng r;
     gp_rng_init(&r, pp.seed, (uint64_t)(int64_t)info.z, GP_TAG_PHOTON + k, draws);
     gp_u4 b = gp_rng_block(&r);
-    const float pe = s_sigmoid(h[0]);
+    const float pe = s_sigmoid((float)h[0]);
     if (!(gp_u01(b.x) <= pe)) return false;
     float lw[SURR_K];
     float mw = -1e30f;
 #pragma unroll
     for (int c = 0; c < SURR_K; ++c) {
-        lw[c] = h[1 + c];
+        lw[c] = (float)h[1 + c];
         mw = fmaxf(mw, lw[c]);
     }
     float sw = 0.0f;
@@ -557,7 +558,7 @@ __device__ __forceinline__ bool d_surr_sample_row(const float *h, const DevQueue
         }
     }
     /* full absorption with the component's photopeak probability */
-    const bool peak = gp_u01(b.z) <= s_sigmoid(h[SURR_PP + kc]);
+    const bool peak = gp_u01(b.z) <= s_sigmoid((float)h[SURR_PP + kc]);
     /* the component's Gaussian restricted to the region the training data
      * cover (continuum fraction in (0, SURR_PP_MIN), depth in [0, 1]): up to
      * 8 draws of four normals, then the last one clamped */
@@ -573,8 +574,8 @@ __device__ __forceinline__ bool d_surr_sample_row(const float *h, const DevQueue
         z[0] = r0 * c0; z[1] = r0 * s0; z[2] = r1 * c1; z[3] = r1 * s1;
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
-            float ls = fminf(fmaxf(h[81 + 4 * kc + d], SURR_LOGSIG_MIN), SURR_LOGSIG_MAX);
-            y[d] = h[17 + 4 * kc + d] + s_exp(ls) * z[d];
+            float ls = fminf(fmaxf((float)h[81 + 4 * kc + d], SURR_LOGSIG_MIN), SURR_LOGSIG_MAX);
+            y[d] = (float)h[17 + 4 * kc + d] + s_exp(ls) * z[d];
         }
         const bool fe_ok = peak || (y[0] > 0.0f && y[0] < SURR_PP_MIN);
         if (fe_ok && y[1] >= 0.0f && y[1] <= 1.0f) break;
@@ -633,9 +634,9 @@ __device__ __forceinline__ void d_append_event(const PanelParams &pp, bool has, 
  * of all 128 rows (2 x 4 tiles, 128 accumulators).  Head: wave w computes
  * outputs [48w, 48w + 48) of the 192 stored (161 used) for all 128 rows on
  * v_mfma_f32_16x16x32_bf16, so each head weight is loaded once per tile;
- * the outputs replace the activations in LDS as fp32, half the rows at a
- * time, and one lane per row samples the row's event (SAMPLE) or copies
- * its outputs to H.  Inputs: the
+ * the outputs replace the activations in LDS, all rows at once as fp16
+ * when the rows are sampled (one lane per row on two waves), or half the
+ * rows at a time as fp32 when they are copied to H.  Inputs: the
  * surrogate queue (features packed in the prologue) or, without SAMPLE, a
  * packed bf16 [n x 8] matrix.  The SiLU here is z / (1 + 2^(-z log2 e))
  * with the hardware exp2 and reciprocal (training uses the exact form; the
@@ -848,10 +849,43 @@ __global__ __launch_bounds__(256 * RG) __attribute__((amdgpu_waves_per_eu(2))) v
                     acc[t][rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], bv, acc[t][rt], 0, 0, 0);
             }
         }
-        /* rows [0, 64) (row tiles 0-3), then rows [64, 128) (4-7): every
-         * wave writes its outputs of the half's rows to LDS (accumulator i:
-         * output 4 (lane / 16) + i of the tile, row lane % 16) and wave 0
-         * handles the 64 rows, one per lane */
+        if (SAMPLE) {
+            /* every row's outputs at once, as fp16 (2^-11 relative: a
+             * deposited-energy mean moves by < 0.3 keV), replace the
+             * activations (accumulator i: output 4 (lane / 16) + i of the
+             * tile, row lane % 16); then waves 0 .. 2 RG - 1 (one per SIMD)
+             * sample 64 rows each, one per lane */
+            typedef _Float16 f16x4_t __attribute__((ext_vector_type(4)));
+            _Float16 *head16 = reinterpret_cast<_Float16 *>(s_buf);
+            __syncthreads();
+#pragma unroll
+            for (int t = 0; t < IHO; ++t) {
+                const int nb = 48 * wv + 16 * t + (lane >> 4) * 4;
+                if (nb < IHLD) {
+#pragma unroll
+                    for (int rt = 0; rt < 8; ++rt) {
+                        f32x4_t v = acc[t][rt];
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) v[e] = fminf(fmaxf(v[e], -65000.0f), 65000.0f);
+                        *reinterpret_cast<f16x4_t *>(head16 + (IB * rg + rt * 16 + l16) * IHLD + nb) =
+                            __builtin_convertvector(v, f16x4_t);
+                    }
+                }
+            }
+            __syncthreads();
+            if (wave < 2 * RG) {
+                const int row = 64 * wave + lane;
+                const int64_t i = b0 + row;
+                bool has = false;
+                gpet_event ev;
+                if (i < n) has = d_surr_sample_row(head16 + row * IHLD, q, row0 + i, det, pp, ev);
+                d_append_event(pp, has, ev);
+            }
+            return;
+        }
+        /* outputs to H (fp32): rows [0, 64) (row tiles 0-3), then rows
+         * [64, 128) (4-7): every wave writes its outputs of the half's rows
+         * to LDS and wave g copies row group g's 64 rows, one per lane */
 #pragma unroll 1
         for (int half = 0; half < 2; ++half) {
             __syncthreads();
@@ -872,14 +906,8 @@ __global__ __launch_bounds__(256 * RG) __attribute__((amdgpu_waves_per_eu(2))) v
                 /* wave g (on SIMD g) handles row group g */
                 const int64_t i = b0 + IB * wave + 64 * half + lane;
                 const float *h = head + (64 * wave + lane) * IHLD;
-                if (SAMPLE) {
-                    bool has = false;
-                    gpet_event ev;
-                    if (i < n) has = d_surr_sample_row(h, q, row0 + i, det, pp, ev);
-                    d_append_event(pp, has, ev);
-                } else if (i < n) {
+                if (i < n)
                     for (int c = 0; c < SURR_NH; ++c) H[i * ldh + c] = h[c];
-                }
             }
         }
     }
