@@ -49,16 +49,18 @@ MAP_BYTES = 16                     # one walk-map record (uniform box) per range
 
 WORKLOADS = {
     # batch: histories per phantom-kernel launch (0: the library's 2^23).  The
-    # cylinder workload's 1e8 decays run in three launches of 2^25: per-launch
-    # fixed costs (queue-region prefixes, counter flushes, the last waves of each
-    # grid) then weigh less.  The body workload keeps the default: its event
-    # buffer (1.2 events per decay) plus a 2^24 batch's headroom does not fit
-    # with the digitizer's work buffers.
+    # cylinder workload's 1e8 decays run in three launches of 2^25 and the
+    # batched workload's 1.47e8 in five: per-launch fixed costs (queue-region
+    # prefixes, counter flushes, the last waves of each grid) then weigh less
+    # (batched: 3.64e9 / 3.70e9 / 3.75e9 histories/s at 2^23 / 2^24 / 2^25).
+    # The body workload keeps the default: its event buffer (1.2 events per
+    # decay) plus a 2^24 batch's headroom does not fit with the digitizer's
+    # work buffers.
     "cylinder": dict(config=2, decays=1e8, seed=2, batch=1 << 25, counters="k_phantom_cylinder200_1e8_counters.md",
                      text="cylinder phantom 200^3 over 1 cm (water r=0.4, bone rod r=0.1, air), uniform F-18 "
                           "with positron range, %.0e decays over 10 s, 8-panel LSO octagon at 8 cm, "
                           "plain MC (no surrogate)"),
-    "batch": dict(config=3, decays=0, seed=3,
+    "batch": dict(config=3, decays=0, seed=3, batch=1 << 25,
                   text="64 blob phantoms 32^3..256^3 over 10 cm (water/bone/lung/air, seed 3), F-18 cylinder "
                        "sources, 1e5..1e7 decays each (%.3g in total) over 10 s, one batched run on the 8-panel "
                        "octagon, plain MC (no surrogate)"),
