@@ -15,7 +15,8 @@ Launch shapes are picked by the library's tuner during the warm-up run.
 
 The same JSON line also carries, measured in the same process:
   host_sink   the headline step with singles and coincidences returned to
-              pinned host buffers (the cost of handing the outputs over);
+              pinned host buffers (the cost of handing the outputs over),
+              with the copies overlapping the next run and without;
   surrogate   the detector-response surrogate on the headline workload:
               harvest of plain-MC panel crossings and GPU training (outside
               the timed region), then timed surrogate-mode steps against
@@ -227,7 +228,10 @@ def summary(M, launch):
 
 def host_sink_step(eng, wl, step_arg, warmup, steps, last):
     """The step with singles and coincidences copied into pinned host
-    buffers (allocated once, reused)."""
+    buffers (allocated once, reused): synchronously at the end of every run,
+    and asynchronously (gpet_sink.async_copy), where each run's copies run on
+    the copy stream beside the next run's transport and the clock stops
+    after gpet_sink_wait for the last run's."""
     import ctypes as C
 
     from gpet_amd import capi, hipmem
@@ -247,18 +251,28 @@ def host_sink_step(eng, wl, step_arg, warmup, steps, last):
     def step():
         eng._ok(eng.f["run_source"](eng.ctx, S, len(src), I, len(iso), float(t0), float(t1), int(step_arg),
                                     C.byref(sink)), "run_source")
-    for _ in range(warmup):
-        step()
-    t = time.perf_counter()
-    for _ in range(steps):
-        step()
-    dt = time.perf_counter() - t
+
+    res = {}
+    for mode in ("sync", "async"):
+        sink.async_copy = 1 if mode == "async" else 0
+        for _ in range(warmup):
+            step()
+        eng.sink_wait()
+        t = time.perf_counter()
+        for _ in range(steps):
+            step()
+        eng.sink_wait()
+        res[mode] = time.perf_counter() - t
     nbytes = 48 * (min(sink.n_singles, ns) + 2 * min(sink.n_coinc, nc))
     sb.free()
     cb.free()
+    dt = res["async"]
     return {"value": steps * int(step_arg) / dt, "unit": "histories/s", "ms_per_step": 1e3 * dt / steps,
+            "sync_value": steps * int(step_arg) / res["sync"], "sync_ms_per_step": 1e3 * res["sync"] / steps,
             "bytes_per_step": nbytes, "singles": int(sink.n_singles), "coincidences": int(sink.n_coinc),
-            "note": "singles and coincidence pairs copied to pinned host buffers at the end of every run"}
+            "note": "singles and coincidence pairs copied to pinned host buffers: value with each run's copies "
+                    "overlapping the next run's transport (async_copy, clock stopped after gpet_sink_wait), "
+                    "sync_value copying at the end of every run"}
 
 
 def train_surrogate(eng, wl, args):

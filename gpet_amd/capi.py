@@ -95,7 +95,8 @@ class Sink(C.Structure):
                 ("psf_source", C.c_void_p), ("psf_source_cap", C.c_int64), ("n_psf_source", C.c_int64),
                 ("psf_phantom", C.c_void_p), ("psf_phantom_cap", C.c_int64), ("n_psf_phantom", C.c_int64),
                 ("out_append", C.c_int32),
-                ("delayed", C.c_void_p), ("delayed_cap", C.c_int64), ("n_delayed", C.c_int64)]
+                ("delayed", C.c_void_p), ("delayed_cap", C.c_int64), ("n_delayed", C.c_int64),
+                ("async_copy", C.c_int32)]
 
 
 PSF_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("vx", "<f4"), ("vy", "<f4"), ("vz", "<f4"),

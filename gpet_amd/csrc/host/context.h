@@ -81,6 +81,11 @@ struct gpet_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
     std::vector<hipEvent_t> kev; /* per-launch timing events */
+    /* asynchronous sink copies (gpet_sink.async_copy): issued on copy_stream
+     * once the digitizer has ended; copy_done follows the last of them */
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t copy_done = nullptr;
+    bool copy_pending = false;
 
     /* tables */
     bool has_tab = false;

@@ -147,6 +147,16 @@ gpet_status check_ready(gpet_ctx *c)
     return GPET_OK;
 }
 
+/* Blocks until the last run's asynchronous sink copies have ended.  They
+ * read the digitizer's output buffers and write the caller's. */
+gpet_status copy_wait(gpet_ctx *c)
+{
+    if (!c->copy_pending) return GPET_OK;
+    c->copy_pending = false;
+    HIPOK(c, hipEventSynchronize(c->copy_done));
+    return GPET_OK;
+}
+
 gpet_status write_outputs(gpet_ctx *c, const std::string &dir, const std::vector<gpet_event> &adder,
                           const std::vector<gpet_event> &singles, const std::vector<gpet_event> &coinc,
                           const std::vector<int32_t> &hid, const std::vector<float> &hf,
@@ -846,6 +856,10 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
         dp.job_bounds = c->d_job_cnt.p;
     }
     char err[256] = {0};
+    /* the previous run's asynchronous sink copies read the buffers the
+     * digitizer writes (or reallocates) next; the transport above has
+     * overlapped them */
+    if ((st = copy_wait(c))) return st;
     if (gpet_digitize(c->dw, dp, c->d_events.p, c->n_events + n_noise, &c->dout, c->stream, err, sizeof(err)))
         return fail(c, GPET_ERR_DEVICE, std::string("digitizer: ") + err);
     HIPOK(c, hipEventRecord(c->ev2, c->stream));
@@ -925,7 +939,19 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
             memcpy(sink->adder, adder.data(), sizeof(gpet_event) * (size_t)std::min<int64_t>(sink->adder_cap, c->n_events));
     }
     /* singles and pairs go straight into the caller's buffers (one copy;
-     * with pinned buffers at full PCIe rate); files need a host copy */
+     * with pinned buffers at full PCIe rate); files need a host copy.  With
+     * async_copy the two copies go on the copy stream and the call returns
+     * at once: they run beside the next run's transport */
+    const bool async = sink->async_copy && !to_files &&
+                       ((sink->singles && sink->singles_cap > 0 && c->dout.n_singles > 0) ||
+                        (sink->coinc && sink->coinc_cap > 0 && c->dout.n_coinc > 0));
+    if (async) {
+        if (!c->copy_stream) {
+            HIPOK(c, hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+            HIPOK(c, hipEventCreateWithFlags(&c->copy_done, hipEventDisableTiming));
+        }
+        HIPOK(c, hipStreamWaitEvent(c->copy_stream, c->ev2, 0));
+    }
     if (to_files) {
         singles.resize((size_t)c->dout.n_singles);
         if (!singles.empty())
@@ -935,7 +961,11 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
                    sizeof(gpet_event) * (size_t)std::min<int64_t>(sink->singles_cap, c->dout.n_singles));
     } else if (sink->singles && sink->singles_cap > 0) {
         const int64_t m = std::min<int64_t>(sink->singles_cap, c->dout.n_singles);
-        if (m) HIPOK(c, hipMemcpy(sink->singles, c->dout.singles, sizeof(gpet_event) * (size_t)m, hipMemcpyDeviceToHost));
+        if (m && async)
+            HIPOK(c, hipMemcpyAsync(sink->singles, c->dout.singles, sizeof(gpet_event) * (size_t)m,
+                                    hipMemcpyDeviceToHost, c->copy_stream));
+        else if (m)
+            HIPOK(c, hipMemcpy(sink->singles, c->dout.singles, sizeof(gpet_event) * (size_t)m, hipMemcpyDeviceToHost));
     }
     if (to_files) {
         coinc.resize((size_t)(2 * c->dout.n_coinc));
@@ -946,7 +976,15 @@ gpet_status run_common(gpet_ctx *c, RunParams rp, int64_t n_hist, gpet_sink *sin
                    sizeof(gpet_event) * (size_t)(2 * std::min<int64_t>(sink->coinc_cap, c->dout.n_coinc)));
     } else if (sink->coinc && sink->coinc_cap > 0) {
         const int64_t m = std::min<int64_t>(sink->coinc_cap, c->dout.n_coinc);
-        if (m) HIPOK(c, hipMemcpy(sink->coinc, c->dout.coinc, sizeof(gpet_event) * (size_t)(2 * m), hipMemcpyDeviceToHost));
+        if (m && async)
+            HIPOK(c, hipMemcpyAsync(sink->coinc, c->dout.coinc, sizeof(gpet_event) * (size_t)(2 * m),
+                                    hipMemcpyDeviceToHost, c->copy_stream));
+        else if (m)
+            HIPOK(c, hipMemcpy(sink->coinc, c->dout.coinc, sizeof(gpet_event) * (size_t)(2 * m), hipMemcpyDeviceToHost));
+    }
+    if (async) {
+        HIPOK(c, hipEventRecord(c->copy_done, c->copy_stream));
+        c->copy_pending = true;
     }
     if ((sink->delayed && sink->delayed_cap > 0) || to_files) {
         int64_t m = to_files ? c->dout.n_delayed : std::min<int64_t>(sink->delayed_cap, c->dout.n_delayed);
@@ -1093,6 +1131,7 @@ gpet_status gpet_destroy(gpet_ctx *c)
     if (!c) return GPET_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->copy_pending) (void)hipEventSynchronize(c->copy_done);
     dfree(c->d_mu4); dfree(c->d_surf[0]); dfree(c->d_surf[1]);
     for (PhSlot &sl : c->slots) {
         dfree(sl.vox); dfree(sl.dens_b); dfree(sl.wck); dfree(sl.wmap);
@@ -1113,6 +1152,8 @@ gpet_status gpet_destroy(gpet_ctx *c)
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev2) (void)hipEventDestroy(c->ev2);
     for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
+    if (c->copy_done) (void)hipEventDestroy(c->copy_done);
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return GPET_OK;
@@ -1590,6 +1631,13 @@ gpet_status gpet_run_psf_noalloc(gpet_ctx *c, const double *d_rec8, int64_t n, i
     e = gpet_launch_panel(pp, c->dtab, c->ddet, q, 2 * n, cfg_of(L, 1), c->n_cu, s, PANEL_PLAIN);
     if (e) return fail(c, GPET_ERR_DEVICE, std::string("panel kernel: ") + hipGetErrorString((hipError_t)e));
     return GPET_OK;
+}
+
+gpet_status gpet_sink_wait(gpet_ctx *c)
+{
+    if (!c) return GPET_ERR_ARG;
+    HIPOK(c, hipSetDevice(c->device));
+    return copy_wait(c);
 }
 
 gpet_status gpet_get_tallies(gpet_ctx *c, gpet_tallies *t)

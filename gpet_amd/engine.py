@@ -496,6 +496,31 @@ class Engine:
                  "run_psf")
         return self._result(s, bufs)
 
+    def run_source_async(self, sources, isotopes, t0: float, t1: float, n_decays: int, singles, coinc):
+        """run_source with the singles and coincidence pairs copied
+        asynchronously into the caller's pinned buffers (hipmem.PinnedBuffer
+        or any host memory given as (address, capacity in events)): the call
+        returns once the digitizer has ended, the copies overlap the next run,
+        and the buffers hold the results after sink_wait() or when the next
+        run reaches its digitizer.  Returns the sink (n_singles, n_coinc)."""
+        s = capi.Sink()
+        s.singles, s.singles_cap = C.c_void_p(singles[0]), int(singles[1])
+        s.coinc, s.coinc_cap = C.c_void_p(coinc[0]), int(coinc[1])
+        s.async_copy = 1
+        S = (capi.Source * len(sources))(*sources)
+        I = (capi.Isotope * len(isotopes))(*isotopes)
+        self._ok(self.f["run_source"](self.ctx, S, len(sources), I, len(isotopes), float(t0), float(t1),
+                                      int(n_decays), C.byref(s)), "run_source")
+        return s
+
+    def sink_wait(self):
+        """Blocks until the last run's asynchronous sink copies have ended
+        (gpet_sink_wait; the HIP library only)."""
+        f = self.lib.gpet_sink_wait
+        f.argtypes = [C.c_void_p]
+        f.restype = C.c_int32
+        self._ok(f(self.ctx), "sink_wait")
+
     def tallies(self, n_crystals=0, n_ebins=0, e_lo=0.0, e_hi=1.0, n_s=0, n_phi=0, s_max=0.0, with_sinogram=False,
                 with_delayed=False):
         """Run statistics, and on request per-crystal singles, the singles

@@ -266,6 +266,13 @@ typedef struct gpet_sink {
     int32_t out_append;  /* 1: append to the files in out_dir (the reference's mode, Q21) */
     /* delayed-window pairs (2 events each; delayed.dat in out_dir) */
     gpet_event *delayed; int64_t delayed_cap; int64_t n_delayed;
+    /* 1 (and out_dir NULL): the singles and coinc copies are queued on a
+     * copy stream and the run returns before they end, so they overlap the
+     * next run's transport.  n_singles / n_coinc are set on return; the
+     * buffers' contents are complete after gpet_sink_wait or when the next
+     * run on the context reaches its digitizer, and must stay allocated
+     * until then (pinned memory, e.g. hipHostMalloc, for a true overlap). */
+    int32_t async_copy;
 } gpet_sink;
 
 /* Run statistics and tallies.  Arrays are caller-provided (may be NULL). */
@@ -551,6 +558,12 @@ gpet_status gpet_get_launch(gpet_ctx *ctx, gpet_launch *l);
 
 /* Tallies of the last run. */
 gpet_status gpet_get_tallies(gpet_ctx *ctx, gpet_tallies *t);
+
+/* Blocks until the asynchronous sink copies of the last run (async_copy)
+ * have ended; returns at once when none are pending.  New: the reference
+ * copies its singles synchronously after each run (outevents,
+ * detector.cu:287-300, called at gPET.cu:424). */
+gpet_status gpet_sink_wait(gpet_ctx *ctx);
 
 /* Woodcock majorant nodes (GPET_NWCK floats) built by gpet_set_phantom
  * (which = 0) or gpet_set_detector (which = 1), or for phantom slot k
