@@ -294,21 +294,49 @@ def train_surrogate(eng, wl, args):
             "loss_first": rep.loss_first, "loss_last": rep.loss_last}
 
 
-def surrogate_accuracy(eng, run, step_arg, n_panels):
-    """Per-crystal singles and the singles energy spectrum of one plain and
-    one surrogate run of the workload (same decays): chi-square homogeneity
-    of the per-crystal counts and of the spectrum (1 keV bins), two-sample
-    KS of the spectrum from the binned CDFs."""
+def independent_context(eng, wl, tabs):
+    """A second context of the workload with another Philox key (seed + 1000),
+    the first one's launch shape and its trained network (saved and loaded,
+    with the digitizer settings it was trained for): its surrogate-mode runs
+    share no random numbers with the plain runs of the first."""
+    import tempfile
+
+    from gpet_amd import engine
+
+    other = engine.Engine(engine.make_options(seed=wl["seed"] + 1000, positron_range=1, batch=eng.options.batch))
+    run = setup(other, wl, tabs)
+    L = eng.get_launch()
+    other.set_launch(block=tuple(L["block"]), wpe=tuple(L["wpe"]), blocks_per_cu=tuple(L["blocks_per_cu"]),
+                     batch=L["batch"], transit=tuple(L["transit"]))
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "surrogate.bin")
+        eng.surrogate_save(path)
+        other.surrogate_load(path)
+    return other, run
+
+
+def surrogate_accuracy(eng, run, step_arg, n_panels, wl, tabs):
+    """Per-crystal singles and the singles energy spectrum of one plain run
+    of the workload and one surrogate-mode run with independent random
+    numbers (a second context, another seed): chi-square homogeneity of the
+    per-crystal counts and of the spectrum (1 keV bins), two-sample KS of the
+    spectrum from the binned CDFs."""
     import numpy as np
     from scipy import stats
 
     res = {}
-    for sur in (False, True):
-        eng.surrogate_enable(sur)
-        r = run(step_arg)
-        t, cs, eh = eng.tallies(n_crystals=n_panels * 400, n_ebins=300, e_lo=350e3, e_hi=650e3)
-        res[sur] = (r.tallies.n_singles, r.tallies.n_coinc, cs.astype(float), eh.astype(float))
     eng.surrogate_enable(False)
+    r = run(step_arg)
+    t, cs, eh = eng.tallies(n_crystals=n_panels * 400, n_ebins=300, e_lo=350e3, e_hi=650e3)
+    res[False] = (r.tallies.n_singles, r.tallies.n_coinc, cs.astype(float), eh.astype(float))
+    other, orun = independent_context(eng, wl, tabs)
+    try:
+        other.surrogate_enable(True)
+        r = orun(step_arg)
+        t, cs, eh = other.tallies(n_crystals=n_panels * 400, n_ebins=300, e_lo=350e3, e_hi=650e3)
+        res[True] = (r.tallies.n_singles, r.tallies.n_coinc, cs.astype(float), eh.astype(float))
+    finally:
+        other.close()
     (sp, cp, csp, ehp), (ss, cs_, css, ehs) = res[False], res[True]
     out = {"singles_ratio": ss / max(sp, 1), "coincidences_ratio": cs_ / max(cp, 1)}
     panel_p = csp.reshape(n_panels, -1).sum(1)
@@ -339,7 +367,7 @@ def surrogate_accuracy(eng, run, step_arg, n_panels):
     return out
 
 
-def surrogate_block(eng, run, step_arg, wl, args, plain):
+def surrogate_block(eng, run, step_arg, wl, args, plain, tabs):
     """The surrogate measured in the same run as plain MC: training outside
     the timed region, then the timed surrogate-mode steps and the accuracy
     runs.  plain: the plain-MC measurement at step_arg (None: measured here,
@@ -352,7 +380,7 @@ def surrogate_block(eng, run, step_arg, wl, args, plain):
     S["steps"] = args.steps
     eng.surrogate_enable(False)
     last = S["last"]
-    acc = surrogate_accuracy(eng, run, step_arg, 48 if wl["config"] == 5 else 8)
+    acc = surrogate_accuracy(eng, run, step_arg, 48 if wl["config"] == 5 else 8, wl, tabs)
     flops = SURR_FLOPS_PER_ROW * last.n_surrogate
     return dict(surr, value=S["value"], unit="histories/s", coincidences_per_s=S["coinc"] / S["wall"],
                 ms_per_step=1e3 * S["wall"] / args.steps,
@@ -386,7 +414,7 @@ def surrogate_fresh(name, args, tabs):
     try:
         run = setup(eng, wl, tabs)
         n = int(wl["surrogate_decays"])
-        out = surrogate_block(eng, run, n, wl, args, None)
+        out = surrogate_block(eng, run, n, wl, args, None, tabs)
         out["decays"] = n
         return out
     finally:
@@ -449,7 +477,7 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-host-sink", action="store_true")
     ap.add_argument("--harvest-runs", type=int, default=7, help="plain runs of the workload harvested for training")
-    ap.add_argument("--train-steps", type=int, default=8000)
+    ap.add_argument("--train-steps", type=int, default=24000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
@@ -499,7 +527,7 @@ def main():
             eng = None
             out["surrogate"] = surrogate_fresh(name, args, tabs)
         else:
-            out["surrogate"] = surrogate_block(eng, run, step_arg, wl, args, M)
+            out["surrogate"] = surrogate_block(eng, run, step_arg, wl, args, M, tabs)
     if eng is not None:
         eng.close()
     if not args.no_cpu:

@@ -131,7 +131,7 @@ class SurrogateReport(C.Structure):
 
 SURR_NX = 6
 SURR_NY = 5
-SURR_NH = 161
+SURR_NH = 190
 
 
 class Job(C.Structure):

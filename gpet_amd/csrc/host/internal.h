@@ -302,9 +302,10 @@ int gpet_transpose_params(const uint16_t *wb, uint16_t *wbT, const LayerShapes &
 int gpet_bias_rows(uint16_t *a, int64_t ld, int r0, int r1, hipStream_t s);
 int gpet_sum_partials(const float *part, int nz, int64_t n, float *out, hipStream_t s);
 int gpet_bias_columns(uint16_t *a, int64_t M, int64_t ld, int c0, hipStream_t s);
-/* HT: head outputs transposed ([output][ldt]); dH row-major (lddh = 192),
+/* HT: head outputs transposed ([output][ldt]); Y: targets with the energy
+ * class in place of the deposited fraction; dH row-major (lddh = 192),
  * dH^T [output][ldt]; one loss partial per 256 rows in part. */
-int gpet_mdn_loss_grad(const float *HT, int64_t ldt, const float *Y, int64_t M, float inv_n, uint16_t *dH,
+int gpet_head_loss_grad(const float *HT, int64_t ldt, const float *Y, int64_t M, float inv_n, uint16_t *dH,
                        int64_t lddh, uint16_t *dHT, float *part, hipStream_t s);
 int gpet_adam(float *w, float *m, float *v, const float *g, const float *gscale, uint16_t *wb, int64_t n, float lr,
               float b1, float b2, float c1, float c2, float eps, hipStream_t s);

@@ -4,9 +4,9 @@
  * (mlp.hip), Adam training on harvested panel crossings, inference inside a
  * run, save/load, and the C ABI of gpet.h's surrogate section.
  *
- * Network: 6 -> 256 -> 256 -> 256 -> 256 -> 161 (SiLU).  Layer l keeps its
+ * Network: 6 -> 256 -> 256 -> 256 -> 256 -> 190 (SiLU).  Layer l keeps its
  * weights as [N_l x Kp_l] with Kp = K + 8 (bias in column K, zeros after);
- * the head is stored with 192 rows (rows 161.. stay zero) so every GEMM
+ * the head is stored with 192 rows (rows 190.. stay zero) so every GEMM
  * dimension is a multiple of 8.
  */
 #include <hip/hip_runtime.h>
@@ -106,9 +106,9 @@ gpet_status put_params(gpet_ctx *c, const std::vector<float> &w)
 
 /* Uniform(-a, a) weights with a = sqrt(3 / fan_in) (unit-variance
  * pre-activations), the head scaled by 0.1; zero biases and padding except
- * the head's: component means start spread over the target ranges
- * (deposited-energy fraction (k + 1/2)/16, depth over [0.1, 0.9], no lateral
- * offset) with log sigma -2, so every component starts on some data. */
+ * the head's: the position components' means start spread over the depths
+ * ((k + 1/2) / 16, no lateral offset) with log sigma -2, so every component
+ * starts on some data. */
 std::vector<float> init_params(uint64_t seed)
 {
     std::vector<float> w((size_t)n_params(), 0.0f);
@@ -123,9 +123,9 @@ std::vector<float> init_params(uint64_t seed)
     }
     const int64_t hb = param_offset(NL - 1) + LK[NL - 1]; /* head bias column */
     for (int k = 0; k < SURR_K; ++k) {
-        w[(size_t)(hb + (int64_t)(17 + 4 * k) * kp(NL - 1))] = ((float)k + 0.5f) / (float)SURR_K;
-        w[(size_t)(hb + (int64_t)(17 + 4 * k + 1) * kp(NL - 1))] = 0.1f + 0.8f * (float)((k * 7) % SURR_K) / (float)SURR_K;
-        for (int d = 0; d < SURR_D; ++d) w[(size_t)(hb + (int64_t)(81 + 4 * k + d) * kp(NL - 1))] = -2.0f;
+        w[(size_t)(hb + (int64_t)(SURR_MU + SURR_D * k) * kp(NL - 1))] = ((float)k + 0.5f) / (float)SURR_K;
+        for (int d = 0; d < SURR_D; ++d)
+            w[(size_t)(hb + (int64_t)(SURR_LS + SURR_D * k + d) * kp(NL - 1))] = -2.0f;
     }
     return w;
 }
@@ -394,7 +394,7 @@ gpet_status gpet_surrogate_train(gpet_ctx *c, const gpet_surrogate_config *cfg, 
         KOK(c, gpet_pack_train(S.set.p, S.set_n, cfg->seed, S.adam_t, M, S.X.p, S.Y.p, S.XT.p, S.ldt, c->stream),
             "pack");
         if ((st = forward(c, M, true))) return st;
-        KOK(c, gpet_mdn_loss_grad(S.HT.p, S.ldt, S.Y.p, M, 1.0f / (float)M, S.dH.p, HEAD_LD, S.dHT.p, S.lpart.p,
+        KOK(c, gpet_head_loss_grad(S.HT.p, S.ldt, S.Y.p, M, 1.0f / (float)M, S.dH.p, HEAD_LD, S.dHT.p, S.lpart.p,
                                   c->stream),
             "loss");
         KOK(c, gpet_loss_reduce(S.lpart.p, nblk, losses.p + step, c->stream), "loss reduce");
@@ -476,7 +476,7 @@ gpet_status gpet_surrogate_save(gpet_ctx *c, const char *path)
     if ((st = gpet_surrogate_get_params(c, w.data(), (int64_t)w.size(), nullptr))) return st;
     FILE *f = fopen(path, "wb");
     if (!f) return fail(c, GPET_ERR_IO, std::string("cannot write ") + path);
-    const char magic[8] = {'G', 'P', 'E', 'T', 'S', 'U', 'R', '1'};
+    const char magic[8] = {'G', 'P', 'E', 'T', 'S', 'U', 'R', '2'};
     int32_t meta[4] = {c->surr.train_depth, c->surr.train_policy, c->surr.train_modules, 0};
     int64_t n = (int64_t)w.size();
     bool ok = fwrite(magic, 1, 8, f) == 8 && fwrite(meta, 4, 4, f) == 4 && fwrite(&n, 8, 1, f) == 1 &&
@@ -493,7 +493,7 @@ gpet_status gpet_surrogate_load(gpet_ctx *c, const char *path)
     char magic[8];
     int32_t meta[4];
     int64_t n = 0;
-    bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, "GPETSUR1", 8) == 0 && fread(meta, 4, 4, f) == 4 &&
+    bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, "GPETSUR2", 8) == 0 && fread(meta, 4, 4, f) == 4 &&
               fread(&n, 8, 1, f) == 1 && n == n_params();
     std::vector<float> w(ok ? (size_t)n : 0);
     ok = ok && fread(w.data(), 4, w.size(), f) == w.size();
@@ -559,10 +559,13 @@ gpet_status gpet_surrogate_grad(gpet_ctx *c, const float *x, const float *y, int
         for (int k = 0; k < SURR_LDX; ++k) XT[(size_t)(k * S.ldt + i)] = X[(size_t)(i * SURR_LDX + k)];
     HIPOK(c, hipMemcpy(S.X.p, X.data(), sizeof(uint16_t) * X.size(), hipMemcpyHostToDevice));
     HIPOK(c, hipMemcpy(S.XT.p, XT.data(), sizeof(uint16_t) * XT.size(), hipMemcpyHostToDevice));
-    HIPOK(c, hipMemcpy(S.Y.p, y, sizeof(float) * (size_t)(n * SURR_NY), hipMemcpyHostToDevice));
+    /* the loss takes the energy class in place of the deposited fraction */
+    std::vector<float> Y(y, y + n * SURR_NY);
+    for (int64_t i = 0; i < n; ++i) Y[(size_t)(i * SURR_NY + 1)] = (float)surr_class(x[i * SURR_NX], y[i * SURR_NY + 1]);
+    HIPOK(c, hipMemcpy(S.Y.p, Y.data(), sizeof(float) * Y.size(), hipMemcpyHostToDevice));
     if ((st = forward(c, n, true))) return st;
     const int nblk = (int)((n + 255) / 256);
-    KOK(c, gpet_mdn_loss_grad(S.HT.p, S.ldt, S.Y.p, n, 1.0f / (float)n, S.dH.p, HEAD_LD, S.dHT.p, S.lpart.p,
+    KOK(c, gpet_head_loss_grad(S.HT.p, S.ldt, S.Y.p, n, 1.0f / (float)n, S.dH.p, HEAD_LD, S.dHT.p, S.lpart.p,
                               c->stream),
         "loss");
     DBuf<float> l1;

@@ -1,13 +1,15 @@
 /*
  * mlp.hip - kernels of the detector-response surrogate: a BF16 MFMA GEMM
- * with FP32 accumulation and fused epilogues, the mixture-density loss and
- * its gradient, Adam, and the inference-side input packing and sampling.
+ * with FP32 accumulation and fused epilogues, the head's loss and its
+ * gradient, Adam, and the inference-side input packing and sampling.
  *
- * The network (6 -> 256 -> 256 -> 256 -> 256 -> 161) maps a photon's panel
- * entry (energy, local direction, entry point) to P(event) and a 16-component
- * diagonal Gaussian mixture over (deposited-energy fraction, depth of the
- * first deposit, its lateral offsets from the straight entry ray), each
- * component with its own full-absorption probability.  It stands
+ * The network (6 -> 256 -> 256 -> 256 -> 256 -> 190) maps a photon's panel
+ * entry (energy, local direction, entry point) to P(event), a histogram of
+ * the deposited-energy fraction over 61 classes (full absorption, 48 bins
+ * up to the entry's Compton edge, 12 above it) and, for each of the two
+ * energy types (full absorption, continuum), a 16-component diagonal
+ * Gaussian mixture over the depth of the first deposit and its lateral
+ * offsets from the straight entry ray.  It stands
  * in for the panel transport of photonde (gPET_kernals.cu:1011-1193) plus the
  * adder/readout (:737-813) inside its validity range; nothing in the reference
  * corresponds to the network itself.
@@ -301,77 +303,77 @@ __global__ void k_bias_columns(uint16_t *a, int64_t M, int64_t ld, int c0)
     a[r * ld + c0 + c] = c == 0 ? (uint16_t)0x3f80 : (uint16_t)0;
 }
 
-/* ---- mixture-density head ------------------------------------------------
- * h[0]            event logit
- * h[1 .. 16]      component logits
- * h[17 + 4k + d]  mean of component k, target d
- * h[81 + 4k + d]  log sigma, clamped to [SURR_LOGSIG_MIN, SURR_LOGSIG_MAX]
- * h[145 + k]      photopeak logit of component k
- * Targets y: {event, fE, depth, dy, dz}.  The deposited-energy fraction has
- * a point mass at 1 (full absorption, ~70% of events): component k gives a
- * photopeak with probability q_k = sigmoid(h[145 + k]), and a continuum
- * fraction from its Gaussian otherwise, so an event's likelihood under k
- * is  q_k N3(depth, dy, dz)  if fE >= SURR_PP_MIN, else
- * (1 - q_k) N(fE) N3(depth, dy, dz). */
+/* ---- head: energy classes and a position mixture -------------------------
+ * Layout in surr_common.h.  Training targets Y (one row of SURR_NY floats):
+ * {event, energy class (surr_class of the entry energy and the deposited
+ * fraction), depth, dy, dz}.  An event's likelihood is
+ *   softmax(h[SURR_CLS ..])[class]
+ *   * sum_k pi_k N(depth) N(dy) N(dz | mean and sigma of component k)
+ * with pi the softmax of the full-absorption weight logits for class 0 and
+ * of the continuum ones otherwise, so the deposited energy comes from a
+ * histogram whose low bins follow the Compton edge of each entry energy
+ * (a Gaussian mixture smears that edge and the full-absorption point mass),
+ * and the first-deposit point from the mixture of its energy type. */
 
 __device__ __forceinline__ float softplus(float x) { return x > 0.0f ? x + gp_logf(1.0f + gp_expf(-x)) : gp_logf(1.0f + gp_expf(x)); }
 
 /* Per-sample loss and gradient dL/dh / n_total (bf16, cols >= SURR_NH zero)
  * from the head outputs transposed (HT: [output][ldt], a wave reads 64
- * consecutive rows of one output); one thread per row.  The row's gradient
- * is built in registers (LOSS_LD bf16 packed in pairs) and leaves as dH
- * (row-major, 16-byte stores) and dH^T ([output][ldt], a wave's stores
- * cover 64 consecutive rows).  Loss partial sums per block to
- * part[blockIdx.x] (fixed order). */
+ * consecutive rows of one output); one thread per row.  Each gradient
+ * value is stored as it is computed into dH^T ([output][ldt], a wave's
+ * stores cover 64 consecutive rows), so no row of gradients is held in
+ * registers; k_rows_from_cols then writes the row-major dH.  Loss partial
+ * sums per block to part[blockIdx.x] (fixed order). */
 constexpr int LOSS_LD = 192;
-struct GradRow {
-    uint32_t w[LOSS_LD / 2];
-    __device__ __forceinline__ void set(int c, uint16_t v)
-    {
-        w[c >> 1] = (c & 1) ? ((w[c >> 1] & 0x0000ffffu) | ((uint32_t)v << 16)) : ((w[c >> 1] & 0xffff0000u) | v);
-    }
-};
-__global__ __launch_bounds__(256) void k_mdn_loss_grad(const float *HT, int64_t ldt, const float *Y, int64_t M,
-                                                       float inv_n, uint16_t *dH, uint16_t *dHT, float *part)
+__global__ __launch_bounds__(256) void k_head_loss_grad(const float *HT, int64_t ldt, const float *Y, int64_t M,
+                                                        float inv_n, uint16_t *dHT, float *part)
 {
     __shared__ float red[256];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     float loss = 0.0f;
-    GradRow g;
-#pragma unroll
-    for (int c = 0; c < LOSS_LD / 2; ++c) g.w[c] = 0u;
     if (i < M) {
         auto hv = [&](int c) -> float { return HT[(int64_t)c * ldt + i]; };
+        auto put = [&](int c, float v) { dHT[(int64_t)c * ldt + i] = f2bf(v * inv_n); };
         const float *y = Y + i * SURR_NY;
         const float ev = y[0];
         const float h0 = hv(0);
         const float p = 1.0f / (1.0f + gp_expf(-h0));
         loss = ev > 0.5f ? softplus(-h0) : softplus(h0);
-        g.set(0, f2bf((p - ev) * inv_n));
-        float lw[SURR_K], lp[SURR_K];
-        float mx = -1e30f, mw = -1e30f;
-#pragma unroll
-        for (int k = 0; k < SURR_K; ++k) {
-            lw[k] = hv(1 + k);
-            mw = fmaxf(mw, lw[k]);
-        }
-        float sw = 0.0f;
-#pragma unroll
-        for (int k = 0; k < SURR_K; ++k) sw += gp_expf(lw[k] - mw);
-        const float lse_w = mw + gp_logf(sw);
+        put(0, p - ev);
         if (ev > 0.5f) {
-            const bool pp = y[1] >= SURR_PP_MIN;
+            /* energy class: softmax cross-entropy */
+            const int cls = (int)y[1];
+            float mc = -1e30f;
+            for (int c = 0; c < SURR_NC; ++c) mc = fmaxf(mc, hv(SURR_CLS + c));
+            float sc = 0.0f;
+            for (int c = 0; c < SURR_NC; ++c) sc += gp_expf(hv(SURR_CLS + c) - mc);
+            const float lse_c = mc + gp_logf(sc);
+            loss += lse_c - hv(SURR_CLS + cls);
+            for (int c = 0; c < SURR_NC; ++c)
+                put(SURR_CLS + c, gp_expf(hv(SURR_CLS + c) - lse_c) - (c == cls ? 1.0f : 0.0f));
+            /* first-deposit point: the mixture of the class's type; the
+             * other type's weights get no gradient */
+            const int wo = cls == 0 ? SURR_WPP : SURR_WCT;
+            const int wz = cls == 0 ? SURR_WCT : SURR_WPP;
+            float lw[SURR_K], lp[SURR_K];
+            float mw = -1e30f, mx = -1e30f;
 #pragma unroll
             for (int k = 0; k < SURR_K; ++k) {
-                const float hq = hv(SURR_PP + k);
-                /* log q_k or log (1 - q_k) */
-                float s = lw[k] - lse_w - (pp ? softplus(-hq) : softplus(hq));
+                lw[k] = hv(wo + k);
+                mw = fmaxf(mw, lw[k]);
+            }
+            float sw = 0.0f;
+#pragma unroll
+            for (int k = 0; k < SURR_K; ++k) sw += gp_expf(lw[k] - mw);
+            const float lse_w = mw + gp_logf(sw);
+#pragma unroll
+            for (int k = 0; k < SURR_K; ++k) {
+                float s = lw[k] - lse_w;
 #pragma unroll
                 for (int d = 0; d < SURR_D; ++d) {
-                    if (d == 0 && pp) continue;
-                    float mu = hv(17 + 4 * k + d);
-                    float ls = fminf(fmaxf(hv(81 + 4 * k + d), SURR_LOGSIG_MIN), SURR_LOGSIG_MAX);
-                    float zz = (y[1 + d] - mu) * gp_expf(-ls);
+                    const float mu = hv(SURR_MU + SURR_D * k + d);
+                    const float ls = fminf(fmaxf(hv(SURR_LS + SURR_D * k + d), SURR_LOGSIG_MIN), SURR_LOGSIG_MAX);
+                    const float zz = (y[2 + d] - mu) * gp_expf(-ls);
                     s += -0.5f * zz * zz - ls - 0.918938533f;
                 }
                 lp[k] = s;
@@ -382,36 +384,27 @@ __global__ __launch_bounds__(256) void k_mdn_loss_grad(const float *HT, int64_t 
             for (int k = 0; k < SURR_K; ++k) se += gp_expf(lp[k] - mx);
             const float lse = mx + gp_logf(se);
             loss += -lse;
-#pragma unroll
             for (int k = 0; k < SURR_K; ++k) {
                 const float gam = gp_expf(lp[k] - lse);
-                const float pik = gp_expf(lw[k] - lse_w);
-                g.set(1 + k, f2bf((pik - gam) * inv_n));
-                const float qk = 1.0f / (1.0f + gp_expf(-hv(SURR_PP + k)));
-                g.set(SURR_PP + k, f2bf((pp ? -gam * (1.0f - qk) : gam * qk) * inv_n));
+                put(wo + k, gp_expf(lw[k] - lse_w) - gam);
+                put(wz + k, 0.0f);
 #pragma unroll
                 for (int d = 0; d < SURR_D; ++d) {
-                    if (d == 0 && pp) {
-                        g.set(17 + 4 * k, 0);
-                        g.set(81 + 4 * k, 0);
-                        continue;
-                    }
-                    float mu = hv(17 + 4 * k + d);
-                    float raw = hv(81 + 4 * k + d);
-                    float ls = fminf(fmaxf(raw, SURR_LOGSIG_MIN), SURR_LOGSIG_MAX);
-                    float is = gp_expf(-ls);
-                    float zz = (y[1 + d] - mu) * is;
-                    g.set(17 + 4 * k + d, f2bf(-gam * zz * is * inv_n));
-                    bool free_ = raw > SURR_LOGSIG_MIN && raw < SURR_LOGSIG_MAX;
-                    g.set(81 + 4 * k + d, f2bf(free_ ? gam * (1.0f - zz * zz) * inv_n : 0.0f));
+                    const float mu = hv(SURR_MU + SURR_D * k + d);
+                    const float raw = hv(SURR_LS + SURR_D * k + d);
+                    const float ls = fminf(fmaxf(raw, SURR_LOGSIG_MIN), SURR_LOGSIG_MAX);
+                    const float is = gp_expf(-ls);
+                    const float zz = (y[2 + d] - mu) * is;
+                    put(SURR_MU + SURR_D * k + d, -gam * zz * is);
+                    const bool free_ = raw > SURR_LOGSIG_MIN && raw < SURR_LOGSIG_MAX;
+                    put(SURR_LS + SURR_D * k + d, free_ ? gam * (1.0f - zz * zz) : 0.0f);
                 }
             }
-        }   /* no event: only the event logit's gradient (the row starts at zero) */
-        uint4 *row = reinterpret_cast<uint4 *>(dH + i * LOSS_LD);
-#pragma unroll
-        for (int v = 0; v < LOSS_LD / 8; ++v) row[v] = make_uint4(g.w[4 * v], g.w[4 * v + 1], g.w[4 * v + 2], g.w[4 * v + 3]);
-#pragma unroll
-        for (int c = 0; c < LOSS_LD; ++c) dHT[(int64_t)c * ldt + i] = (uint16_t)(g.w[c >> 1] >> ((c & 1) * 16));
+        } else {
+            /* no event: only the event logit has a gradient */
+            for (int c = 1; c < SURR_NH; ++c) put(c, 0.0f);
+        }
+        for (int c = SURR_NH; c < LOSS_LD; ++c) put(c, 0.0f);
     }
     red[threadIdx.x] = loss;
     __syncthreads();
@@ -420,6 +413,28 @@ __global__ __launch_bounds__(256) void k_mdn_loss_grad(const float *HT, int64_t 
         __syncthreads();
     }
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+/* dH [M x LOSS_LD] (row-major, 16-byte stores) from dH^T [LOSS_LD x ldt]:
+ * 64 rows per block through LDS. */
+__global__ __launch_bounds__(256) void k_rows_from_cols(const uint16_t *in, int64_t ldt, int64_t M, uint16_t *out)
+{
+    __shared__ uint32_t t[64][LOSS_LD / 2 + 1];
+    const int64_t i0 = (int64_t)blockIdx.x * 64;
+    const int r = (int)threadIdx.x & 63;
+    const bool ok = i0 + r < M;
+    for (int c2 = (int)threadIdx.x >> 6; c2 < LOSS_LD / 2; c2 += 4) {
+        const uint32_t lo = ok ? in[(int64_t)(2 * c2) * ldt + i0 + r] : 0u;
+        const uint32_t hi = ok ? in[(int64_t)(2 * c2 + 1) * ldt + i0 + r] : 0u;
+        t[r][c2] = lo | (hi << 16);
+    }
+    __syncthreads();
+    for (int e = (int)threadIdx.x; e < 64 * (LOSS_LD / 8); e += 256) {
+        const int rr = e / (LOSS_LD / 8), v = e % (LOSS_LD / 8);
+        if (i0 + rr < M)
+            *reinterpret_cast<uint4 *>(out + (i0 + rr) * LOSS_LD + 8 * v) =
+                make_uint4(t[rr][4 * v], t[rr][4 * v + 1], t[rr][4 * v + 2], t[rr][4 * v + 3]);
+    }
 }
 
 /* Squared norm of g: per-block partial sums (fixed order), then one block
@@ -439,13 +454,21 @@ __global__ __launch_bounds__(256) void k_sqnorm_part(const float *g, int64_t n, 
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-__global__ void k_clip_factor(const float *part, int nb, float clip, float *factor)
+__global__ __launch_bounds__(256) void k_clip_factor(const float *part, int nb, float clip, float *factor)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ float red[256];
     float s = 0.0f;
-    for (int i = 0; i < nb; ++i) s += part[i];
-    const float nrm = sqrtf(s);
-    factor[0] = nrm > clip ? clip / nrm : 1.0f;
+    for (int i = (int)threadIdx.x; i < nb; i += 256) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const float nrm = sqrtf(red[0]);
+        factor[0] = nrm > clip ? clip / nrm : 1.0f;
+    }
 }
 
 /* Adam on fp32 master weights (gradient scaled by *gscale); refreshes the
@@ -482,7 +505,8 @@ __global__ void k_loss_reduce(const float *part, int n, float *out)
 
 /* Training minibatch: M rows drawn with replacement from the n harvested
  * records (Philox stream (step, 14), one draw per row) -> input matrix
- * (bf16 [M x 8]: 6 features, 1, 0) and targets (fp32 [M x SURR_NY]). */
+ * (bf16 [M x 8]: 6 features, 1, 0) and targets (fp32 [M x SURR_NY], the
+ * deposited fraction replaced by its energy class). */
 __global__ void k_pack_train(const float *set, int64_t n, uint64_t seed, int64_t step, int64_t M, uint16_t *X,
                              float *Y, uint16_t *XT, int64_t ldt)
 {
@@ -501,6 +525,7 @@ __global__ void k_pack_train(const float *set, int64_t n, uint64_t seed, int64_t
     }
 #pragma unroll
     for (int c = 0; c < SURR_NY; ++c) Y[i * SURR_NY + c] = r[SURR_NX + c];
+    Y[i * SURR_NY + 1] = (float)surr_class(r[0], r[SURR_NX + 1]);
 }
 
 /* Elementary functions of the sampling: the hardware approximations
@@ -513,13 +538,17 @@ __device__ __forceinline__ float s_log(float x) { return __builtin_amdgcn_logf(x
 __device__ __forceinline__ float s_sigmoid(float x) { return __builtin_amdgcn_rcpf(1.0f + s_exp(-x)); }
 
 /* Sampling from the head outputs h of surrogate-queue row j: P(event) on
- * the photon's own stream, a component by its weight, four normals; the
- * event at the first deposit point, with the readout site of the digitizer
- * settings (one event per photon).  Returns whether the photon gives an
- * event (ev filled). */
-template <typename T>
-__device__ __forceinline__ bool d_surr_sample_row(const T *h, const DevQueue &q, int64_t j, const DevDetector &det,
-                                                  const PanelParams &pp, gpet_event &ev)
+ * the photon's own stream, an energy class by its weight and a fraction
+ * uniform in its bin, a position component by its weight among those of
+ * the class's type, three normals; the event at the first deposit point,
+ * with the readout site of the digitizer settings (one event per photon).
+ * Returns whether the photon gives an event (ev filled). */
+typedef _Float16 h16x4_t __attribute__((ext_vector_type(4)));
+static_assert(SURR_CLS + SURR_NC <= 64, "event and class logits in the first 64 outputs");
+static_assert((SURR_WPP & 3) == (SURR_WCT & 3) && (SURR_WCT & ~3) + 20 <= SURR_NH,
+              "both weight sets at the same offset within five 8-byte reads of the row");
+__device__ __forceinline__ bool d_surr_sample_row(const _Float16 *h, const DevQueue &q, int64_t j,
+                                                  const DevDetector &det, const PanelParams &pp, gpet_event &ev)
 {
     const int4 info = q.info[j];
     const DevPanel &P = det.p[info.x];
@@ -528,13 +557,53 @@ __device__ __forceinline__ bool d_surr_sample_row(const T *h, const DevQueue &q,
     gp_rng r;
     gp_rng_init(&r, pp.seed, (uint64_t)(int64_t)info.z, GP_TAG_PHOTON + k, draws);
     gp_u4 b = gp_rng_block(&r);
-    const float pe = s_sigmoid((float)h[0]);
+    /* outputs 0 .. 63 (event and class logits) in registers: 16 8-byte LDS
+     * reads (the row starts 8-byte aligned) */
+    h16x4_t hv[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) hv[e] = *reinterpret_cast<const h16x4_t *>(h + 4 * e);
+    auto out = [&](int c) -> float { return (float)hv[c >> 2][c & 3]; };
+    const float pe = s_sigmoid(out(0));
     if (!(gp_u01(b.x) <= pe)) return false;
+    /* energy class: the first whose running weight reaches u (the last if
+     * rounding leaves u above the total) */
+    float mc = -1e30f;
+#pragma unroll
+    for (int c = 0; c < SURR_NC; ++c) mc = fmaxf(mc, out(SURR_CLS + c));
+    float ew[SURR_NC];
+    float sc = 0.0f;
+#pragma unroll
+    for (int c = 0; c < SURR_NC; ++c) {
+        ew[c] = s_exp(out(SURR_CLS + c) - mc);
+        sc += ew[c];
+    }
+    const float uc = gp_u01(b.y) * sc;
+    float acc = 0.0f;
+    int cls = SURR_NC - 1;
+    bool found = false;
+#pragma unroll
+    for (int c = 0; c < SURR_NC; ++c) {
+        acc += ew[c];
+        if (!found && acc >= uc) {
+            cls = c;
+            found = true;
+        }
+    }
+    const float E = q.E[j];
+    const float fe = surr_class_fe(E * (1.0f / 511.0e3f), cls, gp_u01(b.z));
+    /* position component among those of the class's type: its 16 weight
+     * logits from five 8-byte LDS reads at the 4-aligned output below them */
+    const int wo = cls == 0 ? SURR_WPP : SURR_WCT;
+    const int wa = wo & ~3;
+    constexpr int ws = SURR_WPP & 3;
+    h16x4_t wv[5];
+#pragma unroll
+    for (int e = 0; e < 5; ++e) wv[e] = *reinterpret_cast<const h16x4_t *>(h + wa + 4 * e);
     float lw[SURR_K];
     float mw = -1e30f;
 #pragma unroll
     for (int c = 0; c < SURR_K; ++c) {
-        lw[c] = (float)h[1 + c];
+        lw[c] = (float)wv[(c + ws) >> 2][(c + ws) & 3];
         mw = fmaxf(mw, lw[c]);
     }
     float sw = 0.0f;
@@ -543,12 +612,10 @@ __device__ __forceinline__ bool d_surr_sample_row(const T *h, const DevQueue &q,
         lw[c] = s_exp(lw[c] - mw);
         sw += lw[c];
     }
-    /* the first component whose running weight reaches u (the last if
-     * rounding leaves u above the total) */
-    const float u = gp_u01(b.y) * sw;
-    float acc = 0.0f;
+    const float u = gp_u01(b.w) * sw;
+    acc = 0.0f;
     int kc = SURR_K - 1;
-    bool found = false;
+    found = false;
 #pragma unroll
     for (int c = 0; c < SURR_K; ++c) {
         acc += lw[c];
@@ -557,33 +624,28 @@ __device__ __forceinline__ bool d_surr_sample_row(const T *h, const DevQueue &q,
             found = true;
         }
     }
-    /* full absorption with the component's photopeak probability */
-    const bool peak = gp_u01(b.z) <= s_sigmoid((float)h[SURR_PP + kc]);
-    /* the component's Gaussian restricted to the region the training data
-     * cover (continuum fraction in (0, SURR_PP_MIN), depth in [0, 1]): up to
-     * 8 draws of four normals, then the last one clamped */
-    float y[4];
+    /* the component's Gaussian restricted to the depths the training data
+     * cover ([0, 1]): up to 8 draws of three normals, then the last one
+     * (its depth clamped) */
+    float y[SURR_D];
     for (int tries = 0; tries < 8; ++tries) {
         gp_u4 b2 = gp_rng_block(&r);
         float z[4];
         const float a0 = gp_u01(b2.y), a1 = gp_u01(b2.w);
         const float s0 = __builtin_amdgcn_sinf(a0), c0 = __builtin_amdgcn_cosf(a0);
-        const float s1 = __builtin_amdgcn_sinf(a1), c1 = __builtin_amdgcn_cosf(a1);
+        const float c1 = __builtin_amdgcn_cosf(a1);
         const float r0 = __builtin_amdgcn_sqrtf(-2.0f * s_log(gp_u01(b2.x)));
         const float r1 = __builtin_amdgcn_sqrtf(-2.0f * s_log(gp_u01(b2.z)));
-        z[0] = r0 * c0; z[1] = r0 * s0; z[2] = r1 * c1; z[3] = r1 * s1;
+        z[0] = r0 * c0; z[1] = r0 * s0; z[2] = r1 * c1;
 #pragma unroll
-        for (int d = 0; d < 4; ++d) {
-            float ls = fminf(fmaxf((float)h[81 + 4 * kc + d], SURR_LOGSIG_MIN), SURR_LOGSIG_MAX);
-            y[d] = (float)h[17 + 4 * kc + d] + s_exp(ls) * z[d];
+        for (int d = 0; d < SURR_D; ++d) {
+            const float ls = fminf(fmaxf((float)h[SURR_LS + SURR_D * kc + d], SURR_LOGSIG_MIN), SURR_LOGSIG_MAX);
+            y[d] = (float)h[SURR_MU + SURR_D * kc + d] + s_exp(ls) * z[d];
         }
-        const bool fe_ok = peak || (y[0] > 0.0f && y[0] < SURR_PP_MIN);
-        if (fe_ok && y[1] >= 0.0f && y[1] <= 1.0f) break;
+        if (y[0] >= 0.0f && y[0] <= 1.0f) break;
     }
     float lx, ly, lz;
-    const float E = q.E[j];
-    const float fe = peak ? 1.0f : fminf(y[0], SURR_PP_MIN);
-    bool has = fe > 0.0f && surr_position(P, q.vx[j], q.vy[j], q.vz[j], q.y[j], q.z[j], y[1], y[2], y[3], lx, ly, lz);
+    bool has = fe > 0.0f && surr_position(P, q.vx[j], q.vy[j], q.vz[j], q.y[j], q.z[j], y[0], y[1], y[2], lx, ly, lz);
     int M_id = -1, L_id = -1;
     if (has) has = d_crystal(det, P, lx, ly, lz, M_id, L_id) == 0;
     if (!has) return false;
@@ -632,7 +694,7 @@ __device__ __forceinline__ void d_append_event(const PanelParams &pp, bool has, 
  * the bf16 rounding happen in registers and every 4 outputs go to LDS in
  * one 8-byte write.  Hidden layers: wave w computes neurons [64w, 64w + 64)
  * of all 128 rows (2 x 4 tiles, 128 accumulators).  Head: wave w computes
- * outputs [48w, 48w + 48) of the 192 stored (161 used) for all 128 rows on
+ * outputs [48w, 48w + 48) of the 192 stored (190 used) for all 128 rows on
  * v_mfma_f32_16x16x32_bf16, so each head weight is loaded once per tile;
  * the outputs replace the activations in LDS, all rows at once as fp16
  * when the rows are sampled (one lane per row on two waves), or half the
@@ -646,7 +708,8 @@ constexpr int ILD = 264;
 constexpr int IHO = 3;            /* head: 16-output tiles per wave (4 x 48 = 192 >= SURR_NH) */
 constexpr int IPD = 3;            /* hidden layers: weight loads this many steps ahead */
 constexpr int IHPD = 2;           /* head: the same */
-constexpr int IHLD = 164;         /* head row in LDS (fp32): >= SURR_NH, 16-byte rows */
+constexpr int IHLD = 196;         /* head row in LDS: >= SURR_NH, 16-byte rows (fp32), 4 banks apart */
+static_assert(IHLD >= SURR_NH && IHLD % 4 == 0 && 4 * 48 >= SURR_NH, "head tiles and LDS rows");
 /* LDS of a block of rg row groups: the activations, which the head outputs
  * of half of each group's rows at a time replace */
 constexpr int infer_lds(int rg)
@@ -946,12 +1009,15 @@ int gpet_bias_columns(uint16_t *a, int64_t M, int64_t ld, int c0, hipStream_t s)
     return (int)hipGetLastError();
 }
 
-int gpet_mdn_loss_grad(const float *HT, int64_t ldt, const float *Y, int64_t M, float inv_n, uint16_t *dH,
+int gpet_head_loss_grad(const float *HT, int64_t ldt, const float *Y, int64_t M, float inv_n, uint16_t *dH,
                        int64_t lddh, uint16_t *dHT, float *part, hipStream_t s)
 {
     if (lddh != LOSS_LD) return (int)hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_mdn_loss_grad, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, HT, ldt, Y, M, inv_n, dH,
+    hipLaunchKernelGGL(k_head_loss_grad, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, HT, ldt, Y, M, inv_n,
                        dHT, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL(k_rows_from_cols, dim3((unsigned)((M + 63) / 64)), dim3(256), 0, s, dHT, ldt, M, dH);
     return (int)hipGetLastError();
 }
 
@@ -973,7 +1039,7 @@ int gpet_clip_factor(const float *g, int64_t n, float clip, float *part, float *
 {
     const int nb = 240;
     hipLaunchKernelGGL(k_sqnorm_part, dim3(nb), dim3(256), 0, s, g, n, part);
-    hipLaunchKernelGGL(k_clip_factor, dim3(1), dim3(64), 0, s, part, nb, clip, factor);
+    hipLaunchKernelGGL(k_clip_factor, dim3(1), dim3(256), 0, s, part, nb, clip, factor);
     return (int)hipGetLastError();
 }
 

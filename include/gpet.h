@@ -499,11 +499,12 @@ gpet_status gpet_run_batch(gpet_ctx *ctx, const gpet_job *jobs, int32_t n_jobs,
                            const gpet_isotope *iso, int32_t n_iso, gpet_sink *sink);
 
 /* ---- detector-response surrogate (new; no counterpart in the reference) --
- * A 6 -> 256 -> 256 -> 256 -> 256 -> 161 MLP (SiLU, BF16 weights, FP32
+ * A 6 -> 256 -> 256 -> 256 -> 256 -> 190 MLP (SiLU, BF16 weights, FP32
  * accumulation) maps a photon's panel entry (energy, local direction, entry
- * point) to P(event) and a 16-component Gaussian mixture over the deposited
- * energy fraction, the depth of the first deposit and its lateral offsets,
- * each component with its own probability of a full absorption.
+ * point) to P(event), a 61-class histogram of the deposited energy fraction
+ * (full absorption; 48 bins below the entry's Compton edge, 12 above) and,
+ * per energy type (full absorption or not), a 16-component Gaussian mixture
+ * over the depth of the first deposit and its lateral offsets.
  * It replaces photonde's panel transport, adder and readout
  * (gPET_kernals.cu:1011-1193, 737-813) for entries of 100-520 keV within 60
  * degrees of the panel normal; other entries take the plain transport.  It
@@ -541,7 +542,7 @@ gpet_status gpet_surrogate_load(gpet_ctx *ctx, const char *path);
  * with the bias in column `inputs`); *n receives its length. */
 gpet_status gpet_surrogate_get_params(gpet_ctx *ctx, float *out, int64_t cap, int64_t *n);
 gpet_status gpet_surrogate_set_params(gpet_ctx *ctx, const float *w, int64_t n);
-/* The network on caller data: x is n x 6 features, h receives n x 161 head
+/* The network on caller data: x is n x 6 features, h receives n x 190 head
  * outputs; grad also takes n x 5 targets and returns the mean loss and the
  * parameter gradient (length as get_params). */
 gpet_status gpet_surrogate_forward(gpet_ctx *ctx, const float *x, int64_t n, float *h);

@@ -1964,22 +1964,26 @@ void gpet_cpu_stream(uint64_t seed, uint64_t id, uint32_t aux, int32_t kind, int
 
 /* ======================= surrogate network (CPU restatement) =============
  * The detector-response MLP of the HIP library restated in FP64 for the
- * gradient check: 6 -> 256 -> 256 -> 256 -> 256 -> 161, SiLU, weights
+ * gradient check: 6 -> 256 -> 256 -> 256 -> 256 -> 190, SiLU, weights
  * [outputs x (inputs + 8)] with the bias in column `inputs`, head stored
  * with 192 rows.  By default it rounds what the GPU path stores in BF16
  * (weights, inputs, activations, dL/dh, dL/dz) to BF16 and accumulates in
  * FP64; exact mode (gpet_cpu_surrogate_set_exact) rounds nothing, for
  * finite-difference checks.  Loss: BCE on the event logit plus, for events,
- * the negative log-likelihood of the 16-component diagonal Gaussian mixture
- * over the 4 targets (log sigma clamped to [-5, 2]), averaged over rows;
- * component k gives a full absorption (fE >= 0.999, whose fE term is then
- * left out) with probability sigmoid(h[145 + k]) and a continuum fE from its
- * Gaussian otherwise. */
+ * the softmax cross-entropy of the deposited-energy class (h[1 .. 61]: full
+ * absorption for fE >= 0.999, else 48 bins of fE over [0, edge) and 12 of
+ * [edge, 0.999), edge = 2a / (1 + 2a) the Compton edge of an entry of
+ * a = E / 511 keV, the first feature) and the negative log-likelihood of
+ * the 16-component diagonal Gaussian mixture over (depth, dy, dz), whose
+ * weight logits are h[62 + k] for a full absorption and h[78 + k] otherwise
+ * (means h[94 + 3k + d], log sigmas h[142 + 3k + d] clamped to [-5, 2]),
+ * averaged over rows. */
 namespace {
 
-constexpr int M_NL = 5, M_HID = 256, M_NH = 161, M_K = 16, M_D = 4, M_NX = 6, M_NY = 5;
-constexpr int M_PP = 145;           /* per-component photopeak logits h[145 + k] */
-constexpr double M_PP_MIN = 0.999;  /* fE >= this: full absorption                */
+constexpr int M_NL = 5, M_HID = 256, M_NH = 190, M_K = 16, M_D = 3, M_NX = 6, M_NY = 5;
+constexpr int M_NLO = 48, M_NHI = 12, M_NC = 1 + M_NLO + M_NHI;
+constexpr int M_CLS = 1, M_WPP = 62, M_WCT = 78, M_MU = 94, M_LS = 142;
+constexpr float M_PP_MIN = 0.999f;  /* fE >= this: full absorption                */
 constexpr int M_LN[M_NL] = {256, 256, 256, 256, 192};
 constexpr int M_LK[M_NL] = {6, 256, 256, 256, 256};
 constexpr double M_LSMIN = -5.0, M_LSMAX = 2.0;
@@ -2038,28 +2042,45 @@ void mlp_forward_row(const std::vector<double> &W, const float *x, int exact, Ml
 
 double softplus_d(double x) { return x > 0 ? x + std::log1p(std::exp(-x)) : std::log1p(std::exp(x)); }
 
+/* Energy class of a deposited fraction fe for an entry of e511 = E / 511
+ * keV (float arithmetic, as the library computes it). */
+int m_class(float e511, float fe)
+{
+    if (fe >= M_PP_MIN) return 0;
+    const float a2 = 2.0f * e511;
+    const float ed = a2 / (1.0f + a2);
+    if (fe < ed) return 1 + std::min(std::max((int)(fe / ed * (float)M_NLO), 0), M_NLO - 1);
+    return 1 + M_NLO + std::min(std::max((int)((fe - ed) / (M_PP_MIN - ed) * (float)M_NHI), 0), M_NHI - 1);
+}
+
 /* loss of one row and dL/dh (unscaled) */
-double mlp_loss_row(const double *h, const float *y, double *g)
+double mlp_loss_row(const double *h, const float *x, const float *y, double *g)
 {
     const double ev = y[0];
     double loss = ev > 0.5 ? softplus_d(-h[0]) : softplus_d(h[0]);
     for (int c = 0; c < M_NH; ++c) g[c] = 0.0;
     g[0] = m_sig(h[0]) - ev;
     if (ev > 0.5) {
-        const bool pp = (double)y[1] >= M_PP_MIN;
+        const int cls = m_class(x[0], y[1]);
+        double mc = -1e300;
+        for (int c = 0; c < M_NC; ++c) mc = std::max(mc, h[M_CLS + c]);
+        double sc = 0.0;
+        for (int c = 0; c < M_NC; ++c) sc += std::exp(h[M_CLS + c] - mc);
+        const double lse_c = mc + std::log(sc);
+        loss += lse_c - h[M_CLS + cls];
+        for (int c = 0; c < M_NC; ++c) g[M_CLS + c] = std::exp(h[M_CLS + c] - lse_c) - (c == cls ? 1.0 : 0.0);
+        const int wo = cls == 0 ? M_WPP : M_WCT;
         double mw = -1e300;
-        for (int k = 0; k < M_K; ++k) mw = std::max(mw, h[1 + k]);
+        for (int k = 0; k < M_K; ++k) mw = std::max(mw, h[wo + k]);
         double sw = 0.0;
-        for (int k = 0; k < M_K; ++k) sw += std::exp(h[1 + k] - mw);
+        for (int k = 0; k < M_K; ++k) sw += std::exp(h[wo + k] - mw);
         const double lse_w = mw + std::log(sw);
         double lp[M_K], mx = -1e300;
         for (int k = 0; k < M_K; ++k) {
-            /* log q_k for a full absorption, log(1 - q_k) otherwise */
-            double s = h[1 + k] - lse_w - (pp ? softplus_d(-h[M_PP + k]) : softplus_d(h[M_PP + k]));
+            double s = h[wo + k] - lse_w;
             for (int d = 0; d < M_D; ++d) {
-                if (d == 0 && pp) continue;
-                double ls = std::min(std::max(h[81 + 4 * k + d], M_LSMIN), M_LSMAX);
-                double zz = (y[1 + d] - h[17 + 4 * k + d]) * std::exp(-ls);
+                double ls = std::min(std::max(h[M_LS + 3 * k + d], M_LSMIN), M_LSMAX);
+                double zz = (y[2 + d] - h[M_MU + 3 * k + d]) * std::exp(-ls);
                 s += -0.5 * zz * zz - ls - 0.91893853320467274;
             }
             lp[k] = s;
@@ -2070,18 +2091,15 @@ double mlp_loss_row(const double *h, const float *y, double *g)
         const double lse = mx + std::log(se);
         loss += -lse;
         for (int k = 0; k < M_K; ++k) {
-            const double gam = std::exp(lp[k] - lse), pik = std::exp(h[1 + k] - lse_w);
-            g[1 + k] = pik - gam;
-            const double qk = m_sig(h[M_PP + k]);
-            g[M_PP + k] = pp ? -gam * (1.0 - qk) : gam * qk;
+            const double gam = std::exp(lp[k] - lse), pik = std::exp(h[wo + k] - lse_w);
+            g[wo + k] = pik - gam;
             for (int d = 0; d < M_D; ++d) {
-                if (d == 0 && pp) continue;
-                const double raw = h[81 + 4 * k + d];
+                const double raw = h[M_LS + 3 * k + d];
                 const double ls = std::min(std::max(raw, M_LSMIN), M_LSMAX);
                 const double is = std::exp(-ls);
-                const double zz = (y[1 + d] - h[17 + 4 * k + d]) * is;
-                g[17 + 4 * k + d] = -gam * zz * is;
-                g[81 + 4 * k + d] = (raw > M_LSMIN && raw < M_LSMAX) ? gam * (1.0 - zz * zz) : 0.0;
+                const double zz = (y[2 + d] - h[M_MU + 3 * k + d]) * is;
+                g[M_MU + 3 * k + d] = -gam * zz * is;
+                g[M_LS + 3 * k + d] = (raw > M_LSMIN && raw < M_LSMAX) ? gam * (1.0 - zz * zz) : 0.0;
             }
         }
     }
@@ -2152,7 +2170,7 @@ gpet_status gpet_cpu_surrogate_grad_d(gpet_ctx *c, const float *x, const float *
         MlpRow R;
         mlp_forward_row(W, x + i * M_NX, ex, R);
         double gh[M_NH];
-        L += mlp_loss_row(R.h, y + i * M_NY, gh);
+        L += mlp_loss_row(R.h, x + i * M_NX, y + i * M_NY, gh);
         std::vector<double> dn(M_HID + 8, 0.0);
         for (int k = 0; k < M_NH; ++k) dn[k] = bfr(gh[k] / (double)n, ex);
         int nout = M_NH;
