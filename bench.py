@@ -477,7 +477,7 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-host-sink", action="store_true")
     ap.add_argument("--harvest-runs", type=int, default=7, help="plain runs of the workload harvested for training")
-    ap.add_argument("--train-steps", type=int, default=24000)
+    ap.add_argument("--train-steps", type=int, default=48000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: multi-GPU runs are not part of this build "

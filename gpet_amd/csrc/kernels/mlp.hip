@@ -624,13 +624,16 @@ __device__ __forceinline__ bool d_surr_sample_row(const _Float16 *h, const DevQu
             found = true;
         }
     }
-    /* the component's Gaussian restricted to the depths the training data
-     * cover ([0, 1]): up to 8 draws of three normals, then the last one
-     * (its depth clamped) */
-    float y[SURR_D];
-    for (int tries = 0; tries < 8; ++tries) {
+    /* the component's Gaussian restricted to where first deposits happen:
+     * a depth in [0, 1] and a point inside a crystal of the panel (a photon
+     * that crosses a gap or leaves through a side does not deposit there);
+     * up to 8 draws of three normals, else no event */
+    float lx = 0.0f, ly = 0.0f, lz = 0.0f;
+    int M_id = -1, L_id = -1;
+    bool has = false;
+    for (int tries = 0; tries < 8 && !has; ++tries) {
         gp_u4 b2 = gp_rng_block(&r);
-        float z[4];
+        float y[SURR_D], z[4];
         const float a0 = gp_u01(b2.y), a1 = gp_u01(b2.w);
         const float s0 = __builtin_amdgcn_sinf(a0), c0 = __builtin_amdgcn_cosf(a0);
         const float c1 = __builtin_amdgcn_cosf(a1);
@@ -642,13 +645,11 @@ __device__ __forceinline__ bool d_surr_sample_row(const _Float16 *h, const DevQu
             const float ls = fminf(fmaxf((float)h[SURR_LS + SURR_D * kc + d], SURR_LOGSIG_MIN), SURR_LOGSIG_MAX);
             y[d] = (float)h[SURR_MU + SURR_D * kc + d] + s_exp(ls) * z[d];
         }
-        if (y[0] >= 0.0f && y[0] <= 1.0f) break;
+        if (y[0] >= 0.0f && y[0] <= 1.0f &&
+            surr_position(P, q.vx[j], q.vy[j], q.vz[j], q.y[j], q.z[j], y[0], y[1], y[2], lx, ly, lz))
+            has = d_crystal(det, P, lx, ly, lz, M_id, L_id) == 0;
     }
-    float lx, ly, lz;
-    bool has = fe > 0.0f && surr_position(P, q.vx[j], q.vy[j], q.vz[j], q.y[j], q.z[j], y[0], y[1], y[2], lx, ly, lz);
-    int M_id = -1, L_id = -1;
-    if (has) has = d_crystal(det, P, lx, ly, lz, M_id, L_id) == 0;
-    if (!has) return false;
+    if (!(has && fe > 0.0f)) return false;
     const int pann = P.panel;
     float dx = lx, dy = ly - q.y[j], dz = lz - q.z[j];
     ev.parn = info.y;

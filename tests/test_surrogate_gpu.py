@@ -62,8 +62,8 @@ def test_gradient_matches_oracle(tables):
 
 
 # training as the benchmark does it: 7 plain runs of the cylinder workload
-# (1e8 decays each, ~1.1e8 in-range panel entries), 24000 Adam steps
-HARVEST_RUNS, HARVEST_DECAYS, TRAIN_STEPS = 7, int(1e8), 24000
+# (1e8 decays each, ~1.1e8 in-range panel entries), 48000 Adam steps
+HARVEST_RUNS, HARVEST_DECAYS, TRAIN_STEPS = 7, int(1e8), 48000
 
 
 @pytest.fixture(scope="module")
