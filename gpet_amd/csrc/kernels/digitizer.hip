@@ -460,6 +460,37 @@ __global__ __launch_bounds__(256) void k_tally(const gpet_event *s, int64_t n, i
     }
 }
 
+/* The same tallies with block-private counts in LDS (crystal bins first,
+ * then energy bins; 32-bit, a block sees fewer than 2^32 singles) added to
+ * the global counts once per block: a few hundred energy bins no longer
+ * take every single's atomic on one address. */
+constexpr int TALLY_LDS_BINS = 16000;
+__global__ __launch_bounds__(256) void k_tally_lds(const gpet_event *s, int64_t n, int moduleN, int crystalN,
+                                                   unsigned long long *crystal, int ncrystal,
+                                                   unsigned long long *ehist, int nebins, float elo, float inv)
+{
+    extern __shared__ unsigned tl_bins[];
+    const int nc = crystal ? ncrystal : 0, ne = ehist ? nebins : 0;
+    for (int b = (int)threadIdx.x; b < nc + ne; b += 256) tl_bins[b] = 0u;
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const gpet_event &e = s[i];
+        if (nc) {
+            const int64_t g = (int64_t)e.pann * moduleN * crystalN + (int64_t)e.modn * crystalN + e.cryn;
+            if (g >= 0 && g < nc) atomicAdd(&tl_bins[g], 1u);
+        }
+        if (ne) {
+            const float x = (e.E - elo) * inv;
+            if (x >= 0.0f && x < (float)ne) atomicAdd(&tl_bins[nc + (int)x], 1u);
+        }
+    }
+    __syncthreads();
+    for (int b = (int)threadIdx.x; b < nc + ne; b += 256) {
+        const unsigned v = tl_bins[b];
+        if (v) atomicAdd(b < nc ? &crystal[b] : &ehist[b - nc], (unsigned long long)v);
+    }
+}
+
 /* Sinogram of coincidence pairs: same rule as the oracle (crystal centres,
  * transaxial line as (phi, s), uniform bins). */
 __device__ __forceinline__ void d_crystal_local(const DevPanel &P, int modn, int cryn, int moduleNy, int crystalNy,
@@ -1059,7 +1090,14 @@ int gpet_digi_tallies(DigiWork *w, const gpet_event *singles, int64_t n, int mod
     (void)w;
     if (n <= 0) return 0;
     float inv = (nebins > 0 && ehi > elo) ? (float)nebins / (ehi - elo) : 0.0f;
-    hipLaunchKernelGGL(k_tally, dim3(grid_for(n, 256)), dim3(256), 0, s, singles, n, moduleN, crystalN, crystal, ncrystal,
-                       ehist, nebins, elo, inv);
+    const int64_t bins = (crystal ? (int64_t)ncrystal : 0) + (ehist ? (int64_t)nebins : 0);
+    if (bins > 0 && bins <= TALLY_LDS_BINS) {
+        const int64_t blocks = std::min<int64_t>(grid_for(n, 256), 2048);
+        hipLaunchKernelGGL(k_tally_lds, dim3((unsigned)blocks), dim3(256), (size_t)bins * sizeof(unsigned), s, singles,
+                           n, moduleN, crystalN, crystal, ncrystal, ehist, nebins, elo, inv);
+    } else {
+        hipLaunchKernelGGL(k_tally, dim3(grid_for(n, 256)), dim3(256), 0, s, singles, n, moduleN, crystalN, crystal,
+                           ncrystal, ehist, nebins, elo, inv);
+    }
     return (int)hipGetLastError();
 }
