@@ -39,22 +39,8 @@ __device__ __forceinline__ unsigned long long lanemask_lt()
     return (l == 0) ? 0ull : ((~0ull) >> (64 - l));
 }
 
-/* Wave-aggregated append: every active lane calls it; lanes with want get
- * consecutive slots. */
-__device__ __forceinline__ int wave_append(int *counter, bool want)
-{
-    unsigned long long m = __ballot(want);
-    int base = 0;
-    if (m) {
-        int leader = __ffsll((unsigned long long)m) - 1;
-        if ((int)__lane_id() == leader) base = atomicAdd(counter, (int)__popcll(m));
-        base = __shfl(base, leader);
-    }
-    return base + (int)__popcll(m & lanemask_lt());
-}
-
-/* The same for a 64-bit or unsigned counter, every calling lane appending
- * one item (one atomic per wave instead of one per lane). */
+/* Wave-aggregated append to a 64-bit or unsigned counter, every calling
+ * lane appending one item (one atomic per wave instead of one per lane). */
 template <class T>
 __device__ __forceinline__ T wave_append1(T *counter)
 {
@@ -1343,27 +1329,69 @@ void k_panel(PanelParams pp, DevTables tab, DevDetector det, DevQueue q)
     __syncthreads();
     const int nq = (int)s_pre[G];
     const int stride = gridDim.x * blockDim.x;
-    for (int gi = blockIdx.x * blockDim.x + threadIdx.x; gi < nq; gi += stride) {
-        int lo = 0, hi = G; /* largest r with s_pre[r] <= gi */
+    /* queue item of global index gi: the largest region r with s_pre[r] <= gi */
+    auto item_of = [&](int gi) -> int64_t {
+        int lo = 0, hi = G;
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
             if ((int)s_pre[mid] <= gi) lo = mid; else hi = mid;
         }
-        const int64_t i = (int64_t)lo * q.region + (gi - (int)s_pre[lo]);
-        if (MODE == PANEL_SURROGATE) {
-            const bool sv = surr_valid(q.E[i], q.vx[i]);
-            const int slot = wave_append(pp.sq.count, sv);
-            if (sv) {
-                if (slot < pp.sq.cap) {
-                    pp.sq.y[slot] = q.y[i]; pp.sq.z[slot] = q.z[i];
-                    pp.sq.vx[slot] = q.vx[i]; pp.sq.vy[slot] = q.vy[i]; pp.sq.vz[slot] = q.vz[i];
-                    pp.sq.E[slot] = q.E[i];
-                    pp.sq.t[slot] = q.t[i];
-                    pp.sq.info[slot] = q.info[i];
-                }
-                continue;
-            }
+        return (int64_t)lo * q.region + (gi - (int)s_pre[lo]);
+    };
+    if (MODE == PANEL_SURROGATE) {
+        /* the in-range photons go to the network's queue with one global
+         * reservation per block (one per wave on a single counter took
+         * most of this kernel's time): count the block's photons, reserve,
+         * then place them in passes whose slots come from a scan of the
+         * waves' counts (every thread of the block makes the same passes) */
+        __shared__ unsigned s_wc[4], s_sqbase;
+        unsigned cnt = 0;
+        for (int gi = blockIdx.x * blockDim.x + threadIdx.x; gi < nq; gi += stride) {
+            const int64_t i = item_of(gi);
+            cnt += surr_valid(q.E[i], q.vx[i]) ? 1u : 0u;
         }
+        s_part[threadIdx.x] = cnt;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned tot = 0;
+            for (int t = 0; t < (int)blockDim.x; ++t) tot += s_part[t];
+            s_sqbase = tot ? (unsigned)atomicAdd(pp.sq.count, (int)tot) : 0u;
+        }
+        __syncthreads();
+        const int wv = (int)threadIdx.x >> 6, nw = (int)blockDim.x >> 6;
+        unsigned off = s_sqbase;
+        for (int b0 = blockIdx.x * blockDim.x; b0 < nq; b0 += stride) {
+            const int gi = b0 + (int)threadIdx.x;
+            int64_t i = 0;
+            bool sv = false;
+            if (gi < nq) {
+                i = item_of(gi);
+                sv = surr_valid(q.E[i], q.vx[i]);
+            }
+            const unsigned long long m = __ballot(sv);
+            if (__lane_id() == 0) s_wc[wv] = (unsigned)__popcll(m);
+            __syncthreads();
+            unsigned before = off, pass = 0;
+            for (int w = 0; w < nw; ++w) {
+                before += w < wv ? s_wc[w] : 0u;
+                pass += s_wc[w];
+            }
+            const unsigned slot = before + (unsigned)__popcll(m & lanemask_lt());
+            if (sv && (int64_t)slot < (int64_t)pp.sq.cap) {
+                pp.sq.y[slot] = q.y[i]; pp.sq.z[slot] = q.z[i];
+                pp.sq.vx[slot] = q.vx[i]; pp.sq.vy[slot] = q.vy[i]; pp.sq.vz[slot] = q.vz[i];
+                pp.sq.E[slot] = q.E[i];
+                pp.sq.t[slot] = q.t[i];
+                pp.sq.info[slot] = q.info[i];
+            }
+            off += pass;
+            __syncthreads();
+        }
+    }
+    for (int gi = blockIdx.x * blockDim.x + threadIdx.x; gi < nq; gi += stride) {
+        const int64_t i = item_of(gi);
+        /* in-range photons were handed to the network above */
+        if (MODE == PANEL_SURROGATE && surr_valid(q.E[i], q.vx[i])) continue;
         float fx = 0.0f, fy = 0.0f, fz = 0.0f;
         bool got = false;
         float x2 = 0.0f, y2 = q.y[i], z2 = q.z[i];
