@@ -235,38 +235,65 @@ __device__ __forceinline__ unsigned long long sortable_t(double t)
 }
 
 /* blur (gPET_kernals.cu:814-837) + time blur + thresholder flags. */
+/* Blur and thresholder; flag = passes the thresholder.  With nthr set (no
+ * deadtime: the energy window does not depend on time order, so it is
+ * applied here, before the time sort) flag = passes the thresholder and the
+ * window, and nthr counts the thresholder's passes (one atomic per block). */
 __global__ __launch_bounds__(256) void k_blur_threshold(const gpet_event *in, gpet_event *out, unsigned *flag, int64_t n,
-                                                        uint64_t seed, gpet_digitizer g)
+                                                        uint64_t seed, gpet_digitizer g, unsigned *nthr)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    gpet_event e = in[i];
-    gp_rng r;
-    gp_rng_init(&r, seed, (uint64_t)(uint32_t)e.parn, (uint32_t)GP_TAG_DIGI | ((uint32_t)e.siten << 4), 0u);
-    float R = 0.0f;
-    if (g.blur_policy == 0) R = sqrtf(g.e_ref / e.E) * g.r_ref;
-    if (g.blur_policy == 1) R = g.r_ref + g.slope * (e.E - g.e_ref) / 1.0e6f;
-    if (R < 0.0f) R = 0.0f;
-    float nr = gp_rng_normal(&r);
-    e.E = e.E + ((nr * R) * e.E) / 2.35482f;
-    if (g.space_blur > 0.0f) {
-        e.x = e.x + g.space_blur * gp_rng_normal(&r);
-        e.y = e.y + g.space_blur * gp_rng_normal(&r);
-        e.z = e.z + g.space_blur * gp_rng_normal(&r);
+    __shared__ unsigned s_thr;
+    if (threadIdx.x == 0) s_thr = 0u;
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool thr = false;
+    if (i < n) {
+        gpet_event e = in[i];
+        gp_rng r;
+        gp_rng_init(&r, seed, (uint64_t)(uint32_t)e.parn, (uint32_t)GP_TAG_DIGI | ((uint32_t)e.siten << 4), 0u);
+        float R = 0.0f;
+        if (g.blur_policy == 0) R = sqrtf(g.e_ref / e.E) * g.r_ref;
+        if (g.blur_policy == 1) R = g.r_ref + g.slope * (e.E - g.e_ref) / 1.0e6f;
+        if (R < 0.0f) R = 0.0f;
+        float nr = gp_rng_normal(&r);
+        e.E = e.E + ((nr * R) * e.E) / 2.35482f;
+        if (g.space_blur > 0.0f) {
+            e.x = e.x + g.space_blur * gp_rng_normal(&r);
+            e.y = e.y + g.space_blur * gp_rng_normal(&r);
+            e.z = e.z + g.space_blur * gp_rng_normal(&r);
+        }
+        if (g.time_blur_fwhm_us > 0.0) {
+            float nt = gp_rng_normal(&r);
+            e.t = e.t + (double)nt * (g.time_blur_fwhm_us / 2.35482);
+        }
+        out[i] = e;
+        thr = !(e.E < g.e_threshold || e.E > g.e_upholder);
+        flag[i] = (thr && (!nthr || !(e.E < g.win_min || e.E > g.win_max))) ? 1u : 0u;
     }
-    if (g.time_blur_fwhm_us > 0.0) {
-        float nt = gp_rng_normal(&r);
-        e.t = e.t + (double)nt * (g.time_blur_fwhm_us / 2.35482);
+    if (nthr) {
+        const unsigned long long b = __ballot(thr);
+        if (__lane_id() == 0 && b) atomicAdd(&s_thr, (unsigned)__popcll(b));
+        __syncthreads();
+        if (threadIdx.x == 0 && s_thr) atomicAdd(nthr, s_thr);
     }
-    out[i] = e;
-    flag[i] = (e.E < g.e_threshold || e.E > g.e_upholder) ? 0u : 1u;
 }
 
+/* Compaction (input order kept); with key set it also writes each kept
+ * event's time key and its index, the time sort's input. */
 __global__ __launch_bounds__(256) void k_compact_events(const gpet_event *in, const unsigned *flag, const unsigned *pos,
-                                                        gpet_event *out, int64_t n)
+                                                        gpet_event *out, int64_t n, unsigned long long *key,
+                                                        unsigned *val)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && flag[i]) out[pos[i]] = in[i];
+    if (i < n && flag[i]) {
+        const gpet_event e = in[i];
+        const unsigned j = pos[i];
+        out[j] = e;
+        if (key) {
+            key[j] = sortable_t(e.t);
+            val[j] = j;
+        }
+    }
 }
 
 /* Job of a history in a batched run: the largest j with jf[j] <= h. */
@@ -386,7 +413,9 @@ __global__ __launch_bounds__(256) void k_gather_singles(const gpet_event *ev, co
                                                         const unsigned *pos, gpet_event *out, int64_t n)
 {
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < n && flag[j]) out[pos[j]] = ev[perm[j]];
+    if (j >= n) return;
+    if (!flag) out[j] = ev[perm[j]];   /* every event, in permutation order */
+    else if (flag[j]) out[pos[j]] = ev[perm[j]];
 }
 
 /* Coincidence windows inside one cluster, as the sequential rule: a window
@@ -719,6 +748,7 @@ struct DigiWork {
     gpet_event *singles = nullptr, *coinc = nullptr, *delayed = nullptr;
     int64_t singles_cap = 0, coinc_cap = 0, delayed_cap = 0;
     unsigned long long *ntrue = nullptr;
+    unsigned *nthr = nullptr;
     unsigned *pidx = nullptr;
     int64_t pidx_cap = 0;
     float *rec_tmp = nullptr;
@@ -794,7 +824,7 @@ void gpet_digi_work_destroy(DigiWork *w)
     if (!w) return;
     void *ptrs[] = {w->ev_a,  w->ev_b,  w->flag,      w->pos,  w->alive,   w->key_a,   w->key_b,
                     w->val_a, w->val_b, w->tile_sums, w->hist, w->grand,   w->singles, w->coinc,
-                    w->ntrue, w->delayed, w->noise_cnt, w->noise_pos, w->noise_jobs, w->pidx, w->rec_tmp};
+                    w->ntrue, w->nthr, w->delayed, w->noise_cnt, w->noise_pos, w->noise_jobs, w->pidx, w->rec_tmp};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete w;
@@ -825,18 +855,33 @@ int gpet_digitize(DigiWork *w, const DigiParams &p, const gpet_event *events, in
     if (!w->ntrue) HIPCHK(hipMalloc((void **)&w->ntrue, sizeof(unsigned long long) * CNT_STRIPES));
     HIPCHK(hipMemsetAsync(w->ntrue, 0, sizeof(unsigned long long) * CNT_STRIPES, s));
 
-    /* blur + thresholder, compaction (input order kept) */
-    unsigned nT = 0;
+    /* blur + thresholder, compaction (input order kept).  Without deadtime
+     * the energy window is applied here too (early): the window keeps the
+     * same events in any order, so sorting only the window's events by
+     * time gives the same singles in the same order, and the compaction
+     * writes the time keys as it goes. */
+    const bool early = !(g.dead_time_us > 0.0f);
+    unsigned nT = 0, nsel = 0;
     if (n > 0) {
-        hipLaunchKernelGGL(k_blur_threshold, dim3(grid_for(n, 256)), dim3(256), 0, s, events, w->ev_b, w->flag, n, p.seed, g);
+        if (early) {
+            if (!w->nthr) HIPCHK(hipMalloc((void **)&w->nthr, sizeof(unsigned)));
+            HIPCHK(hipMemsetAsync(w->nthr, 0, sizeof(unsigned), s));
+        }
+        hipLaunchKernelGGL(k_blur_threshold, dim3(grid_for(n, 256)), dim3(256), 0, s, events, w->ev_b, w->flag, n, p.seed, g,
+                           early ? w->nthr : (unsigned *)nullptr);
         HIPCHK(hipGetLastError());
-        if (scan(w, w->flag, w->pos, n, &nT, s, err, errlen)) return -1;
-        hipLaunchKernelGGL(k_compact_events, dim3(grid_for(n, 256)), dim3(256), 0, s, w->ev_b, w->flag, w->pos, w->ev_a, n);
+        if (scan(w, w->flag, w->pos, n, &nsel, s, err, errlen)) return -1;
+        hipLaunchKernelGGL(k_compact_events, dim3(grid_for(n, 256)), dim3(256), 0, s, w->ev_b, w->flag, w->pos, w->ev_a, n,
+                           early ? w->key_a : (unsigned long long *)nullptr, early ? w->val_a : (unsigned *)nullptr);
         HIPCHK(hipGetLastError());
+        if (early)
+            HIPCHK(hipMemcpy(&nT, w->nthr, sizeof(unsigned), hipMemcpyDeviceToHost));
+        else
+            nT = nsel;
     }
     out->n_threshold = nT;
     const gpet_event *ev = w->ev_a;
-    const int64_t m = nT;
+    const int64_t m = nsel;
     if (g.dead_level != 3 && m > 0) {
         hipLaunchKernelGGL(k_set_site, dim3(grid_for(m, 256)), dim3(256), 0, s, w->ev_a, m, g.dead_level, p.moduleN,
                            p.crystalN);
@@ -845,8 +890,10 @@ int gpet_digitize(DigiWork *w, const DigiParams &p, const gpet_event *events, in
 
     /* time order: perm_t in val_a */
     if (m > 0) {
-        hipLaunchKernelGGL(k_time_keys, dim3(grid_for(m, 256)), dim3(256), 0, s, ev, w->key_a, w->val_a, m);
-        HIPCHK(hipGetLastError());
+        if (!early) {
+            hipLaunchKernelGGL(k_time_keys, dim3(grid_for(m, 256)), dim3(256), 0, s, ev, w->key_a, w->val_a, m);
+            HIPCHK(hipGetLastError());
+        }
         if (radix_sort(w, m, 64, s, err, errlen)) return -1;
         if (nj > 1) {
             hipLaunchKernelGGL(k_job_keys, dim3(grid_for(m, 256)), dim3(256), 0, s, ev, w->val_a, w->key_a, m, jf, nj);
@@ -858,7 +905,7 @@ int gpet_digitize(DigiWork *w, const DigiParams &p, const gpet_event *events, in
      * sort sites on (key_b,val_b) after moving perm_t aside into `alive` ... */
     unsigned *perm_t = w->val_a;
     unsigned *alive = nullptr;
-    int64_t n_dead = m;
+    int64_t n_dead = nT;
     if (g.dead_time_us > 0.0f && m > 0) {
         /* move perm_t out of the sort buffers */
         HIPCHK(hipMemcpyAsync(w->pos, perm_t, sizeof(unsigned) * m, hipMemcpyDeviceToDevice, s));
@@ -876,7 +923,14 @@ int gpet_digitize(DigiWork *w, const DigiParams &p, const gpet_event *events, in
     }
     /* window flags in time order, compaction -> singles */
     unsigned ns = 0;
-    if (m > 0) {
+    if (m > 0 && early) {
+        /* the window was applied before the sort: every event is a single */
+        ns = (unsigned)m;
+        if (ensure(w->singles, w->singles_cap, std::max<int64_t>(ns, 1), err, errlen)) return -1;
+        hipLaunchKernelGGL(k_gather_singles, dim3(grid_for(m, 256)), dim3(256), 0, s, ev, perm_t, (const unsigned *)nullptr,
+                           (const unsigned *)nullptr, w->singles, m);
+        HIPCHK(hipGetLastError());
+    } else if (m > 0) {
         if (alive) {
             unsigned long long *tmp = w->key_b; /* reuse as a uint buffer of >= m entries */
             unsigned *flag2 = reinterpret_cast<unsigned *>(tmp);
