@@ -311,8 +311,10 @@ int gpet_adam(float *w, float *m, float *v, const float *g, const float *gscale,
               float b1, float b2, float c1, float c2, float eps, hipStream_t s);
 /* factor[0] = min(1, clip / |g|) (240 partials in part). */
 int gpet_clip_factor(const float *g, int64_t n, float clip, float *part, float *factor, hipStream_t s);
+/* refl: bit 0 / bit 1 mirror half the rows in local y / z (features,
+ * direction and lateral offset negated). */
 int gpet_pack_train(const float *set, int64_t n, uint64_t seed, int64_t step, int64_t M, uint16_t *X, float *Y,
-                    uint16_t *XT, int64_t ldt, hipStream_t s);
+                    uint16_t *XT, int64_t ldt, int refl, hipStream_t s);
 int gpet_loss_reduce(const float *part, int n, float *out, hipStream_t s);
 /* The whole network in one kernel (layer l's bf16 weights w[l]): with X
  * (packed bf16 [n x 8] inputs) the SURR_NH head outputs of each row go to H

@@ -130,6 +130,26 @@ std::vector<float> init_params(uint64_t seed)
     return w;
 }
 
+/* Mirror symmetries of every panel's response (bit 0: local y, bit 1:
+ * local z): the modules fill the panel and the crystals fill each module
+ * exactly, so the crystal grid is its own mirror image and a photon's
+ * response to the mirrored entry is the mirrored response. */
+int surr_reflections(const gpet_ctx *c)
+{
+    const DevDetector &d = c->ddet;
+    if (d.moduleNy <= 0 || d.crystalNy <= 0 || c->panels.empty()) return 0;
+    const int mnz = d.moduleN / d.moduleNy, cnz = d.crystalN / d.crystalNy;
+    auto fills = [](int k, double len, double gap, double total) {
+        return std::fabs((double)k * len + (double)(k - 1) * gap - total) <= 1e-5 * total;
+    };
+    int bits = 3;
+    for (const gpet_panel &p : c->panels) {
+        if (!(fills(d.moduleNy, p.mody, p.mspy, p.leny) && fills(d.crystalNy, p.cry, p.cspy, p.mody))) bits &= ~1;
+        if (!(fills(mnz, p.modz, p.mspz, p.lenz) && fills(cnz, p.crz, p.cspz, p.modz))) bits &= ~2;
+    }
+    return bits;
+}
+
 gpet_status alloc_work(gpet_ctx *c, int64_t rows, bool train)
 {
     SurrState &S = c->surr;
@@ -382,6 +402,7 @@ gpet_status gpet_surrogate_train(gpet_ctx *c, const gpet_surrogate_config *cfg, 
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     const int64_t warm = std::max<int64_t>(1, cfg->steps / 50);
     const int nblk = (int)((M + 255) / 256);
+    const int refl = surr_reflections(c);
     HIPOK(c, hipStreamSynchronize(c->stream));
     auto t0 = std::chrono::steady_clock::now();
     for (int64_t step = 0; step < cfg->steps; ++step) {
@@ -391,7 +412,7 @@ gpet_status gpet_surrogate_train(gpet_ctx *c, const gpet_surrogate_config *cfg, 
             double p = (double)(step - warm) / (double)std::max<int64_t>(1, cfg->steps - warm);
             lr = (float)(cfg->lr * (0.05 + 0.95 * 0.5 * (1.0 + std::cos(3.141592653589793 * p))));
         }
-        KOK(c, gpet_pack_train(S.set.p, S.set_n, cfg->seed, S.adam_t, M, S.X.p, S.Y.p, S.XT.p, S.ldt, c->stream),
+        KOK(c, gpet_pack_train(S.set.p, S.set_n, cfg->seed, S.adam_t, M, S.X.p, S.Y.p, S.XT.p, S.ldt, refl, c->stream),
             "pack");
         if ((st = forward(c, M, true))) return st;
         KOK(c, gpet_head_loss_grad(S.HT.p, S.ldt, S.Y.p, M, 1.0f / (float)M, S.dH.p, HEAD_LD, S.dHT.p, S.lpart.p,

@@ -508,7 +508,7 @@ __global__ void k_loss_reduce(const float *part, int n, float *out)
  * (bf16 [M x 8]: 6 features, 1, 0) and targets (fp32 [M x SURR_NY], the
  * deposited fraction replaced by its energy class). */
 __global__ void k_pack_train(const float *set, int64_t n, uint64_t seed, int64_t step, int64_t M, uint16_t *X,
-                             float *Y, uint16_t *XT, int64_t ldt)
+                             float *Y, uint16_t *XT, int64_t ldt, int refl)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
@@ -517,15 +517,24 @@ __global__ void k_pack_train(const float *set, int64_t n, uint64_t seed, int64_t
     const uint64_t u = ((uint64_t)rr.x << 32) | rr.y;
     const int64_t j = (int64_t)(u % (uint64_t)n);
     const float *r = set + j * SURR_REC;
+    /* mirror images of the record (half the rows each) where the panels'
+     * crystal grid is its own mirror image: (y, vy, dy) -> (-y, -vy, -dy),
+     * and the same in z */
+    const bool ry = (refl & 1) && (rr.z & 1u), rz = (refl & 2) && (rr.z & 2u);
 #pragma unroll
     for (int c = 0; c < SURR_LDX; ++c) {
-        const uint16_t x = c < SURR_NX ? f2bf(r[c]) : (c == SURR_NX ? (uint16_t)0x3f80 : (uint16_t)0);
+        float v = c < SURR_NX ? r[c] : 0.0f;
+        if ((c == 2 || c == 4) && ry) v = -v;
+        if ((c == 3 || c == 5) && rz) v = -v;
+        const uint16_t x = c < SURR_NX ? f2bf(v) : (c == SURR_NX ? (uint16_t)0x3f80 : (uint16_t)0);
         X[i * SURR_LDX + c] = x;
         if (XT) XT[(int64_t)c * ldt + i] = x;
     }
 #pragma unroll
     for (int c = 0; c < SURR_NY; ++c) Y[i * SURR_NY + c] = r[SURR_NX + c];
     Y[i * SURR_NY + 1] = (float)surr_class(r[0], r[SURR_NX + 1]);
+    if (ry) Y[i * SURR_NY + 3] = -r[SURR_NX + 3];
+    if (rz) Y[i * SURR_NY + 4] = -r[SURR_NX + 4];
 }
 
 /* Elementary functions of the sampling: the hardware approximations
@@ -1053,10 +1062,10 @@ int gpet_adam(float *w, float *m, float *v, const float *g, const float *gscale,
 }
 
 int gpet_pack_train(const float *set, int64_t n, uint64_t seed, int64_t step, int64_t M, uint16_t *X, float *Y,
-                    uint16_t *XT, int64_t ldt, hipStream_t s)
+                    uint16_t *XT, int64_t ldt, int refl, hipStream_t s)
 {
     hipLaunchKernelGGL(k_pack_train, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, set, n, seed, step, M, X, Y,
-                       XT, ldt);
+                       XT, ldt, refl);
     return (int)hipGetLastError();
 }
 
