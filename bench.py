@@ -480,8 +480,9 @@ def main():
     ap.add_argument("--train-steps", type=int, default=48000)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
-              "(histories shard by index; see the design notes).", file=sys.stderr)
+        print("bench.py measures one GPU only: multi-GPU runs are not implemented (a run would shard "
+              "its histories by index range; each range gives the same per-history results on any GPU).",
+              file=sys.stderr)
         sys.exit(2)
 
     from gpet_amd import engine
