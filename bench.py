@@ -338,7 +338,8 @@ def surrogate_accuracy(eng, run, step_arg, n_panels, wl, tabs):
     finally:
         other.close()
     (sp, cp, csp, ehp), (ss, cs_, css, ehs) = res[False], res[True]
-    out = {"singles_ratio": ss / max(sp, 1), "coincidences_ratio": cs_ / max(cp, 1)}
+    out = {"seeds": "plain MC with the workload's seed, surrogate mode with seed + 1000 (independent samples)",
+           "singles_ratio": ss / max(sp, 1), "coincidences_ratio": cs_ / max(cp, 1)}
     panel_p = csp.reshape(n_panels, -1).sum(1)
     panel_s = css.reshape(n_panels, -1).sum(1)
     out["per_panel_max_rel_diff"] = float(np.max(np.abs(panel_s / panel_p - 1)))
