@@ -54,9 +54,9 @@ WORKLOADS = {
     # batched workload's 1.47e8 in five: per-launch fixed costs (queue-region
     # prefixes, counter flushes, the last waves of each grid) then weigh less
     # (batched: 3.64e9 / 3.70e9 / 3.75e9 histories/s at 2^23 / 2^24 / 2^25).
-    # The body workload keeps the default: its event buffer (1.2 events per
-    # decay) plus a 2^24 batch's headroom does not fit with the digitizer's
-    # work buffers.
+    # The body workload runs in 2^24 (1.15e9 -> 1.17e9 histories/s against
+    # 2^23): its event buffer (1.2 events per decay) and the digitizer's work
+    # buffers leave no room for a 2^25 batch's headroom.
     "cylinder": dict(config=2, decays=1e8, seed=2, batch=1 << 25, counters="k_phantom_cylinder200_1e8_counters.md",
                      text="cylinder phantom 200^3 over 1 cm (water r=0.4, bone rod r=0.1, air), uniform F-18 "
                           "with positron range, %.0e decays over 10 s, 8-panel LSO octagon at 8 cm, "
@@ -65,7 +65,7 @@ WORKLOADS = {
                   text="64 blob phantoms 32^3..256^3 over 10 cm (water/bone/lung/air, seed 3), F-18 cylinder "
                        "sources, 1e5..1e7 decays each (%.3g in total) over 10 s, one batched run on the 8-panel "
                        "octagon, plain MC (no surrogate)"),
-    "body": dict(config=5, decays=1e9, seed=5, harvest=(1, 1e8), surrogate_decays=2e8,
+    "body": dict(config=5, decays=1e9, seed=5, batch=1 << 24, harvest=(1, 1e8), surrogate_decays=2e8,
                  text="body phantom 256x256x512 at 1 mm (water ellipsoid 23x17x48 cm, lungs 0.3 g/cc, bone spine, "
                       "20 hot spheres 8:1), F-18 with positron range, %.0e decays over 60 s, 4 rings x 12 LSO "
                       "panels (20x20 crystals, 40 cm ring), plain MC (no surrogate)"),

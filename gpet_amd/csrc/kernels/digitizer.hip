@@ -737,8 +737,8 @@ inline int grid_for(int64_t n, int block) { return (int)((n + block - 1) / block
 } // namespace
 
 struct DigiWork {
-    gpet_event *ev_a = nullptr, *ev_b = nullptr;  /* blurred, thresholded */
-    int64_t ev_cap = 0;
+    gpet_event *ev_a = nullptr, *ev_b = nullptr;  /* compacted, blurred */
+    int64_t ev_a_cap = 0, ev_b_cap = 0;
     unsigned *flag = nullptr, *pos = nullptr, *alive = nullptr;
     unsigned long long *key_a = nullptr, *key_b = nullptr;
     unsigned *val_a = nullptr, *val_b = nullptr;
@@ -849,9 +849,7 @@ int gpet_digitize(DigiWork *w, const DigiParams &p, const gpet_event *events, in
         w->cap = std::min({c1, c2, c3, c4, c5, c6, c7});
         (void)c8;
     }
-    int64_t ce = w->ev_cap, ce2 = w->ev_cap;
-    if (ensure(w->ev_a, ce, need, err, errlen) || ensure(w->ev_b, ce2, need, err, errlen)) return -1;
-    w->ev_cap = std::min(ce, ce2);
+    if (ensure(w->ev_b, w->ev_b_cap, need, err, errlen)) return -1;
     if (!w->ntrue) HIPCHK(hipMalloc((void **)&w->ntrue, sizeof(unsigned long long) * CNT_STRIPES));
     HIPCHK(hipMemsetAsync(w->ntrue, 0, sizeof(unsigned long long) * CNT_STRIPES, s));
 
@@ -871,6 +869,9 @@ int gpet_digitize(DigiWork *w, const DigiParams &p, const gpet_event *events, in
                            early ? w->nthr : (unsigned *)nullptr);
         HIPCHK(hipGetLastError());
         if (scan(w, w->flag, w->pos, n, &nsel, s, err, errlen)) return -1;
+        /* the compacted events: the thresholder's (and, early, the window's)
+         * passes only, often well under half of the input */
+        if (ensure(w->ev_a, w->ev_a_cap, std::max<int64_t>(nsel, 1), err, errlen)) return -1;
         hipLaunchKernelGGL(k_compact_events, dim3(grid_for(n, 256)), dim3(256), 0, s, w->ev_b, w->flag, w->pos, w->ev_a, n,
                            early ? w->key_a : (unsigned long long *)nullptr, early ? w->val_a : (unsigned *)nullptr);
         HIPCHK(hipGetLastError());
